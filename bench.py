@@ -1,0 +1,162 @@
+"""Benchmark: jittered slc/slt field mapping of a 2048x2048 DTM on one MI355X.
+
+One step = the full fields pass of the tracing path: integrate_fields
+downstream and upstream over every seed (2x2 sub-pixel seeds per pixel,
+jitter 1.0), i.e. both slc/slt maps of the 2048^2 tile at 2 m.  Inputs (uv,
+mask, seeds) are resident in HBM before timing starts; each step clears and
+re-accumulates the fields.
+
+metric: streamline-steps/s = accepted RK2 steps of all streamlines, both
+directions, counted exactly by the kernel, divided by wall time of the timed
+steps.  ms_per_step is the 2048^2 slc/slt map time.
+
+roofline: dominant kernel k_fields, algorithmic 73 B per step (64 B of uv
+gathers, 1 B mask, 8 B of slc/slt atomics) x steps per launch / mean launch
+time from HIP events on the launch stream; peak 8 TB/s HBM.
+cpu_baseline: the CPU oracle (C restatement of the reference kernels, OpenMP)
+on a bounded random sample of the same seeds, steps/s.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_STEP = 73
+HBM_PEAK_GBS = 8000.0
+
+
+def _cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(job, target_s=12.0):
+    """Oracle steps/s on the first n seeds of the shuffled seed list (a uniform
+    random sample of the map, RNG keys as in the full run), both directions;
+    n grows until one measurement takes a few seconds."""
+    from oracle import oracle
+    cores = _cores()
+    m = job.mask.astype(np.uint8).reshape(-1)
+    uv = job.uv.reshape(-1)
+    n_total = job.seeds.shape[0]
+    n = 4096
+    while True:
+        n = min(n, n_total)
+        steps = 0
+        t0 = time.perf_counter()
+        for sign in (1, -1):
+            p = job.params(sign, n_seed_points=n)
+            steps += oracle.trace_fields(p, job.seeds, m, uv, nthreads=cores)[2]
+        dt = time.perf_counter() - t0
+        if dt > target_s / 4 or n >= n_total:
+            break
+        n *= 4
+    return {"value": steps / dt, "unit": "streamline-steps/s", "cores": cores, "kind": "port",
+            "sample": "first %d of %d shuffled seeds (x%d sub-seeds, both directions): %d steps in "
+                      "%.1f s; C restatement of the reference kernels, OpenMP" %
+                      (n, n_total, int(job.info.subpixel_seed_point_density) ** 2, steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the tracing path is measured on a single MI355X "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from slm_amd import hip, workloads
+
+    dev = torch.device("cuda", 0)
+    job = getattr(workloads, args.workload)()
+    seeds = torch.from_numpy(job.seeds).to(dev)
+    mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)
+    uv = torch.from_numpy(np.ascontiguousarray(job.uv).reshape(-1)).to(dev)
+    pd, pu = job.params(1), job.params(-1)
+    nxy = pd.nxy_padded
+    slc = torch.empty((2, nxy), dtype=torch.int32, device=dev)
+    slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
+    steps = torch.zeros(2, dtype=torch.int64, device=dev)
+    ws = torch.empty(hip.fields_workspace_size(pd), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def one_step(keep_order, events=None):
+        for i, p in enumerate((pd, pu)):
+            if events is not None:
+                events[i][0].record(stream)
+            hip.trace_fields(p, seeds, mask, uv, slc[i], slt[i], steps[i:i + 1], workspace=ws,
+                             stream=stream, keep_order=keep_order or i == 1)
+            if events is not None:
+                events[i][1].record(stream)
+
+    one_step(False)  # builds the seed order
+    for _ in range(args.warmup):
+        one_step(True)
+    torch.cuda.synchronize()
+    steps.zero_()
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(2)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        one_step(True, ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_steps = int(steps.sum().item())
+    launch_ms = [e[i][0].elapsed_time(e[i][1]) for e in ev for i in range(2)]
+    mean_launch_s = float(np.mean(launch_ms)) / 1e3
+    steps_per_launch = total_steps / (2 * args.steps)
+    achieved = steps_per_launch * BYTES_PER_STEP / mean_launch_s / 1e9
+    value = total_steps / wall
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "k_fields_tile_2048_traffic.json")
+    if args.workload == "tile_2048" and os.path.exists(tfile):
+        with open(tfile) as fp:
+            traffic = json.load(fp).get("bytes_per_launch")
+    out = {
+        "metric": "streamline-steps/s",
+        "value": value,
+        "unit": "streamline-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded spectral DTM, V-valleys, sinks filled)",
+        "config": {"workload": job.name, "grid": list(job.roi.shape),
+                   "pixel_size_m": job.pixel_size, "seeds": int(job.seeds.shape[0]),
+                   "subpixel_density": int(job.info.subpixel_seed_point_density),
+                   "streamlines_per_direction": job.n_streamlines(),
+                   "jitter": float(job.info.jitter_magnitude), "max_n_steps": int(job.info.max_n_steps),
+                   "directions": 2, "map_ms": wall / args.steps * 1e3},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fields", "bytes_per_step": BYTES_PER_STEP,
+                     "steps_per_launch": steps_per_launch, "mean_launch_ms": mean_launch_s * 1e3},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(job)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/*
+ * slm_hip.h -- C ABI of libslm_hip.so, the MI355X (gfx950) streamline tracer.
+ *
+ * The reference (atthecodeface/slm) drives OpenCL kernels through PyOpenCL:
+ * every scalar is a compile-time -D macro (python/streamlines/pocl.py:178-299)
+ * and buffers are bound positionally (pocl.py:536-539).  This library replaces
+ * that boundary with plain extern "C" functions:
+ *   - every macro of pocl.py:225-297 becomes a field of SlmParams, passed by
+ *     pointer at run time (no per-call compile);
+ *   - every buffer is a caller-owned device pointer (PyTorch-ROCm tensors in the
+ *     Python facade); the library never allocates caller-visible memory;
+ *   - scratch memory (seed work lists, queue counters) lives in a caller buffer
+ *     whose size the matching *_workspace_size() function reports;
+ *   - work is enqueued on the caller's stream (hipStream_t passed as void*);
+ *     nothing synchronises unless SLM_FLAG_SYNC is set in the call.
+ *
+ * Grid conventions follow the reference: arrays are C-order [nx_padded][ny_padded],
+ * the flat index of pixel (x,y) is ny_padded*x + y (opencl/essentials.cl:56-59),
+ * mask bytes are 1 = masked, uv is interleaved float2 (u,v) per pixel.
+ *
+ * Return codes: SLM_OK (0) or a negative SLM_E_* value; slm_last_error()
+ * returns a thread-local message for the last failure on the calling thread.
+ */
+#ifndef SLM_HIP_H
+#define SLM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLM_ABI_VERSION 1u
+
+#define SLM_OK              0
+#define SLM_E_INVALID_ARG  -1
+#define SLM_E_HIP          -2
+#define SLM_E_OOM          -3
+#define SLM_E_ABI          -4
+#define SLM_E_WORKSPACE    -5
+
+/* Call flags */
+#define SLM_FLAG_SYNC       1u   /* hipStreamSynchronize before returning            */
+#define SLM_FLAG_NO_ZERO    2u   /* do not clear the output fields (accumulate into) */
+#define SLM_FLAG_KEEP_ORDER 4u   /* workspace already holds the seed work order built by
+                                    a previous slm_trace_fields call with the same seeds,
+                                    n_seed_points, seeds_chunk_offset and grid: reuse it */
+
+/*
+ * Every kernel constant of the reference's -D macro set (pocl.py:225-297, values
+ * derived by useful.py:128-220), as run-time data.  Field names are the macro
+ * names in lower case.  All fields are 4 bytes; the struct is 128 bytes.
+ */
+typedef struct SlmParams {
+    uint32_t abi_version;                 /* must equal SLM_ABI_VERSION              */
+    uint32_t nx_padded;                   /* NX_PADDED                               */
+    uint32_t ny_padded;                   /* NY_PADDED                               */
+    uint32_t nxy_padded;                  /* NXY_PADDED                              */
+    uint32_t pad_width;                   /* PAD_WIDTH                               */
+    float    pad_width_pp5;               /* PAD_WIDTH_PP5 = pad+0.5                 */
+    float    nxf_mp5;                     /* NXF_MP5 = nx-0.5                        */
+    float    nyf_mp5;                     /* NYF_MP5 = ny-0.5                        */
+    float    grid_scale;                  /* GRID_SCALE = sqrt(nx*ny)                */
+    float    combo_factor;                /* COMBO_FACTOR, sign of direction applied */
+    float    dt_max;                      /* DT_MAX                                  */
+    float    adjusted_max_error;          /* ADJUSTED_MAX_ERROR                      */
+    float    integration_halt_threshold;  /* INTEGRATION_HALT_THRESHOLD              */
+    float    max_length;                  /* MAX_LENGTH (pixels)                     */
+    uint32_t max_n_steps;                 /* MAX_N_STEPS                             */
+    float    trajectory_resolution;       /* TRAJECTORY_RESOLUTION (128)             */
+    uint32_t subpixel_seed_point_density; /* SUBPIXEL_SEED_POINT_DENSITY (d)         */
+    float    subpixel_seed_halfspan;      /* SUBPIXEL_SEED_HALFSPAN                  */
+    float    subpixel_seed_step;          /* SUBPIXEL_SEED_STEP                      */
+    float    jitter_magnitude;            /* JITTER_MAGNITUDE                        */
+    float    downup_sign;                 /* (float)DOWNUP_SIGN: 1 or 2^32 ("-1u")   */
+    uint32_t n_seed_points;               /* N_SEED_POINTS                           */
+    uint32_t seeds_chunk_offset;          /* SEEDS_CHUNK_OFFSET                      */
+    uint32_t fp_contract;                 /* 1: a*b+c fused as clang's OpenCL default
+                                             FP_CONTRACT ON does; 0: every op rounded */
+    uint32_t reserved[8];
+} SlmParams;
+
+/* Library identity. */
+uint32_t    slm_abi_version(void);
+const char* slm_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int         slm_device_count(void);
+
+/*
+ * Lehmer RNG state after seeding, for n seed indices: out[i] = rng0(idx0+i).
+ * Replaces the seeding in opencl/integratefields.cl:93-99 (byte-reverse,
+ * one lehmer_rand_uint draw of opencl/rng.cl:37-45, byte-reverse).
+ * Also writes n_draws successive lehmer_rand_uint states per seed into
+ * draws[i*n_draws + k] when draws is non-NULL (RNG-stream check).
+ */
+int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws,
+                   uint32_t* rng0_out, uint32_t* draws, void* stream, uint32_t flags);
+
+/*
+ * Jittered streamline fields (slc / slt), one direction.
+ * Replaces kernel integrate_fields (opencl/integratefields.cl:60-114 with
+ * jittertrajectory.cl:42-88) as launched by Fields.gpu_integrate
+ * (python/streamlines/fields.py:233-274).
+ *   seeds_xy : float2 [n_seed_points], the shuffled seed list (seed_point_array);
+ *              the RNG of seed k is keyed on seeds_chunk_offset+k.
+ *   mask     : uint8 [nxy_padded]     uv : float2 [nxy_padded]
+ *   slc, slt : uint32 [nxy_padded], cleared first unless SLM_FLAG_NO_ZERO.
+ *   steps_out: optional uint64 device counter += accepted RK2 steps (not cleared).
+ * All sub-pixel streamlines (d*d per seed) run as independent lanes pulled
+ * from a work queue, so results do not depend on scheduling (integer sums).
+ */
+size_t slm_trace_fields_workspace_size(const SlmParams* p);
+int slm_trace_fields(const SlmParams* p, const float* seeds_xy,
+                     const uint8_t* mask, const float* uv,
+                     uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                     void* workspace, size_t workspace_bytes,
+                     void* stream, uint32_t flags);
+
+/*
+ * Un-jittered trajectory recording, one direction.
+ * Replaces kernel integrate_trajectory (opencl/integratetraj.cl:54-97 with
+ * trajectory.cl:41-80) as launched by Trajectories.gpu_compute_trajectories
+ * (python/streamlines/trajectories.py:257-333).
+ *   traj   : int8x2 [n_seed_points][max_n_steps] compressed steps
+ *   nsteps : uint16 [n_seed_points]      length : float [n_seed_points]
+ * Seed k (k < n_seed_points) is seeds_xy[seeds_chunk_offset + k] and writes
+ * row k of traj / nsteps / length.
+ */
+int slm_trace_paths(const SlmParams* p, const float* seeds_xy,
+                    const uint8_t* mask, const float* uv,
+                    int8_t* traj, uint16_t* nsteps, float* length,
+                    void* stream, uint32_t flags);
+
+/*
+ * Test hook with no reference counterpart: records the positions of selected
+ * jittered fields streamlines exactly as slm_trace_fields integrates them.
+ * For j < n_sel: streamline (seed_sel[j], sub_sel[j]) writes its positions
+ * (start point first) into xy[j][0..max_rec) as float2 and the count into
+ * nrec[j] (count includes the start; positions beyond max_rec are dropped).
+ */
+int slm_trace_fields_record(const SlmParams* p, const float* seeds_xy,
+                            const uint8_t* mask, const float* uv,
+                            const uint32_t* seed_sel, const uint32_t* sub_sel,
+                            uint32_t n_sel, uint32_t max_rec,
+                            float* xy, uint32_t* nrec,
+                            void* stream, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLM_HIP_H */

@@ -1,0 +1,113 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+
+Functions (each restates the reference kernel named in oracle/slm_oracle.c):
+  lehmer, rng_seed, rng_stream         rng.cl:37-67, integratefields.cl:93-99
+  trace_fields                         integratefields.cl:60-114 + jittertrajectory.cl
+  trace_fields_record                  one jittered streamline, positions recorded
+  trace_paths                          integratetraj.cl:54-97 + trajectory.cl
+Each has an FP64 twin (f64=True) built from the same source.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("oracle/liboracle.so is not built (make -C oracle)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for pre in ("slmo_", "slmo64_"):
+            getattr(lib, pre + "lehmer").restype = _U32
+            getattr(lib, pre + "lehmer").argtypes = [ctypes.POINTER(_U32)]
+            getattr(lib, pre + "rng_seed").restype = _U32
+            getattr(lib, pre + "rng_seed").argtypes = [_U32]
+            getattr(lib, pre + "rng_stream").restype = None
+            getattr(lib, pre + "rng_stream").argtypes = [_U32, _U32, _U32, _P, _P]
+            getattr(lib, pre + "trace_fields").restype = ctypes.c_uint64
+            getattr(lib, pre + "trace_fields").argtypes = [_P, _P, _P, _P, _P, _P, ctypes.c_int]
+            getattr(lib, pre + "trace_fields_record").restype = _U32
+            getattr(lib, pre + "trace_fields_record").argtypes = [_P, _P, _P, _P, _U32, _U32, _P, _U32, _P]
+            getattr(lib, pre + "trace_paths").restype = None
+            getattr(lib, pre + "trace_paths").argtypes = [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _pre(f64):
+    return "slmo64_" if f64 else "slmo_"
+
+
+def _c(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a, ctypes.c_void_p(a.ctypes.data)
+
+
+def lehmer(state: int, f64=False):
+    s = _U32(state)
+    v = getattr(load(), _pre(f64) + "lehmer")(ctypes.byref(s))
+    return int(v), int(s.value)
+
+
+def rng_seed(idx: int, f64=False) -> int:
+    return int(getattr(load(), _pre(f64) + "rng_seed")(idx))
+
+
+def rng_stream(idx0: int, n: int, n_draws: int):
+    rng0 = np.zeros(n, dtype=np.uint32)
+    draws = np.zeros((n, max(n_draws, 1)), dtype=np.uint32)
+    load().slmo_rng_stream(idx0, n, n_draws, rng0.ctypes.data, draws.ctypes.data if n_draws else None)
+    return rng0, draws[:, :n_draws]
+
+
+def trace_fields(p, seeds, mask, uv, nthreads=8, f64=False, slc=None, slt=None):
+    """Returns (slc uint32 [nxy], slt uint32 [nxy], accepted steps)."""
+    nxy = p.nxy_padded
+    slc = np.zeros(nxy, dtype=np.uint32) if slc is None else slc
+    slt = np.zeros(nxy, dtype=np.uint32) if slt is None else slt
+    seeds, ps = _c(seeds, np.float32)
+    mask, pm = _c(mask, np.uint8)
+    uv, pu = _c(uv, np.float32)
+    steps = getattr(load(), _pre(f64) + "trace_fields")(ctypes.byref(p), ps, pm, pu, slc.ctypes.data,
+                                                       slt.ctypes.data, int(nthreads))
+    return slc, slt, int(steps)
+
+
+def trace_fields_record(p, seeds, mask, uv, seed_k, sub, max_rec=4096, f64=False):
+    """Positions (float32 [n,2]) of one jittered fields streamline and its step count."""
+    xy = np.zeros((max_rec, 2), dtype=np.float32)
+    nrec = _U32(0)
+    seeds, ps = _c(seeds, np.float32)
+    mask, pm = _c(mask, np.uint8)
+    uv, pu = _c(uv, np.float32)
+    n = getattr(load(), _pre(f64) + "trace_fields_record")(ctypes.byref(p), ps, pm, pu, seed_k, sub,
+                                                          xy.ctypes.data, max_rec, ctypes.byref(nrec))
+    return xy[:min(nrec.value, max_rec)].copy(), int(n)
+
+
+def trace_paths(p, seeds, mask, uv, nthreads=8, f64=False):
+    """Returns (traj int8 [n,max_n_steps,2], nsteps uint16 [n], length float32 [n])."""
+    n = p.n_seed_points - p.seeds_chunk_offset
+    traj = np.zeros((n, p.max_n_steps, 2), dtype=np.int8)
+    nsteps = np.zeros(n, dtype=np.uint16)
+    length = np.zeros(n, dtype=np.float32)
+    seeds, ps = _c(seeds, np.float32)
+    mask, pm = _c(mask, np.uint8)
+    uv, pu = _c(uv, np.float32)
+    getattr(load(), _pre(f64) + "trace_paths")(ctypes.byref(p), ps, pm, pu, traj.ctypes.data,
+                                               nsteps.ctypes.data, length.ctypes.data, int(nthreads))
+    return traj, nsteps, length

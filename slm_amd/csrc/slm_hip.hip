@@ -1,0 +1,529 @@
+// slm_hip.hip -- gfx950 kernels and the extern "C" entry points of libslm_hip.so.
+//
+// Entry points are declared in include/slm_hip.h.  Kernels:
+//   k_rng_stream      Lehmer seeding / draws            (rng.cl, integratefields.cl:93-99)
+//   k_tile_*          counting sort of seeds into Morton-ordered 8x8 pixel tiles
+//   k_fields          persistent lane-refill tracer for integrate_fields
+//                     (integratefields.cl:60-114, jittertrajectory.cl:42-88)
+//   k_paths           one lane per seed for integrate_trajectory
+//                     (integratetraj.cl:54-97, trajectory.cl:41-80)
+//   k_fields_record   fields streamlines with positions recorded (test hook)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdarg.h>
+#include <mutex>
+#include "../../include/slm_hip.h"
+#include "trace_common.hpp"
+
+using namespace slm;
+
+// ------------------------------------------------------------------ errors
+
+static thread_local char g_err[512];
+
+static int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess)                                                      \
+            return fail(SLM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" uint32_t slm_abi_version(void) { return SLM_ABI_VERSION; }
+extern "C" const char* slm_last_error(void) { return g_err; }
+extern "C" int slm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int check_params(const SlmParams* p) {
+    if (!p) return fail(SLM_E_INVALID_ARG, "params is NULL");
+    if (p->abi_version != SLM_ABI_VERSION)
+        return fail(SLM_E_ABI, "SlmParams.abi_version %u != %u", p->abi_version, SLM_ABI_VERSION);
+    if (p->nx_padded == 0 || p->ny_padded == 0 ||
+        (uint64_t)p->nx_padded * p->ny_padded != (uint64_t)p->nxy_padded)
+        return fail(SLM_E_INVALID_ARG, "grid %u x %u != nxy_padded %u", p->nx_padded, p->ny_padded,
+                    p->nxy_padded);
+    if (p->nx_padded > (1u << 20) || p->ny_padded > (1u << 20))
+        return fail(SLM_E_INVALID_ARG, "grid side above 2^20");
+    if (!(p->trajectory_resolution > 0.0f))
+        return fail(SLM_E_INVALID_ARG, "trajectory_resolution must be > 0");
+    if (p->seeds_chunk_offset > p->n_seed_points)
+        return fail(SLM_E_INVALID_ARG, "seeds_chunk_offset %u > n_seed_points %u", p->seeds_chunk_offset,
+                    p->n_seed_points);
+    return SLM_OK;
+}
+
+static int finish(void* stream, uint32_t flags) {
+    HIP_TRY(hipGetLastError());
+    if (flags & SLM_FLAG_SYNC) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return SLM_OK;
+}
+
+struct DevInfo {
+    int cus = 0;
+};
+static std::mutex g_dev_mu;
+static DevInfo g_dev[64];
+
+static int device_info(DevInfo& out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(SLM_E_HIP, "device id %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_dev[dev].cus == 0) {
+        int cus = 0;
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        g_dev[dev].cus = cus > 0 ? cus : 1;
+    }
+    out = g_dev[dev];
+    return SLM_OK;
+}
+
+// ------------------------------------------------------------------ utility kernels
+
+__global__ void k_fill_u32(uint32_t* __restrict__ a, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+static void fill_u32(uint32_t* a, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, a, n, v);
+}
+
+// ------------------------------------------------------------------ RNG stream
+
+__global__ void k_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint32_t* __restrict__ rng0,
+                             uint32_t* __restrict__ draws) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t s = rng_seed(idx0 + i);
+    if (rng0) rng0[i] = s;
+    if (draws)
+        for (uint32_t k = 0; k < n_draws; k++) draws[(size_t)i * n_draws + k] = lehmer(s);
+}
+
+extern "C" int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint32_t* rng0_out,
+                              uint32_t* draws, void* stream, uint32_t flags) {
+    if (n == 0) return finish(stream, flags);
+    if (!rng0_out && !draws) return fail(SLM_E_INVALID_ARG, "no output buffer");
+    if (draws && n_draws == 0) return fail(SLM_E_INVALID_ARG, "n_draws is 0 with a draws buffer");
+    hipLaunchKernelGGL(k_rng_stream, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, idx0, n,
+                       n_draws, rng0_out, draws);
+    return finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ seed tiling
+//
+// Work order for the fields tracer: seeds are bucketed into 8x8-pixel tiles,
+// tiles are visited in Morton (Z) order, so the 64 lanes of a wave start in
+// one tile and neighbouring waves start in neighbouring tiles.  The RNG stays
+// keyed on each seed's index in the caller's (shuffled) list.
+
+static __host__ __device__ inline uint32_t spread16(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+struct TileGeom {
+    uint32_t tx, ty;      // tiles along x and y
+    uint32_t side;        // power-of-two side of the Morton square
+    uint32_t n_keys;      // side*side
+};
+
+static TileGeom tile_geom(const SlmParams* p) {
+    TileGeom g;
+    g.tx = (p->nx_padded + 7u) / 8u;
+    g.ty = (p->ny_padded + 7u) / 8u;
+    uint32_t m = g.tx > g.ty ? g.tx : g.ty, side = 1;
+    while (side < m) side <<= 1;
+    g.side = side;
+    g.n_keys = side * side;
+    return g;
+}
+
+__device__ __forceinline__ uint32_t seed_tile_key(const SlmParams& P, float2 s) {
+    const uint32_t px = pix(s.x, P.pad_width_pp5, P.nx_padded - 1u);
+    const uint32_t py = pix(s.y, P.pad_width_pp5, P.ny_padded - 1u);
+    return spread16(px >> 3) | (spread16(py >> 3) << 1);
+}
+
+__global__ void k_tile_count(SlmParams P, const float2* __restrict__ seeds, uint32_t n,
+                             uint32_t* __restrict__ cnt) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    atomicAdd(&cnt[seed_tile_key(P, seeds[P.seeds_chunk_offset + k])], 1u);
+}
+
+// Exclusive scan of cnt[0..n) in place by one 1024-thread block; writes cursor copy.
+__global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ cnt, uint32_t* __restrict__ cur,
+                                                    uint32_t n) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = t * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = (t == 0) ? 0u : part[t - 1];
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t c = cnt[i];
+        cnt[i] = run;
+        cur[i] = run;
+        run += c;
+    }
+}
+
+__global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, uint32_t n,
+                               uint32_t* __restrict__ cur, uint32_t* __restrict__ order) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = seed_tile_key(P, seeds[P.seeds_chunk_offset + k]);
+    order[atomicAdd(&cur[key], 1u)] = k;
+}
+
+// ------------------------------------------------------------------ fields tracer
+
+// Workspace layout (bytes, 256-aligned blocks):
+//   [0]                 uint32 queue head + pad        (256 B)
+//   [256]               uint32 order[n]                 seed ordinals in tile order
+//   [..]                uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct FieldsWs {
+    size_t off_order, off_cnt, off_cur, total;
+};
+
+static FieldsWs fields_ws(const SlmParams* p) {
+    FieldsWs w;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    const TileGeom g = tile_geom(p);
+    w.off_order = 256;
+    w.off_cnt = w.off_order + align256((size_t)n * 4);
+    w.off_cur = w.off_cnt + align256((size_t)g.n_keys * 4);
+    w.total = w.off_cur + align256((size_t)g.n_keys * 4);
+    return w;
+}
+
+extern "C" size_t slm_trace_fields_workspace_size(const SlmParams* p) {
+    if (check_params(p) != SLM_OK) return 0;
+    return fields_ws(p).total;
+}
+
+// One persistent wave pulls streamlines from the queue; a lane that finishes
+// takes the next item at once, so divergent lengths do not idle the wave.
+// Work item w: seed ordinal order[w / d^2], sub-pixel start w % d^2 = j*d+i.
+template <bool C>
+__global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __restrict__ seeds,
+                                                const uint32_t* __restrict__ order, uint32_t n_work,
+                                                const uint8_t* __restrict__ mask,
+                                                const float2* __restrict__ uv, uint32_t* __restrict__ slc,
+                                                uint32_t* __restrict__ slt, uint64_t* __restrict__ steps,
+                                                uint32_t* __restrict__ queue) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
+    const uint32_t dd = d * d;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+
+    bool active = false;
+    bool exhausted = false;
+    float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
+    uint32_t n = 0, rng = 0;
+    uint64_t my_steps = 0;
+
+    while (true) {
+        if (!exhausted) {
+            const uint64_t need = __ballot(!active);
+            if (need) {
+                const uint32_t cnt = __popcll(need);
+                const uint32_t leader = __ffsll((unsigned long long)need) - 1u;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(queue, cnt);
+                base = __shfl(base, leader);
+                if (base + cnt >= n_work) exhausted = true;
+                if (!active) {
+                    const uint32_t w = base + __popcll(need & lt_mask);
+                    if (w < n_work) {
+                        const uint32_t k = order[w / dd];
+                        const uint32_t sub = w - (w / dd) * dd;
+                        const uint32_t seed_idx = P.seeds_chunk_offset + k;
+                        const float2 s = seeds[seed_idx];
+                        x = s.x + sub_offset<C>(P, sub % d);
+                        y = s.y + sub_offset<C>(P, sub / d);
+                        rng = rng_seed(seed_idx);
+                        l = 0.f;
+                        n = 0;
+                        dt = P.dt_max;
+                        const uint32_t idx = array_idx(P, x, y);
+                        if (!mask[idx]) {
+                            // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
+                            atomicAdd(&slt[idx], slt_inc(0.f));
+                            atomicAdd(&slc[idx], 1u);
+                            active = (0.f < P.max_length) && (0u < P.max_n_steps);
+                        }
+                    }
+                }
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            StepOut o;
+            compute_step<C, true>(P, uv, dt, x, y, rng, o);
+            if (mask[o.idx]) {
+                active = false;  // end pixel masked: step dropped, loop ends
+            } else {
+                const float err = step_error<C>(P, o);
+                const float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+                x = o.next.x;
+                y = o.next.y;
+                const bool stuck = dl < P.integration_halt_threshold;
+                if (!stuck) dt = next_dt(P, dt, err);
+                l += dl;
+                n += 1u;
+                my_steps += 1u;
+                atomicAdd(&slt[o.idx], slt_inc(l));
+                atomicAdd(&slc[o.idx], 1u);
+                active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
+            }
+        }
+    }
+    if (steps) {
+        // wave reduction, one atomic per wave
+        uint64_t v = my_steps;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane == 0 && v) atomicAdd((unsigned long long*)steps, (unsigned long long)v);
+    }
+}
+
+__global__ void k_zero_queue(uint32_t* q) {
+    if (threadIdx.x == 0) q[0] = 0u;
+}
+
+extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                const float* uv, uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                                void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (!mask || !uv || !slc || !slt) return fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    if (!(flags & SLM_FLAG_NO_ZERO)) {
+        fill_u32(slc, p->nxy_padded, 0u, s);
+        fill_u32(slt, p->nxy_padded, 0u, s);
+    }
+    if (n == 0) return finish(stream, flags);
+    if (!seeds_xy) return fail(SLM_E_INVALID_ARG, "NULL seeds");
+    const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+    if ((uint64_t)n * d * d >= (1ull << 32) - 64)
+        return fail(SLM_E_INVALID_ARG, "n_seeds * d^2 = %llu exceeds 2^32", (unsigned long long)n * d * d);
+    const FieldsWs w = fields_ws(p);
+    if (!workspace || workspace_bytes < w.total)
+        return fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
+    DevInfo di;
+    rc = device_info(di);
+    if (rc) return rc;
+
+    char* ws = (char*)workspace;
+    uint32_t* queue = (uint32_t*)ws;
+    uint32_t* order = (uint32_t*)(ws + w.off_order);
+    uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
+    uint32_t* cur = (uint32_t*)(ws + w.off_cur);
+    const TileGeom g = tile_geom(p);
+    const float2* seeds = (const float2*)seeds_xy;
+
+    if (!(flags & SLM_FLAG_KEEP_ORDER)) {
+        fill_u32(cnt, g.n_keys, 0u, s);
+        hipLaunchKernelGGL(k_tile_count, dim3((n + 255) / 256), dim3(256), 0, s, *p, seeds, n, cnt);
+        hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, cnt, cur, g.n_keys);
+        hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255) / 256), dim3(256), 0, s, *p, seeds, n, cur, order);
+    }
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, s, queue);
+
+    const uint32_t n_work = n * d * d;
+    const uint32_t blocks_per_cu = 8;  // 32 waves per CU at 256 threads per block
+    uint32_t grid = (uint32_t)di.cus * blocks_per_cu;
+    const uint32_t max_useful = (n_work + 255) / 256;
+    if (grid > max_useful) grid = max_useful ? max_useful : 1;
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_fields<true>, dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           (const float2*)uv, slc, slt, steps_out, queue);
+    else
+        hipLaunchKernelGGL(k_fields<false>, dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           (const float2*)uv, slc, slt, steps_out, queue);
+    return finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ fields record (test hook)
+
+template <bool C>
+__global__ void k_fields_record(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                                const float2* __restrict__ uv, const uint32_t* __restrict__ seed_sel,
+                                const uint32_t* __restrict__ sub_sel, uint32_t n_sel, uint32_t max_rec,
+                                float2* __restrict__ xy, uint32_t* __restrict__ nrec) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_sel) return;
+    const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
+    const uint32_t seed_idx = P.seeds_chunk_offset + seed_sel[j];
+    const uint32_t sub = sub_sel[j];
+    const float2 s = seeds[seed_idx];
+    float x = s.x + sub_offset<C>(P, sub % d);
+    float y = s.y + sub_offset<C>(P, sub / d);
+    uint32_t rng = rng_seed(seed_idx), n = 0, nr = 0;
+    float l = 0.f, dt = P.dt_max;
+    float2* rec = xy + (size_t)j * max_rec;
+    if (nr < max_rec) rec[nr] = make_float2(x, y);
+    nr++;
+    uint32_t idx = array_idx(P, x, y);
+    bool active = !mask[idx] && (0.f < P.max_length) && (0u < P.max_n_steps);
+    while (active) {
+        StepOut o;
+        compute_step<C, true>(P, uv, dt, x, y, rng, o);
+        if (mask[o.idx]) break;
+        const float err = step_error<C>(P, o);
+        const float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+        x = o.next.x;
+        y = o.next.y;
+        const bool stuck = dl < P.integration_halt_threshold;
+        if (!stuck) dt = next_dt(P, dt, err);
+        l += dl;
+        n += 1u;
+        if (nr < max_rec) rec[nr] = make_float2(x, y);
+        nr++;
+        active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
+    }
+    nrec[j] = nr;
+}
+
+extern "C" int slm_trace_fields_record(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                       const float* uv, const uint32_t* seed_sel, const uint32_t* sub_sel,
+                                       uint32_t n_sel, uint32_t max_rec, float* xy, uint32_t* nrec,
+                                       void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (n_sel == 0) return finish(stream, flags);
+    if (!seeds_xy || !mask || !uv || !seed_sel || !sub_sel || !nrec || (max_rec && !xy))
+        return fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_fields_record<true>, dim3((n_sel + 63) / 64), dim3(64), 0, s, *p,
+                           (const float2*)seeds_xy, mask, (const float2*)uv, seed_sel, sub_sel, n_sel, max_rec,
+                           (float2*)xy, nrec);
+    else
+        hipLaunchKernelGGL(k_fields_record<false>, dim3((n_sel + 63) / 64), dim3(64), 0, s, *p,
+                           (const float2*)seeds_xy, mask, (const float2*)uv, seed_sel, sub_sel, n_sel, max_rec,
+                           (float2*)xy, nrec);
+    return finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ trajectory record
+
+// trajectory_record (trajectory.cl:41-80) with runge_kutta_step_record
+// (rungekutta.cl:49-77) and euler_step_record (:104-120).
+template <bool C>
+__global__ void k_paths(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                        const float2* __restrict__ uv, char2* __restrict__ traj, uint16_t* __restrict__ nsteps,
+                        float* __restrict__ length, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float res = P.trajectory_resolution;
+    const float2 s = seeds[P.seeds_chunk_offset + k];
+    char2* rec = traj + (size_t)k * P.max_n_steps;
+    float x = s.x, y = s.y, px = x, py = y;
+    float l = 0.f, dl = 0.f, dt = P.dt_max;
+    uint32_t nst = 0, rng = 0;
+    uint32_t idx = array_idx(P, x, y);
+    while (!mask[idx] && (l < P.max_length && nst < P.max_n_steps)) {
+        StepOut o;
+        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        idx = o.idx;
+        if (!mask[idx]) {
+            const float err = step_error<C>(P, o);
+            dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+            float2 nx = o.next;
+            if (dl < P.integration_halt_threshold) {
+                nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
+                dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+            }
+            x = nx.x;
+            y = nx.y;
+            idx = array_idx(P, x, y);
+            const bool stuck = dl < P.integration_halt_threshold;
+            if (!stuck) dt = next_dt(P, dt, err);
+            l += dl;
+            rec[nst] = make_char2(compress1(x - px, res), compress1(y - py, res));
+            nst += 1u;
+            if (stuck) break;
+            px = x;
+            py = y;
+        } else {
+            // Euler hop to the nearest pixel edge along DOWNUP_SIGN*uv1
+            const float su = o.uv1.x * P.downup_sign, sv = o.uv1.y * P.downup_sign;
+            float dtx = 0.f, dty = 0.f;
+            if (su > 0.f) dtx = fdiv((float)cvt_i32(x + 1.5f) - (x + 0.5f), su);
+            if (su < 0.f) dtx = fdiv(-((x + 0.5f) - (float)cvt_i32(x + 0.5f)), su);
+            if (sv > 0.f) dty = fdiv((float)cvt_i32(y + 1.5f) - (y + 0.5f), sv);
+            if (sv < 0.f) dty = fdiv(-((y + 0.5f) - (float)cvt_i32(y + 0.5f)), sv);
+            const float ax = fabsf(dtx), ay = fabsf(dty);
+            dt = (ax < ay) ? dtx : ((ay < ax) ? dty : fminf(dtx, dty));
+            x = mad<C>(su, dt, x);
+            y = mad<C>(sv, dt, y);
+            x = approx(x, res);
+            y = approx(y, res);
+            x = fminf(fmaxf(x, -0.5f), P.nxf_mp5);
+            y = fminf(fmaxf(y, -0.5f), P.nyf_mp5);
+            dl = fast_length<C>(x - px, y - py);
+            l += dl;
+            rec[nst] = make_char2(compress1(x - px, res), compress1(y - py, res));
+            nst += 1u;
+            break;
+        }
+    }
+    nsteps[k] = (uint16_t)nst;
+    length[k] = l;
+}
+
+extern "C" int slm_trace_paths(const SlmParams* p, const float* seeds_xy, const uint8_t* mask, const float* uv,
+                               int8_t* traj, uint16_t* nsteps, float* length, void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    if (n == 0) return finish(stream, flags);
+    if (!seeds_xy || !mask || !uv || !traj || !nsteps || !length) return fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (p->max_n_steps > 65535u)
+        return fail(SLM_E_INVALID_ARG, "max_n_steps %u does not fit the uint16 step count", p->max_n_steps);
+    hipStream_t s = (hipStream_t)stream;
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_paths<true>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
+                           (const float2*)uv, (char2*)traj, nsteps, length, n);
+    else
+        hipLaunchKernelGGL(k_paths<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
+                           (const float2*)uv, (char2*)traj, nsteps, length, n);
+    return finish(stream, flags);
+}

@@ -1,0 +1,175 @@
+// trace_common.hpp -- device-side streamline arithmetic shared by every tracing
+// kernel of libslm_hip.so.
+//
+// Each function restates one helper of the reference's OpenCL C (file:line
+// cited per function) in gfx950 HIP.  Rounding is pinned so that results are
+// bit-identical to the CPU oracle:
+//   * built with -ffp-contract=off; fused multiply-adds appear only through
+//     mad<C>() and only where clang's OpenCL default (FP_CONTRACT ON) fuses
+//     (a*b+c / a+=b*c in one expression, left product first) when C is true;
+//   * division and sqrt are IEEE correctly rounded: plain '/' and
+//     __builtin_sqrtf lower to the full-precision gfx950 sequences (HIP's
+//     __fsqrt_rn is the native approximate sqrt unless
+//     OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used);
+//   * fast_length(v) = sqrt(dot(v,v)); fast_normalize(v) = v*(1/sqrt(dot(v,v)));
+//   * float->uint / float->char conversions saturate and map NaN to 0.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/slm_hip.h"
+
+namespace slm {
+
+template <bool C>
+__device__ __forceinline__ float mad(float a, float b, float c) {
+    if constexpr (C) return __builtin_fmaf(a, b, c);
+    else return a * b + c;
+}
+
+__device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
+
+template <bool C>
+__device__ __forceinline__ float dot2(float x, float y) { return mad<C>(x, x, y * y); }
+template <bool C>
+__device__ __forceinline__ float fast_length(float x, float y) { return fsqrt(dot2<C>(x, y)); }
+
+// Pixel coordinate of a position: min(n-1, (uint)max(0, v+PAD_WIDTH_PP5))
+// (essentials.cl:26-27, :57-58).  t is never NaN after fmaxf.
+__device__ __forceinline__ uint32_t pix(float v, float pp5, uint32_t nm1) {
+    const float t = fmaxf(0.0f, v + pp5);
+    return (t >= (float)nm1) ? nm1 : (uint32_t)t;
+}
+
+// get_array_idx (essentials.cl:56-59)
+__device__ __forceinline__ uint32_t array_idx(const SlmParams& P, float x, float y) {
+    return pix(y, P.pad_width_pp5, P.ny_padded - 1u) +
+           P.ny_padded * pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+}
+
+// (int)f with NaN -> 0; callers keep |f| < 2^31.
+__device__ __forceinline__ int32_t cvt_i32(float f) { return (int32_t)((f == f) ? f : 0.0f); }
+
+// approximate (essentials.cl:104-109): (char)(v*RES)/RES.  For RES = 128 the
+// division is an exact scaling, done as a multiply.
+__device__ __forceinline__ float approx(float v, float res) {
+    const int32_t i = cvt_i32(v * res);
+    const float c = (float)(int8_t)(i & 0xff);
+    return (res == 128.0f) ? c * 0.0078125f : fdiv(c, res);
+}
+// compress (essentials.cl:71-74)
+__device__ __forceinline__ int8_t compress1(float v, float res) { return (int8_t)(cvt_i32(v * res) & 0xff); }
+
+// lehmer_rand_uint (rng.cl:37-45): ((u64)(uint)(s+1) * 279470273) mod (2^32-5),
+// evaluated by folding 2^32 == 5 (mod 2^32-5) twice; bit-identical to the 64-bit %.
+__device__ __forceinline__ uint32_t lehmer(uint32_t& s) {
+    const uint32_t a = s + 1u;                 // wraps in 32 bits first
+    const uint32_t lo = a * 279470273u;
+    const uint32_t hi = __umulhi(a, 279470273u);
+    uint64_t t = (uint64_t)hi * 5u + lo;       // < 2^32 + 5*2^29
+    t = (t >> 32) * 5u + (uint32_t)t;          // < 2^32 + 5
+    if (t >= 0xfffffffbull) t -= 0xfffffffbull;
+    s = (uint32_t)t;
+    return s;
+}
+
+// lehmer_rand_vec (rng.cl:62-67): x drawn first; (float)0xfffffffb == 2^32 so
+// the division is an exact scaling by 2^-32.
+__device__ __forceinline__ float2 rand_vec(uint32_t& s) {
+    const float a = (float)lehmer(s);
+    const float b = (float)lehmer(s);
+    return make_float2(a * 2.3283064365386963e-10f - 0.5f, b * 2.3283064365386963e-10f - 0.5f);
+}
+
+__device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
+
+// RNG seeding of integratefields.cl:93-99
+__device__ __forceinline__ uint32_t rng_seed(uint32_t seed_idx) {
+    uint32_t s = bswap32(seed_idx);
+    lehmer(s);
+    return bswap32(s);
+}
+
+// speed_interpolator (essentials.cl:24-45): blend of pixels (xl,yd),(xr,yd),
+// (xl,yu),(xr,yu) with xl = round-half-up of x in padded coordinates, weights
+// rx = x - xl + PAD in [-0.5,0.5) (an extrapolating blend), then fast_normalize.
+template <bool C>
+__device__ __forceinline__ float2 speed_interp(const SlmParams& P, const float2* __restrict__ uv,
+                                               float x, float y) {
+    const uint32_t xl = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+    const uint32_t yd = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
+    const uint32_t xr = min(P.nx_padded - 1u, xl + 1u);
+    const uint32_t yu = min(P.ny_padded - 1u, yd + 1u);
+    const float rx = x - (float)xl + (float)P.pad_width;
+    const float lx = 1.0f - rx;
+    const float uy = y - (float)yd + (float)P.pad_width;
+    const float dy = 1.0f - uy;
+    const float2 a = uv[P.ny_padded * xl + yd];
+    const float2 b = uv[P.ny_padded * xr + yd];
+    const float2 e = uv[P.ny_padded * xl + yu];
+    const float2 f = uv[P.ny_padded * xr + yu];
+    const float dwn0 = mad<C>(a.x, lx, b.x * rx), dwn1 = mad<C>(a.y, lx, b.y * rx);
+    const float upp0 = mad<C>(e.x, lx, f.x * rx), upp1 = mad<C>(e.y, lx, f.y * rx);
+    const float s0 = mad<C>(dwn0, dy, upp0 * uy), s1 = mad<C>(dwn1, dy, upp1 * uy);
+    const float inv = fdiv(1.0f, fsqrt(dot2<C>(s0, s1)));
+    return make_float2(s0 * inv, s1 * inv);
+}
+
+struct StepOut {
+    float2 uv1, dxy1, dxy2, next;
+    uint32_t idx;
+};
+
+// compute_step_vec (computestep.cl:42-57) / compute_step_vec_jittered (:86-106)
+template <bool C, bool JITTER>
+__device__ __forceinline__ void compute_step(const SlmParams& P, const float2* __restrict__ uv,
+                                             float dt, float x, float y, uint32_t& rng, StepOut& o) {
+    const float res = P.trajectory_resolution;
+    const float combo = P.combo_factor;
+    float2 u1 = speed_interp<C>(P, uv, x, y);
+    if constexpr (JITTER) {
+        const float2 r = rand_vec(rng);
+        u1.x = mad<C>(r.x, P.jitter_magnitude, u1.x);
+        u1.y = mad<C>(r.y, P.jitter_magnitude, u1.y);
+        const float len = fast_length<C>(u1.x, u1.y);
+        u1.x = fdiv(u1.x, len);
+        u1.y = fdiv(u1.y, len);
+    }
+    o.uv1 = u1;
+    o.dxy1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+    float2 u2 = speed_interp<C>(P, uv, x + o.dxy1.x, y + o.dxy1.y);
+    if constexpr (JITTER) {
+        const float2 r = rand_vec(rng);
+        u2.x = mad<C>(r.x, P.jitter_magnitude, u2.x);
+        u2.y = mad<C>(r.y, P.jitter_magnitude, u2.y);
+        const float len = fast_length<C>(u2.x, u2.y);
+        u2.x = fdiv(u2.x, len);
+        u2.y = fdiv(u2.y, len);
+    }
+    o.dxy2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, o.dxy1.x), res),
+                         approx(0.5f * mad<C>(u2.y * combo, dt, o.dxy1.y), res));
+    o.next = make_float2(x + o.dxy2.x, y + o.dxy2.y);
+    o.idx = array_idx(P, o.next.x, o.next.y);
+}
+
+// step_error of runge_kutta_step_* (rungekutta.cl:56, :175)
+template <bool C>
+__device__ __forceinline__ float step_error(const SlmParams& P, const StepOut& o) {
+    return fast_length<C>(fdiv(o.dxy2.x - o.dxy1.x, P.grid_scale), fdiv(o.dxy2.y - o.dxy1.y, P.grid_scale));
+}
+// adaptive dt (rungekutta.cl:69-70, :181-182)
+__device__ __forceinline__ float next_dt(const SlmParams& P, float dt, float err) {
+    return (err == 0.0f) ? P.dt_max : fminf(P.dt_max, fdiv(P.adjusted_max_error * dt, err));
+}
+// (uint)(l+0.5f) of writearray.cl:37, :78 (l >= 0, or NaN -> 0)
+__device__ __forceinline__ uint32_t slt_inc(float l) {
+    const float t = l + 0.5f;
+    return (t > 0.0f) ? ((t >= 4294967296.0f) ? 0xffffffffu : (uint32_t)t) : 0u;
+}
+// sub-pixel start offset i*STEP-HALFSPAN (integratefields.cl:108-110)
+template <bool C>
+__device__ __forceinline__ float sub_offset(const SlmParams& P, uint32_t i) {
+    return mad<C>((float)i, P.subpixel_seed_step, -P.subpixel_seed_halfspan);
+}
+
+}  // namespace slm

@@ -1,0 +1,144 @@
+"""ctypes binding of libslm_hip.so (include/slm_hip.h) and torch-tensor wrappers.
+
+The library is the only compute path: if it is missing, importing this module's
+functions raises; nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import SlmParams, ABI_VERSION
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
+
+SLM_FLAG_SYNC = 1
+SLM_FLAG_NO_ZERO = 2
+SLM_FLAG_KEEP_ORDER = 4
+
+_ERR_NAMES = {-1: "SLM_E_INVALID_ARG", -2: "SLM_E_HIP", -3: "SLM_E_OOM", -4: "SLM_E_ABI",
+              -5: "SLM_E_WORKSPACE"}
+
+# Exported symbols and their prototypes: (restype, argtypes)
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_PROTOS = {
+    "slm_abi_version": (_U32, []),
+    "slm_last_error": (ctypes.c_char_p, []),
+    "slm_device_count": (ctypes.c_int, []),
+    "slm_rng_stream": (ctypes.c_int, [_U32, _U32, _U32, _P, _P, _P, _U32]),
+    "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
+    "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
+                                        ctypes.c_size_t, _P, _U32]),
+    "slm_trace_paths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P, _U32]),
+    "slm_trace_fields_record": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _U32,
+                                               _U32, _P, _P, _P, _U32]),
+}
+
+_lib = None
+
+
+class SlmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("{} ({}): {}".format(_ERR_NAMES.get(code, "SLM_E?"), code, msg))
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load and bind libslm_hip.so; raises ImportError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libslm_hip.so not found at {} -- build it with "
+                          "__graft_entry__.build()".format(path))
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.slm_abi_version() != ABI_VERSION:
+        raise ImportError("libslm_hip.so ABI {} != {}".format(lib.slm_abi_version(), ABI_VERSION))
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_PROTOS)
+
+
+def check(rc: int):
+    if rc != 0:
+        raise SlmError(rc, load().slm_last_error().decode(errors="replace"))
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _require_cuda(*tensors):
+    for t in tensors:
+        if t is not None and (not t.is_cuda or not t.is_contiguous()):
+            raise ValueError("tensors must be contiguous CUDA (HIP) tensors")
+
+
+def rng_stream(idx0: int, n: int, n_draws: int, rng0=None, draws=None, stream=None, sync=False):
+    _require_cuda(rng0, draws)
+    check(load().slm_rng_stream(idx0, n, n_draws, _ptr(rng0), _ptr(draws), _stream(stream),
+                                SLM_FLAG_SYNC if sync else 0))
+
+
+def fields_workspace_size(p: SlmParams) -> int:
+    n = load().slm_trace_fields_workspace_size(ctypes.byref(p))
+    if n == 0:
+        check(-1)
+    return int(n)
+
+
+def trace_fields(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=None,
+                 stream=None, accumulate=False, sync=False, keep_order=False):
+    """One direction of integrate_fields on device tensors.
+    seeds float32 [N,2]; mask uint8 [nxp*nyp]; uv float32 [nxp*nyp*2];
+    slc/slt int32 [nxp*nyp] (uint32 bit patterns); steps int64 [1] or None."""
+    import torch
+    _require_cuda(seeds, mask, uv, slc, slt, steps, workspace)
+    if workspace is None:
+        workspace = torch.empty(fields_workspace_size(p), dtype=torch.uint8, device=slc.device)
+    flags = (SLM_FLAG_NO_ZERO if accumulate else 0) | (SLM_FLAG_SYNC if sync else 0) | \
+        (SLM_FLAG_KEEP_ORDER if keep_order else 0)
+    check(load().slm_trace_fields(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(slc),
+                                  _ptr(slt), _ptr(steps), _ptr(workspace), workspace.numel(),
+                                  _stream(stream), flags))
+    return workspace
+
+
+def trace_paths(p: SlmParams, seeds, mask, uv, traj, nsteps, length, stream=None, sync=False):
+    """One direction of integrate_trajectory: traj int8 [n,max_n_steps,2],
+    nsteps int16 [n] (uint16 bits), length float32 [n]."""
+    _require_cuda(seeds, mask, uv, traj, nsteps, length)
+    check(load().slm_trace_paths(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(traj),
+                                 _ptr(nsteps), _ptr(length), _stream(stream),
+                                 SLM_FLAG_SYNC if sync else 0))
+
+
+def trace_fields_record(p: SlmParams, seeds, mask, uv, seed_sel, sub_sel, max_rec, xy, nrec,
+                        stream=None, sync=False):
+    _require_cuda(seeds, mask, uv, seed_sel, sub_sel, xy, nrec)
+    check(load().slm_trace_fields_record(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv),
+                                         _ptr(seed_sel), _ptr(sub_sel), seed_sel.numel(), max_rec,
+                                         _ptr(xy), _ptr(nrec), _stream(stream),
+                                         SLM_FLAG_SYNC if sync else 0))
+
+
+def as_u32(t) -> np.ndarray:
+    """Device int32 tensor holding uint32 bits -> host uint32 array."""
+    return t.detach().cpu().numpy().view(np.uint32)

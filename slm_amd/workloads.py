@@ -1,0 +1,94 @@
+"""Named tracing workloads built from seeded synthetic DTMs.
+
+A Job holds what Trace.compute_fields hands the tracer (trace.py:131-157):
+the bbox-cropped padded mask and uv, the Info object, and the shuffled seed
+list, so tests and benchmarks can call the C ABI directly.
+
+  small_256   256x256 at 1 m, relief 20 m, 5% tilt, 2-px NaN border, d=1
+  tile_2048   2048x2048 at 2 m, relief 60 m, three V-valleys, d=2
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import preprocess, synth
+from .parameters import load_parameters, section
+from .params import SlmParams, kernel_params
+from .useful import Data, Info, create_seeds, get_bbox
+
+
+@dataclass
+class Job:
+    name: str
+    roi: np.ndarray            # float32 [nx, ny] DTM (NaN = no data)
+    pixel_size: float
+    mask: np.ndarray           # bool [nxp, nyp], bbox-cropped, padded
+    uv: np.ndarray             # float32 [nxp, nyp, 2]
+    info: Info
+    seeds: np.ndarray          # float32 [N, 2], shuffled
+    trace: SimpleNamespace
+    state: SimpleNamespace
+
+    @property
+    def shape(self):
+        return self.mask.shape
+
+    def params(self, sign=1, kernel="INTEGRATE_FIELDS", fp_contract=True, n_seed_points=None,
+               jitter=None) -> SlmParams:
+        info = self.info
+        info.n_seed_points = np.uint32(self.seeds.shape[0] if n_seed_points is None else n_seed_points)
+        p = kernel_params(info, sign, kernel, fp_contract)
+        if jitter is not None:
+            p.jitter_magnitude = float(np.float32(jitter))
+        return p
+
+    def n_streamlines(self) -> int:
+        return int(self.seeds.shape[0]) * int(self.info.subpixel_seed_point_density) ** 2
+
+
+def job_from_dtm(name, roi, pixel_size, trace_overrides=None, state_overrides=None, pad_width=2):
+    params = load_parameters(overrides={"trace": trace_overrides or {}, "state": state_overrides or {}})
+    trace = section(params, "trace")
+    state = section(params, "state")
+    mask_full, uv_full = preprocess.prepare(roi, pixel_size, pad_width)
+    bbox, bnx, bny = get_bbox(~mask_full)
+    info = Info(state, trace, pixel_size)
+    info.set_xy(int(bnx), int(bny), pad_width)
+    data = Data(info=info, bbox=bbox, pad=pad_width, mask_array=mask_full, uv_array=uv_full,
+                mapping_array=np.zeros(mask_full.shape, dtype=np.uint32))
+    seeds, n, npad = create_seeds(data.mask_array, pad_width, int(info.n_work_items),
+                                  do_shuffle=info.do_shuffle, rng_seed=info.shuffle_rng_seed)
+    info.n_seed_points = np.uint32(n)
+    info.n_padded_seed_points = np.uint32(npad)
+    return Job(name, roi, pixel_size, data.mask_array, data.uv_array, info, seeds, trace, state)
+
+
+def small_256(jitter=1.0, seed=0, n=256, d=1):
+    roi = synth.make_dtm(n, pixel_size=1.0, relief=20.0, tilt=0.05, seed=seed, nan_border=2)
+    return job_from_dtm("small_256" if n == 256 else "small_%d" % n, roi, 1.0,
+                        {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
+                         "max_length": 1000.0, "integrator_step_factor": 0.5})
+
+
+def tile_2048(jitter=1.0, seed=0, n=2048, d=2):
+    roi = synth.make_dtm(n, pixel_size=2.0, relief=60.0, tilt=0.05, seed=seed, nan_border=2, n_valleys=3)
+    return job_from_dtm("tile_2048" if n == 2048 else "tile_%d" % n, roi, 2.0,
+                        {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
+                         "max_length": 1000.0, "integrator_step_factor": 0.5})
+
+
+def planar(n=64, pixel_size=1.0, slope=(0.3, 0.1), d=1, jitter=0.0, max_length=1000.0):
+    """Planar ramp h = -(sx*x + sy*y)*pixel_size with a NaN border: uv is uniform."""
+    x = np.arange(n, dtype=np.float64)[:, None]
+    y = np.arange(n, dtype=np.float64)[None, :]
+    roi = (-(slope[0] * x + slope[1] * y) * pixel_size).astype(np.float32)
+    roi[:2, :] = np.nan
+    roi[-2:, :] = np.nan
+    roi[:, :2] = np.nan
+    roi[:, -2:] = np.nan
+    return job_from_dtm("planar_%d" % n, roi, pixel_size,
+                        {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
+                         "max_length": max_length, "integrator_step_factor": 0.5})

@@ -1,0 +1,164 @@
+"""GPU parity of the tracing path: libslm_hip.so against the CPU oracle.
+
+Integer fields (slc, slt) and the accepted-step count must be bit-identical
+to the oracle, jitter on or off, in both FP-contraction modes; recorded
+positions and trajectory records must be bit-identical too.  The oracle is
+pinned separately by tests/test_oracle.py.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(job, dev):
+    import torch
+    seeds = torch.from_numpy(np.ascontiguousarray(job.seeds)).to(dev)
+    mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)
+    uv = torch.from_numpy(np.ascontiguousarray(job.uv).reshape(-1)).to(dev)
+    return seeds, mask, uv
+
+
+def _gpu_fields(p, seeds, mask, uv, dev):
+    import torch
+    from slm_amd import hip
+    slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
+    slt = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
+    steps = torch.zeros(1, dtype=torch.int64, device=dev)
+    hip.trace_fields(p, seeds, mask, uv, slc, slt, steps, sync=True)
+    return hip.as_u32(slc), hip.as_u32(slt), int(steps.item())
+
+
+CASES = [
+    # (grid side, d, jitter, fp_contract)
+    (64, 1, 0.0, True),
+    (64, 2, 1.0, True),
+    (96, 3, 1.0, False),
+    (96, 3, 0.0, True),
+    (256, 1, 1.0, True),
+    (256, 1, 0.0, True),
+    (256, 1, 1.0, False),
+    (256, 2, 1.0, True),
+]
+
+
+@pytest.mark.parametrize("n,d,jitter,contract", CASES)
+def test_fields_bit_exact(gpu, n, d, jitter, contract):
+    from slm_amd import workloads
+    job = workloads.small_256(jitter=jitter, n=n, d=d)
+    seeds, mask, uv = _dev(job, gpu)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    for sign in (1, -1):
+        p = job.params(sign, fp_contract=contract)
+        print("case n=%d d=%d jitter=%g contract=%d sign=%+d" % (n, d, jitter, contract, sign), flush=True)
+        gslc, gslt, gsteps = _gpu_fields(p, seeds, mask, uv, gpu)
+        oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1))
+        assert gsteps == osteps
+        assert np.array_equal(gslc, oslc), "slc differs at %d cells" % int((gslc != oslc).sum())
+        assert np.array_equal(gslt, oslt), "slt differs at %d cells" % int((gslt != oslt).sum())
+
+
+def test_fields_planar_and_empty(gpu):
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.planar(n=48, d=2, jitter=0.0)
+    seeds, mask, uv = _dev(job, gpu)
+    p = job.params(1)
+    gslc, gslt, gsteps = _gpu_fields(p, seeds, mask, uv, gpu)
+    oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1))
+    assert gsteps == osteps and np.array_equal(gslc, oslc) and np.array_equal(gslt, oslt)
+    # empty seed list: fields cleared, no work
+    p0 = job.params(1, n_seed_points=0)
+    gslc, gslt, gsteps = _gpu_fields(p0, seeds, mask, uv, gpu)
+    assert gsteps == 0 and not gslc.any() and not gslt.any()
+
+
+def test_fields_chunk_invariance(gpu):
+    """Seeds split into chunks (seeds_chunk_offset) accumulate to the same
+    integer fields as one launch (the reference's adaptive chunking)."""
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=1.0, n=128, d=2)
+    seeds, mask, uv = _dev(job, gpu)
+    p = job.params(1)
+    ref = _gpu_fields(p, seeds, mask, uv, gpu)
+    slc = torch.zeros(p.nxy_padded, dtype=torch.int32, device=gpu)
+    slt = torch.zeros(p.nxy_padded, dtype=torch.int32, device=gpu)
+    steps = torch.zeros(1, dtype=torch.int64, device=gpu)
+    n = int(p.n_seed_points)
+    bounds = [0, 1, 37, n // 2, n - 5, n]
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        q = job.params(1, n_seed_points=b)
+        q.seeds_chunk_offset = a
+        hip.trace_fields(q, seeds, mask, uv, slc, slt, steps, accumulate=True, sync=True)
+    assert int(steps.item()) == ref[2]
+    assert np.array_equal(hip.as_u32(slc), ref[0]) and np.array_equal(hip.as_u32(slt), ref[1])
+
+
+def test_rng_stream_bit_exact(gpu):
+    import torch
+    from slm_amd import hip
+    n, nd = 1 << 16, 64
+    rng0 = torch.empty(n, dtype=torch.int32, device=gpu)
+    draws = torch.empty(n * nd, dtype=torch.int32, device=gpu)
+    for idx0 in (0, 0xfffffff0 - n):
+        hip.rng_stream(idx0, n, nd, rng0, draws, sync=True)
+        o0, od = oracle.rng_stream(idx0, n, nd)
+        assert np.array_equal(hip.as_u32(rng0), o0)
+        assert np.array_equal(hip.as_u32(draws).reshape(n, nd), od)
+
+
+@pytest.mark.parametrize("n,d,jitter,contract", [(256, 1, 1.0, True), (96, 3, 1.0, False), (256, 2, 0.0, True)])
+def test_fields_record_positions(gpu, n, d, jitter, contract):
+    """Recorded jittered positions: bit-identical to the FP32 oracle (so RMS 0
+    <= 1e-4 px); the RMS against the FP64 oracle is reported."""
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=jitter, n=n, d=d)
+    seeds, mask, uv = _dev(job, gpu)
+    p = job.params(1, fp_contract=contract)
+    rs = np.random.default_rng(5)
+    ksel = rs.integers(0, job.seeds.shape[0], 64).astype(np.uint32)
+    ssel = rs.integers(0, d * d, 64).astype(np.uint32)
+    max_rec = 2048
+    xy = torch.zeros((64, max_rec, 2), dtype=torch.float32, device=gpu)
+    nrec = torch.zeros(64, dtype=torch.int32, device=gpu)
+    hip.trace_fields_record(p, seeds, mask, uv, torch.from_numpy(ksel.view(np.int32)).to(gpu),
+                            torch.from_numpy(ssel.view(np.int32)).to(gpu), max_rec, xy, nrec, sync=True)
+    gxy = xy.cpu().numpy()
+    gn = nrec.cpu().numpy()
+    m = job.mask.astype(np.uint8).reshape(-1)
+    sq, cnt = 0.0, 0
+    for j in range(64):
+        oxy, _ = oracle.trace_fields_record(p, job.seeds, m, job.uv.reshape(-1), int(ksel[j]), int(ssel[j]), max_rec)
+        assert gn[j] == len(oxy) or (gn[j] > max_rec and len(oxy) == max_rec)
+        k = min(gn[j], max_rec)
+        assert np.array_equal(gxy[j, :k], oxy[:k])
+        dxy, _ = oracle.trace_fields_record(p, job.seeds, m, job.uv.reshape(-1), int(ksel[j]), int(ssel[j]),
+                                            max_rec, f64=True)
+        kk = min(k, len(dxy))
+        sq += float(((gxy[j, :kk].astype(np.float64) - dxy[:kk]) ** 2).sum())
+        cnt += kk
+    print("RMS vs FP64 oracle over %d positions: %.3g px" % (cnt, (sq / max(cnt, 1)) ** 0.5))
+
+
+@pytest.mark.parametrize("n,contract", [(256, True), (256, False), (96, True)])
+def test_paths_bit_exact(gpu, n, contract):
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=0.0, n=n, d=1)
+    seeds, mask, uv = _dev(job, gpu)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    ns = min(1000, job.seeds.shape[0])
+    for sign in (1, -1):
+        p = job.params(sign, kernel="INTEGRATE_TRAJECTORY", fp_contract=contract, n_seed_points=ns)
+        traj = torch.zeros((ns, p.max_n_steps, 2), dtype=torch.int8, device=gpu)
+        nst = torch.zeros(ns, dtype=torch.int16, device=gpu)
+        ln = torch.zeros(ns, dtype=torch.float32, device=gpu)
+        hip.trace_paths(p, seeds, mask, uv, traj, nst, ln, sync=True)
+        otraj, onst, oln = oracle.trace_paths(p, job.seeds, m, job.uv.reshape(-1))
+        assert np.array_equal(nst.cpu().numpy().view(np.uint16), onst)
+        assert np.array_equal(ln.cpu().numpy().view(np.uint32), oln.view(np.uint32))
+        assert np.array_equal(traj.cpu().numpy(), otraj)
