@@ -46,6 +46,8 @@ extern "C" {
 #define SLM_FLAG_KEEP_ORDER 4u   /* workspace already holds the seed work order built by
                                     a previous slm_trace_fields call with the same seeds,
                                     n_seed_points, seeds_chunk_offset and grid: reuse it */
+#define SLM_FLAG_FORCE_FALLBACK 8u /* test hook: reject the packed 64-bit accumulation so the
+                                      32-bit fallback path runs (results are identical) */
 
 /*
  * Every kernel constant of the reference's -D macro set (pocl.py:225-297, values

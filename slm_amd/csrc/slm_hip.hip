@@ -3,7 +3,8 @@
 // Entry points are declared in include/slm_hip.h.  Kernels:
 //   k_rng_stream      Lehmer seeding / draws            (rng.cl, integratefields.cl:93-99)
 //   k_tile_*          counting sort of seeds into Morton-ordered 8x8 pixel tiles
-//   k_fields          persistent lane-refill tracer for integrate_fields
+//   k_fields          persistent lane-refill tracer for integrate_fields, run-coalesced
+//                     packed 64-bit accumulation checked on device (k_fields_sumlow/commit)
 //                     (integratefields.cl:60-114, jittertrajectory.cl:42-88)
 //   k_paths           one lane per seed for integrate_trajectory
 //                     (integratetraj.cl:54-97, trajectory.cl:41-80)
@@ -207,15 +208,36 @@ __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, ui
 }
 
 // ------------------------------------------------------------------ fields tracer
-
+//
+// Accumulation.  Every write of the reference adds (slc += 1, slt += (uint)(l+0.5f))
+// at one pixel.  Here a lane keeps a run (pixel, count, length-sum) while its
+// streamline stays in one pixel and flushes the run with ONE 64-bit atomic into
+// acc[pixel] = (slt << 32) | slc.  Integer sums commute, so the result equals
+// the reference's per-write uint32 atomics exactly, provided no pixel's slc
+// carries out of the low 32 bits (slt's own carry falls off bit 63, which is
+// exactly uint32 wrap-around).  That proviso is checked on the device, with
+// no host round trip: sum over pixels of low32(acc) must equal the exact
+// number of writes W counted by the kernel.  If it does not, k_fields_commit
+// leaves the outputs untouched and a fallback launch re-traces with separate
+// 32-bit atomics straight into slc/slt; otherwise the fallback launch exits
+// at once.
+//
 // Workspace layout (bytes, 256-aligned blocks):
-//   [0]                 uint32 queue head + pad        (256 B)
-//   [256]               uint32 order[n]                 seed ordinals in tile order
-//   [..]                uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
+//   [0]    FieldsCtrl                      queue head, W, sum of low halves
+//   [256]  uint32 order[n]                 seed ordinals in tile order
+//   [..]   uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
+//   [..]   uint64 acc[nxy_padded]          packed (slt << 32) | slc
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+struct FieldsCtrl {
+    uint32_t queue;
+    uint32_t pad;
+    unsigned long long writes;
+    unsigned long long sumlow;
+};
+
 struct FieldsWs {
-    size_t off_order, off_cnt, off_cur, total;
+    size_t off_order, off_cnt, off_cur, off_acc, total;
 };
 
 static FieldsWs fields_ws(const SlmParams* p) {
@@ -225,7 +247,8 @@ static FieldsWs fields_ws(const SlmParams* p) {
     w.off_order = 256;
     w.off_cnt = w.off_order + align256((size_t)n * 4);
     w.off_cur = w.off_cnt + align256((size_t)g.n_keys * 4);
-    w.total = w.off_cur + align256((size_t)g.n_keys * 4);
+    w.off_acc = w.off_cur + align256((size_t)g.n_keys * 4);
+    w.total = w.off_acc + align256((size_t)p->nxy_padded * 8);
     return w;
 }
 
@@ -234,16 +257,28 @@ extern "C" size_t slm_trace_fields_workspace_size(const SlmParams* p) {
     return fields_ws(p).total;
 }
 
+__device__ __forceinline__ bool fields_packed_ok(const FieldsCtrl* ctrl, uint32_t force_fallback) {
+    return !force_fallback && ctrl->sumlow == ctrl->writes;
+}
+
 // One persistent wave pulls streamlines from the queue; a lane that finishes
 // takes the next item at once, so divergent lengths do not idle the wave.
 // Work item w: seed ordinal order[w / d^2], sub-pixel start w % d^2 = j*d+i.
-template <bool C>
+// PACKED: runs go to acc with one 64-bit atomic; W and the step count are
+// recorded.  !PACKED: the fallback, which runs only when the packed result was
+// rejected and adds runs to slc/slt with two 32-bit atomics.
+template <bool C, bool PACKED>
 __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __restrict__ seeds,
                                                 const uint32_t* __restrict__ order, uint32_t n_work,
                                                 const uint8_t* __restrict__ mask,
-                                                const float2* __restrict__ uv, uint32_t* __restrict__ slc,
-                                                uint32_t* __restrict__ slt, uint64_t* __restrict__ steps,
-                                                uint32_t* __restrict__ queue) {
+                                                const float2* __restrict__ uv,
+                                                unsigned long long* __restrict__ acc,
+                                                uint32_t* __restrict__ slc, uint32_t* __restrict__ slt,
+                                                unsigned long long* __restrict__ steps,
+                                                FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback) {
+    if constexpr (!PACKED) {
+        if (fields_packed_ok(ctrl, force_fallback)) return;
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
     const uint32_t dd = d * d;
@@ -253,7 +288,21 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     bool exhausted = false;
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
-    uint64_t my_steps = 0;
+    uint32_t run_idx = 0, run_n = 0, run_t = 0;   // current same-pixel run
+    uint64_t my_steps = 0, my_writes = 0;
+
+    auto flush = [&]() {
+        if (run_n) {
+            if constexpr (PACKED) {
+                atomicAdd(&acc[run_idx], ((unsigned long long)run_t << 32) | (unsigned long long)run_n);
+            } else {
+                atomicAdd(&slt[run_idx], run_t);
+                atomicAdd(&slc[run_idx], run_n);
+            }
+            run_n = 0;
+            run_t = 0;
+        }
+    };
 
     while (true) {
         if (!exhausted) {
@@ -262,7 +311,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 const uint32_t cnt = __popcll(need);
                 const uint32_t leader = __ffsll((unsigned long long)need) - 1u;
                 uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(queue, cnt);
+                if (lane == leader) base = atomicAdd(&ctrl->queue, cnt);
                 base = __shfl(base, leader);
                 if (base + cnt >= n_work) exhausted = true;
                 if (!active) {
@@ -281,9 +330,12 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                         const uint32_t idx = array_idx(P, x, y);
                         if (!mask[idx]) {
                             // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
-                            atomicAdd(&slt[idx], slt_inc(0.f));
-                            atomicAdd(&slc[idx], 1u);
+                            run_idx = idx;
+                            run_n = 1u;
+                            run_t = slt_inc(0.f);
+                            my_writes += 1u;
                             active = (0.f < P.max_length) && (0u < P.max_n_steps);
+                            if (!active) flush();
                         }
                     }
                 }
@@ -298,6 +350,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             compute_step<C, true>(P, uv, dt, x, y, rng, o);
             if (mask[o.idx]) {
                 active = false;  // end pixel masked: step dropped, loop ends
+                flush();
             } else {
                 const float err = step_error<C>(P, o);
                 const float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
@@ -308,22 +361,91 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 l += dl;
                 n += 1u;
                 my_steps += 1u;
-                atomicAdd(&slt[o.idx], slt_inc(l));
-                atomicAdd(&slc[o.idx], 1u);
+                my_writes += 1u;
+                // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced
+                if (o.idx != run_idx) {
+                    flush();
+                    run_idx = o.idx;
+                }
+                run_n += 1u;
+                run_t += slt_inc(l);
                 active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
+                if (!active) flush();
             }
         }
     }
-    if (steps) {
-        // wave reduction, one atomic per wave
-        uint64_t v = my_steps;
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-        if (lane == 0 && v) atomicAdd((unsigned long long*)steps, (unsigned long long)v);
+    if constexpr (PACKED) {
+        // wave reductions, one atomic per wave
+        uint64_t v = my_steps, wv = my_writes;
+        for (int off = 32; off > 0; off >>= 1) {
+            v += __shfl_down(v, off);
+            wv += __shfl_down(wv, off);
+        }
+        if (lane == 0) {
+            if (steps && v) atomicAdd(steps, (unsigned long long)v);
+            if (wv) atomicAdd(&ctrl->writes, (unsigned long long)wv);
+        }
     }
 }
 
-__global__ void k_zero_queue(uint32_t* q) {
-    if (threadIdx.x == 0) q[0] = 0u;
+__global__ void k_fields_init(FieldsCtrl* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl->queue = 0u;
+        ctrl->writes = 0ull;
+        ctrl->sumlow = 0ull;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ a, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = 0ull;
+}
+
+// sum over pixels of low32(acc), one atomic per block
+__global__ __launch_bounds__(256) void k_fields_sumlow(const unsigned long long* __restrict__ acc, size_t n,
+                                                      FieldsCtrl* __restrict__ ctrl) {
+    __shared__ unsigned long long part[4];
+    unsigned long long v = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v += acc[i] & 0xffffffffull;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(&ctrl->sumlow, t);
+    }
+}
+
+// Split acc into slc/slt (assign, or add when accumulating) if the packed
+// result passed its check; otherwise clear the outputs (unless accumulating)
+// for the fallback.  Resets the queue head for the fallback launch.
+__global__ void k_fields_commit(const unsigned long long* __restrict__ acc, uint32_t* __restrict__ slc,
+                                uint32_t* __restrict__ slt, size_t n, FieldsCtrl* __restrict__ ctrl,
+                                uint32_t accumulate, uint32_t force_fallback) {
+    const bool ok = fields_packed_ok(ctrl, force_fallback);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (ok) {
+            const unsigned long long a = acc[i];
+            const uint32_t lo = (uint32_t)a, hi = (uint32_t)(a >> 32);
+            if (accumulate) {
+                slc[i] += lo;
+                slt[i] += hi;
+            } else {
+                slc[i] = lo;
+                slt[i] = hi;
+            }
+        } else if (!accumulate) {
+            slc[i] = 0u;
+            slt[i] = 0u;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->queue = 0u;
+}
+
+static uint32_t grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    return (uint32_t)(b > 4096 ? 4096 : (b ? b : 1));
 }
 
 extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
@@ -334,11 +456,14 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     if (!mask || !uv || !slc || !slt) return fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
-    if (!(flags & SLM_FLAG_NO_ZERO)) {
-        fill_u32(slc, p->nxy_padded, 0u, s);
-        fill_u32(slt, p->nxy_padded, 0u, s);
+    const bool accumulate = (flags & SLM_FLAG_NO_ZERO) != 0;
+    if (n == 0) {
+        if (!accumulate) {
+            fill_u32(slc, p->nxy_padded, 0u, s);
+            fill_u32(slt, p->nxy_padded, 0u, s);
+        }
+        return finish(stream, flags);
     }
-    if (n == 0) return finish(stream, flags);
     if (!seeds_xy) return fail(SLM_E_INVALID_ARG, "NULL seeds");
     const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
     if ((uint64_t)n * d * d >= (1ull << 32) - 64)
@@ -351,12 +476,15 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     if (rc) return rc;
 
     char* ws = (char*)workspace;
-    uint32_t* queue = (uint32_t*)ws;
+    FieldsCtrl* ctrl = (FieldsCtrl*)ws;
     uint32_t* order = (uint32_t*)(ws + w.off_order);
     uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
     uint32_t* cur = (uint32_t*)(ws + w.off_cur);
+    unsigned long long* acc = (unsigned long long*)(ws + w.off_acc);
     const TileGeom g = tile_geom(p);
     const float2* seeds = (const float2*)seeds_xy;
+    const size_t nxy = p->nxy_padded;
+    const uint32_t force = (flags & SLM_FLAG_FORCE_FALLBACK) ? 1u : 0u;
 
     if (!(flags & SLM_FLAG_KEEP_ORDER)) {
         fill_u32(cnt, g.n_keys, 0u, s);
@@ -364,19 +492,31 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, cnt, cur, g.n_keys);
         hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255) / 256), dim3(256), 0, s, *p, seeds, n, cur, order);
     }
-    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(64), 0, s, queue);
+    hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, ctrl);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy);
 
     const uint32_t n_work = n * d * d;
     const uint32_t blocks_per_cu = 8;  // 32 waves per CU at 256 threads per block
     uint32_t grid = (uint32_t)di.cus * blocks_per_cu;
     const uint32_t max_useful = (n_work + 255) / 256;
     if (grid > max_useful) grid = max_useful ? max_useful : 1;
+    unsigned long long* st = (unsigned long long*)steps_out;
+    const float2* uv2 = (const float2*)uv;
     if (p->fp_contract)
-        hipLaunchKernelGGL(k_fields<true>, dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           (const float2*)uv, slc, slt, steps_out, queue);
+        hipLaunchKernelGGL((k_fields<true, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           uv2, acc, slc, slt, st, ctrl, force);
     else
-        hipLaunchKernelGGL(k_fields<false>, dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           (const float2*)uv, slc, slt, steps_out, queue);
+        hipLaunchKernelGGL((k_fields<false, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           uv2, acc, slc, slt, st, ctrl, force);
+    hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy, ctrl);
+    hipLaunchKernelGGL(k_fields_commit, dim3(grid_for(nxy)), dim3(256), 0, s, acc, slc, slt, nxy, ctrl,
+                       accumulate ? 1u : 0u, force);
+    if (p->fp_contract)
+        hipLaunchKernelGGL((k_fields<true, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           uv2, acc, slc, slt, nullptr, ctrl, force);
+    else
+        hipLaunchKernelGGL((k_fields<false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+                           uv2, acc, slc, slt, nullptr, ctrl, force);
     return finish(stream, flags);
 }
 

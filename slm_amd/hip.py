@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
 SLM_FLAG_SYNC = 1
 SLM_FLAG_NO_ZERO = 2
 SLM_FLAG_KEEP_ORDER = 4
+SLM_FLAG_FORCE_FALLBACK = 8
 
 _ERR_NAMES = {-1: "SLM_E_INVALID_ARG", -2: "SLM_E_HIP", -3: "SLM_E_OOM", -4: "SLM_E_ABI",
               -5: "SLM_E_WORKSPACE"}
@@ -105,7 +106,7 @@ def fields_workspace_size(p: SlmParams) -> int:
 
 
 def trace_fields(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=None,
-                 stream=None, accumulate=False, sync=False, keep_order=False):
+                 stream=None, accumulate=False, sync=False, keep_order=False, force_fallback=False):
     """One direction of integrate_fields on device tensors.
     seeds float32 [N,2]; mask uint8 [nxp*nyp]; uv float32 [nxp*nyp*2];
     slc/slt int32 [nxp*nyp] (uint32 bit patterns); steps int64 [1] or None."""
@@ -114,7 +115,7 @@ def trace_fields(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=
     if workspace is None:
         workspace = torch.empty(fields_workspace_size(p), dtype=torch.uint8, device=slc.device)
     flags = (SLM_FLAG_NO_ZERO if accumulate else 0) | (SLM_FLAG_SYNC if sync else 0) | \
-        (SLM_FLAG_KEEP_ORDER if keep_order else 0)
+        (SLM_FLAG_KEEP_ORDER if keep_order else 0) | (SLM_FLAG_FORCE_FALLBACK if force_fallback else 0)
     check(load().slm_trace_fields(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(slc),
                                   _ptr(slt), _ptr(steps), _ptr(workspace), workspace.numel(),
                                   _stream(stream), flags))

@@ -21,13 +21,13 @@ def _dev(job, dev):
     return seeds, mask, uv
 
 
-def _gpu_fields(p, seeds, mask, uv, dev):
+def _gpu_fields(p, seeds, mask, uv, dev, force_fallback=False):
     import torch
     from slm_amd import hip
     slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
     slt = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
     steps = torch.zeros(1, dtype=torch.int64, device=dev)
-    hip.trace_fields(p, seeds, mask, uv, slc, slt, steps, sync=True)
+    hip.trace_fields(p, seeds, mask, uv, slc, slt, steps, sync=True, force_fallback=force_fallback)
     return hip.as_u32(slc), hip.as_u32(slt), int(steps.item())
 
 
@@ -58,6 +58,28 @@ def test_fields_bit_exact(gpu, n, d, jitter, contract):
         assert gsteps == osteps
         assert np.array_equal(gslc, oslc), "slc differs at %d cells" % int((gslc != oslc).sum())
         assert np.array_equal(gslt, oslt), "slt differs at %d cells" % int((gslt != oslt).sum())
+
+
+@pytest.mark.parametrize("n,d,jitter", [(96, 2, 1.0), (256, 1, 1.0)])
+def test_fields_fallback_path(gpu, n, d, jitter):
+    """The 32-bit fallback accumulation (taken when the packed 64-bit check
+    fails) gives the same fields, also when accumulating into non-zero output."""
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=jitter, n=n, d=d)
+    seeds, mask, uv = _dev(job, gpu)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    p = job.params(-1)
+    gslc, gslt, gsteps = _gpu_fields(p, seeds, mask, uv, gpu, force_fallback=True)
+    oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1))
+    assert gsteps == osteps and np.array_equal(gslc, oslc) and np.array_equal(gslt, oslt)
+    base = np.arange(p.nxy_padded, dtype=np.uint32) * np.uint32(2654435761)
+    for force in (False, True):
+        slc = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
+        slt = torch.from_numpy((base ^ np.uint32(0xdeadbeef)).view(np.int32).copy()).to(gpu)
+        hip.trace_fields(p, seeds, mask, uv, slc, slt, accumulate=True, sync=True, force_fallback=force)
+        assert np.array_equal(hip.as_u32(slc), base + oslc)
+        assert np.array_equal(hip.as_u32(slt), (base ^ np.uint32(0xdeadbeef)) + oslt)
 
 
 def test_fields_planar_and_empty(gpu):

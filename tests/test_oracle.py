@@ -1,0 +1,148 @@
+"""Pins the CPU oracle (oracle/slm_oracle.c) without a GPU.
+
+* RNG: against exact big-integer arithmetic (brute force over many seeds and
+  draws, plus the edge states where the 32-bit +1 wraps and states >= modulus),
+  and against the committed golden vectors.
+* Kernels: against oracle/pyref.py, a separate pure-Python restatement of the
+  OpenCL C (float32 scalars, exact-rational FMA), bit for bit on tiny grids,
+  jitter on and off, both FP-contraction modes, both directions, d = 1, 2, 3.
+* Invariants the mathematics fixes: sum(slc) = seed writes + accepted steps,
+  lattice-exact positions for d in {1, 2}, FP64 twin close to FP32.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle, pyref
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+M = 0xfffffffb
+
+
+def _lehmer_np(s):
+    """Vectorised exact Lehmer step in uint64 (products < 2^61)."""
+    s = s.astype(np.uint64)
+    return (((s + np.uint64(1)) & np.uint64(0xffffffff)) * np.uint64(279470273)) % np.uint64(M)
+
+
+def test_lehmer_edge_states():
+    for s in [0, 1, 2, M - 1, M, M + 1, 0xfffffffe, 0xffffffff, 0x7fffffff, 0x80000000, 123456789]:
+        v, s2 = oracle.lehmer(s)
+        exp = ((s + 1) % (1 << 32)) * 279470273 % M
+        assert v == exp and s2 == exp, s
+    # +1 wraps in 32 bits before widening: state 2^32-1 maps to 0
+    assert oracle.lehmer(0xffffffff)[0] == 0
+
+
+def test_rng_stream_brute_force():
+    n, nd = 1 << 16, 32
+    rng0, draws = oracle.rng_stream(0, n, nd)
+    idx = np.arange(n, dtype=np.uint64)
+    bs = lambda v: ((v >> np.uint64(24)) & np.uint64(0xff)) | ((v >> np.uint64(8)) & np.uint64(0xff00)) | \
+        ((v << np.uint64(8)) & np.uint64(0xff0000)) | ((v << np.uint64(24)) & np.uint64(0xff000000))
+    s = bs(_lehmer_np(bs(idx)))
+    assert np.array_equal(rng0, s.astype(np.uint32))
+    for k in range(nd):
+        s = _lehmer_np(s)
+        assert np.array_equal(draws[:, k], s.astype(np.uint32))
+
+
+def test_rng_golden_vectors():
+    g = np.load(os.path.join(GOLDEN, "rng_vectors.npz"))
+    rng0, draws = oracle.rng_stream(int(g["idx0"]), int(g["rng0"].shape[0]), int(g["draws"].shape[1]))
+    assert np.array_equal(rng0, g["rng0"])
+    assert np.array_equal(draws, g["draws"])
+    for i, si in enumerate(g["seed_idx"]):
+        assert oracle.rng_seed(int(si)) == int(g["seed_rng0"][i])
+
+
+def test_rng_variates_in_range():
+    _, draws = oracle.rng_stream(0, 4096, 8)
+    v = draws.astype(np.float32) / np.float32(float(0xfffffffb)) - np.float32(0.5)
+    assert v.min() >= -0.5 and v.max() <= 0.5
+    assert abs(float(v.mean())) < 0.01
+
+
+def _tiny_job(n, d, jitter, seed=3):
+    from slm_amd import workloads
+    return workloads.small_256(jitter=jitter, n=n, d=d, seed=seed)
+
+
+PYREF_CASES = [
+    # (grid, d, jitter, contract, sign, n_seeds)
+    (24, 1, 1.0, True, 1, 40),
+    (24, 1, 0.0, True, -1, 40),
+    (24, 2, 1.0, False, 1, 20),
+    (28, 3, 1.0, True, -1, 10),
+    (28, 3, 0.0, False, 1, 10),
+]
+
+
+@pytest.mark.parametrize("grid,d,jitter,contract,sign,ns", PYREF_CASES)
+def test_fields_match_pyref(grid, d, jitter, contract, sign, ns):
+    job = _tiny_job(grid, d, jitter)
+    ns = min(ns, job.seeds.shape[0])
+    p = job.params(sign, fp_contract=contract, n_seed_points=ns)
+    p.max_n_steps = min(p.max_n_steps, 300)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1), nthreads=1)
+    rslc, rslt, rsteps = pyref.fields(p, job.seeds, job.mask, job.uv)
+    assert osteps == rsteps
+    assert np.array_equal(oslc, rslc)
+    assert np.array_equal(oslt, rslt)
+    # sum(slc) = one write at each seed + one per accepted step
+    assert int(oslc.sum()) == ns * d * d + osteps
+
+
+@pytest.mark.parametrize("grid,contract,sign", [(24, True, 1), (24, False, -1), (28, True, -1)])
+def test_paths_match_pyref(grid, contract, sign):
+    job = _tiny_job(grid, 1, 0.0)
+    ns = min(30, job.seeds.shape[0])
+    p = job.params(sign, kernel="INTEGRATE_TRAJECTORY", fp_contract=contract, n_seed_points=ns)
+    p.max_n_steps = min(p.max_n_steps, 400)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    otraj, onst, oln = oracle.trace_paths(p, job.seeds, m, job.uv.reshape(-1), nthreads=1)
+    rtraj, rnst, rln = pyref.paths(p, job.seeds, job.mask, job.uv)
+    assert np.array_equal(onst, rnst)
+    assert np.array_equal(oln.view(np.uint32), rln.view(np.uint32))
+    assert np.array_equal(otraj, rtraj)
+
+
+def test_oracle_threads_invariant():
+    """Integer fields do not depend on the order of accumulation."""
+    job = _tiny_job(96, 2, 1.0)
+    p = job.params(1)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    a = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1), nthreads=1)
+    b = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1), nthreads=8)
+    assert a[2] == b[2] and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("d", [1, 2])
+def test_positions_on_lattice(d):
+    """For d in {1, 2} every recorded position is a multiple of 1/128 (the
+    sub-pixel offsets 0 and -1/4 are, and every step is approximate()d)."""
+    job = _tiny_job(96, d, 1.0)
+    p = job.params(1)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    for k in range(0, 40):
+        xy, _ = oracle.trace_fields_record(p, job.seeds, m, job.uv.reshape(-1), k, k % (d * d), 4096)
+        q = xy.astype(np.float64) * 128.0
+        assert np.array_equal(q, np.round(q))
+
+
+def test_fp64_twin_close():
+    """FP64 trace of the same algorithm stays within the 1/128 lattice of the
+    FP32 trace for the first steps (it diverges only after a truncation lands
+    differently)."""
+    job = _tiny_job(96, 1, 1.0)
+    p = job.params(1)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    agree = 0
+    for k in range(40):
+        a, _ = oracle.trace_fields_record(p, job.seeds, m, job.uv.reshape(-1), k, 0, 64)
+        b, _ = oracle.trace_fields_record(p, job.seeds, m, job.uv.reshape(-1), k, 0, 64, f64=True)
+        kk = min(len(a), len(b), 5)
+        agree += int(np.abs(a[:kk].astype(np.float64) - b[:kk]).max() <= 1.0 / 128 + 1e-9)
+    assert agree >= 36
