@@ -80,7 +80,8 @@ typedef struct SlmParams {
     uint32_t seeds_chunk_offset;          /* SEEDS_CHUNK_OFFSET                      */
     uint32_t fp_contract;                 /* 1: a*b+c fused as clang's OpenCL default
                                              FP_CONTRACT ON does; 0: every op rounded */
-    uint32_t reserved[8];
+    float    pixel_size;                  /* PIXEL_SIZE (metres)                     */
+    uint32_t reserved[7];
 } SlmParams;
 
 /* Library identity. */
@@ -147,6 +148,73 @@ int slm_trace_fields_record(const SlmParams* p, const float* seeds_xy,
                             uint32_t n_sel, uint32_t max_rec,
                             float* xy, uint32_t* nrec,
                             void* stream, uint32_t flags);
+
+/* ------------------------------------------------------------------ KDE */
+
+/* Kernel-density filter shapes of opencl/kde.cl:36-92 (KDF_IS_* macros). */
+#define SLM_KDF_TOPHAT       0u
+#define SLM_KDF_TRIANGLE     1u
+#define SLM_KDF_EPANECHNIKOV 2u
+#define SLM_KDF_COSINE       3u
+#define SLM_KDF_GAUSSIAN     4u
+
+/*
+ * Constants of the reference's 'kde' macro set (pocl.py:195-221), as data.
+ * Float fields hold the value of the f-suffixed literal the reference
+ * rendered (KDF_WIDTH_X, BIN_DX, PDF_DX are float literals widened to
+ * double where the kernel computes in double).  64 bytes.
+ */
+typedef struct SlmKdeParams {
+    uint32_t abi_version;          /* SLM_ABI_VERSION               */
+    uint32_t kernel;               /* SLM_KDF_*                     */
+    uint32_t n_data;               /* N_DATA                        */
+    uint32_t n_hist_bins;          /* N_HIST_BINS                   */
+    uint32_t n_pdf_points;         /* N_PDF_POINTS                  */
+    uint32_t n_kdf_part_points_x;  /* N_KDF_PART_POINTS_X           */
+    float    logx_min;             /* LOGX_MIN                      */
+    float    logx_range;           /* LOGX_RANGE                    */
+    float    bin_dx;               /* BIN_DX                        */
+    float    pdf_dx;               /* PDF_DX                        */
+    float    kdf_width_x;          /* KDF_WIDTH_X                   */
+    uint32_t fp_contract;          /* as SlmParams.fp_contract      */
+    uint32_t reserved[4];
+} SlmKdeParams;
+
+/*
+ * Histogram of log values.  Replaces kernel histogram_univariate
+ * (opencl/kde.cl:383-394) launched by kde.estimate_univariate_pdf
+ * (python/streamlines/kde.py:165-169).
+ *   logx : float [n_data]   hist : uint32 [n_hist_bins], cleared first unless
+ *   SLM_FLAG_NO_ZERO.
+ */
+int slm_kde_histogram(const SlmKdeParams* k, const float* logx, uint32_t* hist,
+                      void* stream, uint32_t flags);
+
+/*
+ * Kernel-smoothed pdf of a histogram, FP64 accumulation.  Replaces kernel
+ * pdf_univariate with filter (opencl/kde.cl:426-526), launched by
+ * kde.estimate_univariate_pdf (kde.py:185-189).
+ *   hist : uint32 [n_hist_bins]   pdf : float [n_pdf_points]
+ */
+int slm_kde_pdf(const SlmKdeParams* k, const uint32_t* hist, float* pdf,
+                void* stream, uint32_t flags);
+
+/* ------------------------------------------------------------------ hillslope lengths */
+
+/*
+ * Traced hillslope lengths.  Replaces kernel hillslope_lengths
+ * (opencl/lengths.cl:37-116, with lengths_runge_kutta_step of
+ * rungekutta.cl:492-516) launched by lengths.hsl (python/streamlines/lengths.py:33-92).
+ * From each seed (a mid-slope or ridge pixel) an un-jittered downstream RK2
+ * trace runs until it reaches a pixel flagged thin_flag in mapping; it is
+ * abandoned on a masked pixel, on re-entering a from_flag pixel after leaving
+ * the seed band, or when stuck.  Seeds k in [seeds_chunk_offset, n_seed_points)
+ * that reach a channel write length[k] = l * pixel_size; others leave length[k]
+ * unchanged.  p must be built for downstream tracing.
+ */
+int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                          const float* uv, const uint32_t* mapping, uint32_t thin_flag,
+                          uint32_t from_flag, float* length, void* stream, uint32_t flags);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@ Functions (each restates the reference kernel named in oracle/slm_oracle.c):
   trace_fields                         integratefields.cl:60-114 + jittertrajectory.cl
   trace_fields_record                  one jittered streamline, positions recorded
   trace_paths                          integratetraj.cl:54-97 + trajectory.cl
-Each has an FP64 twin (f64=True) built from the same source.
+  hillslope_lengths                    lengths.cl:37-116
+  kde_histogram, kde_pdf               kde.cl:383-394, :426-526
+The tracing functions have an FP64 twin (f64=True) built from the same source.
 """
 from __future__ import annotations
 
@@ -42,8 +44,14 @@ def load():
             getattr(lib, pre + "trace_fields").argtypes = [_P, _P, _P, _P, _P, _P, ctypes.c_int]
             getattr(lib, pre + "trace_fields_record").restype = _U32
             getattr(lib, pre + "trace_fields_record").argtypes = [_P, _P, _P, _P, _U32, _U32, _P, _U32, _P]
+            getattr(lib, pre + "hillslope_lengths").restype = None
+            getattr(lib, pre + "hillslope_lengths").argtypes = [_P, _P, _P, _P, _P, _U32, _U32, _P]
             getattr(lib, pre + "trace_paths").restype = None
             getattr(lib, pre + "trace_paths").argtypes = [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]
+        lib.slmo_kde_histogram.restype = None
+        lib.slmo_kde_histogram.argtypes = [_P, _P, _P]
+        lib.slmo_kde_pdf.restype = None
+        lib.slmo_kde_pdf.argtypes = [_P, _P, _P]
         _lib = lib
     return _lib
 
@@ -111,3 +119,33 @@ def trace_paths(p, seeds, mask, uv, nthreads=8, f64=False):
     getattr(load(), _pre(f64) + "trace_paths")(ctypes.byref(p), ps, pm, pu, traj.ctypes.data,
                                                nsteps.ctypes.data, length.ctypes.data, int(nthreads))
     return traj, nsteps, length
+
+
+def kde_histogram(k, logx):
+    """histogram_univariate (kde.cl:383-394): uint32 [n_hist_bins]."""
+    logx, px = _c(logx, np.float32)
+    hist = np.zeros(k.n_hist_bins, dtype=np.uint32)
+    load().slmo_kde_histogram(ctypes.byref(k), px, hist.ctypes.data)
+    return hist
+
+
+def kde_pdf(k, hist):
+    """pdf_univariate with filter (kde.cl:426-526): float32 [n_pdf_points]."""
+    hist, ph = _c(hist, np.uint32)
+    pdf = np.zeros(k.n_pdf_points, dtype=np.float32)
+    load().slmo_kde_pdf(ctypes.byref(k), ph, pdf.ctypes.data)
+    return pdf
+
+
+def hillslope_lengths(p, seeds, mask, uv, mapping, thin_flag, from_flag, length=None, f64=False):
+    """hillslope_lengths (lengths.cl:37-116): float32 [n_seed_points], untouched
+    entries keep their initial value (zeros by default)."""
+    n = p.n_seed_points
+    length = np.zeros(n, dtype=np.float32) if length is None else np.ascontiguousarray(length, np.float32)
+    seeds, ps = _c(seeds, np.float32)
+    mask, pm = _c(mask, np.uint8)
+    uv, pu = _c(uv, np.float32)
+    mapping, pmap = _c(mapping, np.uint32)
+    getattr(load(), _pre(f64) + "hillslope_lengths")(ctypes.byref(p), ps, pm, pu, pmap, thin_flag, from_flag,
+                                                     length.ctypes.data)
+    return length

@@ -292,3 +292,44 @@ def paths(p, seeds, mask2d, uv3d):
         nsteps[gid] = n
         length[gid] = l
     return traj, nsteps, length
+
+
+def hillslope_lengths(p, seeds, mask2d, uv3d, mapping2d, thin_flag, from_flag):
+    """hillslope_lengths (lengths.cl:37-116) with lengths_runge_kutta_step
+    (rungekutta.cl:492-516); unreached seeds stay 0."""
+    k = K(p)
+    mask = mask2d.reshape(-1).astype(bool)
+    mapping = mapping2d.reshape(-1).astype(np.int64)
+    out = np.zeros(k.N, dtype=np.float32)
+    for gid in range(k.OFF, k.N):
+        x, y = f32(seeds[gid, 0]), f32(seeds[gid, 1])
+        n, l, dt, moved_off = 0, f32(0.0), k.DT_MAX, False
+        idx = array_index(k, x, y)
+        abandoned = False
+        while ((~mapping[idx]) & thin_flag) and n < k.MAX_N_STEPS:
+            u1, d1, d2, nxt, idx, _ = step(k, uv3d, dt, x, y, 0, False)
+            if mask[idx]:
+                abandoned = True
+                break
+            if not moved_off and ((~mapping[idx]) & from_flag):
+                moved_off = True
+            elif moved_off and (mapping[idx] & from_flag):
+                abandoned = True
+                break
+            with np.errstate(divide="ignore", invalid="ignore"):
+                err = fast_length(k, f32((d2[0] - d1[0]) / k.GRID_SCALE), f32((d2[1] - d1[1]) / k.GRID_SCALE))
+            dl = fast_length(k, d2[0], d2[1])
+            if dl < k.HALT:
+                nxt = (f32(x + d1[0]), f32(y + d1[1]))
+                dl = fast_length(k, d1[0], d1[1])
+            x, y = nxt
+            idx = array_index(k, x, y)
+            if dl < k.HALT:
+                abandoned = True
+                break
+            dt = _dt_update(k, dt, err)
+            l = f32(l + dl)
+            n += 1
+        if not abandoned and (mapping[idx] & thin_flag):
+            out[gid] = f32(l * f32(p.pixel_size))
+    return out

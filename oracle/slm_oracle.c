@@ -426,6 +426,48 @@ void SLM_PREFIX(trace_paths)(const SlmParams* p, const float* seeds_xy,
     }
 }
 
+/*
+ * hillslope_lengths (lengths.cl:37-116) with lengths_runge_kutta_step
+ * (rungekutta.cl:492-516): seed k in [seeds_chunk_offset, n_seed_points)
+ * traces downstream (un-jittered) to a thin_flag pixel and writes
+ * length[k] = l * PIXEL_SIZE; masked pixels, re-crossing a from_flag pixel
+ * after leaving the seed band, and stuck steps abandon the trace.
+ */
+void SLM_PREFIX(hillslope_lengths)(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                   const float* uv, const uint32_t* mapping, uint32_t thin_flag,
+                                   uint32_t from_flag, float* length) {
+    const int c = (int)p->fp_contract;
+    for (uint32_t k = p->seeds_chunk_offset; k < p->n_seed_points; k++) {
+        real vec[2] = {(real)seeds_xy[2 * (size_t)k], (real)seeds_xy[2 * (size_t)k + 1]};
+        real l = 0, dt = (real)p->dt_max;
+        uint32_t n = 0;
+        int moved_off = 0, abandon = 0;
+        uint32_t idx = array_idx(p, vec[0], vec[1]);
+        Step st;
+        while (((~mapping[idx]) & thin_flag) && n < p->max_n_steps) {
+            compute_step(p, uv, dt, vec, NULL, 0, &st);
+            idx = st.idx;
+            if (mask[idx]) { abandon = 1; break; }
+            if (!moved_off && ((~mapping[idx]) & from_flag)) moved_off = 1;
+            else if (moved_off && (mapping[idx] & from_flag)) { abandon = 1; break; }
+            const real err = step_error(p, &st);
+            real dl = fast_length(c, st.dxy2[0], st.dxy2[1]);
+            if (dl < (real)p->integration_halt_threshold) {
+                st.next[0] = vec[0] + st.dxy1[0];
+                st.next[1] = vec[1] + st.dxy1[1];
+                dl = fast_length(c, st.dxy1[0], st.dxy1[1]);
+            }
+            vec[0] = st.next[0]; vec[1] = st.next[1];
+            idx = array_idx(p, vec[0], vec[1]);
+            if (dl < (real)p->integration_halt_threshold) { abandon = 1; break; }
+            dt = next_dt(p, dt, err);
+            l += dl;
+            n += 1u;
+        }
+        if (!abandon && (mapping[idx] & thin_flag)) length[k] = (float)(l * (real)p->pixel_size);
+    }
+}
+
 /* Lehmer draws for n seeds from idx0 (slm_rng_stream restated). */
 void SLM_PREFIX(rng_stream)(uint32_t idx0, uint32_t n, uint32_t n_draws,
                             uint32_t* rng0_out, uint32_t* draws) {

@@ -17,6 +17,7 @@
 #include <mutex>
 #include "../../include/slm_hip.h"
 #include "trace_common.hpp"
+#include "common.hpp"
 
 using namespace slm;
 
@@ -24,7 +25,7 @@ using namespace slm;
 
 static thread_local char g_err[512];
 
-static int fail(int code, const char* fmt, ...) {
+int slm_fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -32,12 +33,11 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                              \
-    do {                                                                           \
-        hipError_t e_ = (expr);                                                    \
-        if (e_ != hipSuccess)                                                      \
-            return fail(SLM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+int slm_finish(void* stream, uint32_t flags) {
+    HIP_TRY(hipGetLastError());
+    if (flags & SLM_FLAG_SYNC) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return SLM_OK;
+}
 
 extern "C" uint32_t slm_abi_version(void) { return SLM_ABI_VERSION; }
 extern "C" const char* slm_last_error(void) { return g_err; }
@@ -48,28 +48,23 @@ extern "C" int slm_device_count(void) {
 }
 
 static int check_params(const SlmParams* p) {
-    if (!p) return fail(SLM_E_INVALID_ARG, "params is NULL");
+    if (!p) return slm_fail(SLM_E_INVALID_ARG, "params is NULL");
     if (p->abi_version != SLM_ABI_VERSION)
-        return fail(SLM_E_ABI, "SlmParams.abi_version %u != %u", p->abi_version, SLM_ABI_VERSION);
+        return slm_fail(SLM_E_ABI, "SlmParams.abi_version %u != %u", p->abi_version, SLM_ABI_VERSION);
     if (p->nx_padded == 0 || p->ny_padded == 0 ||
         (uint64_t)p->nx_padded * p->ny_padded != (uint64_t)p->nxy_padded)
-        return fail(SLM_E_INVALID_ARG, "grid %u x %u != nxy_padded %u", p->nx_padded, p->ny_padded,
+        return slm_fail(SLM_E_INVALID_ARG, "grid %u x %u != nxy_padded %u", p->nx_padded, p->ny_padded,
                     p->nxy_padded);
     if (p->nx_padded > (1u << 20) || p->ny_padded > (1u << 20))
-        return fail(SLM_E_INVALID_ARG, "grid side above 2^20");
+        return slm_fail(SLM_E_INVALID_ARG, "grid side above 2^20");
     if (!(p->trajectory_resolution > 0.0f))
-        return fail(SLM_E_INVALID_ARG, "trajectory_resolution must be > 0");
+        return slm_fail(SLM_E_INVALID_ARG, "trajectory_resolution must be > 0");
     if (p->seeds_chunk_offset > p->n_seed_points)
-        return fail(SLM_E_INVALID_ARG, "seeds_chunk_offset %u > n_seed_points %u", p->seeds_chunk_offset,
+        return slm_fail(SLM_E_INVALID_ARG, "seeds_chunk_offset %u > n_seed_points %u", p->seeds_chunk_offset,
                     p->n_seed_points);
     return SLM_OK;
 }
 
-static int finish(void* stream, uint32_t flags) {
-    HIP_TRY(hipGetLastError());
-    if (flags & SLM_FLAG_SYNC) HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
-    return SLM_OK;
-}
 
 struct DevInfo {
     int cus = 0;
@@ -80,7 +75,7 @@ static DevInfo g_dev[64];
 static int device_info(DevInfo& out) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
-    if (dev < 0 || dev >= 64) return fail(SLM_E_HIP, "device id %d out of range", dev);
+    if (dev < 0 || dev >= 64) return slm_fail(SLM_E_HIP, "device id %d out of range", dev);
     std::lock_guard<std::mutex> lk(g_dev_mu);
     if (g_dev[dev].cus == 0) {
         int cus = 0;
@@ -119,12 +114,12 @@ __global__ void k_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint32
 
 extern "C" int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint32_t* rng0_out,
                               uint32_t* draws, void* stream, uint32_t flags) {
-    if (n == 0) return finish(stream, flags);
-    if (!rng0_out && !draws) return fail(SLM_E_INVALID_ARG, "no output buffer");
-    if (draws && n_draws == 0) return fail(SLM_E_INVALID_ARG, "n_draws is 0 with a draws buffer");
+    if (n == 0) return slm_finish(stream, flags);
+    if (!rng0_out && !draws) return slm_fail(SLM_E_INVALID_ARG, "no output buffer");
+    if (draws && n_draws == 0) return slm_fail(SLM_E_INVALID_ARG, "n_draws is 0 with a draws buffer");
     hipLaunchKernelGGL(k_rng_stream, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, idx0, n,
                        n_draws, rng0_out, draws);
-    return finish(stream, flags);
+    return slm_finish(stream, flags);
 }
 
 // ------------------------------------------------------------------ seed tiling
@@ -453,7 +448,7 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
                                 void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
     int rc = check_params(p);
     if (rc) return rc;
-    if (!mask || !uv || !slc || !slt) return fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
+    if (!mask || !uv || !slc || !slt) return slm_fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
     const bool accumulate = (flags & SLM_FLAG_NO_ZERO) != 0;
@@ -462,15 +457,15 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
             fill_u32(slc, p->nxy_padded, 0u, s);
             fill_u32(slt, p->nxy_padded, 0u, s);
         }
-        return finish(stream, flags);
+        return slm_finish(stream, flags);
     }
-    if (!seeds_xy) return fail(SLM_E_INVALID_ARG, "NULL seeds");
+    if (!seeds_xy) return slm_fail(SLM_E_INVALID_ARG, "NULL seeds");
     const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
     if ((uint64_t)n * d * d >= (1ull << 32) - 64)
-        return fail(SLM_E_INVALID_ARG, "n_seeds * d^2 = %llu exceeds 2^32", (unsigned long long)n * d * d);
+        return slm_fail(SLM_E_INVALID_ARG, "n_seeds * d^2 = %llu exceeds 2^32", (unsigned long long)n * d * d);
     const FieldsWs w = fields_ws(p);
     if (!workspace || workspace_bytes < w.total)
-        return fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
     DevInfo di;
     rc = device_info(di);
     if (rc) return rc;
@@ -517,7 +512,7 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     else
         hipLaunchKernelGGL((k_fields<false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
                            uv2, acc, slc, slt, nullptr, ctrl, force);
-    return finish(stream, flags);
+    return slm_finish(stream, flags);
 }
 
 // ------------------------------------------------------------------ fields record (test hook)
@@ -567,9 +562,9 @@ extern "C" int slm_trace_fields_record(const SlmParams* p, const float* seeds_xy
                                        void* stream, uint32_t flags) {
     int rc = check_params(p);
     if (rc) return rc;
-    if (n_sel == 0) return finish(stream, flags);
+    if (n_sel == 0) return slm_finish(stream, flags);
     if (!seeds_xy || !mask || !uv || !seed_sel || !sub_sel || !nrec || (max_rec && !xy))
-        return fail(SLM_E_INVALID_ARG, "NULL buffer");
+        return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
     if (p->fp_contract)
         hipLaunchKernelGGL(k_fields_record<true>, dim3((n_sel + 63) / 64), dim3(64), 0, s, *p,
@@ -579,7 +574,7 @@ extern "C" int slm_trace_fields_record(const SlmParams* p, const float* seeds_xy
         hipLaunchKernelGGL(k_fields_record<false>, dim3((n_sel + 63) / 64), dim3(64), 0, s, *p,
                            (const float2*)seeds_xy, mask, (const float2*)uv, seed_sel, sub_sel, n_sel, max_rec,
                            (float2*)xy, nrec);
-    return finish(stream, flags);
+    return slm_finish(stream, flags);
 }
 
 // ------------------------------------------------------------------ trajectory record
@@ -654,10 +649,10 @@ extern "C" int slm_trace_paths(const SlmParams* p, const float* seeds_xy, const 
     int rc = check_params(p);
     if (rc) return rc;
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
-    if (n == 0) return finish(stream, flags);
-    if (!seeds_xy || !mask || !uv || !traj || !nsteps || !length) return fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (n == 0) return slm_finish(stream, flags);
+    if (!seeds_xy || !mask || !uv || !traj || !nsteps || !length) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     if (p->max_n_steps > 65535u)
-        return fail(SLM_E_INVALID_ARG, "max_n_steps %u does not fit the uint16 step count", p->max_n_steps);
+        return slm_fail(SLM_E_INVALID_ARG, "max_n_steps %u does not fit the uint16 step count", p->max_n_steps);
     hipStream_t s = (hipStream_t)stream;
     if (p->fp_contract)
         hipLaunchKernelGGL(k_paths<true>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
@@ -665,5 +660,63 @@ extern "C" int slm_trace_paths(const SlmParams* p, const float* seeds_xy, const 
     else
         hipLaunchKernelGGL(k_paths<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
                            (const float2*)uv, (char2*)traj, nsteps, length, n);
-    return finish(stream, flags);
+    return slm_finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ hillslope lengths
+
+// hillslope_lengths (lengths.cl:37-116) with lengths_runge_kutta_step
+// (rungekutta.cl:492-516): one lane per mid-slope/ridge seed.
+template <bool C>
+__global__ void k_hsl(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                      const float2* __restrict__ uv, const uint32_t* __restrict__ mapping, uint32_t thin_flag,
+                      uint32_t from_flag, float* __restrict__ length) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, l = 0.f, dt = P.dt_max;
+    uint32_t n = 0, rng = 0;
+    bool moved_off = false;
+    uint32_t idx = array_idx(P, x, y);
+    while (((~mapping[idx]) & thin_flag) && n < P.max_n_steps) {
+        StepOut o;
+        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        idx = o.idx;
+        if (mask[idx]) return;
+        if (!moved_off && ((~mapping[idx]) & from_flag)) moved_off = true;
+        else if (moved_off && (mapping[idx] & from_flag)) return;
+        const float err = step_error<C>(P, o);
+        float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+        float2 nx = o.next;
+        if (dl < P.integration_halt_threshold) {
+            nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
+            dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+        }
+        x = nx.x;
+        y = nx.y;
+        idx = array_idx(P, x, y);
+        if (dl < P.integration_halt_threshold) return;
+        dt = next_dt(P, dt, err);
+        l += dl;
+        n += 1u;
+    }
+    if (mapping[idx] & thin_flag) length[k] = l * P.pixel_size;
+}
+
+extern "C" int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                     const float* uv, const uint32_t* mapping, uint32_t thin_flag,
+                                     uint32_t from_flag, float* length, void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    if (n == 0) return slm_finish(stream, flags);
+    if (!seeds_xy || !mask || !uv || !mapping || !length) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_hsl<true>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
+                           (const float2*)uv, mapping, thin_flag, from_flag, length);
+    else
+        hipLaunchKernelGGL(k_hsl<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
+                           (const float2*)uv, mapping, thin_flag, from_flag, length);
+    return slm_finish(stream, flags);
 }

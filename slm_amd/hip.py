@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .params import SlmParams, ABI_VERSION
+from .params import SlmKdeParams, SlmParams, ABI_VERSION
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
@@ -37,6 +37,10 @@ _PROTOS = {
     "slm_trace_paths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P, _U32]),
     "slm_trace_fields_record": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _U32,
                                                _U32, _P, _P, _P, _U32]),
+    "slm_hillslope_lengths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _U32, _U32, _P, _P,
+                                             _U32]),
+    "slm_kde_histogram": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
+    "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
 }
 
 _lib = None
@@ -138,6 +142,16 @@ def trace_fields_record(p: SlmParams, seeds, mask, uv, seed_sel, sub_sel, max_re
                                          _ptr(seed_sel), _ptr(sub_sel), seed_sel.numel(), max_rec,
                                          _ptr(xy), _ptr(nrec), _stream(stream),
                                          SLM_FLAG_SYNC if sync else 0))
+
+
+def hillslope_lengths(p: SlmParams, seeds, mask, uv, mapping, thin_flag, from_flag, length, stream=None,
+                      sync=False):
+    """lengths.cl hillslope_lengths on device tensors: mapping int32 [nxy]
+    (uint32 bits), length float32 [n_seed_points] updated in place."""
+    _require_cuda(seeds, mask, uv, mapping, length)
+    check(load().slm_hillslope_lengths(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(mapping),
+                                       int(thin_flag), int(from_flag), _ptr(length), _stream(stream),
+                                       SLM_FLAG_SYNC if sync else 0))
 
 
 def as_u32(t) -> np.ndarray:

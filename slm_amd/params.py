@@ -44,7 +44,8 @@ _FIELD_NAMES = [
     ("n_seed_points", ctypes.c_uint32),
     ("seeds_chunk_offset", ctypes.c_uint32),
     ("fp_contract", ctypes.c_uint32),
-    ("reserved", ctypes.c_uint32 * 8),
+    ("pixel_size", ctypes.c_float),
+    ("reserved", ctypes.c_uint32 * 7),
 ]
 
 
@@ -63,6 +64,20 @@ class SlmParams(ctypes.Structure):
 
 
 assert ctypes.sizeof(SlmParams) == 128
+
+class SlmKdeParams(ctypes.Structure):
+    """Mirror of ``struct SlmKdeParams`` in include/slm_hip.h (64 bytes)."""
+    _fields_ = [("abi_version", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
+                ("n_data", ctypes.c_uint32), ("n_hist_bins", ctypes.c_uint32),
+                ("n_pdf_points", ctypes.c_uint32), ("n_kdf_part_points_x", ctypes.c_uint32),
+                ("logx_min", ctypes.c_float), ("logx_range", ctypes.c_float),
+                ("bin_dx", ctypes.c_float), ("pdf_dx", ctypes.c_float),
+                ("kdf_width_x", ctypes.c_float), ("fp_contract", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+assert ctypes.sizeof(SlmKdeParams) == 64
+
 
 _libc = ctypes.CDLL(None)
 _libc.strtof.restype = ctypes.c_float

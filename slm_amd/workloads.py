@@ -31,6 +31,7 @@ class Job:
     seeds: np.ndarray          # float32 [N, 2], shuffled
     trace: SimpleNamespace
     state: SimpleNamespace
+    origin: tuple = (0, 0)     # full padded index of cropped index (0, 0)
 
     @property
     def shape(self):
@@ -63,7 +64,10 @@ def job_from_dtm(name, roi, pixel_size, trace_overrides=None, state_overrides=No
                                   do_shuffle=info.do_shuffle, rng_seed=info.shuffle_rng_seed)
     info.n_seed_points = np.uint32(n)
     info.n_padded_seed_points = np.uint32(npad)
-    return Job(name, roi, pixel_size, data.mask_array, data.uv_array, info, seeds, trace, state)
+    origin = (int(bbox[0]) - pad_width, int(bbox[2]) - pad_width)
+    return Job(name, roi, pixel_size, data.mask_array, data.uv_array, info, seeds, trace, state, origin)
+
+    # note: a DTM pixel (i, j) sits at cropped padded index (i + pad - origin[0], j + pad - origin[1])
 
 
 def small_256(jitter=1.0, seed=0, n=256, d=1):
@@ -92,3 +96,28 @@ def planar(n=64, pixel_size=1.0, slope=(0.3, 0.1), d=1, jitter=0.0, max_length=1
     return job_from_dtm("planar_%d" % n, roi, pixel_size,
                         {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
                          "max_length": max_length, "integrator_step_factor": 0.5})
+
+
+def v_valley(n=96, pixel_size=1.0, side_slope=0.2, down_slope=0.05, midslope_offset=12):
+    """Planar V-valley along y at x = n//2: h = s*|x-c| - t*y, NaN border.
+    Returns (job, mapping): mapping (uint32, cropped padded grid) flags the
+    valley axis thin-channel (2) and the lines x = c +- midslope_offset
+    mid-slope (2048), the flag values of useful.Info."""
+    c = n // 2
+    x = np.arange(n, dtype=np.float64)[:, None]
+    y = np.arange(n, dtype=np.float64)[None, :]
+    roi = ((side_slope * np.abs(x - c) - down_slope * y) * pixel_size).astype(np.float32)
+    roi[:2, :] = np.nan
+    roi[-2:, :] = np.nan
+    roi[:, :2] = np.nan
+    roi[:, -2:] = np.nan
+    job = job_from_dtm("v_valley_%d" % n, roi, pixel_size, {"subpixel_seed_point_density": 1,
+                                                            "jitter_magnitude": 0.0})
+    mapping = np.zeros(job.mask.shape, dtype=np.uint32)
+    ox, oy = job.origin
+    col = lambda i: i + 2 - ox
+    mapping[col(c), :] |= np.uint32(job.info.is_thinchannel)
+    for xm in (c - midslope_offset, c + midslope_offset):
+        mapping[col(xm), :] |= np.uint32(job.info.is_midslope)
+    mapping[job.mask] = 0
+    return job, mapping

@@ -184,3 +184,25 @@ def test_paths_bit_exact(gpu, n, contract):
         assert np.array_equal(nst.cpu().numpy().view(np.uint16), onst)
         assert np.array_equal(ln.cpu().numpy().view(np.uint32), oln.view(np.uint32))
         assert np.array_equal(traj.cpu().numpy(), otraj)
+
+
+@pytest.mark.parametrize("contract,offset", [(True, 12), (False, 8), (True, 20)])
+def test_hsl_bit_exact(gpu, contract, offset):
+    import torch
+    from slm_amd import hip, workloads
+    from slm_amd.params import kernel_params
+    from slm_amd.useful import pick_seeds
+    job, mapping = workloads.v_valley(midslope_offset=offset)
+    seeds = pick_seeds(mask=job.mask, map=mapping, flag=int(job.info.is_midslope), pad=2)
+    job.info.n_seed_points = np.uint32(seeds.shape[0])
+    p = kernel_params(job.info, 1, "HILLSLOPE_LENGTHS", contract)
+    thin, mid = int(job.info.is_thinchannel), int(job.info.is_midslope)
+    init = np.full(seeds.shape[0], -1.0, dtype=np.float32)
+    o = oracle.hillslope_lengths(p, seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1),
+                                 mapping.reshape(-1), thin, mid, length=init.copy())
+    sd, m, uv = torch.from_numpy(seeds).to(gpu), *_dev(job, gpu)[1:]
+    mp = torch.from_numpy(mapping.reshape(-1).view(np.int32)).to(gpu)
+    L = torch.from_numpy(init.copy()).to(gpu)
+    hip.hillslope_lengths(p, sd, m, uv, mp, thin, mid, L, sync=True)
+    assert np.array_equal(L.cpu().numpy().view(np.uint32), o.view(np.uint32))
+    assert (o == -1.0).sum() < 0.1 * o.size   # most reach the channel; the rest keep their value

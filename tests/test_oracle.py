@@ -146,3 +146,143 @@ def test_fp64_twin_close():
         kk = min(len(a), len(b), 5)
         agree += int(np.abs(a[:kk].astype(np.float64) - b[:kk]).max() <= 1.0 / 128 + 1e-9)
     assert agree >= 36
+
+
+# ---------------------------------------------------------------- KDE
+
+def _kde_case(kernel, n_hist=512, n_pdf=128, n=20000, bandwidth=1.0, seed=0):
+    from slm_amd.kde import Univariate_distribution, kde_params, prepare_filter
+    rs = np.random.default_rng(seed)
+    x = np.concatenate([rs.lognormal(3.0, 0.6, n // 2), rs.lognormal(4.5, 0.3, n - n // 2)]).astype(np.float32)
+    lx = np.log(x).astype(np.float32)
+    d = Univariate_distribution(logx_array=lx, logy_array=lx, n_hist_bins=n_hist, n_pdf_points=n_pdf)
+    d.kernel, d.bandwidth = kernel, bandwidth
+    prepare_filter(d)
+    return d, lx
+
+
+@pytest.mark.parametrize("contract", [True, False])
+def test_kde_histogram_matches_numpy(contract):
+    from slm_amd.kde import kde_params
+    d, lx = _kde_case("epanechnikov")
+    k = kde_params(d, contract)
+    h = oracle.kde_histogram(k, d.logx_data.reshape(-1))
+    bx = ((d.logx_data.reshape(-1) - np.float32(k.logx_min)) / np.float32(k.logx_range)).astype(np.float32)
+    t = (bx * np.float32(k.n_hist_bins)).astype(np.float32)
+    idx = np.minimum(np.trunc(np.maximum(t, 0)).astype(np.int64), k.n_hist_bins - 1)
+    assert np.array_equal(h, np.bincount(idx, minlength=k.n_hist_bins).astype(np.uint32))
+    assert int(h.sum()) == d.n_data
+
+
+def _pdf_pure_python(k, hist):
+    """filter/pdf_univariate (kde.cl:426-526) in plain Python doubles."""
+    import math
+    nbp = k.n_hist_bins // k.n_pdf_points
+    w = float(np.float32(k.kdf_width_x))
+    out = np.zeros(k.n_pdf_points, dtype=np.float32)
+    fma = lambda a, b, c: float(np.float64(a) * np.float64(b) + np.float64(c)) if not k.fp_contract else None
+
+    def ks(x):
+        if k.kernel == 0:
+            return 1.0 if abs(x) < w else 0.0
+        if k.kernel == 1:
+            return max(1.0 - abs(x / w), 0.0)
+        if k.kernel == 2:
+            q = x / w
+            if k.fp_contract:
+                from fractions import Fraction
+                return max(float(Fraction(1) - Fraction(q) * Fraction(q)), 0.0)
+            return max(1.0 - q * q, 0.0)
+        if k.kernel == 3:
+            return max(math.cos(1.57079632679489661923 * x), 0.0)
+        q = x / (w * float(np.float32(0.4)))
+        return math.exp(-0.5 * q * q)
+
+    for p in range(k.n_pdf_points):
+        acc = kw = 0.0
+        for kk in range(k.n_kdf_part_points_x + 1):
+            edge = kk * float(np.float32(k.pdf_dx)) - float(np.float32(np.float32(k.pdf_dx) / np.float32(2.0)))
+            cr, cl = p + kk, p - kk + 1
+            for h in range(nbp):
+                hh = h + 0.5
+                if k.fp_contract:
+                    from fractions import Fraction
+                    x = float(Fraction(float(np.float32(k.bin_dx))) * Fraction(hh) + Fraction(edge))
+                else:
+                    x = edge + float(np.float32(k.bin_dx)) * hh
+                v = ks(x)
+                okr, okl = cr < k.n_pdf_points, (kk != 0 and cl > 0)
+                acc += float(hist[cr * nbp + h]) * v if okr else 0.0
+                acc += float(hist[cl * nbp - h - 1]) * v if okl else 0.0
+                kw += v if okr else 0.0
+                kw += v if okl else 0.0
+        out[p] = np.float32(float(np.float32(acc)) / kw)
+    return out
+
+
+@pytest.mark.parametrize("kernel,contract", [("epanechnikov", True), ("epanechnikov", False),
+                                             ("triangle", True), ("tophat", False), ("gaussian", True)])
+def test_kde_pdf_matches_pure_python(kernel, contract):
+    from slm_amd.kde import kde_params
+    d, lx = _kde_case(kernel, n_hist=256, n_pdf=64, n=5000)
+    k = kde_params(d, contract)
+    h = oracle.kde_histogram(k, d.logx_data.reshape(-1))
+    got = oracle.kde_pdf(k, h)
+    exp = _pdf_pure_python(k, h)
+    assert np.array_equal(got, exp)
+    assert k.n_kdf_part_points_x >= 1
+
+
+def test_kde_pdf_delta_is_kernel_shape():
+    """A histogram with all mass in one pdf bin gives back the (normalised)
+    filter shape around it: symmetric, peaked, zero beyond the half-width."""
+    from slm_amd.kde import kde_params
+    d, lx = _kde_case("epanechnikov", n_hist=512, n_pdf=128, n=20000, bandwidth=3.0)
+    k = kde_params(d, True)
+    nbp = k.n_hist_bins // k.n_pdf_points
+    c = 60
+    hist = np.zeros(k.n_hist_bins, dtype=np.uint32)
+    hist[c * nbp:(c + 1) * nbp] = 100
+    pdf = oracle.kde_pdf(k, hist)
+    K = k.n_kdf_part_points_x
+    assert pdf[c] == pdf.max() and pdf[c] > 0
+    assert np.all(pdf[:c - K - 1] == 0) and np.all(pdf[c + K + 1:] == 0)
+    for j in range(1, K):
+        assert abs(float(pdf[c + j]) - float(pdf[c - j])) <= 0.05 * float(pdf[c])
+
+
+# ---------------------------------------------------------------- hillslope lengths
+
+def _hsl_case(contract=True, offset=12):
+    from slm_amd import workloads
+    from slm_amd.params import kernel_params
+    from slm_amd.useful import pick_seeds
+    job, mapping = workloads.v_valley(midslope_offset=offset)
+    seeds = pick_seeds(mask=job.mask, map=mapping, flag=int(job.info.is_midslope), pad=2)
+    job.info.n_seed_points = np.uint32(seeds.shape[0])
+    p = kernel_params(job.info, 1, "HILLSLOPE_LENGTHS", contract)
+    return job, mapping, seeds, p
+
+
+@pytest.mark.parametrize("contract", [True, False])
+def test_hsl_matches_pyref(contract):
+    job, mapping, seeds, p = _hsl_case(contract)
+    thin, mid = int(job.info.is_thinchannel), int(job.info.is_midslope)
+    p.n_seed_points = 60
+    o = oracle.hillslope_lengths(p, seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1),
+                                 mapping.reshape(-1), thin, mid)
+    r = pyref.hillslope_lengths(p, seeds, job.mask, job.uv, mapping, thin, mid)
+    assert np.array_equal(o.view(np.uint32), r.view(np.uint32))
+
+
+@pytest.mark.parametrize("offset", [8, 12, 20])
+def test_hsl_analytic_v_valley(offset):
+    """On a planar V-valley (side slope s, down-valley slope t) the streamline
+    from x = c +- offset reaches the axis after offset*sqrt(1+(t/s)^2) pixels."""
+    job, mapping, seeds, p = _hsl_case(True, offset)
+    L = oracle.hillslope_lengths(p, seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1),
+                                 mapping.reshape(-1), int(job.info.is_thinchannel), int(job.info.is_midslope))
+    exp = offset * np.sqrt(1 + (0.05 / 0.2) ** 2)
+    reached = L[L > 0]
+    assert reached.size >= 0.9 * L.size
+    assert abs(float(np.median(reached)) - exp) < 0.5
