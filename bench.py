@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is measured on a single MI355X "
@@ -85,6 +87,15 @@ def main():
     from slm_amd import hip, workloads
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n}), flush=True)
+        return
     job = getattr(workloads, args.workload)()
     seeds = torch.from_numpy(job.seeds).to(dev)
     mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)
@@ -125,10 +136,12 @@ def main():
     achieved = steps_per_launch * BYTES_PER_STEP / mean_launch_s / 1e9
     value = total_steps / wall
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_fields_tile_2048_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "k_fields_tile_2048_traffic.txt")
     if args.workload == "tile_2048" and os.path.exists(tfile):
-        with open(tfile) as fp:
-            traffic = json.load(fp).get("bytes_per_launch")
+        # measured HBM-side bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+        for line in open(tfile):
+            if line.startswith("bytes_per_launch:"):
+                traffic = float(line.split(":")[1].split()[0])
     out = {
         "metric": "streamline-steps/s",
         "value": value,

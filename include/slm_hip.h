@@ -43,9 +43,10 @@ extern "C" {
 /* Call flags */
 #define SLM_FLAG_SYNC       1u   /* hipStreamSynchronize before returning            */
 #define SLM_FLAG_NO_ZERO    2u   /* do not clear the output fields (accumulate into) */
-#define SLM_FLAG_KEEP_ORDER 4u   /* workspace already holds the seed work order built by
-                                    a previous slm_trace_fields call with the same seeds,
-                                    n_seed_points, seeds_chunk_offset and grid: reuse it */
+#define SLM_FLAG_KEEP_ORDER 4u   /* workspace already holds the seed work order and the
+                                    uv quads built by a previous slm_trace_fields call
+                                    with the same seeds, n_seed_points, seeds_chunk_offset,
+                                    grid and uv: reuse them */
 #define SLM_FLAG_FORCE_FALLBACK 8u /* test hook: reject the packed 64-bit accumulation so the
                                       32-bit fallback path runs (results are identical) */
 

@@ -222,6 +222,7 @@ __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, ui
 //   [256]  uint32 order[n]                 seed ordinals in tile order
 //   [..]   uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
 //   [..]   uint64 acc[nxy_padded]          packed (slt << 32) | slc
+//   [..]   Quad quad[nxy_padded]           interpolation corners per pixel
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 struct FieldsCtrl {
@@ -232,7 +233,7 @@ struct FieldsCtrl {
 };
 
 struct FieldsWs {
-    size_t off_order, off_cnt, off_cur, off_acc, total;
+    size_t off_order, off_cnt, off_cur, off_acc, off_quad, total;
 };
 
 static FieldsWs fields_ws(const SlmParams* p) {
@@ -243,7 +244,8 @@ static FieldsWs fields_ws(const SlmParams* p) {
     w.off_cnt = w.off_order + align256((size_t)n * 4);
     w.off_cur = w.off_cnt + align256((size_t)g.n_keys * 4);
     w.off_acc = w.off_cur + align256((size_t)g.n_keys * 4);
-    w.total = w.off_acc + align256((size_t)p->nxy_padded * 8);
+    w.off_quad = w.off_acc + align256((size_t)p->nxy_padded * 8);
+    w.total = w.off_quad + align256((size_t)p->nxy_padded * sizeof(Quad));
     return w;
 }
 
@@ -262,15 +264,21 @@ __device__ __forceinline__ bool fields_packed_ok(const FieldsCtrl* ctrl, uint32_
 // PACKED: runs go to acc with one 64-bit atomic; W and the step count are
 // recorded.  !PACKED: the fallback, which runs only when the packed result was
 // rejected and adds runs to slc/slt with two 32-bit atomics.
+// dt_fixed: the host has proven that the adaptive step can never leave DT_MAX
+// (every dxy component lies in [-1, 1), so step_error <= sqrt(8)/GRID_SCALE,
+// and GRID_SCALE*ADJUSTED_MAX_ERROR >= sqrt(8)*(1+2^-20) makes
+// fmin(DT_MAX, ADJUSTED_MAX_ERROR*DT_MAX/step_error) == DT_MAX); step_error
+// is then not computed.
 template <bool C, bool PACKED>
 __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __restrict__ seeds,
                                                 const uint32_t* __restrict__ order, uint32_t n_work,
                                                 const uint8_t* __restrict__ mask,
-                                                const float2* __restrict__ uv,
+                                                const Quad* __restrict__ quad,
                                                 unsigned long long* __restrict__ acc,
                                                 uint32_t* __restrict__ slc, uint32_t* __restrict__ slt,
                                                 unsigned long long* __restrict__ steps,
-                                                FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback) {
+                                                FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
+                                                uint32_t dt_fixed) {
     if constexpr (!PACKED) {
         if (fields_packed_ok(ctrl, force_fallback)) return;
     }
@@ -278,11 +286,14 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
     const uint32_t dd = d * d;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const float res = P.trajectory_resolution;
+    const float combo = P.combo_factor;
 
     bool active = false;
     bool exhausted = false;
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
+    Quad Q;                                        // quad of the current pixel
     uint32_t run_idx = 0, run_n = 0, run_t = 0;   // current same-pixel run
     uint64_t my_steps = 0, my_writes = 0;
 
@@ -325,6 +336,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                         const uint32_t idx = array_idx(P, x, y);
                         if (!mask[idx]) {
                             // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
+                            Q = load_quad(quad, idx);
                             run_idx = idx;
                             run_n = 1u;
                             run_t = slt_inc(0.f);
@@ -341,26 +353,42 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             continue;
         }
         if (active) {
-            StepOut o;
-            compute_step<C, true>(P, uv, dt, x, y, rng, o);
-            if (mask[o.idx]) {
+            // compute_step_vec_jittered (computestep.cl:86-106) on quads
+            const float2 u1 = jitter_unit<C>(P, interp_quad<C>(P, Q, x, y), rng);
+            const float2 d1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+            const float mx = x + d1.x, my = y + d1.y;
+            const Quad Q1 = load_quad(quad, array_idx(P, mx, my));
+            const float2 u2 = jitter_unit<C>(P, interp_quad<C>(P, Q1, mx, my), rng);
+            const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
+                                          approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
+            const float nx = x + d2.x, ny = y + d2.y;
+            const uint32_t nidx = array_idx(P, nx, ny);
+            const uint8_t mk = mask[nidx];
+            const Quad Qn = load_quad(quad, nidx);
+            if (mk) {
                 active = false;  // end pixel masked: step dropped, loop ends
                 flush();
             } else {
-                const float err = step_error<C>(P, o);
-                const float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
-                x = o.next.x;
-                y = o.next.y;
+                // runge_kutta_step_write_sl_data (rungekutta.cl:163-186)
+                const float dl = fast_length<C>(d2.x, d2.y);
+                x = nx;
+                y = ny;
+                Q = Qn;
                 const bool stuck = dl < P.integration_halt_threshold;
-                if (!stuck) dt = next_dt(P, dt, err);
+                if (!stuck && !dt_fixed) {
+                    StepOut o;
+                    o.dxy1 = d1;
+                    o.dxy2 = d2;
+                    dt = next_dt(P, dt, step_error<C>(P, o));
+                }
                 l += dl;
                 n += 1u;
                 my_steps += 1u;
                 my_writes += 1u;
                 // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced
-                if (o.idx != run_idx) {
+                if (nidx != run_idx) {
                     flush();
-                    run_idx = o.idx;
+                    run_idx = nidx;
                 }
                 run_n += 1u;
                 run_t += slt_inc(l);
@@ -381,6 +409,27 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             if (wv) atomicAdd(&ctrl->writes, (unsigned long long)wv);
         }
     }
+}
+
+// quad[NY_PADDED*x + y] from uv (layout described in trace_common.hpp)
+__global__ void k_build_quads(SlmParams P, const float2* __restrict__ uv, Quad* __restrict__ quad) {
+    const size_t n = P.nxy_padded;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)(i / P.ny_padded), y = (uint32_t)(i - (size_t)x * P.ny_padded);
+        const uint32_t xr = min(P.nx_padded - 1u, x + 1u), yu = min(P.ny_padded - 1u, y + 1u);
+        const float2 a = uv[(size_t)P.ny_padded * x + y], b = uv[(size_t)P.ny_padded * xr + y];
+        const float2 e = uv[(size_t)P.ny_padded * x + yu], f = uv[(size_t)P.ny_padded * xr + yu];
+        Quad q;
+        q.lo = make_float4(a.x, a.y, b.x, b.y);
+        q.hi = make_float4(e.x, e.y, f.x, f.y);
+        quad[i] = q;
+    }
+}
+
+// True when next_dt can only return DT_MAX (see k_fields).
+static uint32_t dt_is_fixed(const SlmParams* p) {
+    return ((double)p->grid_scale * (double)p->adjusted_max_error >= 2.8284271247461903 * (1.0 + 1.0 / 1048576.0))
+               ? 1u : 0u;
 }
 
 __global__ void k_fields_init(FieldsCtrl* ctrl) {
@@ -476,6 +525,7 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
     uint32_t* cur = (uint32_t*)(ws + w.off_cur);
     unsigned long long* acc = (unsigned long long*)(ws + w.off_acc);
+    Quad* quad = (Quad*)(ws + w.off_quad);
     const TileGeom g = tile_geom(p);
     const float2* seeds = (const float2*)seeds_xy;
     const size_t nxy = p->nxy_padded;
@@ -487,31 +537,39 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
         hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, cnt, cur, g.n_keys);
         hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255) / 256), dim3(256), 0, s, *p, seeds, n, cur, order);
     }
+    if (!(flags & SLM_FLAG_KEEP_ORDER))
+        hipLaunchKernelGGL(k_build_quads, dim3(grid_for(nxy)), dim3(256), 0, s, *p, (const float2*)uv, quad);
     hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, ctrl);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy);
 
     const uint32_t n_work = n * d * d;
-    const uint32_t blocks_per_cu = 8;  // 32 waves per CU at 256 threads per block
-    uint32_t grid = (uint32_t)di.cus * blocks_per_cu;
+    // persistent grid: as many 256-thread blocks as can be resident at once
+    int per_cu = 0;
+    if (p->fp_contract)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<true, true>, 256, 0));
+    else
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<false, true>, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint32_t grid = (uint32_t)di.cus * (uint32_t)per_cu;
     const uint32_t max_useful = (n_work + 255) / 256;
     if (grid > max_useful) grid = max_useful ? max_useful : 1;
     unsigned long long* st = (unsigned long long*)steps_out;
-    const float2* uv2 = (const float2*)uv;
+    const uint32_t fixed = dt_is_fixed(p);
     if (p->fp_contract)
         hipLaunchKernelGGL((k_fields<true, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           uv2, acc, slc, slt, st, ctrl, force);
+                           quad, acc, slc, slt, st, ctrl, force, fixed);
     else
         hipLaunchKernelGGL((k_fields<false, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           uv2, acc, slc, slt, st, ctrl, force);
+                           quad, acc, slc, slt, st, ctrl, force, fixed);
     hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy, ctrl);
     hipLaunchKernelGGL(k_fields_commit, dim3(grid_for(nxy)), dim3(256), 0, s, acc, slc, slt, nxy, ctrl,
                        accumulate ? 1u : 0u, force);
     if (p->fp_contract)
         hipLaunchKernelGGL((k_fields<true, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           uv2, acc, slc, slt, nullptr, ctrl, force);
+                           quad, acc, slc, slt, nullptr, ctrl, force, fixed);
     else
         hipLaunchKernelGGL((k_fields<false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           uv2, acc, slc, slt, nullptr, ctrl, force);
+                           quad, acc, slc, slt, nullptr, ctrl, force, fixed);
     return slm_finish(stream, flags);
 }
 

@@ -172,4 +172,53 @@ __device__ __forceinline__ float sub_offset(const SlmParams& P, uint32_t i) {
     return mad<C>((float)i, P.subpixel_seed_step, -P.subpixel_seed_halfspan);
 }
 
+
+// ------------------------------------------------------------ quad layout
+//
+// quad[NY_PADDED*x + y] holds the four uv values speed_interpolator reads for
+// a position whose pixel is (x, y): (x,y), (xr,y), (x,yu), (xr,yu) with
+// xr = min(NX_PADDED-1, x+1), yu = min(NY_PADDED-1, y+1).  The interpolation
+// pixel of a position is its get_array_idx pixel, so one aligned 32-byte load
+// replaces four scattered float2 loads, and the quad of a step's end pixel is
+// fetched together with that pixel's mask byte.
+struct alignas(32) Quad {
+    float4 lo;  // (x,y).u, (x,y).v, (xr,y).u, (xr,y).v
+    float4 hi;  // (x,yu).u, (x,yu).v, (xr,yu).u, (xr,yu).v
+};
+
+__device__ __forceinline__ Quad load_quad(const Quad* __restrict__ q, uint32_t idx) {
+    Quad r;
+    r.lo = q[idx].lo;
+    r.hi = q[idx].hi;
+    return r;
+}
+
+// speed_interpolator (essentials.cl:24-45) on a quad; same arithmetic as
+// speed_interp, so bit-identical.
+template <bool C>
+__device__ __forceinline__ float2 interp_quad(const SlmParams& P, const Quad& Q, float x, float y) {
+    const uint32_t xl = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+    const uint32_t yd = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
+    const float rx = x - (float)xl + (float)P.pad_width;
+    const float lx = 1.0f - rx;
+    const float uy = y - (float)yd + (float)P.pad_width;
+    const float dy = 1.0f - uy;
+    const float dwn0 = mad<C>(Q.lo.x, lx, Q.lo.z * rx), dwn1 = mad<C>(Q.lo.y, lx, Q.lo.w * rx);
+    const float upp0 = mad<C>(Q.hi.x, lx, Q.hi.z * rx), upp1 = mad<C>(Q.hi.y, lx, Q.hi.w * rx);
+    const float s0 = mad<C>(dwn0, dy, upp0 * uy), s1 = mad<C>(dwn1, dy, upp1 * uy);
+    const float inv = fdiv(1.0f, fsqrt(dot2<C>(s0, s1)));
+    return make_float2(s0 * inv, s1 * inv);
+}
+
+// Jitter of compute_step_vec_jittered (computestep.cl:96-98): add the scaled
+// variate, renormalise.
+template <bool C>
+__device__ __forceinline__ float2 jitter_unit(const SlmParams& P, float2 u, uint32_t& rng) {
+    const float2 r = rand_vec(rng);
+    u.x = mad<C>(r.x, P.jitter_magnitude, u.x);
+    u.y = mad<C>(r.y, P.jitter_magnitude, u.y);
+    const float len = fast_length<C>(u.x, u.y);
+    return make_float2(fdiv(u.x, len), fdiv(u.y, len));
+}
+
 }  // namespace slm

@@ -32,7 +32,9 @@ def _gpu_fields(p, seeds, mask, uv, dev, force_fallback=False):
 
 
 CASES = [
-    # (grid side, d, jitter, fp_contract)
+    # (grid side, d, jitter, fp_contract[, max_integration_step_error])
+    (96, 2, 1.0, True, 1e-6),
+    (128, 1, 0.0, False, 1e-5),
     (64, 1, 0.0, True),
     (64, 2, 1.0, True),
     (96, 3, 1.0, False),
@@ -44,10 +46,11 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("n,d,jitter,contract", CASES)
-def test_fields_bit_exact(gpu, n, d, jitter, contract):
+@pytest.mark.parametrize("case", CASES)
+def test_fields_bit_exact(gpu, case):
     from slm_amd import workloads
-    job = workloads.small_256(jitter=jitter, n=n, d=d)
+    n, d, jitter, contract = case[:4]
+    job = workloads.small_256(jitter=jitter, n=n, d=d, max_error=case[4] if len(case) > 4 else None)
     seeds, mask, uv = _dev(job, gpu)
     m = job.mask.astype(np.uint8).reshape(-1)
     for sign in (1, -1):

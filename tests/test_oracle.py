@@ -64,24 +64,27 @@ def test_rng_variates_in_range():
     assert abs(float(v.mean())) < 0.01
 
 
-def _tiny_job(n, d, jitter, seed=3):
+def _tiny_job(n, d, jitter, seed=3, max_error=None):
     from slm_amd import workloads
-    return workloads.small_256(jitter=jitter, n=n, d=d, seed=seed)
+    return workloads.small_256(jitter=jitter, n=n, d=d, seed=seed, max_error=max_error)
 
 
 PYREF_CASES = [
-    # (grid, d, jitter, contract, sign, n_seeds)
-    (24, 1, 1.0, True, 1, 40),
-    (24, 1, 0.0, True, -1, 40),
-    (24, 2, 1.0, False, 1, 20),
-    (28, 3, 1.0, True, -1, 10),
-    (28, 3, 0.0, False, 1, 10),
+    # (grid, d, jitter, contract, sign, n_seeds, max_integration_step_error)
+    (24, 1, 1.0, True, 1, 40, None),
+    (24, 1, 0.0, True, -1, 40, None),
+    (24, 2, 1.0, False, 1, 20, None),
+    (28, 3, 1.0, True, -1, 10, None),
+    (28, 3, 0.0, False, 1, 10, None),
+    # adaptive step actually adapting (dt leaves DT_MAX)
+    (32, 1, 1.0, True, 1, 30, 1e-6),
+    (32, 2, 0.0, False, -1, 15, 1e-5),
 ]
 
 
-@pytest.mark.parametrize("grid,d,jitter,contract,sign,ns", PYREF_CASES)
-def test_fields_match_pyref(grid, d, jitter, contract, sign, ns):
-    job = _tiny_job(grid, d, jitter)
+@pytest.mark.parametrize("grid,d,jitter,contract,sign,ns,max_error", PYREF_CASES)
+def test_fields_match_pyref(grid, d, jitter, contract, sign, ns, max_error):
+    job = _tiny_job(grid, d, jitter, max_error=max_error)
     ns = min(ns, job.seeds.shape[0])
     p = job.params(sign, fp_contract=contract, n_seed_points=ns)
     p.max_n_steps = min(p.max_n_steps, 300)
@@ -95,9 +98,10 @@ def test_fields_match_pyref(grid, d, jitter, contract, sign, ns):
     assert int(oslc.sum()) == ns * d * d + osteps
 
 
-@pytest.mark.parametrize("grid,contract,sign", [(24, True, 1), (24, False, -1), (28, True, -1)])
-def test_paths_match_pyref(grid, contract, sign):
-    job = _tiny_job(grid, 1, 0.0)
+@pytest.mark.parametrize("grid,contract,sign,max_error", [(24, True, 1, None), (24, False, -1, None),
+                                                          (28, True, -1, None), (32, True, 1, 1e-6)])
+def test_paths_match_pyref(grid, contract, sign, max_error):
+    job = _tiny_job(grid, 1, 0.0, max_error=max_error)
     ns = min(30, job.seeds.shape[0])
     p = job.params(sign, kernel="INTEGRATE_TRAJECTORY", fp_contract=contract, n_seed_points=ns)
     p.max_n_steps = min(p.max_n_steps, 400)
@@ -262,6 +266,17 @@ def _hsl_case(contract=True, offset=12):
     job.info.n_seed_points = np.uint32(seeds.shape[0])
     p = kernel_params(job.info, 1, "HILLSLOPE_LENGTHS", contract)
     return job, mapping, seeds, p
+
+
+def test_dt_adapts_with_small_error():
+    """With a tiny step-error bound the step size really adapts: steps are
+    shorter and more numerous than with the default bound."""
+    a = _tiny_job(32, 1, 0.0)
+    b = _tiny_job(32, 1, 0.0, max_error=1e-6)
+    m = a.mask.astype(np.uint8).reshape(-1)
+    sa = oracle.trace_fields(a.params(1), a.seeds, m, a.uv.reshape(-1))[2]
+    sb = oracle.trace_fields(b.params(1), b.seeds, m, b.uv.reshape(-1))[2]
+    assert sb > 1.5 * sa
 
 
 @pytest.mark.parametrize("contract", [True, False])
