@@ -122,6 +122,23 @@ int slm_trace_fields(const SlmParams* p, const float* seeds_xy,
                      void* stream, uint32_t flags);
 
 /*
+ * Same result as slm_trace_fields (bit for bit), computed by LDS-owned tiles:
+ * the grid is cut into tile x tile pixel tiles (8 <= tile <= 128); each
+ * 256-thread block steps the streamlines currently in one tile with that
+ * tile's uv, mask and a 64-bit accumulator in LDS, and hands streamlines that
+ * leave the tile (or exceed a per-visit step cap) to the owning tile for the
+ * next round.  Rounds are driven from the host: this call synchronises the
+ * stream once per round (so it cannot be captured in a graph).  Flags as for
+ * slm_trace_fields except SLM_FLAG_KEEP_ORDER, which is ignored.
+ */
+size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, uint32_t tile);
+int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const float* seeds_xy,
+                           const uint8_t* mask, const float* uv,
+                           uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                           void* workspace, size_t workspace_bytes,
+                           void* stream, uint32_t flags);
+
+/*
  * Un-jittered trajectory recording, one direction.
  * Replaces kernel integrate_trajectory (opencl/integratetraj.cl:54-97 with
  * trajectory.cl:41-80) as launched by Trajectories.gpu_compute_trajectories

@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 if (!active) {
                     const uint32_t w = base + __popcll(need & lt_mask);
                     if (w < n_work) {
-                        const uint32_t k = order[w / dd];
+                        const uint32_t k = order ? order[w / dd] : w / dd;
                         const uint32_t sub = w - (w / dd) * dd;
                         const uint32_t seed_idx = P.seeds_chunk_offset + k;
                         const float2 s = seeds[seed_idx];
@@ -776,5 +776,575 @@ extern "C" int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, 
     else
         hipLaunchKernelGGL(k_hsl<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
                            (const float2*)uv, mapping, thin_flag, from_flag, length);
+    return slm_finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ tiled fields tracer
+//
+// LDS-owned tiles.  The padded grid is cut into T x T pixel tiles.  A unit of
+// work is (tile, up to U streamline states queued in that tile); one 256-thread
+// block processes a unit with the tile's region in LDS:
+//   region   = tile + H-pixel hysteresis band + M-pixel access margin, side R
+//   uv_l     = uv of the region plus one row/column (interpolation corners)
+//   mask_l   = mask of the region          acc_l = packed (slt<<32)|slc sums
+// A streamline steps while its pixel stays inside tile+band (a step moves a
+// position by less than one pixel, M = 2 covers the mid and end pixels) and
+// at most cap_steps times per visit; then its state is emitted to the queue
+// of the tile that owns its pixel.  Rounds repeat: bin emitted states by tile
+// (LDS-histogram counting sort), cut bins into units, process, until no state
+// is left.  Each unit ends by adding its non-zero acc_l entries to the global
+// acc with one 64-bit atomic per pixel.  All sums are integers, so the
+// result is the reference's per-write atomics exactly; the packed-sum check
+// and 32-bit fallback of the queue tracer apply unchanged.
+
+struct TState {  // 32 bytes
+    float x, y, l, dt;
+    uint32_t n, rng, tile, fresh;
+};
+
+struct TileCfg {
+    uint32_t T, H, M, R, RU;   // R = T + 2H + 2M; RU = R + 1 (uv side)
+    uint32_t ntx, nty, n_tiles;
+    uint32_t unit, cap_steps;
+};
+
+struct TUnit {
+    uint32_t tile, begin, end, pad;
+};
+
+struct TileCtrl {
+    FieldsCtrl f;               // queue (unused here), W, sum of low halves
+    uint32_t n_out;             // states emitted by the current round
+    uint32_t unit_head;         // next unit to take
+    uint32_t n_units;           // units of the current round
+    uint32_t pad;
+};
+
+static TileCfg tile_cfg(const SlmParams* p, uint32_t T) {
+    TileCfg c;
+    c.T = T;
+    c.H = 4;
+    c.M = 2;
+    c.R = c.T + 2 * c.H + 2 * c.M;
+    c.RU = c.R + 1;
+    c.ntx = (p->nx_padded + T - 1) / T;
+    c.nty = (p->ny_padded + T - 1) / T;
+    c.n_tiles = c.ntx * c.nty;
+    c.unit = 2048;
+    c.cap_steps = 4 * T;
+    return c;
+}
+
+static size_t tile_lds_bytes(const TileCfg& c) {
+    return (size_t)c.RU * c.RU * 8 + (size_t)c.R * c.R * 8 + ((size_t)c.R * c.R + 15) / 16 * 16;
+}
+
+__device__ __forceinline__ uint32_t tile_of(const TileCfg& c, uint32_t px, uint32_t py) {
+    return (px / c.T) * c.nty + (py / c.T);
+}
+
+// fresh states: work item w = seed ordinal w/d^2, sub-pixel start w%d^2
+template <bool C>
+__global__ void k_tl_fresh(SlmParams P, TileCfg c, const float2* __restrict__ seeds, uint32_t n_work,
+                           TState* __restrict__ out) {
+    const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
+    const uint32_t dd = d * d;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n_work; w += gridDim.x * blockDim.x) {
+        const uint32_t k = w / dd, sub = w - k * dd;
+        const uint32_t seed_idx = P.seeds_chunk_offset + k;
+        const float2 s = seeds[seed_idx];
+        TState t;
+        t.x = s.x + sub_offset<C>(P, sub % d);
+        t.y = s.y + sub_offset<C>(P, sub / d);
+        t.l = 0.f;
+        t.dt = P.dt_max;
+        t.n = 0u;
+        t.rng = rng_seed(seed_idx);
+        t.tile = tile_of(c, pix(t.x, P.pad_width_pp5, P.nx_padded - 1u), pix(t.y, P.pad_width_pp5, P.ny_padded - 1u));
+        t.fresh = 1u;
+        out[w] = t;
+    }
+}
+
+// Per-tile counts of n states; LDS histogram when the tiles fit (lds_bins).
+__global__ __launch_bounds__(256) void k_tl_count(const TState* __restrict__ st, uint32_t n, uint32_t n_tiles,
+                                                  uint32_t* __restrict__ cnt, uint32_t use_lds) {
+    extern __shared__ uint32_t h[];
+    if (use_lds) {
+        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x) h[i] = 0u;
+        __syncthreads();
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t t = st[i].tile;
+        if (use_lds) atomicAdd(&h[t], 1u);
+        else atomicAdd(&cnt[t], 1u);
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x)
+            if (h[i]) atomicAdd(&cnt[i], h[i]);
+    }
+}
+
+// Exclusive scan of tile counts -> off (and cursor copy), units per tile ->
+// unit offsets; total units into ctrl.  One 1024-thread block.
+__global__ __launch_bounds__(1024) void k_tl_scan(const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off,
+                                                  uint32_t* __restrict__ cur, uint32_t* __restrict__ uoff,
+                                                  uint32_t n_tiles, uint32_t unit, TileCtrl* __restrict__ ctrl) {
+    __shared__ uint32_t ps[1024], pu[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n_tiles + 1023u) / 1024u;
+    const uint32_t b = t * per, e = min(n_tiles, b + per);
+    uint32_t s = 0, u = 0;
+    for (uint32_t i = b; i < e; i++) {
+        s += cnt[i];
+        u += (cnt[i] + unit - 1u) / unit;
+    }
+    ps[t] = s;
+    pu[t] = u;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const uint32_t vs = (t >= o) ? ps[t - o] : 0u, vu = (t >= o) ? pu[t - o] : 0u;
+        __syncthreads();
+        ps[t] += vs;
+        pu[t] += vu;
+        __syncthreads();
+    }
+    uint32_t rs = t ? ps[t - 1] : 0u, ru = t ? pu[t - 1] : 0u;
+    for (uint32_t i = b; i < e; i++) {
+        off[i] = rs;
+        cur[i] = rs;
+        uoff[i] = ru;
+        rs += cnt[i];
+        ru += (cnt[i] + unit - 1u) / unit;
+    }
+    if (t == 1023) {
+        ctrl->n_units = pu[1023];
+        ctrl->unit_head = 0u;
+        ctrl->n_out = 0u;
+    }
+}
+
+// Scatter states into tile order: per block, an LDS histogram reserves one
+// global range per tile, then each state takes a slot in its tile's range.
+__global__ __launch_bounds__(256) void k_tl_scatter(const TState* __restrict__ st, uint32_t n, uint32_t n_tiles,
+                                                    uint32_t* __restrict__ cur, TState* __restrict__ out,
+                                                    uint32_t use_lds) {
+    extern __shared__ uint32_t h[];  // [0, n_tiles): counts then bases; [n_tiles, 2n_tiles): ranks
+    uint32_t* rank = h + n_tiles;
+    const uint32_t per_block = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t b = blockIdx.x * per_block, e = min(n, b + per_block);
+    if (use_lds) {
+        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x) {
+            h[i] = 0u;
+            rank[i] = 0u;
+        }
+        __syncthreads();
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) atomicAdd(&h[st[i].tile], 1u);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n_tiles; i += blockDim.x)
+            if (h[i]) h[i] = atomicAdd(&cur[i], h[i]);
+        __syncthreads();
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const TState s = st[i];
+            out[h[s.tile] + atomicAdd(&rank[s.tile], 1u)] = s;
+        }
+    } else {
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            const TState s = st[i];
+            out[atomicAdd(&cur[s.tile], 1u)] = s;
+        }
+    }
+}
+
+__global__ void k_tl_units(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off,
+                           const uint32_t* __restrict__ uoff, uint32_t n_tiles, uint32_t unit,
+                           TUnit* __restrict__ units) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_tiles; t += gridDim.x * blockDim.x) {
+        const uint32_t c = cnt[t];
+        uint32_t u = uoff[t];
+        for (uint32_t b = 0; b < c; b += unit, u++) {
+            TUnit x;
+            x.tile = t;
+            x.begin = off[t] + b;
+            x.end = off[t] + min(c, b + unit);
+            x.pad = 0u;
+            units[u] = x;
+        }
+    }
+}
+
+// Process units: see the section comment.  in: binned states; out: emitted.
+template <bool C>
+__global__ __launch_bounds__(256) void k_tl_process(SlmParams P, TileCfg c, const TState* __restrict__ in,
+                                                    TState* __restrict__ out, const TUnit* __restrict__ units,
+                                                    const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                                    unsigned long long* __restrict__ acc,
+                                                    unsigned long long* __restrict__ steps,
+                                                    TileCtrl* __restrict__ ctrl, uint32_t dt_fixed,
+                                                    uint32_t out_cap) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    float2* uv_l = (float2*)lds;                                          // RU*RU
+    unsigned long long* acc_l = (unsigned long long*)(lds + (size_t)c.RU * c.RU * 8);   // R*R
+    uint8_t* mask_l = lds + (size_t)c.RU * c.RU * 8 + (size_t)c.R * c.R * 8;            // R*R
+    __shared__ uint32_t s_unit, s_next;
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    const float res = P.trajectory_resolution;
+    const float combo = P.combo_factor;
+    const uint32_t R = c.R, RU = c.RU;
+    uint64_t my_steps = 0, my_writes = 0;
+
+    while (true) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(&ctrl->unit_head, 1u);
+        __syncthreads();
+        const uint32_t u = s_unit;
+        if (u >= ctrl->n_units) break;
+        const TUnit U = units[u];
+        const int32_t ox = (int32_t)((U.tile / c.nty) * c.T) - (int32_t)(c.H + c.M);
+        const int32_t oy = (int32_t)((U.tile % c.nty) * c.T) - (int32_t)(c.H + c.M);
+        const int32_t sx0 = ox + (int32_t)c.M, sy0 = oy + (int32_t)c.M;   // stay band origin
+        const int32_t sw = (int32_t)(c.T + 2 * c.H);                         // stay band side
+        for (uint32_t i = threadIdx.x; i < RU * RU; i += blockDim.x) {
+            const int32_t gx = min(max(ox + (int32_t)(i / RU), 0), (int32_t)P.nx_padded - 1);
+            const int32_t gy = min(max(oy + (int32_t)(i % RU), 0), (int32_t)P.ny_padded - 1);
+            uv_l[i] = uv[(size_t)gx * P.ny_padded + gy];
+        }
+        for (uint32_t i = threadIdx.x; i < R * R; i += blockDim.x) {
+            const int32_t gx = min(max(ox + (int32_t)(i / R), 0), (int32_t)P.nx_padded - 1);
+            const int32_t gy = min(max(oy + (int32_t)(i % R), 0), (int32_t)P.ny_padded - 1);
+            mask_l[i] = mask[(size_t)gx * P.ny_padded + gy];
+            acc_l[i] = 0ull;
+        }
+        if (threadIdx.x == 0) s_next = U.begin;
+        __syncthreads();
+
+        // local index of a global pixel, or 0xffffffff outside the mask/acc region
+        auto loc = [&](uint32_t px, uint32_t py) -> uint32_t {
+            const uint32_t li = (uint32_t)((int32_t)px - ox), lj = (uint32_t)((int32_t)py - oy);
+            return (li < R && lj < R) ? li * R + lj : 0xffffffffu;
+        };
+        // speed_interpolator from the region (corners read at +1 row/column of uv_l)
+        auto interp = [&](float x, float y) -> float2 {
+            const uint32_t xl = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+            const uint32_t yd = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
+            const uint32_t li = (uint32_t)((int32_t)xl - ox), lj = (uint32_t)((int32_t)yd - oy);
+            Quad Q;
+            if (li < R && lj < R) {
+                const float2 a = uv_l[li * RU + lj], b = uv_l[(li + 1) * RU + lj];
+                const float2 e = uv_l[li * RU + lj + 1], f = uv_l[(li + 1) * RU + lj + 1];
+                Q.lo = make_float4(a.x, a.y, b.x, b.y);
+                Q.hi = make_float4(e.x, e.y, f.x, f.y);
+            } else {  // outside the region: read HBM (never taken: steps move < 1 pixel)
+                const uint32_t xr = min(P.nx_padded - 1u, xl + 1u), yu = min(P.ny_padded - 1u, yd + 1u);
+                const float2 a = uv[(size_t)P.ny_padded * xl + yd], b = uv[(size_t)P.ny_padded * xr + yd];
+                const float2 e = uv[(size_t)P.ny_padded * xl + yu], f = uv[(size_t)P.ny_padded * xr + yu];
+                Q.lo = make_float4(a.x, a.y, b.x, b.y);
+                Q.hi = make_float4(e.x, e.y, f.x, f.y);
+            }
+            return interp_quad<C>(P, Q, x, y);
+        };
+
+        bool active = false, done = false;
+        TState S;
+        uint32_t run_li = 0, run_n = 0, run_t = 0, visit = 0;
+        auto flush = [&]() {
+            if (run_n) {
+                atomicAdd(&acc_l[run_li], ((unsigned long long)run_t << 32) | (unsigned long long)run_n);
+                run_n = 0;
+                run_t = 0;
+            }
+        };
+        while (true) {
+            if (!done) {
+                const uint64_t need = __ballot(!active);
+                if (need) {
+                    const uint32_t cnt = __popcll(need);
+                    const uint32_t leader = __ffsll((unsigned long long)need) - 1u;
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&s_next, cnt);
+                    base = __shfl(base, leader);
+                    if (base + cnt >= U.end) done = true;
+                    if (!active) {
+                        const uint32_t w = base + __popcll(need & lt_mask);
+                        if (w < U.end) {
+                            S = in[w];
+                            visit = 0;
+                            active = true;
+                            if (S.fresh) {
+                                // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
+                                S.fresh = 0u;
+                                const uint32_t px = pix(S.x, P.pad_width_pp5, P.nx_padded - 1u);
+                                const uint32_t py = pix(S.y, P.pad_width_pp5, P.ny_padded - 1u);
+                                const uint32_t li = loc(px, py);
+                                const uint8_t mk = (li != 0xffffffffu) ? mask_l[li] : mask[(size_t)px * P.ny_padded + py];
+                                if (mk) {
+                                    active = false;
+                                } else {
+                                    my_writes += 1u;
+                                    if (li != 0xffffffffu) {
+                                        run_li = li;
+                                        run_n = 1u;
+                                        run_t = slt_inc(0.f);
+                                    } else {
+                                        atomicAdd(&acc[(size_t)px * P.ny_padded + py], 1ull);
+                                    }
+                                    active = (0.f < P.max_length) && (0u < P.max_n_steps);
+                                    if (!active) flush();
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (!__any(active)) {
+                if (done) break;
+                continue;
+            }
+            if (active) {
+                // compute_step_vec_jittered (computestep.cl:86-106) from LDS
+                const float2 u1 = jitter_unit<C>(P, interp(S.x, S.y), S.rng);
+                const float2 d1 = make_float2(approx(u1.x * combo * S.dt, res), approx(u1.y * combo * S.dt, res));
+                const float2 u2 = jitter_unit<C>(P, interp(S.x + d1.x, S.y + d1.y), S.rng);
+                const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, S.dt, d1.x), res),
+                                              approx(0.5f * mad<C>(u2.y * combo, S.dt, d1.y), res));
+                const float nx = S.x + d2.x, ny = S.y + d2.y;
+                const uint32_t npx = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
+                const uint32_t npy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
+                const uint32_t nli = loc(npx, npy);
+                const size_t ngi = (size_t)npx * P.ny_padded + npy;
+                const uint8_t mk = (nli != 0xffffffffu) ? mask_l[nli] : mask[ngi];
+                if (mk) {
+                    active = false;  // end pixel masked: step dropped, loop ends
+                    flush();
+                } else {
+                    // runge_kutta_step_write_sl_data (rungekutta.cl:163-186)
+                    const float dl = fast_length<C>(d2.x, d2.y);
+                    S.x = nx;
+                    S.y = ny;
+                    const bool stuck = dl < P.integration_halt_threshold;
+                    if (!stuck && !dt_fixed) {
+                        StepOut o;
+                        o.dxy1 = d1;
+                        o.dxy2 = d2;
+                        S.dt = next_dt(P, S.dt, step_error<C>(P, o));
+                    }
+                    S.l += dl;
+                    S.n += 1u;
+                    my_steps += 1u;
+                    my_writes += 1u;
+                    if (nli != 0xffffffffu) {
+                        if (nli != run_li || run_n == 0u) {
+                            flush();
+                            run_li = nli;
+                        }
+                        run_n += 1u;
+                        run_t += slt_inc(S.l);
+                    } else {
+                        flush();
+                        atomicAdd(&acc[ngi], ((unsigned long long)slt_inc(S.l) << 32) | 1ull);
+                    }
+                    active = !stuck && (S.l < P.max_length) && (S.n < P.max_n_steps);
+                    visit += 1u;
+                    if (!active) {
+                        flush();
+                    } else {
+                        const bool inside = (uint32_t)((int32_t)npx - sx0) < (uint32_t)sw &&
+                                            (uint32_t)((int32_t)npy - sy0) < (uint32_t)sw;
+                        if (!inside || visit >= c.cap_steps) {
+                            flush();
+                            S.tile = tile_of(c, npx, npy);
+                            // wave-aggregated append to the emitted list
+                            const uint64_t em = __ballot(true);
+                            const uint32_t lead = __ffsll((unsigned long long)em) - 1u;
+                            uint32_t ob = 0;
+                            if (lane == lead) ob = atomicAdd(&ctrl->n_out, (uint32_t)__popcll(em));
+                            ob = __shfl(ob, lead);
+                            const uint32_t slot = ob + __popcll(em & lt_mask);
+                            if (slot < out_cap) out[slot] = S;   // overflow is reported by the host
+                            active = false;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // add this unit's sums to the grid (one 64-bit atomic per touched pixel)
+        for (uint32_t i = threadIdx.x; i < R * R; i += blockDim.x) {
+            const unsigned long long v = acc_l[i];
+            const int32_t gx = ox + (int32_t)(i / R), gy = oy + (int32_t)(i % R);
+            if (v && gx >= 0 && gy >= 0 && gx < (int32_t)P.nx_padded && gy < (int32_t)P.ny_padded)
+                atomicAdd(&acc[(size_t)gx * P.ny_padded + gy], v);
+        }
+        __syncthreads();
+    }
+    uint64_t v = my_steps, wv = my_writes;
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off);
+        wv += __shfl_down(wv, off);
+    }
+    if (lane == 0) {
+        if (steps && v) atomicAdd(steps, (unsigned long long)v);
+        if (wv) atomicAdd(&ctrl->f.writes, (unsigned long long)wv);
+    }
+}
+
+struct TiledWs {
+    size_t off_acc, off_quad, off_a, off_b, off_cnt, off_off, off_cur, off_uoff, off_units, total;
+    uint32_t max_units;
+};
+
+static TiledWs tiled_ws(const SlmParams* p, const TileCfg& c) {
+    TiledWs w;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+    const size_t n_work = (size_t)n * d * d;
+    w.max_units = (uint32_t)((n_work + c.unit - 1) / c.unit + c.n_tiles);
+    w.off_acc = 256;
+    w.off_quad = w.off_acc + align256((size_t)p->nxy_padded * 8);
+    w.off_a = w.off_quad + align256((size_t)p->nxy_padded * sizeof(Quad));
+    w.off_b = w.off_a + align256(n_work * sizeof(TState));
+    w.off_cnt = w.off_b + align256(n_work * sizeof(TState));
+    w.off_off = w.off_cnt + align256((size_t)c.n_tiles * 4);
+    w.off_cur = w.off_off + align256((size_t)c.n_tiles * 4);
+    w.off_uoff = w.off_cur + align256((size_t)c.n_tiles * 4);
+    w.off_units = w.off_uoff + align256((size_t)c.n_tiles * 4);
+    w.total = w.off_units + align256((size_t)w.max_units * sizeof(TUnit));
+    return w;
+}
+
+extern "C" size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, uint32_t tile) {
+    if (check_params(p) != SLM_OK) return 0;
+    if (tile < 8 || tile > 128) return 0;
+    return tiled_ws(p, tile_cfg(p, tile)).total;
+}
+
+extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const float* seeds_xy,
+                                      const uint8_t* mask, const float* uv, uint32_t* slc, uint32_t* slt,
+                                      uint64_t* steps_out, void* workspace, size_t workspace_bytes,
+                                      void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (tile < 8 || tile > 128) return slm_fail(SLM_E_INVALID_ARG, "tile side %u outside [8, 128]", tile);
+    if (!mask || !uv || !slc || !slt) return slm_fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+    const bool accumulate = (flags & SLM_FLAG_NO_ZERO) != 0;
+    if (n == 0) {
+        if (!accumulate) {
+            fill_u32(slc, p->nxy_padded, 0u, s);
+            fill_u32(slt, p->nxy_padded, 0u, s);
+        }
+        return slm_finish(stream, flags);
+    }
+    if (!seeds_xy) return slm_fail(SLM_E_INVALID_ARG, "NULL seeds");
+    const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+    if ((uint64_t)n * d * d >= (1ull << 31))
+        return slm_fail(SLM_E_INVALID_ARG, "n_seeds * d^2 = %llu exceeds 2^31", (unsigned long long)n * d * d);
+    const TileCfg c = tile_cfg(p, tile);
+    const TiledWs w = tiled_ws(p, c);
+    if (!workspace || workspace_bytes < w.total)
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
+    DevInfo di;
+    rc = device_info(di);
+    if (rc) return rc;
+    const size_t lds = tile_lds_bytes(c);
+    if (lds > 64 * 1024) return slm_fail(SLM_E_INVALID_ARG, "tile %u needs %zu bytes of LDS", tile, lds);
+
+    char* ws = (char*)workspace;
+    TileCtrl* ctrl = (TileCtrl*)ws;
+    unsigned long long* acc = (unsigned long long*)(ws + w.off_acc);
+    Quad* quad = (Quad*)(ws + w.off_quad);
+    TState* A = (TState*)(ws + w.off_a);
+    TState* B = (TState*)(ws + w.off_b);
+    uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
+    uint32_t* off = (uint32_t*)(ws + w.off_off);
+    uint32_t* cur = (uint32_t*)(ws + w.off_cur);
+    uint32_t* uoff = (uint32_t*)(ws + w.off_uoff);
+    TUnit* units = (TUnit*)(ws + w.off_units);
+    const size_t nxy = p->nxy_padded;
+    const uint32_t n_work = n * d * d;
+    const uint32_t force = (flags & SLM_FLAG_FORCE_FALLBACK) ? 1u : 0u;
+    const uint32_t fixed = dt_is_fixed(p);
+    const uint32_t use_lds_hist = (c.n_tiles <= 8192u) ? 1u : 0u;
+    const size_t hist_lds = use_lds_hist ? (size_t)c.n_tiles * 4 : 0;
+
+    hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, &ctrl->f);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy);
+    hipLaunchKernelGGL(k_build_quads, dim3(grid_for(nxy)), dim3(256), 0, s, *p, (const float2*)uv, quad);
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_tl_fresh<true>, dim3(grid_for(n_work)), dim3(256), 0, s, *p, c,
+                           (const float2*)seeds_xy, n_work, A);
+    else
+        hipLaunchKernelGGL(k_tl_fresh<false>, dim3(grid_for(n_work)), dim3(256), 0, s, *p, c,
+                           (const float2*)seeds_xy, n_work, A);
+
+    int per_cu = 0;
+    if (p->fp_contract)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tl_process<true>, 256, lds));
+    else
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tl_process<false>, 256, lds));
+    if (per_cu < 1) per_cu = 1;
+    const uint32_t pgrid = (uint32_t)di.cus * (uint32_t)per_cu;
+
+    uint32_t* h_nout = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&h_nout, sizeof(uint32_t), hipHostMallocDefault));
+    uint32_t n_in = n_work;
+    int rounds = 0;
+    hipError_t herr = hipSuccess;
+    // every visit accepts at least one step or ends the streamline, so the
+    // number of rounds is bounded by max_n_steps + 1
+    const int max_rounds = (int)p->max_n_steps + 2;
+    while (n_in > 0 && rounds < max_rounds) {
+        const uint32_t g = grid_for(n_in) < 1024u ? grid_for(n_in) : 1024u;
+        hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(c.n_tiles)), dim3(256), 0, s, cnt, (size_t)c.n_tiles, 0u);
+        hipLaunchKernelGGL(k_tl_count, dim3(g), dim3(256), hist_lds, s, A, n_in, c.n_tiles, cnt, use_lds_hist);
+        hipLaunchKernelGGL(k_tl_scan, dim3(1), dim3(1024), 0, s, cnt, off, cur, uoff, c.n_tiles, c.unit, ctrl);
+        hipLaunchKernelGGL(k_tl_scatter, dim3(g), dim3(256), 2 * hist_lds, s, A, n_in, c.n_tiles, cur, B,
+                           use_lds_hist);
+        hipLaunchKernelGGL(k_tl_units, dim3(grid_for(c.n_tiles)), dim3(256), 0, s, cnt, off, uoff, c.n_tiles,
+                           c.unit, units);
+        if (p->fp_contract)
+            hipLaunchKernelGGL(k_tl_process<true>, dim3(pgrid), dim3(256), lds, s, *p, c, B, A, units, mask,
+                               (const float2*)uv, acc, (unsigned long long*)steps_out, ctrl, fixed, n_work);
+        else
+            hipLaunchKernelGGL(k_tl_process<false>, dim3(pgrid), dim3(256), lds, s, *p, c, B, A, units, mask,
+                               (const float2*)uv, acc, (unsigned long long*)steps_out, ctrl, fixed, n_work);
+        herr = hipMemcpyAsync(h_nout, &ctrl->n_out, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+        if (herr == hipSuccess) herr = hipStreamSynchronize(s);
+        if (herr != hipSuccess) break;
+        n_in = *h_nout;
+        if (n_in > n_work) {
+            (void)hipHostFree(h_nout);
+            return slm_fail(SLM_E_HIP, "tiled round %d emitted %u states, more than the %u streamlines", rounds,
+                            n_in, n_work);
+        }
+        rounds++;
+    }
+    (void)hipHostFree(h_nout);
+    if (herr != hipSuccess) return slm_fail(SLM_E_HIP, "tiled round %d failed: %s", rounds, hipGetErrorString(herr));
+    if (n_in > 0) return slm_fail(SLM_E_HIP, "tiled tracer did not finish in %d rounds", rounds);
+
+    hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy, &ctrl->f);
+    hipLaunchKernelGGL(k_fields_commit, dim3(grid_for(nxy)), dim3(256), 0, s, acc, slc, slt, nxy, &ctrl->f,
+                       accumulate ? 1u : 0u, force);
+    // fallback: the queue tracer with 32-bit atomics, identity order; exits at once when committed
+    int fb_cu = 0;
+    if (p->fp_contract)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<true, false>, 256, 0));
+    else
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<false, false>, 256, 0));
+    if (fb_cu < 1) fb_cu = 1;
+    uint32_t fgrid = (uint32_t)di.cus * (uint32_t)fb_cu;
+    const uint32_t max_useful = (n_work + 255) / 256;
+    if (fgrid > max_useful) fgrid = max_useful ? max_useful : 1;
+    const float2* seeds = (const float2*)seeds_xy;
+    if (p->fp_contract)
+        hipLaunchKernelGGL((k_fields<true, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work, mask,
+                           quad, acc, slc, slt, nullptr, &ctrl->f, force, fixed);
+    else
+        hipLaunchKernelGGL((k_fields<false, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work,
+                           mask, quad, acc, slc, slt, nullptr, &ctrl->f, force, fixed);
     return slm_finish(stream, flags);
 }

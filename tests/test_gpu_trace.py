@@ -209,3 +209,51 @@ def test_hsl_bit_exact(gpu, contract, offset):
     hip.hillslope_lengths(p, sd, m, uv, mp, thin, mid, L, sync=True)
     assert np.array_equal(L.cpu().numpy().view(np.uint32), o.view(np.uint32))
     assert (o == -1.0).sum() < 0.1 * o.size   # most reach the channel; the rest keep their value
+
+
+TILED_CASES = [
+    # (grid side, d, jitter, fp_contract, tile, max_error)
+    (256, 2, 1.0, True, 32, None),
+    (256, 1, 0.0, False, 16, None),
+    (200, 3, 1.0, True, 64, None),
+    (96, 2, 1.0, True, 8, 1e-6),
+    (128, 1, 1.0, False, 24, 1e-5),
+]
+
+
+@pytest.mark.parametrize("n,d,jitter,contract,tile,max_error", TILED_CASES)
+def test_fields_tiled_bit_exact(gpu, n, d, jitter, contract, tile, max_error):
+    """The LDS-tiled tracer gives the oracle's fields, both directions."""
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=jitter, n=n, d=d, max_error=max_error)
+    seeds, mask, uv = _dev(job, gpu)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    for sign in (1, -1):
+        p = job.params(sign, fp_contract=contract)
+        print("tiled n=%d d=%d jitter=%g tile=%d sign=%+d" % (n, d, jitter, tile, sign), flush=True)
+        slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=gpu)
+        slt = torch.empty(p.nxy_padded, dtype=torch.int32, device=gpu)
+        steps = torch.zeros(1, dtype=torch.int64, device=gpu)
+        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, steps, tile=tile, sync=True)
+        oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1))
+        assert int(steps.item()) == osteps
+        assert np.array_equal(hip.as_u32(slc), oslc)
+        assert np.array_equal(hip.as_u32(slt), oslt)
+
+
+def test_fields_tiled_fallback_and_accumulate(gpu):
+    import torch
+    from slm_amd import hip, workloads
+    job = workloads.small_256(jitter=1.0, n=128, d=2)
+    seeds, mask, uv = _dev(job, gpu)
+    p = job.params(1)
+    oslc, oslt, _ = oracle.trace_fields(p, job.seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1))
+    base = np.arange(p.nxy_padded, dtype=np.uint32) * np.uint32(40503)
+    for force in (False, True):
+        slc = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
+        slt = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
+        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, tile=32, accumulate=True, sync=True,
+                               force_fallback=force)
+        assert np.array_equal(hip.as_u32(slc), base + oslc)
+        assert np.array_equal(hip.as_u32(slt), base + oslt)
