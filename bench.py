@@ -75,6 +75,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--path", default="queue", choices=["tiled", "queue"],
+                    help="tiled: LDS-owned tiles (slm_trace_fields_tiled); queue: global work queue")
+    ap.add_argument("--tile", type=int, default=0, help="tile side for --path tiled (0: library default)")
+    ap.add_argument("--block", type=int, default=0, help="threads per block for --path tiled")
+    ap.add_argument("--unit", type=int, default=0, help="states per unit for --path tiled")
+    ap.add_argument("--cap", type=int, default=0, help="steps per tile visit for --path tiled")
+    ap.add_argument("--halo", type=int, default=0, help="hysteresis band for --path tiled")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales PMC byte counters)")
     args = ap.parse_args()
@@ -105,15 +112,23 @@ def main():
     slc = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     steps = torch.zeros(2, dtype=torch.int64, device=dev)
-    ws = torch.empty(hip.fields_workspace_size(pd), dtype=torch.uint8, device=dev)
+    topts = hip.tile_opts(args.tile, args.block, args.unit, args.cap, args.halo)
+    if args.path == "tiled":
+        ws = torch.empty(hip.tiled_workspace_size(pd, topts), dtype=torch.uint8, device=dev)
+    else:
+        ws = torch.empty(hip.fields_workspace_size(pd), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
 
     def one_step(keep_order, events=None):
         for i, p in enumerate((pd, pu)):
             if events is not None:
                 events[i][0].record(stream)
-            hip.trace_fields(p, seeds, mask, uv, slc[i], slt[i], steps[i:i + 1], workspace=ws,
-                             stream=stream, keep_order=keep_order or i == 1)
+            if args.path == "tiled":
+                hip.trace_fields_tiled(p, seeds, mask, uv, slc[i], slt[i], steps[i:i + 1], workspace=ws,
+                                       opts=topts, stream=stream)
+            else:
+                hip.trace_fields(p, seeds, mask, uv, slc[i], slt[i], steps[i:i + 1], workspace=ws,
+                                 stream=stream, keep_order=keep_order or i == 1)
             if events is not None:
                 events[i][1].record(stream)
 
@@ -136,7 +151,8 @@ def main():
     achieved = steps_per_launch * BYTES_PER_STEP / mean_launch_s / 1e9
     value = total_steps / wall
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_fields_tile_2048_traffic.txt")
+    tfile = os.path.join(ROOT, "profiles", ("k_tl_process" if args.path == "tiled" else "k_fields") +
+                         "_tile_2048_traffic.txt")
     if args.workload == "tile_2048" and os.path.exists(tfile):
         # measured HBM-side bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
         for line in open(tfile):
@@ -163,7 +179,11 @@ def main():
                    "directions": 2, "map_ms": wall / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fields", "bytes_per_step": BYTES_PER_STEP,
+                     "kernel": "k_tl_process (with its per-round binning)" if args.path == "tiled" else "k_fields",
+                     "path": args.path,
+                     "tile_opts": {"tile": topts.tile, "block": topts.block, "unit": topts.unit,
+                                   "cap": topts.cap_steps, "halo": topts.halo} if args.path == "tiled" else None,
+                     "bytes_per_step": BYTES_PER_STEP,
                      "steps_per_launch": steps_per_launch, "mean_launch_ms": mean_launch_s * 1e3},
     }
     if not args.no_cpu:

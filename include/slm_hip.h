@@ -123,16 +123,26 @@ int slm_trace_fields(const SlmParams* p, const float* seeds_xy,
 
 /*
  * Same result as slm_trace_fields (bit for bit), computed by LDS-owned tiles:
- * the grid is cut into tile x tile pixel tiles (8 <= tile <= 128); each
- * 256-thread block steps the streamlines currently in one tile with that
+ * the grid is cut into tile x tile pixel tiles; each block steps the
+ * streamlines currently in one tile with that
  * tile's uv, mask and a 64-bit accumulator in LDS, and hands streamlines that
  * leave the tile (or exceed a per-visit step cap) to the owning tile for the
  * next round.  Rounds are driven from the host: this call synchronises the
  * stream once per round (so it cannot be captured in a graph).  Flags as for
- * slm_trace_fields except SLM_FLAG_KEEP_ORDER, which is ignored.
+ * slm_trace_fields except SLM_FLAG_KEEP_ORDER, which is ignored.  opts may be
+ * NULL (all defaults); zero fields take their defaults.
  */
-size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, uint32_t tile);
-int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const float* seeds_xy,
+typedef struct SlmTileOpts {
+    uint32_t tile;       /* tile side in pixels, 8..128 (0: 32)                    */
+    uint32_t block;      /* threads per block: 256, 512 or 1024 (0: 256)           */
+    uint32_t unit;       /* streamline states per unit of work (0: 8*block)        */
+    uint32_t cap_steps;  /* steps per tile visit before re-queueing (0: 4*tile)    */
+    uint32_t halo;       /* hysteresis band in pixels, <= 32 (0: 4)                */
+    uint32_t reserved[3];
+} SlmTileOpts;
+
+size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, const SlmTileOpts* opts);
+int slm_trace_fields_tiled(const SlmParams* p, const SlmTileOpts* opts, const float* seeds_xy,
                            const uint8_t* mask, const float* uv,
                            uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
                            void* workspace, size_t workspace_bytes,

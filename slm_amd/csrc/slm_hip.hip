@@ -68,6 +68,7 @@ static int check_params(const SlmParams* p) {
 
 struct DevInfo {
     int cus = 0;
+    int lds_per_block = 0;
 };
 static std::mutex g_dev_mu;
 static DevInfo g_dev[64];
@@ -81,6 +82,9 @@ static int device_info(DevInfo& out) {
         int cus = 0;
         HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         g_dev[dev].cus = cus > 0 ? cus : 1;
+        int lds = 0;
+        HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        g_dev[dev].lds_per_block = lds > 0 ? lds : 65536;
     }
     out = g_dev[dev];
     return SLM_OK;
@@ -194,12 +198,24 @@ __global__ __launch_bounds__(1024) void k_tile_scan(uint32_t* __restrict__ cnt, 
     }
 }
 
+// Seed record of the work order: position and RNG key of one seed.
+struct SeedRec {
+    float x, y;
+    uint32_t seed_idx, pad;
+};
+
 __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, uint32_t n,
-                               uint32_t* __restrict__ cur, uint32_t* __restrict__ order) {
+                               uint32_t* __restrict__ cur, SeedRec* __restrict__ order) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const uint32_t key = seed_tile_key(P, seeds[P.seeds_chunk_offset + k]);
-    order[atomicAdd(&cur[key], 1u)] = k;
+    const float2 s = seeds[P.seeds_chunk_offset + k];
+    const uint32_t key = seed_tile_key(P, s);
+    SeedRec r;
+    r.x = s.x;
+    r.y = s.y;
+    r.seed_idx = P.seeds_chunk_offset + k;
+    r.pad = 0u;
+    order[atomicAdd(&cur[key], 1u)] = r;
 }
 
 // ------------------------------------------------------------------ fields tracer
@@ -219,7 +235,7 @@ __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, ui
 //
 // Workspace layout (bytes, 256-aligned blocks):
 //   [0]    FieldsCtrl                      queue head, W, sum of low halves
-//   [256]  uint32 order[n]                 seed ordinals in tile order
+//   [256]  SeedRec order[n]                seeds in tile order with their RNG keys
 //   [..]   uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
 //   [..]   uint64 acc[nxy_padded]          packed (slt << 32) | slc
 //   [..]   Quad quad[nxy_padded]           interpolation corners per pixel
@@ -241,7 +257,7 @@ static FieldsWs fields_ws(const SlmParams* p) {
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
     const TileGeom g = tile_geom(p);
     w.off_order = 256;
-    w.off_cnt = w.off_order + align256((size_t)n * 4);
+    w.off_cnt = w.off_order + align256((size_t)n * sizeof(SeedRec));
     w.off_cur = w.off_cnt + align256((size_t)g.n_keys * 4);
     w.off_acc = w.off_cur + align256((size_t)g.n_keys * 4);
     w.off_quad = w.off_acc + align256((size_t)p->nxy_padded * 8);
@@ -271,7 +287,7 @@ __device__ __forceinline__ bool fields_packed_ok(const FieldsCtrl* ctrl, uint32_
 // is then not computed.
 template <bool C, bool PACKED>
 __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __restrict__ seeds,
-                                                const uint32_t* __restrict__ order, uint32_t n_work,
+                                                const SeedRec* __restrict__ order, uint32_t n_work,
                                                 const uint8_t* __restrict__ mask,
                                                 const Quad* __restrict__ quad,
                                                 unsigned long long* __restrict__ acc,
@@ -310,23 +326,41 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
         }
     };
 
+    // wave-private chunk of the work queue: one global atomic per kChunk items
+    constexpr uint32_t kChunk = 256;
+    uint32_t c_next = 0, c_end = 0;
     while (true) {
         if (!exhausted) {
-            const uint64_t need = __ballot(!active);
-            if (need) {
-                const uint32_t cnt = __popcll(need);
-                const uint32_t leader = __ffsll((unsigned long long)need) - 1u;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&ctrl->queue, cnt);
-                base = __shfl(base, leader);
-                if (base + cnt >= n_work) exhausted = true;
-                if (!active) {
-                    const uint32_t w = base + __popcll(need & lt_mask);
-                    if (w < n_work) {
-                        const uint32_t k = order ? order[w / dd] : w / dd;
-                        const uint32_t sub = w - (w / dd) * dd;
-                        const uint32_t seed_idx = P.seeds_chunk_offset + k;
-                        const float2 s = seeds[seed_idx];
+            uint64_t need = __ballot(!active);
+            while (need) {
+                if (c_next >= c_end) {
+                    uint32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&ctrl->queue, kChunk);
+                    base = __shfl(base, 0);
+                    if (base >= n_work) {
+                        exhausted = true;
+                        break;
+                    }
+                    c_next = base;
+                    c_end = min(base + kChunk, n_work);
+                }
+                const uint32_t take = min((uint32_t)__popcll(need), c_end - c_next);
+                const uint32_t rank = __popcll(need & lt_mask);
+                const bool mine = ((need >> lane) & 1ull) && rank < take;
+                if (mine) {
+                    const uint32_t w = c_next + rank;
+                    {
+                        const uint32_t sub = w % dd;
+                        float2 s;
+                        uint32_t seed_idx;
+                        if (order) {
+                            const SeedRec r = order[w / dd];
+                            s = make_float2(r.x, r.y);
+                            seed_idx = r.seed_idx;
+                        } else {
+                            seed_idx = P.seeds_chunk_offset + w / dd;
+                            s = seeds[seed_idx];
+                        }
                         x = s.x + sub_offset<C>(P, sub % d);
                         y = s.y + sub_offset<C>(P, sub / d);
                         rng = rng_seed(seed_idx);
@@ -346,6 +380,8 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                         }
                     }
                 }
+                need &= ~__ballot(mine);
+                c_next += take;
             }
         }
         if (!__any(active)) {
@@ -521,7 +557,7 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
 
     char* ws = (char*)workspace;
     FieldsCtrl* ctrl = (FieldsCtrl*)ws;
-    uint32_t* order = (uint32_t*)(ws + w.off_order);
+    SeedRec* order = (SeedRec*)(ws + w.off_order);
     uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
     uint32_t* cur = (uint32_t*)(ws + w.off_cur);
     unsigned long long* acc = (unsigned long long*)(ws + w.off_acc);
@@ -820,18 +856,35 @@ struct TileCtrl {
     uint32_t pad;
 };
 
-static TileCfg tile_cfg(const SlmParams* p, uint32_t T) {
+static void tile_opts_defaults(SlmTileOpts* o) {
+    o->tile = o->tile ? o->tile : 32u;
+    o->block = o->block ? o->block : 256u;
+    o->unit = o->unit ? o->unit : 8u * o->block;
+    o->cap_steps = o->cap_steps ? o->cap_steps : 4u * o->tile;
+    o->halo = o->halo ? o->halo : 4u;
+}
+
+static int check_tile_opts(const SlmTileOpts* o) {
+    if (o->tile < 8 || o->tile > 128) return slm_fail(SLM_E_INVALID_ARG, "tile side %u outside [8, 128]", o->tile);
+    if (o->block != 256 && o->block != 512 && o->block != 1024)
+        return slm_fail(SLM_E_INVALID_ARG, "block %u is not 256, 512 or 1024", o->block);
+    if (o->halo > 32) return slm_fail(SLM_E_INVALID_ARG, "halo %u above 32", o->halo);
+    if (o->unit == 0 || o->cap_steps == 0) return slm_fail(SLM_E_INVALID_ARG, "unit and cap_steps must be > 0");
+    return SLM_OK;
+}
+
+static TileCfg tile_cfg(const SlmParams* p, const SlmTileOpts& o) {
     TileCfg c;
-    c.T = T;
-    c.H = 4;
+    c.T = o.tile;
+    c.H = o.halo;
     c.M = 2;
     c.R = c.T + 2 * c.H + 2 * c.M;
     c.RU = c.R + 1;
-    c.ntx = (p->nx_padded + T - 1) / T;
-    c.nty = (p->ny_padded + T - 1) / T;
+    c.ntx = (p->nx_padded + c.T - 1) / c.T;
+    c.nty = (p->ny_padded + c.T - 1) / c.T;
     c.n_tiles = c.ntx * c.nty;
-    c.unit = 2048;
-    c.cap_steps = 4 * T;
+    c.unit = o.unit;
+    c.cap_steps = o.cap_steps;
     return c;
 }
 
@@ -975,8 +1028,8 @@ __global__ void k_tl_units(const uint32_t* __restrict__ cnt, const uint32_t* __r
 }
 
 // Process units: see the section comment.  in: binned states; out: emitted.
-template <bool C>
-__global__ __launch_bounds__(256) void k_tl_process(SlmParams P, TileCfg c, const TState* __restrict__ in,
+template <bool C, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tl_process(SlmParams P, TileCfg c, const TState* __restrict__ in,
                                                     TState* __restrict__ out, const TUnit* __restrict__ units,
                                                     const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
                                                     unsigned long long* __restrict__ acc,
@@ -1214,19 +1267,49 @@ static TiledWs tiled_ws(const SlmParams* p, const TileCfg& c) {
     return w;
 }
 
-extern "C" size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, uint32_t tile) {
+extern "C" size_t slm_trace_fields_tiled_workspace_size(const SlmParams* p, const SlmTileOpts* opts) {
     if (check_params(p) != SLM_OK) return 0;
-    if (tile < 8 || tile > 128) return 0;
-    return tiled_ws(p, tile_cfg(p, tile)).total;
+    SlmTileOpts o = {};
+    if (opts) o = *opts;
+    tile_opts_defaults(&o);
+    if (check_tile_opts(&o) != SLM_OK) return 0;
+    return tiled_ws(p, tile_cfg(p, o)).total;
 }
 
-extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const float* seeds_xy,
+template <bool C>
+static hipError_t tl_occupancy(int* per_cu, uint32_t block, size_t lds) {
+    if (block == 1024) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_tl_process<C, 1024>, 1024, lds);
+    if (block == 512) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_tl_process<C, 512>, 512, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_tl_process<C, 256>, 256, lds);
+}
+
+template <bool C>
+static void tl_launch(uint32_t block, uint32_t grid, size_t lds, hipStream_t s, const SlmParams& P, const TileCfg& c,
+                      const TState* in, TState* out, const TUnit* units, const uint8_t* mask, const float2* uv,
+                      unsigned long long* acc, unsigned long long* steps, TileCtrl* ctrl, uint32_t fixed,
+                      uint32_t cap) {
+    if (block == 1024)
+        hipLaunchKernelGGL((k_tl_process<C, 1024>), dim3(grid), dim3(1024), lds, s, P, c, in, out, units, mask, uv,
+                           acc, steps, ctrl, fixed, cap);
+    else if (block == 512)
+        hipLaunchKernelGGL((k_tl_process<C, 512>), dim3(grid), dim3(512), lds, s, P, c, in, out, units, mask, uv,
+                           acc, steps, ctrl, fixed, cap);
+    else
+        hipLaunchKernelGGL((k_tl_process<C, 256>), dim3(grid), dim3(256), lds, s, P, c, in, out, units, mask, uv,
+                           acc, steps, ctrl, fixed, cap);
+}
+
+extern "C" int slm_trace_fields_tiled(const SlmParams* p, const SlmTileOpts* opts, const float* seeds_xy,
                                       const uint8_t* mask, const float* uv, uint32_t* slc, uint32_t* slt,
                                       uint64_t* steps_out, void* workspace, size_t workspace_bytes,
                                       void* stream, uint32_t flags) {
     int rc = check_params(p);
     if (rc) return rc;
-    if (tile < 8 || tile > 128) return slm_fail(SLM_E_INVALID_ARG, "tile side %u outside [8, 128]", tile);
+    SlmTileOpts o = {};
+    if (opts) o = *opts;
+    tile_opts_defaults(&o);
+    rc = check_tile_opts(&o);
+    if (rc) return rc;
     if (!mask || !uv || !slc || !slt) return slm_fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
@@ -1242,7 +1325,7 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const f
     const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
     if ((uint64_t)n * d * d >= (1ull << 31))
         return slm_fail(SLM_E_INVALID_ARG, "n_seeds * d^2 = %llu exceeds 2^31", (unsigned long long)n * d * d);
-    const TileCfg c = tile_cfg(p, tile);
+    const TileCfg c = tile_cfg(p, o);
     const TiledWs w = tiled_ws(p, c);
     if (!workspace || workspace_bytes < w.total)
         return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
@@ -1250,7 +1333,9 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const f
     rc = device_info(di);
     if (rc) return rc;
     const size_t lds = tile_lds_bytes(c);
-    if (lds > 64 * 1024) return slm_fail(SLM_E_INVALID_ARG, "tile %u needs %zu bytes of LDS", tile, lds);
+    if (lds > (size_t)di.lds_per_block)
+        return slm_fail(SLM_E_INVALID_ARG, "tile %u needs %zu bytes of LDS, the device allows %d per block", o.tile,
+                        lds, di.lds_per_block);
 
     char* ws = (char*)workspace;
     TileCtrl* ctrl = (TileCtrl*)ws;
@@ -1282,9 +1367,9 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const f
 
     int per_cu = 0;
     if (p->fp_contract)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tl_process<true>, 256, lds));
+        HIP_TRY(tl_occupancy<true>(&per_cu, o.block, lds));
     else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tl_process<false>, 256, lds));
+        HIP_TRY(tl_occupancy<false>(&per_cu, o.block, lds));
     if (per_cu < 1) per_cu = 1;
     const uint32_t pgrid = (uint32_t)di.cus * (uint32_t)per_cu;
 
@@ -1306,11 +1391,11 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, uint32_t tile, const f
         hipLaunchKernelGGL(k_tl_units, dim3(grid_for(c.n_tiles)), dim3(256), 0, s, cnt, off, uoff, c.n_tiles,
                            c.unit, units);
         if (p->fp_contract)
-            hipLaunchKernelGGL(k_tl_process<true>, dim3(pgrid), dim3(256), lds, s, *p, c, B, A, units, mask,
-                               (const float2*)uv, acc, (unsigned long long*)steps_out, ctrl, fixed, n_work);
+            tl_launch<true>(o.block, pgrid, lds, s, *p, c, B, A, units, mask, (const float2*)uv, acc,
+                            (unsigned long long*)steps_out, ctrl, fixed, n_work);
         else
-            hipLaunchKernelGGL(k_tl_process<false>, dim3(pgrid), dim3(256), lds, s, *p, c, B, A, units, mask,
-                               (const float2*)uv, acc, (unsigned long long*)steps_out, ctrl, fixed, n_work);
+            tl_launch<false>(o.block, pgrid, lds, s, *p, c, B, A, units, mask, (const float2*)uv, acc,
+                             (unsigned long long*)steps_out, ctrl, fixed, n_work);
         herr = hipMemcpyAsync(h_nout, &ctrl->n_out, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
         if (herr == hipSuccess) herr = hipStreamSynchronize(s);
         if (herr != hipSuccess) break;

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .params import SlmKdeParams, SlmParams, ABI_VERSION
+from .params import SlmKdeParams, SlmParams, SlmTileOpts, ABI_VERSION
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
@@ -34,9 +34,10 @@ _PROTOS = {
     "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _U32]),
-    "slm_trace_fields_tiled_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams), _U32]),
-    "slm_trace_fields_tiled": (ctypes.c_int, [ctypes.POINTER(SlmParams), _U32, _P, _P, _P, _P, _P, _P, _P,
-                                              ctypes.c_size_t, _P, _U32]),
+    "slm_trace_fields_tiled_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams),
+                                                                ctypes.POINTER(SlmTileOpts)]),
+    "slm_trace_fields_tiled": (ctypes.c_int, [ctypes.POINTER(SlmParams), ctypes.POINTER(SlmTileOpts), _P, _P, _P,
+                                              _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_trace_paths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P, _U32]),
     "slm_trace_fields_record": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _U32,
                                                _U32, _P, _P, _P, _U32]),
@@ -129,27 +130,33 @@ def trace_fields(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=
     return workspace
 
 
-DEFAULT_TILE = 32
+def tile_opts(tile=0, block=0, unit=0, cap_steps=0, halo=0) -> SlmTileOpts:
+    o = SlmTileOpts()
+    o.tile, o.block, o.unit, o.cap_steps, o.halo = tile, block, unit, cap_steps, halo
+    return o
 
 
-def tiled_workspace_size(p: SlmParams, tile: int = DEFAULT_TILE) -> int:
-    n = load().slm_trace_fields_tiled_workspace_size(ctypes.byref(p), tile)
+def tiled_workspace_size(p: SlmParams, opts: SlmTileOpts = None) -> int:
+    opts = opts if opts is not None else tile_opts()
+    n = load().slm_trace_fields_tiled_workspace_size(ctypes.byref(p), ctypes.byref(opts))
     if n == 0:
         check(-1)
     return int(n)
 
 
-def trace_fields_tiled(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=None, tile=DEFAULT_TILE,
+def trace_fields_tiled(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=None, opts=None,
                        stream=None, accumulate=False, sync=False, force_fallback=False):
     """slm_trace_fields_tiled: same result as trace_fields, LDS-owned tiles."""
     import torch
     _require_cuda(seeds, mask, uv, slc, slt, steps, workspace)
-    need = tiled_workspace_size(p, tile)
+    opts = opts if opts is not None else tile_opts()
+    need = tiled_workspace_size(p, opts)
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=slc.device)
     flags = (SLM_FLAG_NO_ZERO if accumulate else 0) | (SLM_FLAG_SYNC if sync else 0) | \
         (SLM_FLAG_FORCE_FALLBACK if force_fallback else 0)
-    check(load().slm_trace_fields_tiled(ctypes.byref(p), tile, _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(slc),
+    check(load().slm_trace_fields_tiled(ctypes.byref(p), ctypes.byref(opts), _ptr(seeds), _ptr(mask), _ptr(uv),
+                                        _ptr(slc),
                                         _ptr(slt), _ptr(steps), _ptr(workspace), workspace.numel(),
                                         _stream(stream), flags))
     return workspace

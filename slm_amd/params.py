@@ -79,6 +79,12 @@ class SlmKdeParams(ctypes.Structure):
 assert ctypes.sizeof(SlmKdeParams) == 64
 
 
+class SlmTileOpts(ctypes.Structure):
+    """Mirror of ``struct SlmTileOpts`` in include/slm_hip.h (32 bytes)."""
+    _fields_ = [("tile", ctypes.c_uint32), ("block", ctypes.c_uint32), ("unit", ctypes.c_uint32),
+                ("cap_steps", ctypes.c_uint32), ("halo", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
 _libc = ctypes.CDLL(None)
 _libc.strtof.restype = ctypes.c_float
 _libc.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]

@@ -212,17 +212,18 @@ def test_hsl_bit_exact(gpu, contract, offset):
 
 
 TILED_CASES = [
-    # (grid side, d, jitter, fp_contract, tile, max_error)
-    (256, 2, 1.0, True, 32, None),
-    (256, 1, 0.0, False, 16, None),
-    (200, 3, 1.0, True, 64, None),
-    (96, 2, 1.0, True, 8, 1e-6),
-    (128, 1, 1.0, False, 24, 1e-5),
+    # (grid side, d, jitter, fp_contract, (tile, block, unit, cap, halo), max_error)
+    (256, 2, 1.0, True, (32, 256, 0, 0, 0), None),
+    (256, 1, 0.0, False, (16, 512, 0, 0, 2), None),
+    (200, 3, 1.0, True, (48, 1024, 4096, 64, 4), None),
+    (96, 2, 1.0, True, (8, 256, 100, 3, 1), 1e-6),
+    (128, 1, 1.0, False, (24, 1024, 0, 0, 8), 1e-5),
+    (256, 2, 1.0, True, (64, 1024, 0, 0, 4), None),
 ]
 
 
-@pytest.mark.parametrize("n,d,jitter,contract,tile,max_error", TILED_CASES)
-def test_fields_tiled_bit_exact(gpu, n, d, jitter, contract, tile, max_error):
+@pytest.mark.parametrize("n,d,jitter,contract,topts,max_error", TILED_CASES)
+def test_fields_tiled_bit_exact(gpu, n, d, jitter, contract, topts, max_error):
     """The LDS-tiled tracer gives the oracle's fields, both directions."""
     import torch
     from slm_amd import hip, workloads
@@ -231,11 +232,11 @@ def test_fields_tiled_bit_exact(gpu, n, d, jitter, contract, tile, max_error):
     m = job.mask.astype(np.uint8).reshape(-1)
     for sign in (1, -1):
         p = job.params(sign, fp_contract=contract)
-        print("tiled n=%d d=%d jitter=%g tile=%d sign=%+d" % (n, d, jitter, tile, sign), flush=True)
+        print("tiled n=%d d=%d jitter=%g opts=%s sign=%+d" % (n, d, jitter, topts, sign), flush=True)
         slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=gpu)
         slt = torch.empty(p.nxy_padded, dtype=torch.int32, device=gpu)
         steps = torch.zeros(1, dtype=torch.int64, device=gpu)
-        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, steps, tile=tile, sync=True)
+        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, steps, opts=hip.tile_opts(*topts), sync=True)
         oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1))
         assert int(steps.item()) == osteps
         assert np.array_equal(hip.as_u32(slc), oslc)
@@ -253,7 +254,6 @@ def test_fields_tiled_fallback_and_accumulate(gpu):
     for force in (False, True):
         slc = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
         slt = torch.from_numpy(base.view(np.int32).copy()).to(gpu)
-        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, tile=32, accumulate=True, sync=True,
-                               force_fallback=force)
+        hip.trace_fields_tiled(p, seeds, mask, uv, slc, slt, accumulate=True, sync=True, force_fallback=force)
         assert np.array_equal(hip.as_u32(slc), base + oslc)
         assert np.array_equal(hip.as_u32(slt), base + oslt)
