@@ -234,18 +234,22 @@ __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, ui
 // at once.
 //
 // Workspace layout (bytes, 256-aligned blocks):
-//   [0]    FieldsCtrl                      queue head, W, sum of low halves
-//   [256]  SeedRec order[n]                seeds in tile order with their RNG keys
+//   [0]    FieldsCtrl                      W, sum of low halves, queue heads
+//   [4096] SeedRec order[n]                seeds in tile order with their RNG keys
 //   [..]   uint32 cnt[n_keys], cur[n_keys] counting-sort buffers
 //   [..]   uint64 acc[nxy_padded]          packed (slt << 32) | slc
 //   [..]   Quad quad[nxy_padded]           interpolation corners per pixel
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Work-queue heads: one per XCD slice of the work order, 256 bytes apart.
+constexpr uint32_t kQueues = 8;
+constexpr uint32_t kQueueStride = 64;  // uint32 words
+
 struct FieldsCtrl {
-    uint32_t queue;
-    uint32_t pad;
     unsigned long long writes;
     unsigned long long sumlow;
+    uint32_t pad[60];
+    uint32_t queue[kQueues * kQueueStride];
 };
 
 struct FieldsWs {
@@ -256,7 +260,7 @@ static FieldsWs fields_ws(const SlmParams* p) {
     FieldsWs w;
     const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
     const TileGeom g = tile_geom(p);
-    w.off_order = 256;
+    w.off_order = align256(sizeof(FieldsCtrl)) > 4096 ? align256(sizeof(FieldsCtrl)) : 4096;
     w.off_cnt = w.off_order + align256((size_t)n * sizeof(SeedRec));
     w.off_cur = w.off_cnt + align256((size_t)g.n_keys * 4);
     w.off_acc = w.off_cur + align256((size_t)g.n_keys * 4);
@@ -334,15 +338,29 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             uint64_t need = __ballot(!active);
             while (need) {
                 if (c_next >= c_end) {
-                    uint32_t base = 0;
-                    if (lane == 0) base = atomicAdd(&ctrl->queue, kChunk);
-                    base = __shfl(base, 0);
-                    if (base >= n_work) {
+                    // The work order is cut into kQueues slices; a wave starts on
+                    // the slice of its XCD (its L2 then holds that part of the
+                    // grid) and steals from the other slices when it runs dry.
+                    uint32_t base = 0xffffffffu, lim = 0;
+                    const uint32_t me = __builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11));  // XCC_ID
+                    const uint32_t slice = (n_work + kQueues - 1u) / kQueues;
+                    for (uint32_t t = 0; t < kQueues && base == 0xffffffffu; t++) {
+                        const uint32_t q = (me + t) % kQueues;
+                        const uint32_t lo = q * slice, hi = min(n_work, lo + slice);
+                        uint32_t b = 0;
+                        if (lane == 0) b = atomicAdd(&ctrl->queue[q * kQueueStride], kChunk);
+                        b = __shfl(b, 0);
+                        if (b < slice && lo + b < hi) {
+                            base = lo + b;
+                            lim = hi;
+                        }
+                    }
+                    if (base == 0xffffffffu) {
                         exhausted = true;
                         break;
                     }
                     c_next = base;
-                    c_end = min(base + kChunk, n_work);
+                    c_end = min(base + kChunk, lim);
                 }
                 const uint32_t take = min((uint32_t)__popcll(need), c_end - c_next);
                 const uint32_t rank = __popcll(need & lt_mask);
@@ -469,8 +487,8 @@ static uint32_t dt_is_fixed(const SlmParams* p) {
 }
 
 __global__ void k_fields_init(FieldsCtrl* ctrl) {
+    if (threadIdx.x < kQueues) ctrl->queue[threadIdx.x * kQueueStride] = 0u;
     if (threadIdx.x == 0) {
-        ctrl->queue = 0u;
         ctrl->writes = 0ull;
         ctrl->sumlow = 0ull;
     }
@@ -520,7 +538,7 @@ __global__ void k_fields_commit(const unsigned long long* __restrict__ acc, uint
             slt[i] = 0u;
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctrl->queue = 0u;
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) ctrl->queue[threadIdx.x * kQueueStride] = 0u;
 }
 
 static uint32_t grid_for(size_t n) {
@@ -1254,7 +1272,7 @@ static TiledWs tiled_ws(const SlmParams* p, const TileCfg& c) {
     const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
     const size_t n_work = (size_t)n * d * d;
     w.max_units = (uint32_t)((n_work + c.unit - 1) / c.unit + c.n_tiles);
-    w.off_acc = 256;
+    w.off_acc = align256(sizeof(TileCtrl));
     w.off_quad = w.off_acc + align256((size_t)p->nxy_padded * 8);
     w.off_a = w.off_quad + align256((size_t)p->nxy_padded * sizeof(Quad));
     w.off_b = w.off_a + align256(n_work * sizeof(TState));
