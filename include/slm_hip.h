@@ -82,7 +82,9 @@ typedef struct SlmParams {
     uint32_t fp_contract;                 /* 1: a*b+c fused as clang's OpenCL default
                                              FP_CONTRACT ON does; 0: every op rounded */
     float    pixel_size;                  /* PIXEL_SIZE (metres)                     */
-    uint32_t reserved[7];
+    uint32_t interchannel_max_n_steps;    /* INTERCHANNEL_MAX_N_STEPS (connect_channels) */
+    uint32_t segmentation_threshold;      /* SEGMENTATION_THRESHOLD (0 when not rendered) */
+    uint32_t reserved[5];
 } SlmParams;
 
 /* Library identity. */
@@ -243,6 +245,149 @@ int slm_kde_pdf(const SlmKdeParams* k, const uint32_t* hist, float* pdf,
 int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
                           const float* uv, const uint32_t* mapping, uint32_t thin_flag,
                           uint32_t from_flag, float* length, void* stream, uint32_t flags);
+
+/* ------------------------------------------------- channel mapping and segmentation */
+
+/*
+ * Mapping flag bits: bit k of the list in python/streamlines/useful.py:249-268
+ * (the Python list is canonical; opencl/info.h:114-131 differs).
+ */
+#define SLM_IS_CHANNEL          0x00001u
+#define SLM_IS_THINCHANNEL      0x00002u
+#define SLM_IS_INTERCHANNEL     0x00004u
+#define SLM_IS_CHANNELHEAD      0x00008u
+#define SLM_IS_CHANNELTAIL      0x00010u
+#define SLM_IS_MAJORCONFLUENCE  0x00020u
+#define SLM_IS_MINORCONFLUENCE  0x00040u
+#define SLM_IS_MAJORINFLOW      0x00080u
+#define SLM_IS_MINORINFLOW      0x00100u
+#define SLM_IS_LEFTFLANK        0x00200u
+#define SLM_IS_RIGHTFLANK       0x00400u
+#define SLM_IS_MIDSLOPE         0x00800u
+#define SLM_IS_RIDGE            0x01000u
+#define SLM_WAS_CHANNELHEAD     0x02000u
+#define SLM_IS_SUBSEGMENTHEAD   0x04000u
+#define SLM_IS_LOOP             0x08000u
+#define SLM_IS_BLOCKAGE         0x10000u
+#define SLM_LEFT_FLANK_ADDITION 0x80000000u  /* label bit of a left-flank pixel (useful.py:196) */
+
+/*
+ * Conventions shared by the mapping entry points:
+ *   seeds_xy : float2 [n_seed_points], the pixel list of pick_seeds
+ *              (useful.py:519-538: argwhere order, minus pad_width); seeds k in
+ *              [seeds_chunk_offset, n_seed_points) are processed;
+ *   mask uint8, uv float2, mapping / count / link / label uint32, all
+ *   [nxy_padded]; label holds the reference's int32 label bits.
+ * The traces use the un-jittered RK2 step (computestep.cl:42-57) with p built
+ * for downstream tracing; nxy_padded must be below 2^31.
+ * Where the reference kernel reads values that other work items write in the
+ * same launch, the result here follows a fixed, documented order of phases, so
+ * repeated calls give identical output; on inputs where the reference's
+ * outcome does not depend on its schedule, the outputs are the reference's.
+ * Calls taking a workspace need slm_map_workspace_size(p) bytes; calls that
+ * share a workspace must be ordered on one stream.
+ */
+size_t slm_map_workspace_size(const SlmParams* p);
+
+/* connect_channels (opencl/connect.cl:26-108), launched by
+ * connect.connect_channel_pixels (python/streamlines/connect.py:32-67):
+ * seeds = IS_CHANNEL pixels; ORs IS_INTERCHANNEL into unflagged pixels of
+ * short traces that join two channel pixels. */
+int slm_connect_channels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                         const float* uv, uint32_t* mapping, void* stream, uint32_t flags);
+
+/* Stand-in for Mapping.thin_channels (mapping.py:707-719), which calls
+ * scikit-image skeletonize(medial_axis(.)): Zhang-Suen thinning of the
+ * IS_CHANNEL | IS_INTERCHANNEL pixels, result ORed as IS_THINCHANNEL.
+ * Synchronises the stream once per iteration; *n_iter (may be NULL) receives
+ * the iteration count. */
+int slm_thin_channels(const SlmParams* p, uint32_t* mapping, void* workspace,
+                      size_t workspace_bytes, uint32_t* n_iter, void* stream, uint32_t flags);
+
+/* map_channel_heads (opencl/channelheads.cl:32-93), launched by
+ * channelheads.map_channel_heads (channelheads.py:33-79): seeds = all unmasked
+ * pixels; clears IS_CHANNELHEAD where a trace arrives, unless it arrives on a
+ * thin channel pixel from a non-thin one. */
+int slm_map_channel_heads(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                          const float* uv, uint32_t* mapping, void* stream, uint32_t flags);
+
+/* count_downchannels (opencl/countlink.cl:34-114), launched by
+ * countlink.count_downchannels (countlink.py:36-80): seeds = channel heads;
+ * traces downstream numbering pixels, flags them thin channel, links each
+ * pixel to the next.  Phases: full traces rank pixels by (counter, seed);
+ * traces stop on entering a pixel ranked higher by another trace; each
+ * pixel links to the next pixel of its highest-counter hop. */
+int slm_count_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                           const float* uv, uint32_t* mapping, uint32_t* count, uint32_t* link,
+                           void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
+
+/* flag_downchannels (opencl/countlink.cl:134-210), launched by
+ * countlink.flag_downchannels (countlink.py:83-130): seeds = channel heads;
+ * walks link, flagging thin channel, count = max(count, hops+1), channel
+ * tail at the last unmasked pixel; unflags heads of short minor inflows.
+ * Phases: head counts = 1, walks, head test on the final counts. */
+int slm_flag_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                          uint32_t* mapping, uint32_t* count, const uint32_t* link,
+                          void* stream, uint32_t flags);
+
+/* label_confluences (opencl/label.cl:21-134), launched by
+ * label.label_confluences (label.py:33-75): seeds = thin channel pixels;
+ * flags confluences and their major / minor inflows (slt: float [nxy_padded],
+ * the downstream slt field, breaks ties). */
+int slm_label_confluences(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                          const float* slt, uint32_t* mapping, const uint32_t* count,
+                          const uint32_t* link, void* stream, uint32_t flags);
+
+/* segment_downchannels (opencl/segment.cl:38-107), launched by
+ * segment.segment_channels (segment.py:34-93): seeds = channel heads; labels
+ * channel pixels with the index of their segment's first pixel and flags
+ * segment starts IS_SUBSEGMENTHEAD.  Phases: each channel pixel is owned by
+ * its hop of largest (count, major inflow, index); a walk follows its owned
+ * hops and marks segment starts, including where it joins another's reach;
+ * then labels. */
+int slm_segment_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                             uint32_t* mapping, const uint32_t* count, const uint32_t* link,
+                             uint32_t* label, void* workspace, size_t workspace_bytes,
+                             void* stream, uint32_t flags);
+
+/* hillslopes (opencl/hillslopes.cl:31-91), launched by
+ * hillslopes.link_hillslopes (hillslopes.py:32-75): seeds = unmasked
+ * non-thin pixels; links each to the pixel its trace enters. */
+int slm_link_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                        const float* uv, uint32_t* link, void* stream, uint32_t flags);
+
+/* segment_hillslopes (opencl/segment.cl:133-188), launched by
+ * segment.segment_hillslopes (segment.py:95-146): seeds = unmasked non-thin
+ * pixels; copies the label of the thin channel pixel the trace reaches. */
+int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                           const float* uv, const uint32_t* mapping, uint32_t* label,
+                           void* stream, uint32_t flags);
+
+/* subsegment_channel_edges (opencl/segment.cl:195-334), first launch of
+ * segment.subsegment_flanks (segment.py:148-243): seeds = subsegment heads;
+ * flags the left-hand neighbours of each channel reach IS_LEFTFLANK and sets
+ * LEFT_FLANK_ADDITION in their label. */
+int slm_subsegment_channel_edges(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                 uint32_t* mapping, const uint32_t* link, uint32_t* label,
+                                 void* stream, uint32_t flags);
+
+/* subsegment_flanks (opencl/segment.cl:360-414), second launch of
+ * segment.subsegment_flanks: marks LEFT_FLANK_ADDITION on pixels whose trace
+ * reaches a left-flank pixel before a thin channel pixel. */
+int slm_subsegment_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                          const float* uv, const uint32_t* mapping, uint32_t* label,
+                          void* stream, uint32_t flags);
+
+/* fix_right_flanks (opencl/segment.cl:441-504) and fix_left_flanks (:530-594),
+ * third and fourth launches of segment.subsegment_flanks: relabel a pixel
+ * from the pixels its trace crosses on its own side of the channel.  Reads of
+ * other pixels' labels see the labels as they were when the call started. */
+int slm_fix_right_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                         const float* uv, const uint32_t* mapping, uint32_t* label,
+                         void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
+int slm_fix_left_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                        const float* uv, const uint32_t* mapping, uint32_t* label,
+                        void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
 
 #ifdef __cplusplus
 }

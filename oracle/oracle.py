@@ -10,6 +10,8 @@ Functions (each restates the reference kernel named in oracle/slm_oracle.c):
   trace_paths                          integratetraj.cl:54-97 + trajectory.cl
   hillslope_lengths                    lengths.cl:37-116
   kde_histogram, kde_pdf               kde.cl:383-394, :426-526
+  connect_channels ... fix_left_flanks channel mapping and segmentation
+                                       (oracle/slm_oracle_map.c), see map_call
 The tracing functions have an FP64 twin (f64=True) built from the same source.
 """
 from __future__ import annotations
@@ -25,6 +27,64 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
+
+
+# name -> number of pointer arguments (SlmParams* first), in C order
+_MAP_FUNCS = {
+    "connect_channels": 5,          # p, seeds, mask, uv, mapping
+    "thin_channels": 2,             # p, mapping
+    "map_channel_heads": 5,         # p, seeds, mask, uv, mapping
+    "count_downchannels": 7,        # p, seeds, mask, uv, mapping, count, link
+    "flag_downchannels": 6,         # p, seeds, mask, mapping, count, link
+    "label_confluences": 7,         # p, seeds, mask, slt, mapping, count, link
+    "segment_downchannels": 7,      # p, seeds, mask, mapping, count, link, label
+    "link_hillslopes": 5,           # p, seeds, mask, uv, link
+    "segment_hillslopes": 6,        # p, seeds, mask, uv, mapping, label
+    "subsegment_channel_edges": 6,  # p, seeds, mask, mapping, link, label
+    "subsegment_flanks": 6,         # p, seeds, mask, uv, mapping, label
+    "fix_right_flanks": 6,          # p, seeds, mask, uv, mapping, label
+    "fix_left_flanks": 6,           # p, seeds, mask, uv, mapping, label
+}
+
+_MAP_DTYPES = {"seeds": np.float32, "mask": np.uint8, "uv": np.float32, "slt": np.float32,
+               "mapping": np.uint32, "count": np.uint32, "link": np.uint32, "label": np.uint32}
+_MAP_ARGS = {
+    "connect_channels": ("seeds", "mask", "uv", "mapping"),
+    "thin_channels": ("mapping",),
+    "map_channel_heads": ("seeds", "mask", "uv", "mapping"),
+    "count_downchannels": ("seeds", "mask", "uv", "mapping", "count", "link"),
+    "flag_downchannels": ("seeds", "mask", "mapping", "count", "link"),
+    "label_confluences": ("seeds", "mask", "slt", "mapping", "count", "link"),
+    "segment_downchannels": ("seeds", "mask", "mapping", "count", "link", "label"),
+    "link_hillslopes": ("seeds", "mask", "uv", "link"),
+    "segment_hillslopes": ("seeds", "mask", "uv", "mapping", "label"),
+    "subsegment_channel_edges": ("seeds", "mask", "mapping", "link", "label"),
+    "subsegment_flanks": ("seeds", "mask", "uv", "mapping", "label"),
+    "fix_right_flanks": ("seeds", "mask", "uv", "mapping", "label"),
+    "fix_left_flanks": ("seeds", "mask", "uv", "mapping", "label"),
+}
+
+
+def map_call(name, p, **arrays):
+    """Run oracle mapping function ``name`` (slm_oracle_map.c) in place.
+
+    Arrays are passed by keyword (names as in _MAP_ARGS); the uint32 grids
+    (mapping/count/link/label) must be C-contiguous numpy arrays of the right
+    dtype, since they are updated in place.  Returns the C return value
+    (thin_channels: iteration count) or None."""
+    keep = []
+    ptrs = []
+    for a in _MAP_ARGS[name]:
+        arr = arrays[a]
+        want = _MAP_DTYPES[a]
+        if a in ("mapping", "count", "link", "label"):
+            if arr.dtype != want or not arr.flags.c_contiguous:
+                raise TypeError("%s must be a C-contiguous %s array" % (a, np.dtype(want)))
+        else:
+            arr = np.ascontiguousarray(arr, dtype=want)
+        keep.append(arr)
+        ptrs.append(ctypes.c_void_p(arr.ctypes.data))
+    return getattr(load(), "slmo_" + name)(ctypes.byref(p), *ptrs)
 
 
 def load():
@@ -48,6 +108,10 @@ def load():
             getattr(lib, pre + "hillslope_lengths").argtypes = [_P, _P, _P, _P, _P, _U32, _U32, _P]
             getattr(lib, pre + "trace_paths").restype = None
             getattr(lib, pre + "trace_paths").argtypes = [_P, _P, _P, _P, _P, _P, _P, ctypes.c_int]
+        for name, nargs in _MAP_FUNCS.items():
+            f = getattr(lib, "slmo_" + name)
+            f.restype = _U32 if name == "thin_channels" else None
+            f.argtypes = [_P] * nargs
         lib.slmo_kde_histogram.restype = None
         lib.slmo_kde_histogram.argtypes = [_P, _P, _P]
         lib.slmo_kde_pdf.restype = None

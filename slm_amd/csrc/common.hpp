@@ -7,6 +7,8 @@
 
 // Records a printf-style message for slm_last_error() and returns code.
 int slm_fail(int code, const char* fmt, ...);
+// Validates the grid and ABI fields of an SlmParams (SLM_OK or an error).
+int slm_check_params(const SlmParams* p);
 // Checks the last launch, synchronises the stream when SLM_FLAG_SYNC is set.
 int slm_finish(void* stream, uint32_t flags);
 

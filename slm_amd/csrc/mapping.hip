@@ -1,0 +1,857 @@
+// mapping.hip -- channel mapping and segmentation kernels of libslm_hip.so.
+//
+// Reference kernels (opencl/): connect.cl, channelheads.cl, countlink.cl,
+// label.cl, hillslopes.cl, segment.cl, launched from
+// python/streamlines/mapping.py:627-776 through connect.py, channelheads.py,
+// countlink.py, label.py, hillslopes.py and segment.py.
+//
+// One thread per seed pixel: the work is a short un-jittered RK2 trace
+// (compute_step_vec, computestep.cl:42-57) or an integer walk along link.
+// Where the reference reads values that other work items write in the same
+// launch (count_downchannels, flag_downchannels, segment_downchannels,
+// fix_right/left_flanks) the work is split into launches with a fixed order
+// of phases and order-free updates (atomic max / or / and), so every run
+// gives the same bits; the phases are documented at each entry point.
+//
+// Workspace (slm_map_workspace_size): two 64-bit per-pixel arrays A and B
+// (16 bytes per pixel) plus 256 bytes of counters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "trace_common.hpp"
+
+using namespace slm;
+
+namespace {
+
+constexpr uint32_t F_CHANNEL = SLM_IS_CHANNEL;
+constexpr uint32_t F_THIN = SLM_IS_THINCHANNEL;
+constexpr uint32_t F_INTER = SLM_IS_INTERCHANNEL;
+constexpr uint32_t F_HEAD = SLM_IS_CHANNELHEAD;
+constexpr uint32_t F_TAIL = SLM_IS_CHANNELTAIL;
+constexpr uint32_t F_MAJCONF = SLM_IS_MAJORCONFLUENCE;
+constexpr uint32_t F_MAJIN = SLM_IS_MAJORINFLOW;
+constexpr uint32_t F_MININ = SLM_IS_MINORINFLOW;
+constexpr uint32_t F_LEFT = SLM_IS_LEFTFLANK;
+constexpr uint32_t F_SSH = SLM_IS_SUBSEGMENTHEAD;
+constexpr uint32_t LFA = SLM_LEFT_FLANK_ADDITION;
+constexpr int kBlock = 256;
+
+typedef unsigned long long u64;
+
+__device__ __forceinline__ u64 key2(uint32_t hi, uint32_t lo) { return ((u64)hi << 32) | (u64)lo; }
+
+// Clears n 64-bit words (a kernel rather than a memset node, so that a
+// captured graph replays it correctly).
+__global__ void k_zero_u64(u64* __restrict__ a, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        a[i] = 0ull;
+}
+
+// link[i]; a link outside the grid reads as a self link (end of the chain).
+__device__ __forceinline__ uint32_t lk(const SlmParams& P, const uint32_t* __restrict__ link, uint32_t i) {
+    const uint32_t j = link[i];
+    return j < P.nxy_padded ? j : i;
+}
+
+__device__ __forceinline__ uint32_t seed_pixel(const SlmParams& P, const float2* __restrict__ seeds, uint32_t k) {
+    const float2 s = seeds[k];
+    return array_idx(P, s.x, s.y);
+}
+
+// The shared head of the mapping RK steps (rungekutta.cl:280-459): dxy1
+// retry of a short step, move, new pixel; returns the step length.
+template <bool C>
+__device__ __forceinline__ float map_move(const SlmParams& P, const StepOut& o, float& x, float& y,
+                                          uint32_t& idx) {
+    float2 nx = o.next;
+    float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+    if (dl < P.integration_halt_threshold) {
+        nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
+        dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+    }
+    x = nx.x;
+    y = nx.y;
+    idx = array_idx(P, x, y);
+    return dl;
+}
+
+// --------------------------------------------------------------- connect_channels
+
+// connect_channels (connect.cl:26-108).  Pass 0 traces and counts the
+// recorded steps n; if 2 < n < INTERCHANNEL_MAX_N_STEPS, pass 1 traces again
+// (the trace is deterministic) and replays each record compress(vec-prev_vec)
+// as it is produced, flagging visited pixels whose mapping is 0.  The first
+// loop is bounded by INTERCHANNEL_MAX_N_STEPS as well (such a trace flags
+// nothing in the reference either).
+template <bool C>
+__global__ void k_connect(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                          const float2* __restrict__ uv, uint32_t* __restrict__ mapping) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t cap = P.interchannel_max_n_steps;
+    const float res = P.trajectory_resolution;
+    const float2 s0 = seeds[k];
+    uint32_t n_total = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1 && !(n_total > 2 && n_total < cap)) return;
+        float x = s0.x, y = s0.y, px = x, py = y, dt = P.dt_max;
+        uint32_t n = 0, rng = 0, idx = array_idx(P, x, y);
+        const uint32_t seed_idx = idx;
+        float rx = s0.x, ry = s0.y;
+        uint32_t ridx = idx;
+        for (int loop = 0; loop < 2; loop++) {
+            for (;;) {
+                if (loop == 0) {
+                    if (!(seed_idx == idx && !mask[idx] && n != P.max_n_steps && n < cap)) break;
+                } else {
+                    if (!((mapping[idx] & F_CHANNEL) == 0 && !mask[idx] && n < cap)) break;
+                }
+                StepOut o;
+                compute_step<C, false>(P, uv, dt, x, y, rng, o);
+                idx = o.idx;
+                if (mask[idx]) continue;
+                const float err = step_error<C>(P, o);
+                const float dl = map_move<C>(P, o, x, y, idx);
+                const bool stuck = dl < P.integration_halt_threshold;
+                if (!stuck) dt = next_dt(P, dt, err);
+                if (pass == 1) {
+                    // replay record n: the flag loop's step n
+                    if (mask[ridx] || n >= n_total - 1u) return;
+                    if (mapping[ridx] == 0u) atomicOr(&mapping[ridx], F_INTER);
+                    rx = rx + fdiv((float)compress1(x - px, res), res);
+                    ry = ry + fdiv((float)compress1(y - py, res), res);
+                    ridx = array_idx(P, rx, ry);
+                }
+                n++;
+                if (!stuck) {
+                    px = x;
+                    py = y;
+                }
+            }
+        }
+        n_total = n;
+    }
+}
+
+// ----------------------------------------------------------------- thinning
+
+__global__ void k_zs_init(uint32_t nxy, const uint32_t* __restrict__ mapping, uint8_t* __restrict__ a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy) a[i] = (mapping[i] & (F_CHANNEL | F_INTER)) ? 1 : 0;
+}
+
+// One Zhang-Suen subiteration a -> b; *changed += pixels removed.
+__global__ void k_zs_sub(uint32_t nx, uint32_t ny, int sub, const uint8_t* __restrict__ a,
+                         uint8_t* __restrict__ b, uint32_t* __restrict__ changed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool del = false;
+    if (i < nx * ny) {
+        const uint8_t v = a[i];
+        uint8_t out = v;
+        if (v) {
+            const int x = (int)(i / ny), y = (int)(i % ny);
+            auto g = [&](int xx, int yy) -> int {
+                return (xx < 0 || yy < 0 || xx >= (int)nx || yy >= (int)ny) ? 0 : a[(size_t)xx * ny + yy];
+            };
+            // P2..P9 = N, NE, E, SE, S, SW, W, NW (N = row x-1)
+            const int q[8] = {g(x - 1, y), g(x - 1, y + 1), g(x, y + 1), g(x + 1, y + 1),
+                              g(x + 1, y), g(x + 1, y - 1), g(x, y - 1), g(x - 1, y - 1)};
+            int B = 0, A = 0;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                B += q[j];
+                A += (q[j] == 0 && q[(j + 1) & 7] == 1);
+            }
+            const int c1 = sub == 0 ? q[0] * q[2] * q[4] : q[0] * q[2] * q[6];
+            const int c2 = sub == 0 ? q[2] * q[4] * q[6] : q[0] * q[4] * q[6];
+            if (B >= 2 && B <= 6 && A == 1 && c1 == 0 && c2 == 0) {
+                out = 0;
+                del = true;
+            }
+        }
+        b[i] = out;
+    }
+    const u64 m = __ballot(del);
+    if (m && (threadIdx.x & 63) == 0) atomicAdd(changed, (uint32_t)__popcll(m));
+}
+
+__global__ void k_zs_final(uint32_t nxy, const uint8_t* __restrict__ a, uint32_t* __restrict__ mapping) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy && a[i]) mapping[i] |= F_THIN;
+}
+
+// ---------------------------------------------------------- map_channel_heads
+
+// map_channel_heads (channelheads.cl:32-93) with channelheads_runge_kutta_step
+// (rungekutta.cl:335-353: no stuck exit, dt always updated).
+template <bool C>
+__global__ void k_heads(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                        const float2* __restrict__ uv, uint32_t* __restrict__ mapping) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t n = 0, rng = 0, idx = array_idx(P, x, y);
+    const uint32_t prev_idx = idx;
+    while (prev_idx == idx && !mask[idx] && n < P.max_n_steps) {
+        StepOut o;
+        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        idx = o.idx;
+        const float err = step_error<C>(P, o);
+        map_move<C>(P, o, x, y, idx);
+        n++;
+        dt = next_dt(P, dt, err);
+    }
+    if (n >= P.max_n_steps) return;
+    if (!mask[idx]) {
+        if (((~mapping[idx]) & F_THIN) || (mapping[prev_idx] & F_THIN)) atomicAnd(&mapping[idx], ~F_HEAD);
+    }
+}
+
+// --------------------------------------------------------- count_downchannels
+
+// countlink_runge_kutta_step (rungekutta.cl:382-406): true when stuck.
+template <bool C>
+__device__ __forceinline__ bool countlink_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
+                                               float& x, float& y, uint32_t& idx) {
+    StepOut o;
+    uint32_t rng = 0;
+    compute_step<C, false>(P, uv, dt, x, y, rng, o);
+    idx = o.idx;
+    const float err = step_error<C>(P, o);
+    const float dl = map_move<C>(P, o, x, y, idx);
+    if (dl < P.integration_halt_threshold) return true;
+    dt = next_dt(P, dt, err);
+    return false;
+}
+
+// count_downchannels (countlink.cl:34-114).  PASS 0: full traces rank each
+// pixel by max (counter, seed) in keyA.  PASS 1: traces stop on entering a
+// pixel ranked higher by another trace, flag it, count = max, and offer
+// link keys (counter at prev, next pixel) into keyL.
+template <bool C, int PASS>
+__global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                        const float2* __restrict__ uv, uint32_t* __restrict__ mapping,
+                        uint32_t* __restrict__ count, u64* __restrict__ keyA, u64* __restrict__ keyL) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t cap = 1000u * P.max_n_steps;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t idx = array_idx(P, x, y), prev = idx, n = 0, counter = 2, cprev = 1;
+    if (PASS == 0)
+        atomicMax(&keyA[idx], key2(1u, k));
+    else
+        atomicMax(&count[idx], 1u);
+    while (!mask[idx] && n < cap) {
+        if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
+        n++;
+        if (prev != idx) {
+            ++counter;
+            if (PASS == 0) {
+                atomicMax(&keyA[idx], key2(counter, k));
+            } else {
+                atomicAnd(&mapping[idx], ~F_HEAD);
+                atomicOr(&mapping[idx], F_THIN);
+                atomicMax(&keyL[prev], key2(cprev, idx));
+                const u64 a = keyA[idx];
+                if (counter < (uint32_t)(a >> 32) && (uint32_t)a != k) break;
+                atomicMax(&count[idx], counter);
+                cprev = counter;
+            }
+            prev = idx;
+        }
+    }
+    if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
+}
+
+__global__ void k_link_final(uint32_t nxy, const u64* __restrict__ keyL, uint32_t* __restrict__ link) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy) {
+        const u64 v = keyL[i];
+        if (v) link[i] = (uint32_t)v;
+    }
+}
+
+// ---------------------------------------------------------- flag_downchannels
+
+__global__ void k_flag_init(SlmParams P, const float2* __restrict__ seeds, uint32_t* __restrict__ mapping,
+                            uint32_t* __restrict__ count) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t h = seed_pixel(P, seeds, k);
+    count[h] = 1u;
+    atomicOr(&mapping[h], F_THIN);
+}
+
+// flag_downchannels (countlink.cl:134-210).  PHASE 2: walks (flags, count
+// max, tail).  PHASE 3: the head test on the final counts.
+template <int PHASE>
+__global__ void k_flag_walk(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                            uint32_t* __restrict__ mapping, uint32_t* __restrict__ count,
+                            const uint32_t* __restrict__ link) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t cap = 1000u * P.max_n_steps;
+    const uint32_t h = seed_pixel(P, seeds, k);
+    uint32_t idx = h, prev = h + 1u, counter = 1;
+    while (!mask[idx] && prev != idx && counter < cap) {
+        prev = idx;
+        idx = lk(P, link, idx);
+        counter++;
+        if (mask[idx]) break;
+        if (PHASE == 3) {
+            if (counter >= 20u) break;
+            if ((mapping[idx] & F_MININ) || (counter == 2u && count[idx] > 2u)) atomicAnd(&mapping[h], ~F_HEAD);
+        } else {
+            atomicOr(&mapping[idx], F_THIN);
+            atomicMax(&count[idx], counter);
+        }
+    }
+    if (PHASE == 2 && prev < P.nxy_padded && !mask[prev] && counter < cap) atomicOr(&mapping[prev], F_TAIL);
+}
+
+// ---------------------------------------------------------- label_confluences
+
+// label_confluences (label.cl:21-134).
+__global__ void k_confluences(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                              const float* __restrict__ slt, uint32_t* __restrict__ mapping,
+                              const uint32_t* __restrict__ count, const uint32_t* __restrict__ link) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    const uint32_t idx = array_idx(P, s.x, s.y);
+    // N, S, E, W, NE, SE, NW, SW
+    const float ox[8] = {0.f, 0.f, 1.f, -1.f, 1.f, 1.f, -1.f, -1.f};
+    const float oy[8] = {1.f, -1.f, 0.f, 0.f, 1.f, -1.f, 1.f, -1.f};
+    uint32_t in[8], eq[8], n_in = 0, n_eq = 0, dom_idx = 0;
+    float dom = -3.402823466e+38f;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const uint32_t nb = array_idx(P, s.x + ox[d], s.y + oy[d]);
+        if (!mask[nb] && (mapping[nb] & F_THIN) && link[nb] == idx) in[n_in++] = nb;
+    }
+    if (n_in > 1) {
+        atomicOr(&mapping[idx], F_MAJCONF);
+        const uint32_t ci = count[idx];
+        for (uint32_t i = 0; i < n_in; i++) {
+            if (ci != count[in[i]] + 1u) {
+                atomicOr(&mapping[in[i]], F_MININ);
+            } else {
+                eq[n_eq++] = in[i];
+                if (slt[in[i]] > dom) {
+                    dom_idx = in[i];
+                    dom = slt[dom_idx];
+                }
+            }
+        }
+        for (uint32_t i = 0; i < n_eq; i++) atomicOr(&mapping[eq[i]], eq[i] == dom_idx ? F_MAJIN : F_MININ);
+    }
+}
+
+// ------------------------------------------------------- segment_downchannels
+
+__device__ __forceinline__ u64 own_key(const uint32_t* __restrict__ mapping, const uint32_t* __restrict__ count,
+                                       uint32_t prev) {
+    return ((u64)count[prev] << 32) | ((mapping[prev] & F_MAJIN) ? 0x80000000ull : 0ull) |
+           (u64)(prev & 0x7fffffffu);
+}
+
+__global__ void k_seg_heads(SlmParams P, const float2* __restrict__ seeds, uint32_t* __restrict__ mapping,
+                            uint8_t* __restrict__ mark) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t h = seed_pixel(P, seeds, k);
+    atomicOr(&mapping[h], F_SSH);
+    mark[h] = 1;
+}
+
+// Full walks: own[q] = max own_key(prev) over hops prev -> q.
+__global__ void k_seg_own(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                          const uint32_t* __restrict__ mapping, const uint32_t* __restrict__ count,
+                          const uint32_t* __restrict__ link, u64* __restrict__ own) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    uint32_t prev = seed_pixel(P, seeds, k), idx = lk(P, link, prev);
+    for (uint32_t hop = 0; hop < P.nxy_padded && !mask[idx] && prev != idx && !(mapping[idx] & F_SSH); hop++) {
+        atomicMax(&own[idx], own_key(mapping, count, prev));
+        prev = idx;
+        idx = lk(P, link, idx);
+    }
+}
+
+// PHASE 2: owned walks mark segment starts (also where they join another
+// walk's reach, then stop).  PHASE 3: owned walks label.
+template <int PHASE>
+__global__ void k_seg_walk(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                           const uint32_t* __restrict__ mapping, const uint32_t* __restrict__ count,
+                           const uint32_t* __restrict__ link, const u64* __restrict__ own,
+                           uint8_t* __restrict__ mark, uint32_t* __restrict__ label) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t T = P.segmentation_threshold;
+    const uint32_t h = seed_pixel(P, seeds, k);
+    uint32_t prev = h, idx = lk(P, link, h), seg_label = h, segc = 0;
+    if (PHASE == 3) atomicMax(&label[h], h);
+    for (uint32_t hop = 0; hop < P.nxy_padded && !mask[idx] && prev != idx && !(mapping[idx] & F_SSH); hop++) {
+        const bool owned = own[idx] == own_key(mapping, count, prev);
+        if (PHASE == 2) {
+            const uint32_t c = count[idx];
+            if (c >= segc + T / 2u) {
+                mark[idx] = 1;
+                segc = c + T;
+            }
+            if (!owned) break;
+        } else {
+            if (!owned) break;
+            if (mark[idx]) seg_label = idx;
+            atomicMax(&label[idx], seg_label);
+        }
+        prev = idx;
+        idx = lk(P, link, idx);
+    }
+}
+
+__global__ void k_seg_final(uint32_t nxy, const uint8_t* __restrict__ mark, uint32_t* __restrict__ mapping) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy && mark[i]) mapping[i] |= F_SSH;
+}
+
+// ------------------------------------------------------------ link_hillslopes
+
+// hillslopes (hillslopes.cl:31-91).
+template <bool C>
+__global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                          const float2* __restrict__ uv, uint32_t* __restrict__ link) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t cap = 100u * P.max_n_steps;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t idx = array_idx(P, x, y), prev = idx, n = 0;
+    while (prev == idx && !mask[idx] && n < cap) {
+        prev = idx;
+        if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
+        n++;
+    }
+    if (prev != idx && !mask[idx]) link[prev] = idx;
+}
+
+// ------------------------------------------------------ hillslope segmentation
+
+// segment_runge_kutta_step (rungekutta.cl:435-459); probe = the pixel of
+// compute_step_vec's next point, before the dxy1 retry.
+template <bool C>
+__device__ __forceinline__ bool segment_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
+                                             float& x, float& y, uint32_t& idx, uint32_t& n, uint32_t& probe) {
+    StepOut o;
+    uint32_t rng = 0;
+    compute_step<C, false>(P, uv, dt, x, y, rng, o);
+    probe = o.idx;
+    const float err = step_error<C>(P, o);
+    const float dl = map_move<C>(P, o, x, y, idx);
+    n += 1u;
+    if (dl < P.integration_halt_threshold) return true;
+    dt = next_dt(P, dt, err);
+    return false;
+}
+
+// segment_hillslopes (segment.cl:133-188).
+template <bool C>
+__global__ void k_seg_hs(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                         const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
+                         uint32_t* __restrict__ label) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t n = 0, probe, idx = array_idx(P, x, y);
+    const uint32_t hs = idx;
+    while (!mask[idx] && ((~mapping[idx]) & F_THIN) && n < P.max_n_steps)
+        if (segment_step<C>(P, uv, dt, x, y, idx, n, probe)) break;
+    if (mapping[idx] & F_THIN) label[hs] = label[idx];
+}
+
+// rotate_left (segment.cl:195-200) in char arithmetic (abs of a char is a
+// uchar); the index is computed in uint arithmetic like the reference's.
+__device__ __forceinline__ uint32_t rotate_left(const SlmParams& P, uint32_t px, uint32_t py, int8_t& dx,
+                                                int8_t& dy) {
+    const int8_t rx = (int8_t)(dx - dy), ry = (int8_t)(dx + dy);
+    int ax = (int8_t)(uint8_t)(rx < 0 ? -rx : rx), ay = (int8_t)(uint8_t)(ry < 0 ? -ry : ry);
+    ax = ax < 1 ? 1 : (ax > 2 ? 2 : ax);
+    ay = ay < 1 ? 1 : (ay > 2 ? 2 : ay);
+    dx = (int8_t)(rx / ax);
+    dy = (int8_t)(ry / ay);
+    return (px + (uint32_t)(int32_t)dx) * P.ny_padded + (py + (uint32_t)(int32_t)dy);
+}
+
+// subsegment_channel_edges (segment.cl:226-334); an index outside the grid
+// reads as masked.
+__global__ void k_sub_edges(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                            uint32_t* __restrict__ mapping, const uint32_t* __restrict__ link,
+                            uint32_t* __restrict__ label) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const uint32_t nyp = P.ny_padded, nxy = P.nxy_padded;
+    uint32_t prev = seed_pixel(P, seeds, k), idx = lk(P, link, prev), n = 0;
+    while (prev != idx && n++ < P.max_n_steps) {
+        const uint32_t px = prev / nyp, py = prev % nyp, x = idx / nyp, y = idx % nyp;
+        int8_t dx = (int8_t)(x - px), dy = (int8_t)(y - py);
+        uint32_t left = idx, turns = 0;
+        bool do_left = true;
+        while (left != prev && ++turns <= 7) {
+            left = rotate_left(P, px, py, dx, dy);
+            if (turns >= 2 && left < nxy && !mask[left] && (mapping[left] & F_THIN)) {
+                if (left == lk(P, link, idx)) do_left = false;
+                break;
+            }
+        }
+        if (do_left) {
+            turns = 0;
+            left = idx;
+            dx = (int8_t)(x - px);
+            dy = (int8_t)(y - py);
+            while (left != prev && ++turns <= 4) {
+                left = rotate_left(P, px, py, dx, dy);
+                if (turns >= 2 && left < nxy && !mask[left]) {
+                    if (mapping[left] & F_THIN) break;
+                    atomicOr(&mapping[left], F_LEFT);
+                    atomicOr(&label[left], LFA);
+                }
+            }
+        }
+        prev = idx;
+        idx = lk(P, link, idx);
+        if (mask[idx] || (mapping[prev] & F_SSH)) break;
+    }
+}
+
+// subsegment_flanks (segment.cl:360-414).
+template <bool C>
+__global__ void k_sub_flanks(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                             const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
+                             uint32_t* __restrict__ label) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t n = 0, probe, idx = array_idx(P, x, y);
+    const uint32_t hs = idx;
+    while (!mask[idx] && !(mapping[idx] & F_LEFT) && !(mapping[idx] & F_THIN) && n < P.max_n_steps)
+        if (segment_step<C>(P, uv, dt, x, y, idx, n, probe)) break;
+    if (mapping[idx] & F_LEFT) label[hs] |= LFA;
+}
+
+// fix_right_flanks (segment.cl:441-504) / fix_left_flanks (:530-594); other
+// pixels' labels are read from the snapshot taken before the launch.
+template <bool C, bool LEFT>
+__global__ void k_fix(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                      const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
+                      const uint32_t* __restrict__ snap, uint32_t* __restrict__ label) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t n = 0, idx = array_idx(P, x, y);
+    const uint32_t hs = idx;
+    uint32_t own = snap[hs];
+    auto lab = [&](uint32_t q) -> uint32_t { return q == hs ? own : snap[q]; };
+    for (;;) {
+        const bool side = LEFT ? ((lab(idx) & LFA) != 0u) : ((lab(idx) & LFA) == 0u);
+        if (!(!mask[idx] && side && !(mapping[idx] & F_THIN) && n < P.max_n_steps)) break;
+        uint32_t probe;
+        const bool stuck = segment_step<C>(P, uv, dt, x, y, idx, n, probe);
+        if ((own & ~LFA) != (lab(probe) & ~LFA)) own = lab(probe);
+        if (stuck) break;
+    }
+    if ((own & ~LFA) != (lab(idx) & ~LFA)) own = lab(idx);
+    if (LEFT) {
+        if (!(lab(idx) & LFA) && !(mapping[idx] & F_THIN)) own &= ~LFA;
+    } else {
+        if (lab(idx) & LFA) own |= LFA;
+    }
+    label[hs] = own;
+}
+
+// ------------------------------------------------------------------ host side
+
+int check_grid(const SlmParams* p) {
+    int rc = slm_check_params(p);
+    if (rc) return rc;
+    if (p->nxy_padded >= 0x80000000u) return slm_fail(SLM_E_INVALID_ARG, "nxy_padded must be below 2^31");
+    return SLM_OK;
+}
+
+int check_map(const SlmParams* p, const void* seeds) {
+    int rc = check_grid(p);
+    if (rc) return rc;
+    if (p->n_seed_points > p->seeds_chunk_offset && !seeds) return slm_fail(SLM_E_INVALID_ARG, "seeds is NULL");
+    return SLM_OK;
+}
+
+size_t map_ws_bytes(const SlmParams* p) { return (size_t)16 * p->nxy_padded + 256; }
+
+int check_ws(const SlmParams* p, void* ws, size_t bytes) {
+    if (!ws) return slm_fail(SLM_E_WORKSPACE, "workspace is NULL");
+    if (bytes < map_ws_bytes(p))
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes < %zu", bytes, map_ws_bytes(p));
+    return SLM_OK;
+}
+
+inline dim3 seed_grid(const SlmParams* p) {
+    return dim3((p->n_seed_points - p->seeds_chunk_offset + kBlock - 1) / kBlock);
+}
+inline dim3 pixel_grid(const SlmParams* p) { return dim3((p->nxy_padded + kBlock - 1) / kBlock); }
+inline uint32_t n_work(const SlmParams* p) { return p->n_seed_points - p->seeds_chunk_offset; }
+
+inline void zero_u64(u64* a, size_t n, hipStream_t s) {
+    const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_zero_u64, dim3((uint32_t)std::max<size_t>(blocks, 1)), dim3(kBlock), 0, s, a, n);
+}
+
+// Workspace views.
+inline u64* ws_a(void* ws) { return (u64*)ws; }
+inline u64* ws_b(void* ws, const SlmParams* p) { return (u64*)ws + p->nxy_padded; }
+inline uint32_t* ws_ctr(void* ws, const SlmParams* p) { return (uint32_t*)((u64*)ws + 2 * (size_t)p->nxy_padded); }
+
+}  // namespace
+
+extern "C" size_t slm_map_workspace_size(const SlmParams* p) {
+    if (slm_check_params(p)) return 0;
+    return map_ws_bytes(p);
+}
+
+#define SEED_ARGS(p) (*p), (const float2*)seeds_xy
+#define LAUNCH_C(kern, grid, ...)                                                           \
+    do {                                                                                    \
+        if (p->fp_contract)                                                                 \
+            hipLaunchKernelGGL(kern<true>, grid, dim3(kBlock), 0, s, __VA_ARGS__);          \
+        else                                                                                \
+            hipLaunchKernelGGL(kern<false>, grid, dim3(kBlock), 0, s, __VA_ARGS__);         \
+    } while (0)
+
+extern "C" int slm_connect_channels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                    const float* uv, uint32_t* mapping, void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !uv || !mapping) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH_C(k_connect, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_thin_channels(const SlmParams* p, uint32_t* mapping, void* workspace, size_t workspace_bytes,
+                                 uint32_t* n_iter, void* stream, uint32_t flags) {
+    int rc = check_grid(p);
+    if (rc) return rc;
+    if ((rc = check_ws(p, workspace, workspace_bytes))) return rc;
+    if (!mapping) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t nxy = p->nxy_padded;
+    uint8_t* a = (uint8_t*)ws_a(workspace);
+    uint8_t* b = a + nxy;
+    uint32_t* ctr = ws_ctr(workspace, p);
+    hipLaunchKernelGGL(k_zs_init, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint32_t*)mapping, a);
+    uint32_t iters = 0;
+    const uint32_t max_iters = p->nx_padded + p->ny_padded + 2;
+    for (;;) {
+        zero_u64((u64*)ctr, 1, s);
+        hipLaunchKernelGGL(k_zs_sub, pixel_grid(p), dim3(kBlock), 0, s, p->nx_padded, p->ny_padded, 0,
+                           (const uint8_t*)a, b, ctr);
+        hipLaunchKernelGGL(k_zs_sub, pixel_grid(p), dim3(kBlock), 0, s, p->nx_padded, p->ny_padded, 1,
+                           (const uint8_t*)b, a, ctr);
+        uint32_t changed = 0;
+        HIP_TRY(hipMemcpyAsync(&changed, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        iters++;
+        if (!changed) break;
+        if (iters >= max_iters) return slm_fail(SLM_E_HIP, "thinning did not converge in %u iterations", iters);
+    }
+    hipLaunchKernelGGL(k_zs_final, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint8_t*)a, mapping);
+    if (n_iter) *n_iter = iters;
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_map_channel_heads(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                     const float* uv, uint32_t* mapping, void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !uv || !mapping) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH_C(k_heads, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_count_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                      const float* uv, uint32_t* mapping, uint32_t* count, uint32_t* link,
+                                      void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if ((rc = check_ws(p, workspace, workspace_bytes))) return rc;
+    if (!mask || !uv || !mapping || !count || !link) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    u64* keyA = ws_a(workspace);
+    u64* keyL = ws_b(workspace, p);
+    zero_u64(keyA, (size_t)2 * p->nxy_padded, s);
+    const dim3 g = seed_grid(p);
+    if (p->fp_contract) {
+        hipLaunchKernelGGL((k_count<true, 0>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
+                           mapping, count, keyA, keyL);
+        hipLaunchKernelGGL((k_count<true, 1>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
+                           mapping, count, keyA, keyL);
+    } else {
+        hipLaunchKernelGGL((k_count<false, 0>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
+                           mapping, count, keyA, keyL);
+        hipLaunchKernelGGL((k_count<false, 1>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
+                           mapping, count, keyA, keyL);
+    }
+    hipLaunchKernelGGL(k_link_final, pixel_grid(p), dim3(kBlock), 0, s, p->nxy_padded, (const u64*)keyL, link);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_flag_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                     uint32_t* mapping, uint32_t* count, const uint32_t* link, void* stream,
+                                     uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !mapping || !count || !link) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 g = seed_grid(p);
+    hipLaunchKernelGGL(k_flag_init, g, dim3(kBlock), 0, s, SEED_ARGS(p), mapping, count);
+    hipLaunchKernelGGL(k_flag_walk<2>, g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, mapping, count, link);
+    hipLaunchKernelGGL(k_flag_walk<3>, g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, mapping, count, link);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_label_confluences(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                     const float* slt, uint32_t* mapping, const uint32_t* count,
+                                     const uint32_t* link, void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !slt || !mapping || !count || !link) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_confluences, seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask, slt, mapping, count,
+                       link);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_segment_downchannels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                        uint32_t* mapping, const uint32_t* count, const uint32_t* link,
+                                        uint32_t* label, void* workspace, size_t workspace_bytes, void* stream,
+                                        uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if ((rc = check_ws(p, workspace, workspace_bytes))) return rc;
+    if (!mask || !mapping || !count || !link || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    u64* own = ws_a(workspace);
+    uint8_t* mark = (uint8_t*)ws_b(workspace, p);
+    zero_u64(own, (size_t)p->nxy_padded + ((size_t)p->nxy_padded + 7) / 8, s);
+    const dim3 g = seed_grid(p);
+    hipLaunchKernelGGL(k_seg_heads, g, dim3(kBlock), 0, s, SEED_ARGS(p), mapping, mark);
+    hipLaunchKernelGGL(k_seg_own, g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const uint32_t*)mapping, count, link,
+                       own);
+    hipLaunchKernelGGL(k_seg_walk<2>, g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const uint32_t*)mapping, count,
+                       link, (const u64*)own, mark, label);
+    hipLaunchKernelGGL(k_seg_walk<3>, g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const uint32_t*)mapping, count,
+                       link, (const u64*)own, mark, label);
+    hipLaunchKernelGGL(k_seg_final, pixel_grid(p), dim3(kBlock), 0, s, p->nxy_padded, (const uint8_t*)mark,
+                       mapping);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_link_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                   const float* uv, uint32_t* link, void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !uv || !link) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH_C(k_link_hs, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, link);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                      const float* uv, const uint32_t* mapping, uint32_t* label, void* stream,
+                                      uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH_C(k_seg_hs, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping, label);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_subsegment_channel_edges(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                            uint32_t* mapping, const uint32_t* link, uint32_t* label,
+                                            void* stream, uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !mapping || !link || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_sub_edges, seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask, mapping, link, label);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_subsegment_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                     const float* uv, const uint32_t* mapping, uint32_t* label, void* stream,
+                                     uint32_t flags) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    LAUNCH_C(k_sub_flanks, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping, label);
+    return slm_finish(stream, flags);
+}
+
+static int fix_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask, const float* uv,
+                      const uint32_t* mapping, uint32_t* label, void* workspace, size_t workspace_bytes,
+                      void* stream, uint32_t flags, bool left) {
+    int rc = check_map(p, seeds_xy);
+    if (rc) return rc;
+    if (!n_work(p)) return slm_finish(stream, flags);
+    if ((rc = check_ws(p, workspace, workspace_bytes))) return rc;
+    if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* snap = (uint32_t*)ws_a(workspace);
+    HIP_TRY(hipMemcpyAsync(snap, label, (size_t)4 * p->nxy_padded, hipMemcpyDeviceToDevice, s));
+    const dim3 g = seed_grid(p);
+    const float2* sd = (const float2*)seeds_xy;
+    const float2* u = (const float2*)uv;
+    if (p->fp_contract) {
+        if (left)
+            hipLaunchKernelGGL((k_fix<true, true>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+        else
+            hipLaunchKernelGGL((k_fix<true, false>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+    } else {
+        if (left)
+            hipLaunchKernelGGL((k_fix<false, true>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+        else
+            hipLaunchKernelGGL((k_fix<false, false>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+    }
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_fix_right_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                    const float* uv, const uint32_t* mapping, uint32_t* label, void* workspace,
+                                    size_t workspace_bytes, void* stream, uint32_t flags) {
+    return fix_flanks(p, seeds_xy, mask, uv, mapping, label, workspace, workspace_bytes, stream, flags, false);
+}
+
+extern "C" int slm_fix_left_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                   const float* uv, const uint32_t* mapping, uint32_t* label, void* workspace,
+                                   size_t workspace_bytes, void* stream, uint32_t flags) {
+    return fix_flanks(p, seeds_xy, mask, uv, mapping, label, workspace, workspace_bytes, stream, flags, true);
+}

@@ -47,7 +47,7 @@ extern "C" int slm_device_count(void) {
     return n;
 }
 
-static int check_params(const SlmParams* p) {
+int slm_check_params(const SlmParams* p) {
     if (!p) return slm_fail(SLM_E_INVALID_ARG, "params is NULL");
     if (p->abi_version != SLM_ABI_VERSION)
         return slm_fail(SLM_E_ABI, "SlmParams.abi_version %u != %u", p->abi_version, SLM_ABI_VERSION);
@@ -64,6 +64,7 @@ static int check_params(const SlmParams* p) {
                     p->n_seed_points);
     return SLM_OK;
 }
+static inline int check_params(const SlmParams* p) { return slm_check_params(p); }
 
 
 struct DevInfo {

@@ -45,7 +45,29 @@ _PROTOS = {
                                              _U32]),
     "slm_kde_histogram": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
+    "slm_map_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
+    "slm_thin_channels": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, ctypes.c_size_t,
+                                         ctypes.POINTER(_U32), _P, _U32]),
 }
+
+# Mapping entry points: C name -> (device array arguments in C order, takes a workspace)
+MAP_ARGS = {
+    "connect_channels": (("seeds", "mask", "uv", "mapping"), False),
+    "map_channel_heads": (("seeds", "mask", "uv", "mapping"), False),
+    "count_downchannels": (("seeds", "mask", "uv", "mapping", "count", "link"), True),
+    "flag_downchannels": (("seeds", "mask", "mapping", "count", "link"), False),
+    "label_confluences": (("seeds", "mask", "slt", "mapping", "count", "link"), False),
+    "segment_downchannels": (("seeds", "mask", "mapping", "count", "link", "label"), True),
+    "link_hillslopes": (("seeds", "mask", "uv", "link"), False),
+    "segment_hillslopes": (("seeds", "mask", "uv", "mapping", "label"), False),
+    "subsegment_channel_edges": (("seeds", "mask", "mapping", "link", "label"), False),
+    "subsegment_flanks": (("seeds", "mask", "uv", "mapping", "label"), False),
+    "fix_right_flanks": (("seeds", "mask", "uv", "mapping", "label"), True),
+    "fix_left_flanks": (("seeds", "mask", "uv", "mapping", "label"), True),
+}
+for _name, (_args, _ws) in MAP_ARGS.items():
+    _PROTOS["slm_" + _name] = (ctypes.c_int, [ctypes.POINTER(SlmParams)] + [_P] * len(_args) +
+                               ([_P, ctypes.c_size_t] if _ws else []) + [_P, _U32])
 
 _lib = None
 
@@ -188,6 +210,46 @@ def hillslope_lengths(p: SlmParams, seeds, mask, uv, mapping, thin_flag, from_fl
     check(load().slm_hillslope_lengths(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(mapping),
                                        int(thin_flag), int(from_flag), _ptr(length), _stream(stream),
                                        SLM_FLAG_SYNC if sync else 0))
+
+
+def map_workspace_size(p: SlmParams) -> int:
+    n = load().slm_map_workspace_size(ctypes.byref(p))
+    if n == 0:
+        check(-1)
+    return int(n)
+
+
+def map_call(name: str, p: SlmParams, workspace=None, stream=None, sync=False, **arrays):
+    """Call mapping entry point slm_<name> with device tensors passed by
+    keyword (names of MAP_ARGS: seeds float32 [N,2], mask uint8, uv float32,
+    slt float32, mapping/count/link/label int32 holding uint32 bits, all
+    [nxy_padded]); p.n_seed_points must equal the seed count."""
+    import torch
+    args, takes_ws = MAP_ARGS[name]
+    ts = [arrays[a] for a in args]
+    _require_cuda(*ts)
+    extra = []
+    if takes_ws:
+        need = map_workspace_size(p)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(need, dtype=torch.uint8, device=ts[-1].device)
+        extra = [_ptr(workspace), workspace.numel()]
+    check(getattr(load(), "slm_" + name)(ctypes.byref(p), *[_ptr(t) for t in ts], *extra, _stream(stream),
+                                         SLM_FLAG_SYNC if sync else 0))
+    return workspace
+
+
+def thin_channels(p: SlmParams, mapping, workspace=None, stream=None) -> int:
+    """Zhang-Suen stand-in for skeletonize(medial_axis(.)); returns iterations."""
+    import torch
+    _require_cuda(mapping, workspace)
+    need = map_workspace_size(p)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=mapping.device)
+    it = _U32(0)
+    check(load().slm_thin_channels(ctypes.byref(p), _ptr(mapping), _ptr(workspace), workspace.numel(),
+                                   ctypes.byref(it), _stream(stream), 0))
+    return int(it.value)
 
 
 def as_u32(t) -> np.ndarray:

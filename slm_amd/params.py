@@ -45,7 +45,9 @@ _FIELD_NAMES = [
     ("seeds_chunk_offset", ctypes.c_uint32),
     ("fp_contract", ctypes.c_uint32),
     ("pixel_size", ctypes.c_float),
-    ("reserved", ctypes.c_uint32 * 7),
+    ("interchannel_max_n_steps", ctypes.c_uint32),
+    ("segmentation_threshold", ctypes.c_uint32),
+    ("reserved", ctypes.c_uint32 * 5),
 ]
 
 

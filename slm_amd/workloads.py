@@ -125,3 +125,42 @@ def v_valley(n=96, pixel_size=1.0, side_slope=0.2, down_slope=0.05, midslope_off
         mapping[col(xm), :] |= np.uint32(job.info.is_midslope)
     mapping[job.mask] = 0
     return job, mapping
+
+
+def y_valley(n=64, pixel_size=1.0, side_slope=0.2, down_slope=0.05, junction=0.6, skew=0.6,
+             channel_width=1):
+    """A main V-valley along y at x = n//2 joined by a tributary V-valley whose
+    axis x = c + skew*(yj - y) meets it at y = yj = junction*n; h is the lower
+    of the two, so the tributary drains into the main valley.  NaN border.
+    Returns (job, mapping): mapping (uint32, cropped padded grid) flags
+    IS_CHANNEL on the pixels within channel_width//2 of either axis (the
+    tributary only upstream of the junction), as Mapping.map_channels would
+    on a well-resolved slt field."""
+    c = n // 2
+    yj = junction * n
+    x = np.arange(n, dtype=np.float64)[:, None]
+    y = np.arange(n, dtype=np.float64)[None, :]
+    h_main = side_slope * np.abs(x - c) - down_slope * y
+    xt = c + skew * (yj - y)
+    h_trib = np.where(y <= yj, side_slope * np.abs(x - xt) - down_slope * y, np.inf)
+    roi = (np.minimum(h_main, h_trib) * pixel_size).astype(np.float32)
+    roi[:2, :] = np.nan
+    roi[-2:, :] = np.nan
+    roi[:, :2] = np.nan
+    roi[:, -2:] = np.nan
+    job = job_from_dtm("y_valley_%d" % n, roi, pixel_size, {"subpixel_seed_point_density": 1,
+                                                            "jitter_magnitude": 0.0})
+    mapping = np.zeros(job.mask.shape, dtype=np.uint32)
+    ox, oy = job.origin
+    half = channel_width // 2
+    on_main = np.abs(x - c) <= half
+    on_trib = (np.abs(x - np.round(xt)) <= half) & (y <= yj)
+    chan = np.zeros((n, n), dtype=bool)
+    chan |= np.broadcast_to(on_main, (n, n))
+    chan |= on_trib
+    full = np.zeros((n + 4, n + 4), dtype=bool)
+    full[2:-2, 2:-2] = chan
+    nxp, nyp = job.mask.shape
+    mapping[full[ox:ox + nxp, oy:oy + nyp]] = np.uint32(job.info.is_channel)
+    mapping[job.mask] = 0
+    return job, mapping

@@ -1,0 +1,143 @@
+"""GPU parity of the channel mapping and segmentation chain (libslm_hip.so,
+slm_amd.mapping.ChannelMapper) against the CPU oracle, bit for bit.
+
+Each step of Mapping.map_channels_segments runs on the GPU; before the step
+the oracle is loaded with the GPU's current grids, runs the same step, and
+every grid (mapping, count, link, label) must match exactly.  The chain is
+also run end to end on the oracle alone and compared at the end."""
+import numpy as np
+import pytest
+
+from slm_amd import workloads
+
+from map_chain import OracleMapper, STEPS, channels_from_mapping
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_step(gm, name):
+    return {"connect": gm.connect_channel_pixels, "thin": gm.thin_channels, "heads": gm.map_channel_heads,
+            "count": gm.count_downchannels, "flag": gm.flag_downchannels, "confluences": gm.label_confluences,
+            "segment": gm.segment_downchannels, "link_hs": gm.link_hillslopes, "segment_hs": gm.segment_hillslopes,
+            "flanks": gm.subsegment_flanks}[name]()
+
+
+def _u32(t):
+    return None if t is None else t.detach().cpu().numpy().view(np.uint32).copy()
+
+
+def _case(kind):
+    if kind == "v":
+        job, m = workloads.v_valley(n=96)
+        m = channels_from_mapping(m, job.info)
+        slt = np.zeros(m.shape, np.float32)
+    elif kind == "y":
+        job, m = workloads.y_valley(n=128, channel_width=3)
+        slt = np.random.default_rng(1).random(m.shape).astype(np.float32)
+    else:
+        job, m, slt = _traced_case()
+    job.info.set_thresholds(segmentation_threshold=np.uint32(20))
+    return job, m, slt
+
+
+def _traced_case():
+    """small_256 DTM: downstream fields traced on the GPU, channels where
+    Mapping.map_channels' test holds for the 97th-percentile slt."""
+    import torch
+    from slm_amd import hip
+    from slm_amd.fields import postprocess_raw
+    job = workloads.small_256(jitter=1.0)
+    dev = torch.device("cuda", 0)
+    p = job.params(1)
+    seeds = torch.from_numpy(job.seeds).to(dev)
+    mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)
+    uv = torch.from_numpy(job.uv.reshape(-1)).to(dev)
+    slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
+    slt = torch.empty_like(slc)
+    hip.trace_fields(p, seeds, mask, uv, slc, slt, sync=True)
+    slc_raw = hip.as_u32(slc).reshape(job.mask.shape)[..., None].repeat(2, axis=2)
+    slt_raw = hip.as_u32(slt).astype(np.float32).reshape(job.mask.shape)[..., None].repeat(2, axis=2)
+    c, t, _ = postprocess_raw(slc_raw, slt_raw, 1)
+    t = t[..., 0]
+    thr = np.float32(np.quantile(t[~job.mask], 0.97))
+    job.info.set_thresholds(channel_threshold=thr)
+    m = np.zeros(job.mask.shape, np.uint32)
+    m[(t >= thr) & (t.astype(np.float64) * 2 >= c[..., 0])] = np.uint32(job.info.is_channel)
+    return job, m, t.astype(np.float32)
+
+
+def _load_oracle(om, gm):
+    om.mapping = _u32(gm.mapping).reshape(gm.mapping.shape)
+    for k in ("count", "link", "label", "channel_label"):
+        v = getattr(gm, k)
+        setattr(om, k, None if v is None else _u32(v).reshape(v.shape))
+
+
+@pytest.mark.parametrize("kind,contract", [("v", True), ("y", True), ("y", False), ("traced", True)])
+def test_chain_stepwise_bit_exact(gpu, kind, contract):
+    import torch
+    from slm_amd.mapping import ChannelMapper
+    job, m0, slt = _case(kind)
+    print("case", kind, "contract", contract, "grid", job.mask.shape, flush=True)
+    dev = gpu
+    gm = ChannelMapper(job.info, torch.from_numpy(job.mask).to(dev), torch.from_numpy(job.uv).to(dev),
+                       torch.from_numpy(m0.view(np.int32)).to(dev), slt=torch.from_numpy(slt).to(dev),
+                       fp_contract=contract)
+    om = OracleMapper(job.info, job.mask, job.uv, m0, slt=slt, fp_contract=contract)
+    solo = OracleMapper(job.info, job.mask, job.uv, m0, slt=slt, fp_contract=contract)
+    for name in STEPS:
+        print("step", name, flush=True)
+        _load_oracle(om, gm)
+        r_o = om.step(name)
+        r_g = _gpu_step(gm, name)
+        torch.cuda.synchronize()
+        assert bool(r_o) == bool(r_g), name
+        assert np.array_equal(_u32(gm.mapping).reshape(-1), om.mapping.reshape(-1)), (name, "mapping")
+        for k in ("count", "link", "label"):
+            if getattr(om, k) is not None:
+                assert np.array_equal(_u32(getattr(gm, k)).reshape(-1), getattr(om, k).reshape(-1)), (name, k)
+        if name == "segment":
+            assert gm.n_segments == om.n_segments
+        solo.step(name)
+    # the oracle alone, end to end, lands on the same grids
+    assert np.array_equal(_u32(gm.mapping).reshape(-1), solo.mapping.reshape(-1))
+    assert np.array_equal(_u32(gm.label).reshape(-1), solo.label.reshape(-1))
+    assert np.array_equal(_u32(gm.link).reshape(-1), solo.link.reshape(-1))
+    assert gm.n_segments == solo.n_segments and gm.n_segments > 0
+
+
+def test_thinning_bit_exact(gpu):
+    import torch
+    from scipy import ndimage
+    from oracle import oracle
+    from slm_amd import hip
+    rs = np.random.default_rng(7)
+    img = ndimage.binary_dilation(rs.random((300, 420)) > 0.985, iterations=4)
+    p = workloads.planar(n=8).params()
+    p.nx_padded, p.ny_padded, p.nxy_padded = img.shape[0], img.shape[1], img.size
+    m = np.where(img, np.uint32(1), np.uint32(0)).astype(np.uint32).reshape(-1)
+    it_o = oracle.map_call("thin_channels", p, mapping=m)
+    mg = torch.from_numpy(np.where(img, 1, 0).astype(np.int32).reshape(-1)).to(gpu)
+    it_g = hip.thin_channels(p, mg)
+    torch.cuda.synchronize()
+    assert it_g == it_o
+    assert np.array_equal(hip.as_u32(mg), m)
+
+
+def test_mapping_entry_points_reject_bad_input(gpu):
+    import ctypes
+    import torch
+    from slm_amd import hip
+    job, m0, _ = _case("v")
+    p = job.params(1)
+    p.n_seed_points = 4
+    seeds = torch.zeros((4, 2), dtype=torch.float32, device=gpu)
+    mask = torch.zeros(p.nxy_padded, dtype=torch.uint8, device=gpu)
+    mp = torch.zeros(p.nxy_padded, dtype=torch.int32, device=gpu)
+    small = torch.empty(16, dtype=torch.uint8, device=gpu)
+    lib = hip.load()
+    rc = lib.slm_count_downchannels(ctypes.byref(p), hip._ptr(seeds), hip._ptr(mask), hip._ptr(mp), hip._ptr(mp),
+                                    hip._ptr(mp), hip._ptr(mp), hip._ptr(small), 16, hip._stream(None), 0)
+    assert rc == -5                                       # SLM_E_WORKSPACE
+    p.n_seed_points = 0                                  # empty seed list: nothing to do, no error
+    assert lib.slm_connect_channels(ctypes.byref(p), None, None, None, None, hip._stream(None), 1) == 0
