@@ -246,6 +246,43 @@ int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8
                           const float* uv, const uint32_t* mapping, uint32_t thin_flag,
                           uint32_t from_flag, float* length, void* stream, uint32_t flags);
 
+/* ------------------------------------------------- grids either side of the tracer */
+
+/*
+ * The tracer's input: unit flow vectors and the mask from a DTM.
+ * Replaces the host numpy of python/streamlines/preprocess.py:51-62
+ * (np.gradient / pixel_size), :100-193 (u,v = -grad; speed = hypot, non-finite
+ * or 0 -> 1; normalise), :584-600 (normalise again when do_normalize_speed;
+ * pad 'edge'), :602-621 (NaN uv -> masked, uv = 0), merged with the DTM mask
+ * (NaN -> masked, padding masked).  Conditioning of loops and blockages
+ * (preprocess.py:195-473) is not applied.
+ *   roi  : float [nx][ny] (NaN = no data), nx, ny >= 2
+ *   uv   : float2 [nx+2pad][ny+2pad]     mask : uint8 [nx+2pad][ny+2pad]
+ */
+int slm_gradient_uv(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size,
+                    int do_normalize_speed, float* uv, uint8_t* mask, void* stream, uint32_t flags);
+
+/*
+ * Finishing of the raw fields (fields.py:296-309): from the per-direction raw
+ * sums of slm_trace_fields, slc_raw / slt_raw uint32 [2][nxy] (direction k at
+ * k*nxy), write slt float [nxy][2] = slt_raw / d^2 and sla float [nxy][2] =
+ * sqrt(2) slt_raw / slc_raw where slc_raw > 0 (slt = slt_raw, sla = 0
+ * elsewhere), with the reference's float32 / float64 roundings.
+ */
+int slm_fields_finish_raw(const uint32_t* slc_raw, const uint32_t* slt_raw, uint32_t nxy,
+                          uint32_t density, float* slt, float* sla, void* stream, uint32_t flags);
+
+/*
+ * Scaling of the fields (fields.py:141-164): slt [nxy][2] (from
+ * slm_fields_finish_raw) *= ds_scale_k, then sqrt, in float32, where
+ * ds_scale_k = float32(mean ds_k / pixel_size) per direction; slc [nxy][2]
+ * = sqrt(2) (slc_raw / d^2)^(2/3) in float64 (the device pow may differ from
+ * the host libm pow in the last place).
+ */
+int slm_fields_finish_scaled(const uint32_t* slc_raw, uint32_t nxy, uint32_t density,
+                             float ds_scale_down, float ds_scale_up, float* slt, double* slc,
+                             void* stream, uint32_t flags);
+
 /* ------------------------------------------------- channel mapping and segmentation */
 
 /*

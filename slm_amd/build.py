@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip"]
+HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip", "prep.hip"]
 
 
 def _run(cmd):

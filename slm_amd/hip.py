@@ -46,6 +46,10 @@ _PROTOS = {
     "slm_kde_histogram": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_map_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
+    "slm_gradient_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, _P, _P, _P, _U32]),
+    "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),
+    "slm_fields_finish_scaled": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _P, _P, _P,
+                                                _U32]),
     "slm_thin_channels": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, ctypes.c_size_t,
                                          ctypes.POINTER(_U32), _P, _U32]),
 }
@@ -250,6 +254,47 @@ def thin_channels(p: SlmParams, mapping, workspace=None, stream=None) -> int:
     check(load().slm_thin_channels(ctypes.byref(p), _ptr(mapping), _ptr(workspace), workspace.numel(),
                                    ctypes.byref(it), _stream(stream), 0))
     return int(it.value)
+
+
+def gradient_uv(roi, pad: int, pixel_size: float, do_normalize_speed=True, stream=None, sync=False):
+    """slm_gradient_uv: roi float32 [nx, ny] device tensor -> (uv float32
+    [nx+2pad, ny+2pad, 2], mask bool [nx+2pad, ny+2pad])."""
+    import torch
+    _require_cuda(roi)
+    nx, ny = roi.shape
+    nxp, nyp = nx + 2 * pad, ny + 2 * pad
+    uv = torch.empty((nxp, nyp, 2), dtype=torch.float32, device=roi.device)
+    mask = torch.empty((nxp, nyp), dtype=torch.uint8, device=roi.device)
+    check(load().slm_gradient_uv(_ptr(roi), nx, ny, pad, float(np.float32(pixel_size)),
+                                 1 if do_normalize_speed else 0, _ptr(uv), _ptr(mask), _stream(stream),
+                                 SLM_FLAG_SYNC if sync else 0))
+    return uv, mask.to(torch.bool)
+
+
+def fields_finish_raw(slc_raw, slt_raw, density: int, stream=None, sync=False):
+    """slm_fields_finish_raw: slc_raw / slt_raw int32 [2, nxy] (uint32 bits)
+    -> (slt, sla) float32 device tensors [nxy, 2]."""
+    import torch
+    _require_cuda(slc_raw, slt_raw)
+    nxy = slc_raw.shape[1]
+    slt = torch.empty((nxy, 2), dtype=torch.float32, device=slc_raw.device)
+    sla = torch.empty_like(slt)
+    check(load().slm_fields_finish_raw(_ptr(slc_raw), _ptr(slt_raw), nxy, int(density), _ptr(slt), _ptr(sla),
+                                       _stream(stream), SLM_FLAG_SYNC if sync else 0))
+    return slt, sla
+
+
+def fields_finish_scaled(slc_raw, slt, density: int, ds_scale_down, ds_scale_up, stream=None, sync=False):
+    """slm_fields_finish_scaled: slt float32 [nxy, 2] scaled and square-rooted
+    in place; returns slc float64 [nxy, 2]."""
+    import torch
+    _require_cuda(slc_raw, slt)
+    nxy = slc_raw.shape[1]
+    slc = torch.empty((nxy, 2), dtype=torch.float64, device=slc_raw.device)
+    check(load().slm_fields_finish_scaled(_ptr(slc_raw), nxy, int(density), float(np.float32(ds_scale_down)),
+                                          float(np.float32(ds_scale_up)), _ptr(slt), _ptr(slc), _stream(stream),
+                                          SLM_FLAG_SYNC if sync else 0))
+    return slc
 
 
 def as_u32(t) -> np.ndarray:

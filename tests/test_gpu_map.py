@@ -45,7 +45,7 @@ def _traced_case():
     Mapping.map_channels' test holds for the 97th-percentile slt."""
     import torch
     from slm_amd import hip
-    from slm_amd.fields import postprocess_raw
+    from oracle.postprocess_ref import postprocess_raw
     job = workloads.small_256(jitter=1.0)
     dev = torch.device("cuda", 0)
     p = job.params(1)

@@ -105,7 +105,7 @@ def test_json_merge_by_key(tmp_path):
 
 
 def test_postprocess_dtypes():
-    from slm_amd.fields import postprocess_raw, postprocess_scaled
+    from oracle.postprocess_ref import postprocess_raw, postprocess_scaled
     slc = np.array([[[0, 3], [7, 1]]], dtype=np.uint32)
     slt_u = np.array([[[0, 10], [16777217, 1]]], dtype=np.uint32)   # 2^24+1 rounds in float32
     slc2, slt2, sla = postprocess_raw(slc, slt_u.astype(np.float32), 2)
@@ -149,3 +149,32 @@ def test_library_rejects_bad_params_without_gpu():
     assert lib.slm_trace_paths(ctypes.byref(p), None, None, None, None, None, None, None, 0) == -1
     assert b"nxy_padded" in lib.slm_last_error()
     assert lib.slm_trace_fields_workspace_size(ctypes.byref(p)) == 0
+
+
+def test_postprocess_known_answers():
+    """fields.py:296-309 and :141-164 by hand: slc 4, slt 8 (d=2) gives
+    sla = sqrt(2)*8/4, slt = 8/4 = 2, then slt*(0.5/1) = 1 -> sqrt 1 and
+    slc = sqrt(2)*(4/4)^(2/3) = sqrt(2); empty pixels stay 0."""
+    from oracle.postprocess_ref import postprocess_raw, postprocess_scaled
+    slc = np.array([[[4, 0]]], dtype=np.uint32)
+    slt = np.array([[[8.0, 0.0]]], dtype=np.float32)
+    c, t, a = postprocess_raw(slc, slt, 2)
+    assert a[0, 0, 0] == np.float32(np.float64(np.float32(np.sqrt(2.0)) * np.float32(8.0)) / 4.0)
+    assert t[0, 0, 0] == 2.0 and a[0, 0, 1] == 0.0 and t[0, 0, 1] == 0.0
+    c2, t2 = postprocess_scaled(c, t, 2, np.float32(0.5), np.float32(0.5), 1.0)
+    assert t2[0, 0, 0] == 1.0 and t2[0, 0, 1] == 0.0
+    assert c2[0, 0, 0] == np.sqrt(2.0) and c2[0, 0, 1] == 0.0
+
+
+def test_gradient_uv_planar_closed_form():
+    """A planar ramp h = -(a x + b y) p has grad = -(a, b) everywhere, so the
+    unit flow vector is (a, b)/|(a, b)| at every pixel, edges included."""
+    from slm_amd import preprocess
+    a, b, p = 0.75, 0.25, 2.0
+    x = np.arange(20, dtype=np.float32)[:, None]
+    y = np.arange(30, dtype=np.float32)[None, :]
+    roi = (-(a * x + b * y) * p).astype(np.float32)
+    uv, m = preprocess.gradient_uv(roi, p, 2)
+    assert not m.any()
+    n = np.hypot(a, b)
+    assert np.allclose(uv[..., 0], a / n, atol=1e-6) and np.allclose(uv[..., 1], b / n, atol=1e-6)
