@@ -3,8 +3,8 @@
 One step = the full fields pass of the tracing path: integrate_fields
 downstream and upstream over every seed (2x2 sub-pixel seeds per pixel,
 jitter 1.0), i.e. both slc/slt maps of the 2048^2 tile at 2 m.  Inputs (uv,
-mask, seeds) are resident in HBM before timing starts; each step clears and
-re-accumulates the fields.
+mask, seeds) are resident in HBM before timing starts; each step sorts the
+seeds into tile order, clears and re-accumulates the fields.
 
 metric: streamline-steps/s = accepted RK2 steps of all streamlines, both
 directions, counted exactly by the kernel, divided by wall time of the timed
@@ -132,16 +132,16 @@ def main():
             if events is not None:
                 events[i][1].record(stream)
 
-    one_step(False)  # builds the seed order
+    one_step(False)
     for _ in range(args.warmup):
-        one_step(True)
+        one_step(False)
     torch.cuda.synchronize()
     steps.zero_()
     ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(2)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        one_step(True, ev[k])
+        one_step(False, ev[k])  # each step re-sorts the seeds (downstream) and reuses them upstream
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_steps = int(steps.sum().item())
