@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--unit", type=int, default=0, help="states per unit for --path tiled")
     ap.add_argument("--cap", type=int, default=0, help="steps per tile visit for --path tiled")
     ap.add_argument("--halo", type=int, default=0, help="hysteresis band for --path tiled")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="also time the full tile pipeline stage by stage (uv, trajectories, fields, "
+                         "finishing, channel mapping and segmentation, hillslope lengths, KDE)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales PMC byte counters)")
     args = ap.parse_args()
@@ -186,6 +189,12 @@ def main():
                      "bytes_per_step": BYTES_PER_STEP,
                      "steps_per_launch": steps_per_launch, "mean_launch_ms": mean_launch_s * 1e3},
     }
+    if args.pipeline:
+        from slm_amd.pipeline import map_tile
+        map_tile(job)  # warm-up
+        res, tim = map_tile(job)
+        out["pipeline"] = {"stages": {k: round(v["gpu_ms"], 3) for k, v in tim.items()},
+                           "total_ms": round(sum(v["gpu_ms"] for v in tim.values()), 3), "results": res}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(job)
     print(json.dumps(out), flush=True)

@@ -321,6 +321,10 @@ int slm_fields_finish_scaled(const uint32_t* slc_raw, uint32_t nxy, uint32_t den
  * same launch, the result here follows a fixed, documented order of phases, so
  * repeated calls give identical output; on inputs where the reference's
  * outcome does not depend on its schedule, the outputs are the reference's.
+ * Traces that circle (pits, flats) and walks on link cycles run on to the
+ * reference's caps (1000*MAX_N_STEPS steps or hops); they are detected and
+ * fast-forwarded by whole laps, which leaves every output bit unchanged
+ * because all shared updates are max / or / and of lap-periodic values.
  * Calls taking a workspace need slm_map_workspace_size(p) bytes; calls that
  * share a workspace must be ordered on one stream.
  */

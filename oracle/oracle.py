@@ -87,6 +87,11 @@ def map_call(name, p, **arrays):
     return getattr(load(), "slmo_" + name)(ctypes.byref(p), *ptrs)
 
 
+def map_set_fast_forward(on: bool):
+    """Cycle fast-forward of the mapping oracle (exact); off runs every step."""
+    load().slmo_map_set_fast_forward(1 if on else 0)
+
+
 def load():
     global _lib
     if _lib is None:
@@ -112,6 +117,8 @@ def load():
             f = getattr(lib, "slmo_" + name)
             f.restype = _U32 if name == "thin_channels" else None
             f.argtypes = [_P] * nargs
+        lib.slmo_map_set_fast_forward.restype = None
+        lib.slmo_map_set_fast_forward.argtypes = [ctypes.c_int]
         lib.slmo_kde_histogram.restype = None
         lib.slmo_kde_histogram.argtypes = [_P, _P, _P]
         lib.slmo_kde_pdf.restype = None

@@ -52,6 +52,19 @@ static inline uint32_t lk(const SlmParams* p, const uint32_t* link, uint32_t i) 
     return j < p->nxy_padded ? j : i;
 }
 
+/*
+ * Cycle fast-forward.  A trace (or link walk) whose state repeats is periodic
+ * and would run on to its step cap.  Every shared-grid update in the loops
+ * below is a max of a counter that grows by the same amount each lap, an or,
+ * or an and, so all laps but the last two are dominated: on detecting a cycle
+ * (Brent's algorithm on the whole state) the loop skips them, advancing the
+ * step and counter by whole laps, and runs the rest normally.  The result is
+ * the same bits as running every step; slmo_map_set_fast_forward(0) runs every
+ * step (tests compare the two).
+ */
+static int g_fast_forward = 1;
+void slmo_map_set_fast_forward(int on) { g_fast_forward = on; }
+
 static inline uint32_t seed_pixel(const SlmParams* p, const float* seeds, uint32_t k) {
     return array_idx(p, (real)seeds[2 * (size_t)k], (real)seeds[2 * (size_t)k + 1]);
 }
@@ -300,6 +313,10 @@ void slmo_count_downchannels(const SlmParams* p, const float* seeds, const uint8
             if (pass == 0) kmax(&keyA[idx], KEY(1, k));
             else umax(&count[idx], 1);
             counter = 2;
+            /* Brent state: position, dt, previous pixel */
+            real sx = vec[0], sy = vec[1], sdt = dt;
+            uint32_t sprev = prev_idx, sn = n, scounter = counter, power = 1;
+            int ff = g_fast_forward;
             while (!mask[idx] && n < cap) {
                 if (countlink_step(p, uv, &dt, vec, &idx)) break;
                 n++;
@@ -317,6 +334,20 @@ void slmo_count_downchannels(const SlmParams* p, const float* seeds, const uint8
                         cprev = counter;
                     }
                     prev_idx = idx;
+                }
+                if (ff) {
+                    if (vec[0] == sx && vec[1] == sy && dt == sdt && prev_idx == sprev) {
+                        const uint32_t lam = n - sn, clap = counter - scounter, laps = (cap - n) / lam;
+                        if (laps >= 2) {
+                            n += (laps - 1) * lam;
+                            counter += (laps - 1) * clap;
+                            cprev += (laps - 1) * clap;
+                        }
+                        ff = 0;
+                    } else if (n - sn == power) {
+                        sx = vec[0]; sy = vec[1]; sdt = dt; sprev = prev_idx;
+                        sn = n; scounter = counter; power <<= 1;
+                    }
                 }
             }
             if (pass == 1) kmax(&keyL[prev_idx], KEY(cprev, idx));
@@ -352,6 +383,8 @@ void slmo_flag_downchannels(const SlmParams* p, const float* seeds, const uint8_
         for (uint32_t k = p->seeds_chunk_offset; k < p->n_seed_points; k++) {
             const uint32_t h = seed_pixel(p, seeds, k);
             uint32_t idx = h, prev_idx = h + 1u, counter = 1;
+            uint32_t sidx = idx, scounter = counter, power = 1;
+            int ff = g_fast_forward && phase == 2;
             while (!mask[idx] && prev_idx != idx && counter < cap) {
                 prev_idx = idx;
                 idx = lk(p, link, idx);
@@ -366,6 +399,15 @@ void slmo_flag_downchannels(const SlmParams* p, const float* seeds, const uint8_
                     }
                 } else {
                     break;
+                }
+                if (ff) {   /* a walk's state is its pixel; one hop per counter */
+                    if (idx == sidx && counter != scounter) {
+                        const uint32_t lam = counter - scounter, laps = (cap - counter) / lam;
+                        if (laps >= 2) counter += (laps - 1) * lam;
+                        ff = 0;
+                    } else if (counter - scounter == power) {
+                        sidx = idx; scounter = counter; power <<= 1;
+                    }
                 }
             }
             if (phase == 2 && prev_idx < p->nxy_padded && !mask[prev_idx] && counter < cap)
@@ -502,10 +544,17 @@ void slmo_link_hillslopes(const SlmParams* p, const float* seeds, const uint8_t*
         real vec[2] = {(real)seeds[2 * (size_t)k], (real)seeds[2 * (size_t)k + 1]};
         real dt = (real)p->dt_max;
         uint32_t idx = array_idx(p, vec[0], vec[1]), prev_idx = idx, n = 0;
+        real sx = vec[0], sy = vec[1], sdt = dt;
+        uint32_t sn = 0, power = 1;
         while (prev_idx == idx && !mask[idx] && n < cap) {
             prev_idx = idx;
             if (countlink_step(p, uv, &dt, vec, &idx)) break;
             n++;
+            if (g_fast_forward && prev_idx == idx) {
+                /* a repeated state inside the seed pixel never leaves it: no link */
+                if (vec[0] == sx && vec[1] == sy && dt == sdt) { n = cap; break; }
+                if (n - sn == power) { sx = vec[0]; sy = vec[1]; sdt = dt; sn = n; power <<= 1; }
+            }
         }
         if (prev_idx != idx && !mask[idx]) link[prev_idx] = idx;
     }

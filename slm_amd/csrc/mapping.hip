@@ -246,6 +246,10 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
         atomicMax(&keyA[idx], key2(1u, k));
     else
         atomicMax(&count[idx], 1u);
+    // Brent cycle detection on (x, y, dt, prev); see k_flag_walk for the skip rule
+    float sx = x, sy = y, sdt = dt;
+    uint32_t sprev = prev, sn = n, scounter = counter, power = 1;
+    bool ff = true;
     while (!mask[idx] && n < cap) {
         if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
         n++;
@@ -263,6 +267,25 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
                 cprev = counter;
             }
             prev = idx;
+        }
+        if (ff) {
+            if (x == sx && y == sy && dt == sdt && prev == sprev) {
+                const uint32_t lam = n - sn, clap = counter - scounter, laps = (cap - n) / lam;
+                if (laps >= 2) {
+                    n += (laps - 1) * lam;
+                    counter += (laps - 1) * clap;
+                    cprev += (laps - 1) * clap;
+                }
+                ff = false;
+            } else if (n - sn == power) {
+                sx = x;
+                sy = y;
+                sdt = dt;
+                sprev = prev;
+                sn = n;
+                scounter = counter;
+                power <<= 1;
+            }
         }
     }
     if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
@@ -298,6 +321,12 @@ __global__ void k_flag_walk(SlmParams P, const float2* __restrict__ seeds, const
     const uint32_t cap = 1000u * P.max_n_steps;
     const uint32_t h = seed_pixel(P, seeds, k);
     uint32_t idx = h, prev = h + 1u, counter = 1;
+    // A walk that returns to a pixel is on a link cycle and would circle to the
+    // hop cap.  Its updates are or and max of a counter that grows by the cycle
+    // length each lap, so all laps but the last two are dominated: skip them
+    // (Brent's cycle detection on the pixel).  Same bits as walking every hop.
+    uint32_t sidx = idx, scounter = counter, power = 1;
+    bool ff = PHASE == 2;
     while (!mask[idx] && prev != idx && counter < cap) {
         prev = idx;
         idx = lk(P, link, idx);
@@ -309,6 +338,17 @@ __global__ void k_flag_walk(SlmParams P, const float2* __restrict__ seeds, const
         } else {
             atomicOr(&mapping[idx], F_THIN);
             atomicMax(&count[idx], counter);
+        }
+        if (ff) {
+            if (idx == sidx && counter != scounter) {
+                const uint32_t lam = counter - scounter, laps = (cap - counter) / lam;
+                if (laps >= 2) counter += (laps - 1) * lam;
+                ff = false;
+            } else if (counter - scounter == power) {
+                sidx = idx;
+                scounter = counter;
+                power <<= 1;
+            }
         }
     }
     if (PHASE == 2 && prev < P.nxy_padded && !mask[prev] && counter < cap) atomicOr(&mapping[prev], F_TAIL);
@@ -432,10 +472,23 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
     const float2 s = seeds[k];
     float x = s.x, y = s.y, dt = P.dt_max;
     uint32_t idx = array_idx(P, x, y), prev = idx, n = 0;
+    float sx = x, sy = y, sdt = dt;
+    uint32_t sn = 0, power = 1;
     while (prev == idx && !mask[idx] && n < cap) {
         prev = idx;
         if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
         n++;
+        if (prev == idx) {
+            // a repeated state inside the seed pixel never leaves it: no link
+            if (x == sx && y == sy && dt == sdt) return;
+            if (n - sn == power) {
+                sx = x;
+                sy = y;
+                sdt = dt;
+                sn = n;
+                power <<= 1;
+            }
+        }
     }
     if (prev != idx && !mask[idx]) link[prev] = idx;
 }

@@ -1,0 +1,146 @@
+"""The full mapping pipeline of one DTM tile on the GPU, stage by stage.
+
+The device-side sequence behind Trace.do() and the mapping passes
+(python/streamlines/trace.py:74-166, mapping.py:627-776, lengths.py,
+kde.py): uv from the DTM, the trajectory pass for the ds statistics, both
+fields directions, the fields finishing, channel mapping and segmentation,
+traced hillslope lengths from every hillslope pixel, and the Gaussian KDE of
+those lengths.  Inputs stay in HBM between stages; only the small statistics
+(1000 trajectories, the KDE histogram) come back to the host.  Each stage is
+timed with HIP events on the current stream.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import hip
+from .kde import Univariate_distribution
+from .mapping import ChannelMapper, pick_seeds
+from .params import kernel_params
+from .useful import compute_stats, create_seeds
+
+
+class _Stages:
+    def __init__(self):
+        import torch
+        self.torch = torch
+        self.marks = []
+
+    def mark(self, name):
+        ev = self.torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.marks.append((name, ev, time.perf_counter()))
+
+    def report(self):
+        self.torch.cuda.synchronize()
+        out = {}
+        for (name, e0, w0), (_, e1, w1) in zip(self.marks[:-1], self.marks[1:]):
+            out[name] = {"gpu_ms": e0.elapsed_time(e1), "wall_ms": (w1 - w0) * 1e3}
+        return out
+
+
+def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None):
+    """Run the pipeline on a workloads.Job; returns (results dict, stage timings)."""
+    import torch
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    info = job.info
+    pad = int(info.pad_width)
+    st = _Stages()
+    roi = torch.from_numpy(np.ascontiguousarray(job.roi, dtype=np.float32)).to(dev)
+    mask_np = job.mask
+    nxp, nyp = mask_np.shape
+    seeds = torch.from_numpy(job.seeds).to(dev)
+    # host seed lists (useful.create_seeds, legacy MT19937 shuffle) are inputs, made first
+    tseeds, n_t, _ = create_seeds(mask_np, pad, int(info.n_work_items), n_seed_points=info.n_trajectory_seed_points,
+                                  do_shuffle=info.do_shuffle, rng_seed=info.shuffle_rng_seed)
+    ts = torch.from_numpy(tseeds).to(dev)
+    torch.cuda.synchronize()
+
+    st.mark("uv")
+    uv_full, mask_full = hip.gradient_uv(roi, pad, float(job.pixel_size))
+    ox, oy = job.origin
+    # the tracer's grids: the bbox crop of the full padded grids (Data, useful.py:37-115)
+    uv = uv_full[ox:ox + nxp, oy:oy + nyp].contiguous()
+    mask_b = mask_full[ox:ox + nxp, oy:oy + nyp].contiguous()
+    mask = mask_b.to(torch.uint8).reshape(-1)
+
+    st.mark("trajectories")
+    lengths, nsteps = [], []
+    for sign in (1, -1):
+        info.n_seed_points = np.uint32(n_t)
+        p = kernel_params(info, sign, "INTEGRATE_TRAJECTORY", fp_contract)
+        traj = torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=dev)
+        ns = torch.empty(n_t, dtype=torch.int16, device=dev)
+        ln = torch.empty(n_t, dtype=torch.float32, device=dev)
+        hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln)
+        lengths.append(ln)
+        nsteps.append(ns)
+    tl = torch.stack(lengths, 1).cpu().numpy()
+    tn = torch.stack(nsteps, 1).cpu().numpy().view(np.uint16)
+    stats = compute_stats(tl, tn, info.pixel_size)
+
+    st.mark("fields")
+    nxy = nxp * nyp
+    slc = torch.empty((2, nxy), dtype=torch.int32, device=dev)
+    slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
+    steps = torch.zeros(2, dtype=torch.int64, device=dev)
+    ws = None
+    info.n_seed_points = np.uint32(job.seeds.shape[0])
+    for k, sign in enumerate((1, -1)):
+        p = kernel_params(info, sign, "INTEGRATE_FIELDS", fp_contract)
+        ws = hip.trace_fields(p, seeds, mask, uv.reshape(-1), slc[k], slt[k], steps[k:k + 1], workspace=ws,
+                              keep_order=(k == 1))
+
+    st.mark("finish")
+    slt_f, sla_f = hip.fields_finish_raw(slc, slt, int(info.subpixel_seed_point_density))
+    ps = np.float32(info.pixel_size)
+    sd = np.float32(np.float32(stats["ds"]["downstream", "mean"]) / ps)
+    su = np.float32(np.float32(stats["ds"]["upstream", "mean"]) / ps)
+    slc_f = hip.fields_finish_scaled(slc, slt_f, int(info.subpixel_seed_point_density), sd, su)
+
+    st.mark("mapping")
+    slt_dn = slt_f[:, 0].reshape(nxp, nyp).contiguous()
+    slc_dn = slc_f[:, 0].reshape(nxp, nyp).contiguous()
+    ok = ~mask_b
+    thr = float(torch.quantile(slt_dn[ok][:: max(1, int(ok.sum().item()) // 1000000)].double(),
+                               channel_quantile).item())
+    info.set_thresholds(channel_threshold=np.float32(thr), segmentation_threshold=np.uint32(segmentation_threshold))
+    mapping0 = torch.zeros((nxp, nyp), dtype=torch.int32, device=dev)
+    cm = ChannelMapper(info, mask_b, uv, mapping0, slt=slt_dn, slc=slc_dn, fp_contract=fp_contract)
+    ok_chain = cm.map_channels_segments()
+
+    st.mark("hsl")
+    hs_seeds = pick_seeds(mask=mask_b, map=~cm.mapping, flag=info.is_thinchannel, pad=pad)
+    info.n_seed_points = np.uint32(hs_seeds.shape[0])
+    p = kernel_params(info, 1, "HILLSLOPE_LENGTHS", fp_contract)
+    hsl = torch.zeros(hs_seeds.shape[0], dtype=torch.float32, device=dev)
+    if hs_seeds.shape[0]:
+        hip.hillslope_lengths(p, hs_seeds, mask, uv.reshape(-1), cm.mapping.reshape(-1),
+                              int(info.is_thinchannel), int(info.is_midslope), hsl)
+
+    st.mark("kde")
+    pos = hsl[hsl > 0]
+    dist = None
+    if pos.numel() > 1:
+        logx = torch.log(pos).cpu().numpy().astype(np.float32)
+        dist = Univariate_distribution(logx_array=logx, logy_array=logx, n_hist_bins=512, n_pdf_points=256)
+        dist.compute_kde_opencl(kernel="gaussian", bandwidth=0.15, fp_contract=fp_contract)
+    st.mark("end")
+    timings = st.report()
+    results = {
+        "steps": [int(v) for v in steps.cpu()],
+        "channel_threshold": thr,
+        "chain_ok": bool(ok_chain),
+        "n_segments": int(cm.n_segments),
+        "channel_pixels": int(((cm.mapping & int(info.is_thinchannel)) != 0).sum().item()),
+        "hsl_seeds": int(hs_seeds.shape[0]),
+        "hsl_reached": int(pos.numel()),
+        "hsl_median_m": float(pos.median().item()) if pos.numel() else None,
+        "kde_mode_x": None,
+    }
+    if dist is not None:
+        dist.find_modes()
+        results["kde_mode_x"] = float(np.asarray(dist.mode_x).reshape(-1)[0])
+    return results, timings

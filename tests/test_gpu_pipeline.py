@@ -1,0 +1,33 @@
+"""The whole device pipeline of one tile (slm_amd.pipeline.map_tile) on the
+256^2 workload: the stages run end to end and their outputs agree with the
+stage-level parity tests' subjects (uv on the device equals the host uv the
+job was built with, so the fields equal a direct slm_trace_fields call)."""
+import numpy as np
+import pytest
+
+from slm_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def test_map_tile_small_256(gpu):
+    import torch
+    from slm_amd import hip
+    from slm_amd.pipeline import map_tile
+    job = workloads.small_256(jitter=1.0, d=2)
+    res, tim = map_tile(job)
+    print(res, {k: round(v["gpu_ms"], 2) for k, v in tim.items()}, flush=True)
+    assert set(tim) == {"uv", "trajectories", "fields", "finish", "mapping", "hsl", "kde"}
+    assert res["chain_ok"] and res["n_segments"] > 0 and res["channel_pixels"] > 0
+    assert res["hsl_reached"] > 0.5 * res["hsl_seeds"]
+    assert res["kde_mode_x"] is not None and np.isfinite(res["kde_mode_x"])
+    # fields steps equal a direct call on the host-prepared inputs
+    seeds = torch.from_numpy(job.seeds).to(gpu)
+    mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(gpu)
+    uv = torch.from_numpy(job.uv.reshape(-1)).to(gpu)
+    nxy = job.mask.size
+    slc = torch.empty(nxy, dtype=torch.int32, device=gpu)
+    slt = torch.empty_like(slc)
+    steps = torch.zeros(1, dtype=torch.int64, device=gpu)
+    hip.trace_fields(job.params(1), seeds, mask, uv, slc, slt, steps, sync=True)
+    assert int(steps.item()) == res["steps"][0]

@@ -253,3 +253,38 @@ def test_y_valley_chain_invariants():
         region = chan & (np.abs(lab) == s)
         if region.any():
             assert ndimage.label(region, structure=np.ones((3, 3)))[1] == 1, s
+
+
+def test_cycle_fast_forward_is_exact():
+    """Traces that circle in pits and link-graph cycles run to the reference's
+    1000*MAX_N_STEPS cap; the oracle (and the GPU) skip whole laps.  On the
+    traced 256^2 workload, which has such cycles, every grid of the chain
+    must be the same with the skip and without it."""
+    import time
+    from oracle.postprocess_ref import postprocess_raw
+    job = workloads.small_256(jitter=1.0)
+    p = job.params(1)
+    c, t, _ = oracle.trace_fields(p, job.seeds, job.mask.reshape(-1).astype(np.uint8), job.uv.reshape(-1))
+    sh = job.mask.shape
+    cc, tt, _ = postprocess_raw(c.reshape(sh)[..., None].repeat(2, 2), t.astype(np.float32).reshape(sh)[..., None]
+                                .repeat(2, 2), 1)
+    tt = tt[..., 0]
+    thr = np.float32(np.quantile(tt[~job.mask], 0.97))
+    m0 = np.zeros(sh, np.uint32)
+    m0[(tt >= thr) & (tt.astype(np.float64) * 2 >= cc[..., 0])] = np.uint32(job.info.is_channel)
+    job.info.set_thresholds(segmentation_threshold=np.uint32(20))
+    states, times = [], []
+    try:
+        for ff in (True, False):
+            oracle.map_set_fast_forward(ff)
+            om = OracleMapper(job.info, job.mask, job.uv, m0, slt=tt.astype(np.float32))
+            t0 = time.perf_counter()
+            for s in ("connect", "thin", "heads", "count", "flag", "confluences", "segment", "link_hs"):
+                om.step(s)
+            times.append(time.perf_counter() - t0)
+            states.append(om.state())
+    finally:
+        oracle.map_set_fast_forward(True)
+    print("chain seconds with / without the skip:", times)
+    for k in states[1]:
+        assert np.array_equal(states[0][k], states[1][k]), k
