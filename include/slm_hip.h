@@ -151,6 +151,41 @@ int slm_trace_fields_tiled(const SlmParams* p, const SlmTileOpts* opts, const fl
                            void* stream, uint32_t flags);
 
 /*
+ * Batched fields: many tiles (each its own grid, seeds, macro set and
+ * direction) traced by one persistent launch with XCD-sliced work-stealing
+ * queues.  Each job's slc/slt are bit-identical to a slm_trace_fields call
+ * with the same job.  No host synchronisation, allocation or memset: the call
+ * can be captured in a hipGraph and replayed.  All jobs must share
+ * fp_contract.  Flags: SLM_FLAG_NO_ZERO, SLM_FLAG_KEEP_ORDER (reuse the seed
+ * order and uv quads a previous call with the same jobs built),
+ * SLM_FLAG_FORCE_FALLBACK, SLM_FLAG_SYNC.  steps_out (optional) += accepted
+ * RK2 steps of the whole batch.  No counterpart in the reference, which
+ * traces one tile per Trace.do() (trace.py:131-157); the tiles, sizes and
+ * queue parameters are those of the batched configuration.
+ */
+typedef struct SlmFieldsJob {
+    SlmParams params;        /* the tile's macro set (downup sign in combo_factor) */
+    const float* seeds_xy;   /* float2 [n_seed_points] (shuffled seed list)        */
+    const uint8_t* mask;     /* uint8 [nxy_padded]                                 */
+    const float* uv;         /* float2 [nxy_padded]                                */
+    uint32_t* slc;           /* uint32 [nxy_padded]                                */
+    uint32_t* slt;           /* uint32 [nxy_padded]                                */
+    uint64_t reserved[3];
+} SlmFieldsJob;
+
+typedef struct SlmBatchOpts {
+    uint32_t chunk;          /* work items per queue grab (0: 256)                  */
+    uint32_t blocks_per_cu;  /* resident 256-thread blocks per CU (0: occupancy max) */
+    uint32_t queue_slices;   /* work-list slices / queue heads, 1..8 (0: 8, one per XCD) */
+    uint32_t reserved[5];
+} SlmBatchOpts;
+
+size_t slm_trace_fields_batch_workspace_size(const SlmFieldsJob* jobs, uint32_t n_jobs);
+int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs, const SlmBatchOpts* opts,
+                           uint64_t* steps_out, void* workspace, size_t workspace_bytes,
+                           void* stream, uint32_t flags);
+
+/*
  * Un-jittered trajectory recording, one direction.
  * Replaces kernel integrate_trajectory (opencl/integratetraj.cl:54-97 with
  * trajectory.cl:41-80) as launched by Trajectories.gpu_compute_trajectories

@@ -1,0 +1,88 @@
+"""Autotuner for the batched fields tracer's queue parameters.
+
+slm_trace_fields_batch takes SlmBatchOpts: the number of work items a wave
+grabs per queue atomic (chunk), the resident blocks per CU, and the number of
+queue heads the work list is cut into (one per XCD by default).  The best
+values depend on the batch: many short trajectories favour small chunks
+(less tail imbalance), long uniform ones large chunks (fewer atomics).  The
+tuner times candidates on the batch itself with HIP events (coordinate
+descent: chunk, then queue slices, then occupancy) and keeps the winner per
+batch signature in memory and, if given, in a JSON file.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+from . import hip
+
+CHUNKS = (32, 64, 128, 256, 512, 1024)
+SLICES = (1, 2, 4, 8)
+BLOCKS = (0, 4, 3, 2)
+
+_cache: dict = {}
+
+
+def signature(job_array) -> str:
+    """Batch signature: grid sizes, seed counts, sub-pixel density and step caps."""
+    h = hashlib.sha1()
+    for j in job_array:
+        p = j.params
+        h.update(("%d,%d,%d,%d,%d;" % (p.nx_padded, p.ny_padded, p.n_seed_points, p.subpixel_seed_point_density,
+                                       p.max_n_steps)).encode())
+    return h.hexdigest()[:16]
+
+
+def time_batch(job_array, opts, workspace, repeats=2, stream=None) -> float:
+    """Mean ms of slm_trace_fields_batch with opts (seed order reused)."""
+    import torch
+    s = stream or torch.cuda.current_stream()
+    hip.trace_fields_batch(job_array, workspace=workspace, opts=opts, stream=s, keep_order=True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(repeats)]
+    for a, b in ev:
+        a.record(s)
+        hip.trace_fields_batch(job_array, workspace=workspace, opts=opts, stream=s, keep_order=True)
+        b.record(s)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in ev) / repeats
+
+
+def tune_batch(job_array, workspace=None, cache_path=None, repeats=2, verbose=False):
+    """Returns (SlmBatchOpts, table of (chunk, blocks, slices, ms))."""
+    import torch
+    sig = signature(job_array)
+    if cache_path and os.path.exists(cache_path) and not _cache:
+        _cache.update(json.load(open(cache_path)))
+    if sig in _cache:
+        c = _cache[sig]
+        return hip.batch_opts(c["chunk"], c["blocks_per_cu"], c["queue_slices"]), c.get("table", [])
+    need = hip.batch_workspace_size(job_array)
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=torch.device("cuda", torch.cuda.current_device()))
+    hip.trace_fields_batch(job_array, workspace=workspace)   # builds the seed order and quads
+    table = []
+    best = {"chunk": 256, "blocks_per_cu": 0, "queue_slices": 8}
+
+    def run(cfg):
+        ms = time_batch(job_array, hip.batch_opts(cfg["chunk"], cfg["blocks_per_cu"], cfg["queue_slices"]),
+                        workspace, repeats)
+        table.append((cfg["chunk"], cfg["blocks_per_cu"], cfg["queue_slices"], ms))
+        if verbose:
+            print("autotune", cfg, "%.2f ms" % ms, flush=True)
+        return ms
+
+    best_ms = run(best)
+    for key, values in (("chunk", CHUNKS), ("queue_slices", SLICES), ("blocks_per_cu", BLOCKS)):
+        for v in values:
+            if v == best[key]:
+                continue
+            cfg = dict(best, **{key: v})
+            ms = run(cfg)
+            if ms < best_ms:
+                best, best_ms = cfg, ms
+    _cache[sig] = dict(best, ms=best_ms, table=table)
+    if cache_path:
+        with open(cache_path, "w") as f:
+            json.dump(_cache, f, indent=1)
+    return hip.batch_opts(best["chunk"], best["blocks_per_cu"], best["queue_slices"]), table

@@ -1452,3 +1452,408 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, const SlmTileOpts* opt
                            mask, quad, acc, slc, slt, nullptr, &ctrl->f, force, fixed);
     return slm_finish(stream, flags);
 }
+
+// ------------------------------------------------------------------ batched fields tracer
+//
+// Many tiles (grids of any size, each with its own macro set: DT_MAX,
+// COMBO_FACTOR and direction differ per tile) traced by ONE persistent
+// launch.  The work list is the concatenation of every job's Morton-ordered
+// work items; waves take wave-private chunks from XCD-sliced queues exactly
+// as k_fields does, so a lane can hold a streamline of any job.  At each
+// refill a lane finds its job by binary search over the jobs' work offsets
+// and loads that job's constants and pointers into registers.  All jobs
+// accumulate into one concatenated acc with the same packed-sum check and
+// fallback as a single tile.  No host synchronisation and no memset nodes:
+// the whole call can be captured in a hipGraph.
+
+struct BatchDesc {
+    SlmParams P;
+    const float2* seeds;
+    const uint8_t* mask;
+    const Quad* quad;
+    unsigned long long* acc;
+    uint32_t* slc;
+    uint32_t* slt;
+    const SeedRec* order;
+    uint32_t work_begin, n_work;
+    uint32_t pix_begin, dt_fixed;
+};
+
+constexpr uint32_t kDescPerLaunch = 8;
+struct DescPack {
+    BatchDesc d[kDescPerLaunch];
+};
+
+__global__ void k_put_desc(DescPack pack, uint32_t first, uint32_t count, BatchDesc* __restrict__ out) {
+    const uint32_t i = threadIdx.x;
+    if (i < count) out[first + i] = pack.d[i];
+}
+
+template <bool C, bool PACKED>
+__global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restrict__ desc, uint32_t n_jobs,
+                                                      uint32_t n_work, unsigned long long* __restrict__ steps,
+                                                      FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
+                                                      uint32_t chunk, uint32_t n_slices) {
+    if constexpr (!PACKED) {
+        if (fields_packed_ok(ctrl, force_fallback)) return;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+
+    // per-lane job context
+    SlmParams P;
+    const uint8_t* mask = nullptr;
+    const Quad* quad = nullptr;
+    unsigned long long* acc = nullptr;
+    uint32_t* slc = nullptr;
+    uint32_t* slt = nullptr;
+    uint32_t dt_fixed = 0;
+
+    bool active = false, exhausted = false;
+    float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
+    uint32_t n = 0, rng = 0;
+    Quad Q;
+    uint32_t run_idx = 0, run_n = 0, run_t = 0;
+    uint64_t my_steps = 0, my_writes = 0;
+
+    auto flush = [&]() {
+        if (run_n) {
+            if constexpr (PACKED) {
+                atomicAdd(&acc[run_idx], ((unsigned long long)run_t << 32) | (unsigned long long)run_n);
+            } else {
+                atomicAdd(&slt[run_idx], run_t);
+                atomicAdd(&slc[run_idx], run_n);
+            }
+            run_n = 0;
+            run_t = 0;
+        }
+    };
+
+    uint32_t c_next = 0, c_end = 0;
+    const uint32_t slice = (n_work + n_slices - 1u) / n_slices;
+    while (true) {
+        if (!exhausted) {
+            uint64_t need = __ballot(!active);
+            while (need) {
+                if (c_next >= c_end) {
+                    uint32_t base = 0xffffffffu, lim = 0;
+                    const uint32_t me = __builtin_amdgcn_s_getreg((20) | (0 << 6) | (3 << 11)) % n_slices;
+                    for (uint32_t t = 0; t < n_slices && base == 0xffffffffu; t++) {
+                        const uint32_t q = (me + t) % n_slices;
+                        const uint32_t lo = q * slice, hi = min(n_work, lo + slice);
+                        uint32_t b = 0;
+                        if (lane == 0) b = atomicAdd(&ctrl->queue[q * kQueueStride], chunk);
+                        b = __shfl(b, 0);
+                        if (b < slice && lo + b < hi) {
+                            base = lo + b;
+                            lim = hi;
+                        }
+                    }
+                    if (base == 0xffffffffu) {
+                        exhausted = true;
+                        break;
+                    }
+                    c_next = base;
+                    c_end = min(base + chunk, lim);
+                }
+                const uint32_t take = min((uint32_t)__popcll(need), c_end - c_next);
+                const uint32_t rank = __popcll(need & lt_mask);
+                const bool mine = ((need >> lane) & 1ull) && rank < take;
+                if (mine) {
+                    const uint32_t w = c_next + rank;
+                    // the job holding work item w
+                    uint32_t lo = 0, hi = n_jobs;
+                    while (hi - lo > 1u) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (desc[mid].work_begin <= w) lo = mid; else hi = mid;
+                    }
+                    const BatchDesc* D = desc + lo;
+                    P = D->P;
+                    mask = D->mask;
+                    quad = D->quad;
+                    acc = D->acc;
+                    slc = D->slc;
+                    slt = D->slt;
+                    dt_fixed = D->dt_fixed;
+                    const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
+                    const uint32_t dd = d * d;
+                    const uint32_t wl = w - D->work_begin;
+                    const uint32_t sub = wl % dd;
+                    const SeedRec r = D->order[wl / dd];
+                    x = r.x + sub_offset<C>(P, sub % d);
+                    y = r.y + sub_offset<C>(P, sub / d);
+                    rng = rng_seed(r.seed_idx);
+                    l = 0.f;
+                    n = 0;
+                    dt = P.dt_max;
+                    const uint32_t idx = array_idx(P, x, y);
+                    if (!mask[idx]) {
+                        // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
+                        Q = load_quad(quad, idx);
+                        run_idx = idx;
+                        run_n = 1u;
+                        run_t = slt_inc(0.f);
+                        my_writes += 1u;
+                        active = (0.f < P.max_length) && (0u < P.max_n_steps);
+                        if (!active) flush();
+                    }
+                }
+                need &= ~__ballot(mine);
+                c_next += take;
+            }
+        }
+        if (!__any(active)) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            const float res = P.trajectory_resolution;
+            const float combo = P.combo_factor;
+            const float2 u1 = jitter_unit<C>(P, interp_quad<C>(P, Q, x, y), rng);
+            const float2 d1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+            const float mx = x + d1.x, my = y + d1.y;
+            const Quad Q1 = load_quad(quad, array_idx(P, mx, my));
+            const float2 u2 = jitter_unit<C>(P, interp_quad<C>(P, Q1, mx, my), rng);
+            const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
+                                          approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
+            const float nx = x + d2.x, ny = y + d2.y;
+            const uint32_t nidx = array_idx(P, nx, ny);
+            const uint8_t mk = mask[nidx];
+            const Quad Qn = load_quad(quad, nidx);
+            if (mk) {
+                active = false;
+                flush();
+            } else {
+                const float dl = fast_length<C>(d2.x, d2.y);
+                x = nx;
+                y = ny;
+                Q = Qn;
+                const bool stuck = dl < P.integration_halt_threshold;
+                if (!stuck && !dt_fixed) {
+                    StepOut o;
+                    o.dxy1 = d1;
+                    o.dxy2 = d2;
+                    dt = next_dt(P, dt, step_error<C>(P, o));
+                }
+                l += dl;
+                n += 1u;
+                my_steps += 1u;
+                my_writes += 1u;
+                if (nidx != run_idx) {
+                    flush();
+                    run_idx = nidx;
+                }
+                run_n += 1u;
+                run_t += slt_inc(l);
+                active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
+                if (!active) flush();
+            }
+        }
+    }
+    if constexpr (PACKED) {
+        uint64_t v = my_steps, wv = my_writes;
+        for (int off = 32; off > 0; off >>= 1) {
+            v += __shfl_down(v, off);
+            wv += __shfl_down(wv, off);
+        }
+        if (lane == 0) {
+            if (steps && v) atomicAdd(steps, (unsigned long long)v);
+            if (wv) atomicAdd(&ctrl->writes, (unsigned long long)wv);
+        }
+    }
+}
+
+// Split the concatenated acc into every job's slc/slt (or clear them for the
+// fallback), as k_fields_commit does for one grid.
+__global__ void k_batch_commit(const BatchDesc* __restrict__ desc, uint32_t n_jobs, size_t n_pix,
+                               FieldsCtrl* __restrict__ ctrl, uint32_t accumulate, uint32_t force_fallback,
+                               const unsigned long long* __restrict__ acc_all) {
+    const bool ok = fields_packed_ok(ctrl, force_fallback);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = n_jobs;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (desc[mid].pix_begin <= i) lo = mid; else hi = mid;
+        }
+        const BatchDesc* D = desc + lo;
+        const size_t j = i - D->pix_begin;
+        if (ok) {
+            const unsigned long long a = acc_all[i];
+            const uint32_t l32 = (uint32_t)a, h32 = (uint32_t)(a >> 32);
+            if (accumulate) {
+                D->slc[j] += l32;
+                D->slt[j] += h32;
+            } else {
+                D->slc[j] = l32;
+                D->slt[j] = h32;
+            }
+        } else if (!accumulate) {
+            D->slc[j] = 0u;
+            D->slt[j] = 0u;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) ctrl->queue[threadIdx.x * kQueueStride] = 0u;
+}
+
+struct BatchWs {
+    size_t off_desc, off_cnt, off_cur, off_order, off_acc, off_quad, total;
+    uint32_t max_keys;
+    uint64_t n_seeds, n_pix, n_work;
+};
+
+static int batch_ws(const SlmFieldsJob* jobs, uint32_t n_jobs, BatchWs& w) {
+    w.max_keys = 1;
+    w.n_seeds = w.n_pix = w.n_work = 0;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const SlmParams* p = &jobs[j].params;
+        int rc = check_params(p);
+        if (rc) return rc;
+        const TileGeom g = tile_geom(p);
+        if (g.n_keys > w.max_keys) w.max_keys = g.n_keys;
+        const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+        const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+        w.n_seeds += n;
+        w.n_pix += p->nxy_padded;
+        w.n_work += (uint64_t)n * d * d;
+    }
+    if (w.n_work >= (1ull << 32) - 4096 || w.n_pix >= (1ull << 32))
+        return slm_fail(SLM_E_INVALID_ARG, "batch too large: %llu work items, %llu pixels",
+                        (unsigned long long)w.n_work, (unsigned long long)w.n_pix);
+    w.off_desc = align256(sizeof(FieldsCtrl)) > 4096 ? align256(sizeof(FieldsCtrl)) : 4096;
+    w.off_cnt = w.off_desc + align256((size_t)n_jobs * sizeof(BatchDesc));
+    w.off_cur = w.off_cnt + align256((size_t)w.max_keys * 4);
+    w.off_order = w.off_cur + align256((size_t)w.max_keys * 4);
+    w.off_acc = w.off_order + align256((size_t)w.n_seeds * sizeof(SeedRec));
+    w.off_quad = w.off_acc + align256((size_t)w.n_pix * 8);
+    w.total = w.off_quad + align256((size_t)w.n_pix * sizeof(Quad));
+    return SLM_OK;
+}
+
+extern "C" size_t slm_trace_fields_batch_workspace_size(const SlmFieldsJob* jobs, uint32_t n_jobs) {
+    if (!jobs || n_jobs == 0) return 0;
+    BatchWs w;
+    if (batch_ws(jobs, n_jobs, w) != SLM_OK) return 0;
+    return w.total;
+}
+
+extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs, const SlmBatchOpts* opts,
+                                      uint64_t* steps_out, void* workspace, size_t workspace_bytes, void* stream,
+                                      uint32_t flags) {
+    if (!jobs || n_jobs == 0) return slm_fail(SLM_E_INVALID_ARG, "empty batch");
+    BatchWs w;
+    int rc = batch_ws(jobs, n_jobs, w);
+    if (rc) return rc;
+    if (!workspace || workspace_bytes < w.total)
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes < required %zu", workspace_bytes, w.total);
+    const uint32_t C0 = jobs[0].params.fp_contract ? 1u : 0u;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const SlmFieldsJob& J = jobs[j];
+        if ((J.params.fp_contract ? 1u : 0u) != C0)
+            return slm_fail(SLM_E_INVALID_ARG, "job %u: fp_contract differs within the batch", j);
+        if (!J.mask || !J.uv || !J.slc || !J.slt) return slm_fail(SLM_E_INVALID_ARG, "job %u: NULL buffer", j);
+        if (J.params.n_seed_points > J.params.seeds_chunk_offset && !J.seeds_xy)
+            return slm_fail(SLM_E_INVALID_ARG, "job %u: NULL seeds", j);
+    }
+    uint32_t chunk = 256, blocks_per_cu = 0, slices = kQueues;
+    if (opts) {
+        if (opts->chunk) chunk = opts->chunk;
+        if (opts->blocks_per_cu) blocks_per_cu = opts->blocks_per_cu;
+        if (opts->queue_slices) slices = opts->queue_slices;
+    }
+    if (chunk < 1 || chunk > 65536) return slm_fail(SLM_E_INVALID_ARG, "chunk %u outside [1, 65536]", chunk);
+    if (slices < 1 || slices > kQueues) return slm_fail(SLM_E_INVALID_ARG, "queue_slices %u outside [1, 8]", slices);
+    DevInfo di;
+    rc = device_info(di);
+    if (rc) return rc;
+
+    hipStream_t s = (hipStream_t)stream;
+    const bool accumulate = (flags & SLM_FLAG_NO_ZERO) != 0;
+    const uint32_t force = (flags & SLM_FLAG_FORCE_FALLBACK) ? 1u : 0u;
+    char* ws = (char*)workspace;
+    FieldsCtrl* ctrl = (FieldsCtrl*)ws;
+    BatchDesc* desc = (BatchDesc*)(ws + w.off_desc);
+    uint32_t* cnt = (uint32_t*)(ws + w.off_cnt);
+    uint32_t* cur = (uint32_t*)(ws + w.off_cur);
+    SeedRec* order_all = (SeedRec*)(ws + w.off_order);
+    unsigned long long* acc_all = (unsigned long long*)(ws + w.off_acc);
+    Quad* quad_all = (Quad*)(ws + w.off_quad);
+
+    // descriptors, sort and quads per job
+    DescPack pack;
+    uint32_t in_pack = 0, pack_first = 0;
+    uint64_t work_off = 0, seed_off = 0, pix_off = 0;
+    for (uint32_t j = 0; j < n_jobs; j++) {
+        const SlmFieldsJob& J = jobs[j];
+        const SlmParams* p = &J.params;
+        const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
+        const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+        BatchDesc& D = pack.d[in_pack];
+        D.P = *p;
+        D.seeds = (const float2*)J.seeds_xy;
+        D.mask = J.mask;
+        D.quad = quad_all + pix_off;
+        D.acc = acc_all + pix_off;
+        D.slc = J.slc;
+        D.slt = J.slt;
+        D.order = order_all + seed_off;
+        D.work_begin = (uint32_t)work_off;
+        D.n_work = n * d * d;
+        D.pix_begin = (uint32_t)pix_off;
+        D.dt_fixed = dt_is_fixed(p);
+        if (!(flags & SLM_FLAG_KEEP_ORDER)) {
+            if (n) {
+                const TileGeom g = tile_geom(p);
+                fill_u32(cnt, g.n_keys, 0u, s);
+                hipLaunchKernelGGL(k_tile_count, dim3((n + 255) / 256), dim3(256), 0, s, *p, D.seeds, n, cnt);
+                hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, cnt, cur, g.n_keys);
+                hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255) / 256), dim3(256), 0, s, *p, D.seeds, n, cur,
+                                   order_all + seed_off);
+            }
+            hipLaunchKernelGGL(k_build_quads, dim3(grid_for(p->nxy_padded)), dim3(256), 0, s, *p,
+                               (const float2*)J.uv, quad_all + pix_off);
+        }
+        work_off += (uint64_t)n * d * d;
+        seed_off += n;
+        pix_off += p->nxy_padded;
+        if (++in_pack == kDescPerLaunch || j + 1 == n_jobs) {
+            hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, s, pack, pack_first, in_pack, desc);
+            pack_first += in_pack;
+            in_pack = 0;
+        }
+    }
+    hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, ctrl);
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix);
+
+    const uint32_t n_work = (uint32_t)w.n_work;
+    int per_cu = 0;
+    if (C0)
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<true, true>, 256, 0));
+    else
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<false, true>, 256, 0));
+    if (per_cu < 1) per_cu = 1;
+    if (blocks_per_cu && (int)blocks_per_cu < per_cu) per_cu = (int)blocks_per_cu;
+    uint32_t grid = (uint32_t)di.cus * (uint32_t)per_cu;
+    const uint32_t max_useful = (n_work + 255) / 256;
+    if (grid > max_useful) grid = max_useful ? max_useful : 1;
+    unsigned long long* st = (unsigned long long*)steps_out;
+    if (n_work) {
+        if (C0)
+            hipLaunchKernelGGL((k_fields_batch<true, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
+                               n_work, st, ctrl, force, chunk, slices);
+        else
+            hipLaunchKernelGGL((k_fields_batch<false, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
+                               n_jobs, n_work, st, ctrl, force, chunk, slices);
+    }
+    hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix, ctrl);
+    hipLaunchKernelGGL(k_batch_commit, dim3(grid_for(w.n_pix)), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
+                       (size_t)w.n_pix, ctrl, accumulate ? 1u : 0u, force, (const unsigned long long*)acc_all);
+    if (n_work) {
+        if (C0)
+            hipLaunchKernelGGL((k_fields_batch<true, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices);
+        else
+            hipLaunchKernelGGL((k_fields_batch<false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices);
+    }
+    return slm_finish(stream, flags);
+}

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .params import SlmKdeParams, SlmParams, SlmTileOpts, ABI_VERSION
+from .params import SlmFieldsJob, SlmBatchOpts, SlmKdeParams, SlmParams, SlmTileOpts, ABI_VERSION
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
@@ -46,6 +46,9 @@ _PROTOS = {
     "slm_kde_histogram": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_map_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
+    "slm_trace_fields_batch_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmFieldsJob), _U32]),
+    "slm_trace_fields_batch": (ctypes.c_int, [ctypes.POINTER(SlmFieldsJob), _U32, ctypes.POINTER(SlmBatchOpts), _P,
+                                              _P, ctypes.c_size_t, _P, _U32]),
     "slm_gradient_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, _P, _P, _P, _U32]),
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),
     "slm_fields_finish_scaled": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _P, _P, _P,
@@ -214,6 +217,50 @@ def hillslope_lengths(p: SlmParams, seeds, mask, uv, mapping, thin_flag, from_fl
     check(load().slm_hillslope_lengths(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(mapping),
                                        int(thin_flag), int(from_flag), _ptr(length), _stream(stream),
                                        SLM_FLAG_SYNC if sync else 0))
+
+
+def batch_opts(chunk=0, blocks_per_cu=0, queue_slices=0) -> SlmBatchOpts:
+    o = SlmBatchOpts()
+    o.chunk, o.blocks_per_cu, o.queue_slices = chunk, blocks_per_cu, queue_slices
+    return o
+
+
+def fields_jobs(jobs) -> "ctypes.Array":
+    """ctypes SlmFieldsJob array from (params, seeds, mask, uv, slc, slt) tuples
+    of an SlmParams and device tensors."""
+    arr = (SlmFieldsJob * len(jobs))()
+    for i, (p, seeds, mask, uv, slc, slt) in enumerate(jobs):
+        _require_cuda(seeds, mask, uv, slc, slt)
+        arr[i].params = p
+        arr[i].seeds_xy = seeds.data_ptr()
+        arr[i].mask = mask.data_ptr()
+        arr[i].uv = uv.data_ptr()
+        arr[i].slc = slc.data_ptr()
+        arr[i].slt = slt.data_ptr()
+    return arr
+
+
+def batch_workspace_size(job_array) -> int:
+    n = load().slm_trace_fields_batch_workspace_size(job_array, len(job_array))
+    if n == 0:
+        check(-1)
+    return int(n)
+
+
+def trace_fields_batch(job_array, steps=None, workspace=None, opts=None, stream=None, accumulate=False,
+                       keep_order=False, force_fallback=False, sync=False):
+    """slm_trace_fields_batch over a fields_jobs() array; returns the workspace."""
+    import torch
+    _require_cuda(steps, workspace)
+    need = batch_workspace_size(job_array)
+    if workspace is None or workspace.numel() < need:
+        dev = torch.device("cuda", torch.cuda.current_device())
+        workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+    flags = (SLM_FLAG_NO_ZERO if accumulate else 0) | (SLM_FLAG_SYNC if sync else 0) | \
+        (SLM_FLAG_KEEP_ORDER if keep_order else 0) | (SLM_FLAG_FORCE_FALLBACK if force_fallback else 0)
+    check(load().slm_trace_fields_batch(job_array, len(job_array), ctypes.byref(opts) if opts is not None else None,
+                                        _ptr(steps), _ptr(workspace), workspace.numel(), _stream(stream), flags))
+    return workspace
 
 
 def map_workspace_size(p: SlmParams) -> int:

@@ -87,6 +87,22 @@ class SlmTileOpts(ctypes.Structure):
                 ("cap_steps", ctypes.c_uint32), ("halo", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
+class SlmFieldsJob(ctypes.Structure):
+    """Mirror of ``struct SlmFieldsJob`` in include/slm_hip.h (192 bytes)."""
+    _fields_ = [("params", SlmParams), ("seeds_xy", ctypes.c_void_p), ("mask", ctypes.c_void_p),
+                ("uv", ctypes.c_void_p), ("slc", ctypes.c_void_p), ("slt", ctypes.c_void_p),
+                ("reserved", ctypes.c_uint64 * 3)]
+
+
+assert ctypes.sizeof(SlmFieldsJob) == 192
+
+
+class SlmBatchOpts(ctypes.Structure):
+    """Mirror of ``struct SlmBatchOpts`` in include/slm_hip.h (32 bytes)."""
+    _fields_ = [("chunk", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("queue_slices", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
+
+
 _libc = ctypes.CDLL(None)
 _libc.strtof.restype = ctypes.c_float
 _libc.strtof.argtypes = [ctypes.c_char_p, ctypes.c_void_p]

@@ -164,3 +164,30 @@ def y_valley(n=64, pixel_size=1.0, side_slope=0.2, down_slope=0.05, junction=0.6
     mapping[full[ox:ox + nxp, oy:oy + nyp]] = np.uint32(job.info.is_channel)
     mapping[job.mask] = 0
     return job, mapping
+
+
+def batch_c3(n_tiles=64, seed=1, n_cones=16, d=1, jitter=1.0, max_length=2500.0, side_range=(256, 1536),
+             relief=40.0):
+    """Batched uneven tiles: n_tiles DTMs whose sides are drawn log-uniformly
+    from side_range (x and y independently) with RNG seed ``seed``, pixel size
+    1 m or 2 m; n_cones of them are alluvial-fan-like cones with pits left
+    unfilled, so trajectories range from a few steps to the step cap.
+    max_length 2500 m gives MAX_N_STEPS = 5000 at the 0.5 step factor.
+    Returns a list of Jobs."""
+    rs = np.random.default_rng(seed)
+    lo, hi = np.log(side_range[0]), np.log(side_range[1])
+    sides = np.exp(rs.uniform(lo, hi, size=(n_tiles, 2))).astype(int)
+    pixel = rs.choice([1.0, 2.0], size=n_tiles)
+    cones = set(int(i) for i in rs.choice(n_tiles, size=min(n_cones, n_tiles), replace=False))
+    jobs = []
+    for i in range(n_tiles):
+        nx, ny = int(sides[i, 0]), int(sides[i, 1])
+        ps = float(pixel[i])
+        cone = i in cones
+        roi = synth.make_dtm((nx, ny), pixel_size=ps, relief=relief, tilt=0.05, seed=seed * 1000 + i,
+                             nan_border=2, n_valleys=0 if cone else 1, cone=cone,
+                             n_pits=max(4, nx * ny // 20000) if cone else 0, fill=not cone)
+        jobs.append(job_from_dtm("c3_%02d_%dx%d_%gm%s" % (i, nx, ny, ps, "_cone" if cone else ""), roi, ps,
+                                 {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
+                                  "max_length": float(max_length), "integrator_step_factor": 0.5}))
+    return jobs
