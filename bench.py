@@ -68,12 +68,98 @@ def cpu_baseline(job, target_s=12.0):
                       (n, n_total, int(job.info.subpixel_seed_point_density) ** 2, steps, dt)}
 
 
+def bench_batch(args):
+    """One step = every tile of the batched workload traced in both directions by
+    one slm_trace_fields_batch call (seed sort included), replayed from a hipGraph."""
+    import torch
+    from slm_amd import autotune, hip, workloads
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    tiles = workloads.batch_c3()
+    gen_s = time.perf_counter() - t0
+    jobs = []
+    for t in tiles:
+        seeds = torch.from_numpy(t.seeds).to(dev)
+        mask = torch.from_numpy(t.mask.astype(np.uint8).reshape(-1)).to(dev)
+        uv = torch.from_numpy(np.ascontiguousarray(t.uv).reshape(-1)).to(dev)
+        for sign in (1, -1):
+            p = t.params(sign)
+            slc = torch.empty(p.nxy_padded, dtype=torch.int32, device=dev)
+            jobs.append((p, seeds, mask, uv, slc, torch.empty_like(slc)))
+    arr = hip.fields_jobs(jobs)
+    ws = torch.empty(hip.batch_workspace_size(arr), dtype=torch.uint8, device=dev)
+    steps = torch.zeros(1, dtype=torch.int64, device=dev)
+    hip.trace_fields_batch(arr, steps=steps, workspace=ws, sync=True)
+    default_ms = autotune.time_batch(arr, hip.batch_opts(), ws, repeats=2)
+    table = []
+    if args.no_tune:
+        opts = hip.batch_opts()
+    else:
+        opts, table = autotune.tune_batch(arr, workspace=ws, repeats=2)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            hip.trace_fields_batch(arr, steps=steps, workspace=ws, opts=opts)
+    torch.cuda.current_stream().wait_stream(side)
+    for _ in range(args.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    steps.zero_()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        g.replay()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_steps = int(steps.item())
+    mean_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    steps_per_call = total_steps / args.steps
+    achieved = steps_per_call * BYTES_PER_STEP / (mean_ms / 1e3) / 1e9
+    out = {
+        "metric": "streamline-steps/s",
+        "value": total_steps / wall,
+        "unit": "streamline-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (64 seeded DTM tiles, 16 fans with unfilled pits)",
+        "config": {"workload": "batch_c3", "tiles": len(tiles), "jobs": len(jobs),
+                   "sides": [list(t.roi.shape) for t in tiles],
+                   "pixels": int(sum(t.mask.size for t in tiles)), "seeds": int(sum(t.seeds.shape[0] for t in tiles)),
+                   "max_n_steps": int(tiles[0].info.max_n_steps), "subpixel_density": 1, "jitter": 1.0,
+                   "directions": 2, "host_generation_s": round(gen_s, 1)},
+        "batch": {"graph": True, "opts": {"chunk": opts.chunk, "blocks_per_cu": opts.blocks_per_cu,
+                                          "queue_slices": opts.queue_slices},
+                  "default_opts_ms": round(default_ms, 3), "graph_replay_ms": round(mean_ms, 3),
+                  "autotune_table": [list(r) for r in table]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_fields_batch",
+                     "bytes_per_step": BYTES_PER_STEP, "steps_per_launch": steps_per_call,
+                     "mean_launch_ms": mean_ms},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256"])
+    ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256", "batch_c3"],
+                    help="batch_c3: 64 uneven tiles (log-uniform 256..1536 px, 1 or 2 m, 16 fans with pits, "
+                         "MAX_N_STEPS 5000), both directions, one persistent launch captured in a hipGraph "
+                         "with autotuned queue parameters")
+    ap.add_argument("--no-tune", action="store_true", help="batch_c3: skip the autotuner (library defaults)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", default="queue", choices=["tiled", "queue"],
                     help="tiled: LDS-owned tiles (slm_trace_fields_tiled); queue: global work queue")
@@ -106,6 +192,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n}), flush=True)
         return
+    if args.workload == "batch_c3":
+        return bench_batch(args)
     job = getattr(workloads, args.workload)()
     seeds = torch.from_numpy(job.seeds).to(dev)
     mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)

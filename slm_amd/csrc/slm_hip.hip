@@ -1477,9 +1477,108 @@ struct BatchDesc {
     const SeedRec* order;
     uint32_t work_begin, n_work;
     uint32_t pix_begin, dt_fixed;
+    uint32_t seed_begin, n_seeds;   // this job's slice of the concatenated seed records
+    uint32_t key_begin, n_keys;     // and of the concatenated Morton-tile counters
 };
 
 constexpr uint32_t kDescPerLaunch = 8;
+
+// Binary search: the job whose [begin, next begin) holds i, for the member
+// `field` (jobs are stored in increasing order of every begin).
+#define BATCH_FIND(desc, n_jobs, field, i, out)                    \
+    do {                                                           \
+        uint32_t lo_ = 0, hi_ = (n_jobs);                          \
+        while (hi_ - lo_ > 1u) {                                   \
+            const uint32_t mid_ = (lo_ + hi_) >> 1;                \
+            if ((desc)[mid_].field <= (i)) lo_ = mid_; else hi_ = mid_; \
+        }                                                          \
+        (out) = lo_;                                               \
+    } while (0)
+
+// The seed sort and quad build of every job, one launch each for all jobs.
+__global__ void k_batch_count(const BatchDesc* __restrict__ desc, uint32_t n_jobs, uint32_t n_seeds,
+                              uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_seeds) return;
+    uint32_t j;
+    BATCH_FIND(desc, n_jobs, seed_begin, i, j);
+    const BatchDesc& D = desc[j];
+    const uint32_t k = i - D.seed_begin;
+    atomicAdd(&cnt[D.key_begin + seed_tile_key(D.P, D.seeds[D.P.seeds_chunk_offset + k])], 1u);
+}
+
+// Exclusive scan of each job's counters: one 1024-thread block per job.
+__global__ __launch_bounds__(1024) void k_batch_scan(const BatchDesc* __restrict__ desc, uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ cur) {
+    const BatchDesc& D = desc[blockIdx.x];
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x, n = D.n_keys;
+    uint32_t* c = cnt + D.key_begin;
+    uint32_t* u = cur + D.key_begin;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = t * per, e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += c[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = (t == 0) ? 0u : part[t - 1];
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t x = c[i];
+        c[i] = run;
+        u[i] = run;
+        run += x;
+    }
+}
+
+__global__ void k_batch_scatter(const BatchDesc* __restrict__ desc, uint32_t n_jobs, uint32_t n_seeds,
+                                uint32_t* __restrict__ cur, SeedRec* __restrict__ order_all) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_seeds) return;
+    uint32_t j;
+    BATCH_FIND(desc, n_jobs, seed_begin, i, j);
+    const BatchDesc& D = desc[j];
+    const uint32_t k = i - D.seed_begin;
+    const float2 sd = D.seeds[D.P.seeds_chunk_offset + k];
+    SeedRec r;
+    r.x = sd.x;
+    r.y = sd.y;
+    r.seed_idx = D.P.seeds_chunk_offset + k;
+    r.pad = 0u;
+    order_all[D.seed_begin + atomicAdd(&cur[D.key_begin + seed_tile_key(D.P, sd)], 1u)] = r;
+}
+
+__global__ void k_batch_quads(const BatchDesc* __restrict__ desc, uint32_t n_jobs, size_t n_pix,
+                              const float2* const* __restrict__ uvs, Quad* __restrict__ quad_all) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix; i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t j;
+        BATCH_FIND(desc, n_jobs, pix_begin, i, j);
+        const SlmParams& P = desc[j].P;
+        const float2* uv = uvs[j];
+        const size_t li = i - desc[j].pix_begin;
+        const uint32_t x = (uint32_t)(li / P.ny_padded), y = (uint32_t)(li - (size_t)x * P.ny_padded);
+        const uint32_t xr = min(P.nx_padded - 1u, x + 1u), yu = min(P.ny_padded - 1u, y + 1u);
+        const float2 a = uv[(size_t)P.ny_padded * x + y], b = uv[(size_t)P.ny_padded * xr + y];
+        const float2 e = uv[(size_t)P.ny_padded * x + yu], f = uv[(size_t)P.ny_padded * xr + yu];
+        Quad q;
+        q.lo = make_float4(a.x, a.y, b.x, b.y);
+        q.hi = make_float4(e.x, e.y, f.x, f.y);
+        quad_all[i] = q;
+    }
+}
+
+struct UvPack {
+    const float2* uv[kDescPerLaunch];
+};
+__global__ void k_put_uv(UvPack pack, uint32_t first, uint32_t count, const float2** __restrict__ out) {
+    const uint32_t i = threadIdx.x;
+    if (i < count) out[first + i] = pack.uv[i];
+}
 struct DescPack {
     BatchDesc d[kDescPerLaunch];
 };
@@ -1696,33 +1795,33 @@ __global__ void k_batch_commit(const BatchDesc* __restrict__ desc, uint32_t n_jo
 }
 
 struct BatchWs {
-    size_t off_desc, off_cnt, off_cur, off_order, off_acc, off_quad, total;
-    uint32_t max_keys;
-    uint64_t n_seeds, n_pix, n_work;
+    size_t off_desc, off_uvs, off_cnt, off_cur, off_order, off_acc, off_quad, total;
+    uint64_t n_keys, n_seeds, n_pix, n_work;
 };
 
 static int batch_ws(const SlmFieldsJob* jobs, uint32_t n_jobs, BatchWs& w) {
-    w.max_keys = 1;
-    w.n_seeds = w.n_pix = w.n_work = 0;
+    w.n_keys = w.n_seeds = w.n_pix = w.n_work = 0;
     for (uint32_t j = 0; j < n_jobs; j++) {
         const SlmParams* p = &jobs[j].params;
         int rc = check_params(p);
         if (rc) return rc;
         const TileGeom g = tile_geom(p);
-        if (g.n_keys > w.max_keys) w.max_keys = g.n_keys;
+        w.n_keys += g.n_keys;
         const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
         const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
         w.n_seeds += n;
         w.n_pix += p->nxy_padded;
         w.n_work += (uint64_t)n * d * d;
     }
-    if (w.n_work >= (1ull << 32) - 4096 || w.n_pix >= (1ull << 32))
+    if (w.n_work >= (1ull << 32) - 4096 || w.n_pix >= (1ull << 32) || w.n_keys >= (1ull << 32) ||
+        w.n_seeds >= (1ull << 32))
         return slm_fail(SLM_E_INVALID_ARG, "batch too large: %llu work items, %llu pixels",
                         (unsigned long long)w.n_work, (unsigned long long)w.n_pix);
     w.off_desc = align256(sizeof(FieldsCtrl)) > 4096 ? align256(sizeof(FieldsCtrl)) : 4096;
-    w.off_cnt = w.off_desc + align256((size_t)n_jobs * sizeof(BatchDesc));
-    w.off_cur = w.off_cnt + align256((size_t)w.max_keys * 4);
-    w.off_order = w.off_cur + align256((size_t)w.max_keys * 4);
+    w.off_uvs = w.off_desc + align256((size_t)n_jobs * sizeof(BatchDesc));
+    w.off_cnt = w.off_uvs + align256((size_t)n_jobs * sizeof(void*));
+    w.off_cur = w.off_cnt + align256((size_t)w.n_keys * 4);
+    w.off_order = w.off_cur + align256((size_t)w.n_keys * 4);
     w.off_acc = w.off_order + align256((size_t)w.n_seeds * sizeof(SeedRec));
     w.off_quad = w.off_acc + align256((size_t)w.n_pix * 8);
     w.total = w.off_quad + align256((size_t)w.n_pix * sizeof(Quad));
@@ -1778,15 +1877,19 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     unsigned long long* acc_all = (unsigned long long*)(ws + w.off_acc);
     Quad* quad_all = (Quad*)(ws + w.off_quad);
 
-    // descriptors, sort and quads per job
+    const float2** uvs = (const float2**)(ws + w.off_uvs);
+    // descriptors (and the uv pointer table) through kernel arguments, so the
+    // call stays capturable; then the seed sort and quads of all jobs at once
     DescPack pack;
+    UvPack upack;
     uint32_t in_pack = 0, pack_first = 0;
-    uint64_t work_off = 0, seed_off = 0, pix_off = 0;
+    uint64_t work_off = 0, seed_off = 0, pix_off = 0, key_off = 0;
     for (uint32_t j = 0; j < n_jobs; j++) {
         const SlmFieldsJob& J = jobs[j];
         const SlmParams* p = &J.params;
         const uint32_t n = p->n_seed_points - p->seeds_chunk_offset;
         const uint32_t d = p->subpixel_seed_point_density ? p->subpixel_seed_point_density : 1u;
+        const TileGeom g = tile_geom(p);
         BatchDesc& D = pack.d[in_pack];
         D.P = *p;
         D.seeds = (const float2*)J.seeds_xy;
@@ -1800,26 +1903,34 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
         D.n_work = n * d * d;
         D.pix_begin = (uint32_t)pix_off;
         D.dt_fixed = dt_is_fixed(p);
-        if (!(flags & SLM_FLAG_KEEP_ORDER)) {
-            if (n) {
-                const TileGeom g = tile_geom(p);
-                fill_u32(cnt, g.n_keys, 0u, s);
-                hipLaunchKernelGGL(k_tile_count, dim3((n + 255) / 256), dim3(256), 0, s, *p, D.seeds, n, cnt);
-                hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, cnt, cur, g.n_keys);
-                hipLaunchKernelGGL(k_tile_scatter, dim3((n + 255) / 256), dim3(256), 0, s, *p, D.seeds, n, cur,
-                                   order_all + seed_off);
-            }
-            hipLaunchKernelGGL(k_build_quads, dim3(grid_for(p->nxy_padded)), dim3(256), 0, s, *p,
-                               (const float2*)J.uv, quad_all + pix_off);
-        }
+        D.seed_begin = (uint32_t)seed_off;
+        D.n_seeds = n;
+        D.key_begin = (uint32_t)key_off;
+        D.n_keys = g.n_keys;
+        upack.uv[in_pack] = (const float2*)J.uv;
         work_off += (uint64_t)n * d * d;
         seed_off += n;
         pix_off += p->nxy_padded;
+        key_off += g.n_keys;
         if (++in_pack == kDescPerLaunch || j + 1 == n_jobs) {
             hipLaunchKernelGGL(k_put_desc, dim3(1), dim3(64), 0, s, pack, pack_first, in_pack, desc);
+            hipLaunchKernelGGL(k_put_uv, dim3(1), dim3(64), 0, s, upack, pack_first, in_pack, uvs);
             pack_first += in_pack;
             in_pack = 0;
         }
+    }
+    if (!(flags & SLM_FLAG_KEEP_ORDER)) {
+        const uint32_t ns = (uint32_t)w.n_seeds;
+        fill_u32(cnt, (size_t)w.n_keys, 0u, s);
+        if (ns) {
+            hipLaunchKernelGGL(k_batch_count, dim3((ns + 255) / 256), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
+                               ns, cnt);
+            hipLaunchKernelGGL(k_batch_scan, dim3(n_jobs), dim3(1024), 0, s, (const BatchDesc*)desc, cnt, cur);
+            hipLaunchKernelGGL(k_batch_scatter, dim3((ns + 255) / 256), dim3(256), 0, s, (const BatchDesc*)desc,
+                               n_jobs, ns, cur, order_all);
+        }
+        hipLaunchKernelGGL(k_batch_quads, dim3(grid_for(w.n_pix)), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
+                           (size_t)w.n_pix, (const float2* const*)uvs, quad_all);
     }
     hipLaunchKernelGGL(k_fields_init, dim3(1), dim3(64), 0, s, ctrl);
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix);
