@@ -150,16 +150,130 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_surrogate(args):
+    """Surrogate training on traced tiles, then the surrogate-gated map of a
+    large DTM against plain tracing of the same DTM.  One step = one gated
+    map (inference on overlapping 256^2 tiles, 2% of the tiles re-traced and
+    checked, failures replaced); the plain map (every pixel traced) is timed
+    the same way.  Both start from the DTM's uv, mask and h resident in HBM."""
+    import torch
+    from slm_amd import gated
+    from slm_amd import surrogate as S
+    dev = torch.device("cuda", 0)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.weights and os.path.exists(args.weights):
+        z = np.load(args.weights)
+        params = {name: (z[name + ".w"], z[name + ".b"]) for name, *_ in S.LAYERS}
+        theta = float(z["theta"])
+        train_rep = {"loaded": args.weights}
+    else:
+        data = gated.training_set(n_dtms=args.train_dtms, n=2048, size=128, log=log, device=dev)
+        const = gated.constant_loss(data)
+        net, hist = gated.train(data, epochs=args.epochs, batch=64, lr=1e-3, log=log, device=dev)
+        params, theta = net.params_numpy(), data.theta
+        train_rep = {"tiles": data.n_tiles, "tile": data.size, "epochs": args.epochs, "batch": 64, "lr": 1e-3,
+                     "theta": theta, "constant_loss": const, "loss_first_epoch": hist["loss"][0],
+                     "loss_last_epoch": hist["loss"][-1], "train_s": hist["train_ms"] / 1e3,
+                     "tiles_per_s": hist["tiles_per_s"], "train_tflops": hist["tflops"],
+                     "data_generate_s": data.timings["generate_s"], "data_trace_s": data.timings["trace_s"]}
+        del data, net
+        if args.save_weights:
+            np.savez(args.save_weights, theta=theta,
+                     **{k + ".w": w for k, (w, b) in params.items()}, **{k + ".b": b for k, (w, b) in params.items()})
+    torch.cuda.empty_cache()
+    inf = S.UNet(params, device=dev, inference=True)
+    t0 = time.perf_counter()
+    roi = gated.family_dtm(args.map_n, args.map_seed)
+    gen_s = time.perf_counter() - t0
+    grid = gated.DeviceGrid(roi, device=dev)
+    log("map grid %d x %d, generated in %.1f s" % (grid.nx, grid.ny, gen_s))
+
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        r = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return r, a.elapsed_time(b)
+
+    steps = torch.zeros(1, dtype=torch.int64, device=dev)
+    plain, _ = gated.plain_map(grid, theta, steps=steps)            # warm-up (and the reference fields)
+    plain_ms = []
+    for _ in range(args.steps):
+        grid._seeds = None                                          # seed list built inside the timed map
+        (plain, _), ms = timed(lambda: gated.plain_map(grid, theta))
+        plain_ms.append(ms)
+    plain = plain.clone()
+    gkw = dict(size=256, halo=16, batch=64, check_frac=0.02, rng_seed=2, max_err=0.05, min_f1=0.95)
+    for _ in range(args.warmup):
+        gated.gated_map(inf, grid, theta, **gkw)
+    gated_ms = []
+    for _ in range(args.steps):
+        (res, ms) = timed(lambda: gated.gated_map(inf, grid, theta, **gkw))
+        gated_ms.append(ms)
+    out, rep = res
+    (pred, _), pred_ms = timed(lambda: gated.predict(inf, grid, 256, 16, 64))
+    fwd = []
+    for _ in range(3):
+        _, ms = timed(lambda: inf.forward(inf.A["x"], 64, 256, 256))
+        fwd.append(ms)
+    fwd_ms = float(np.median(fwd))
+    fwd_tflops = inf.flops_per_pixel() * 64 * 256 * 256 / (fwd_ms / 1e3) / 1e12
+    m_gated = gated.compare(out, plain, grid.mask)
+    m_pred = gated.compare(pred, plain, grid.mask)
+    tp, tg = float(np.median(plain_ms)), float(np.median(gated_ms))
+    peak = 2500.0
+    res = {
+        "metric": "surrogate-gated vs plain mapping speedup",
+        "value": tp / tg,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tg,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded DTMs, surrogate trained on their traces; random-init weights trained here)",
+        "config": {"workload": "surrogate_gated_%d" % args.map_n, "map_px": grid.nx * grid.ny,
+                   "tile": 256, "halo": 16, "check_frac": 0.02, "check_rng_seed": 2,
+                   "tolerance": {"median_rel_err": 0.05, "channel_f1": 0.95},
+                   "trace": gated.TRACE, "directions": 1, "train_dtms": args.train_dtms},
+        "surrogate": {"train": train_rep,
+                      "map": {"plain_ms": tp, "gated_ms": tg, "predict_only_ms": pred_ms,
+                              "plain_steps": int(steps.item()), "host_generation_s": round(gen_s, 1),
+                              "gated_vs_plain": m_gated, "prediction_vs_plain": m_pred,
+                              "tiles": rep["tiles"], "checked": rep["checked"], "failed": rep["failed"],
+                              "fallback": rep["fallback"], "check_steps": rep["check_steps"],
+                              "checks": [{k: c[k] for k in ("median_rel_err", "f1", "channel_px", "ok")}
+                                         for c in rep["checks"]]}},
+        "roofline": {"bound": "mfma", "achieved": fwd_tflops, "peak": peak, "unit": "TFLOP/s",
+                     "frac": fwd_tflops / peak, "traffic": None, "kernel": "network forward (k_conv)",
+                     "flop_per_px": inf.flops_per_pixel(), "px_per_launch": 64 * 256 * 256,
+                     "mean_forward_ms": fwd_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256", "batch_c3"],
+    ap.add_argument("--workload", default="tile_2048", choices=["tile_2048", "small_256", "batch_c3", "surrogate"],
                     help="batch_c3: 64 uneven tiles (log-uniform 256..1536 px, 1 or 2 m, 16 fans with pits, "
                          "MAX_N_STEPS 5000), both directions, one persistent launch captured in a hipGraph "
                          "with autotuned queue parameters")
     ap.add_argument("--no-tune", action="store_true", help="batch_c3: skip the autotuner (library defaults)")
+    ap.add_argument("--train-dtms", type=int, default=32, help="surrogate: traced 2048^2 DTMs for training")
+    ap.add_argument("--epochs", type=int, default=20, help="surrogate: training epochs")
+    ap.add_argument("--map-n", type=int, default=8192, help="surrogate: side of the gated map's DTM")
+    ap.add_argument("--map-seed", type=int, default=2, help="surrogate: seed of the gated map's DTM")
+    ap.add_argument("--weights", default="", help="surrogate: load trained weights (.npz) instead of training")
+    ap.add_argument("--save-weights", default="", help="surrogate: save the trained weights (.npz)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", default="queue", choices=["tiled", "queue"],
                     help="tiled: LDS-owned tiles (slm_trace_fields_tiled); queue: global work queue")
@@ -194,6 +308,8 @@ def main():
         return
     if args.workload == "batch_c3":
         return bench_batch(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     job = getattr(workloads, args.workload)()
     seeds = torch.from_numpy(job.seeds).to(dev)
     mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(dev)

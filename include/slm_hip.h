@@ -465,6 +465,75 @@ int slm_fix_left_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t
                         const float* uv, const uint32_t* mapping, uint32_t* label,
                         void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
 
+/* ------------------------------------------------------------------------
+ * Surrogate kernels (no counterpart in the reference): a BF16 convolutional
+ * U-Net trained on the tracer's own fields, used to predict slt/slc/channel
+ * masks on large DTMs with a traced check and fallback.  Activations are
+ * NHWC bf16: pixel p's channels start at base + p*ld (ld in elements), so a
+ * concatenation is channel slices of one buffer.  Weights are bf16
+ * [Cout][taps][Cin]; tap t of a 3x3 kernel reads row offset t/3-1 and
+ * column offset t%3-1 (zero padding).  Products accumulate in fp32 on MFMA.
+ * Requirements: N*H*W a multiple of 32, Cin a multiple of 16, Cout of 32,
+ * ld a multiple of 8, buffers 16-byte aligned.
+ * ---------------------------------------------------------------------- */
+
+/* y = conv(x, w) + bias, then ReLU when relu != 0 (bias may be NULL). */
+int slm_nn_conv(const void* x, int ldx, const void* w, const float* bias, void* y, int ldy,
+                int N, int H, int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags);
+/* wt[ci][taps-1-t][co] = w[co][t][ci]: weights of the backward-data
+ * convolution (slm_nn_conv on dY with Cin and Cout exchanged). */
+int slm_nn_wflip(const void* w, void* wt, int taps, int Cin, int Cout, void* stream, uint32_t flags);
+/* dw[co][t][ci] += sum_p dy[p][co] x[p + off(t)][ci] (fp32), db[co] += sum_p dy[p][co]. */
+int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int lddy, float* dw, float* db,
+                      int N, int H, int W, int Cin, int Cout, int taps, void* stream, uint32_t flags);
+/* dy[p][c] = 0 where y[p][c] <= 0 (ReLU backward on the stored output). */
+int slm_nn_relu_bwd(const void* y, int ldy, void* dy, int lddy, int M, int C, void* stream, uint32_t flags);
+/* 2x2 max pool, stride 2; H, W are the input's (even) sizes. */
+int slm_nn_pool(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
+                uint32_t flags);
+/* Max-pool backward: each window's gradient goes to its first maximum (row-major). */
+int slm_nn_pool_bwd(const void* x, int ldx, const void* y, int ldy, const void* dy, int lddy, void* dx,
+                    int lddx, int N, int H, int W, int C, void* stream, uint32_t flags);
+/* Nearest 2x upsample; H, W are the output's sizes. */
+int slm_nn_up(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
+              uint32_t flags);
+/* Upsample backward: dx = sum of the 2x2 block of dy; H, W are dx's sizes. */
+int slm_nn_up_bwd(const void* dy, int lddy, void* dx, int lddx, int N, int H, int W, int C, void* stream,
+                  uint32_t flags);
+/* a += b over M pixels x C channels. */
+int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C, void* stream, uint32_t flags);
+/* Loss over valid pixels: squared error on head channels 0 and 1, binary
+ * cross-entropy with logits (weight w_bce) on channel 2; dhead = gradient
+ * of the total times inv_n (channels >= 3 zero); out[0..2] += the summed terms. */
+int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead,
+                int lddh, int M, float inv_n, float w_bce, float* out, void* stream, uint32_t flags);
+/* One Adam step (bias-corrected, step >= 1) on fp32 p with moments m, v;
+ * writes the bf16 copy pb when not NULL. */
+int slm_nn_adam(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1,
+                float b2, float eps, int step, void* stream, uint32_t flags);
+
+/* Tiles: S x S windows of an nx x ny grid (x-major, pixel (i, j) at i*ny + j)
+ * with top-left corners tiles[2t], tiles[2t+1] (may lie outside the grid;
+ * pixels outside it count as masked).  Tile t's pixels are rows
+ * t*S*S .. (t+1)*S*S - 1 of the NHWC buffers. */
+/* Network input, 16 bf16 channels: (h - tile mean of valid h) / h_scale, u,
+ * v, valid (1/0), then zeros; mean_ws holds n_tiles floats of workspace. */
+int slm_nn_tile_inputs(const float* h, const float* uv, const uint8_t* mask, int nx, int ny, const int* tiles,
+                       int n_tiles, int S, float h_scale, float* mean_ws, void* x, void* stream, uint32_t flags);
+/* Training targets from the raw downstream fields (uint32): log1p(slt/d^2),
+ * log1p(slc/d^2), slc/d^2 >= threshold, as float32 [.., 3]; valid [..] u8. */
+int slm_nn_tile_targets(const uint32_t* slc, const uint32_t* slt, const uint8_t* mask, int nx, int ny,
+                        const int* tiles, int n_tiles, int S, int density, float threshold, float* target,
+                        uint8_t* valid, void* stream, uint32_t flags);
+/* Writes head channels 0..2 of each tile's core [halo, S-halo)^2 to
+ * out[k*nx*ny + pixel] (float32, planar), for core pixels inside the grid. */
+int slm_nn_tile_stitch(const void* head, int ldh, const int* tiles, int n_tiles, int S, int halo, int nx, int ny,
+                       float* out, void* stream, uint32_t flags);
+/* Traced raw fields (uint32) -> planar float32 out[3][nxy]: log1p(slt/d^2),
+ * log1p(slc/d^2), +1/-1 for slc/d^2 >= threshold; masked pixels 0, 0, -1. */
+int slm_nn_fields_log(const uint32_t* slc, const uint32_t* slt, const uint8_t* mask, size_t nxy, int density,
+                      float threshold, float* out, void* stream, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

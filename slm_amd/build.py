@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip", "prep.hip"]
+HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip", "prep.hip", "surrogate.hip"]
 
 
 def _run(cmd):
@@ -46,7 +46,7 @@ def build_hip(out=None):
 def build_host(out=None):
     out = out or os.path.join(HERE, "libslm_host.so")
     tmp = out + ".tmp"
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", os.path.join(CSRC, "dtm_fill.c"), "-o", tmp, "-lm"])
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", os.path.join(CSRC, "dtm_fill.c"), "-o", tmp, "-lm"])
     os.replace(tmp, out)
     return out
 

@@ -81,3 +81,35 @@ int slm_fill_sinks(double* h, int64_t nx, int64_t ny, double eps) {
     free(hp);
     return 0;
 }
+
+/* V-shaped incisions for the synthetic DTM generator: for each valley k,
+ * out[x][y] -= depth * clip(1 - |(x - cx)*s - (y - cy)*c| / hw, 0, 1), valleys
+ * applied in order.  Only the band of columns where the distance can be
+ * below hw is visited (outside it the term is exactly zero), so the result
+ * has the bits of the same expression evaluated over the whole grid.
+ * out: float64 [nx*ny] C-order (x major), accumulated in place. */
+int slm_v_valleys(double* out, int64_t nx, int64_t ny, int n, const double* cx, const double* cy,
+                  const double* s, const double* c, double depth, double hw) {
+    if (!out || nx <= 0 || ny <= 0 || hw <= 0.0) return -1;
+    for (int k = 0; k < n; k++) {
+        for (int64_t x = 0; x < nx; x++) {
+            const double a = ((double)x - cx[k]) * s[k];
+            int64_t y0 = 0, y1 = ny - 1;
+            if (fabs(c[k]) > 1e-9) {
+                const double ya = cy[k] + (a - hw) / c[k], yb = cy[k] + (a + hw) / c[k];
+                const double lo = floor(fmin(ya, yb)) - 2.0, hi = ceil(fmax(ya, yb)) + 2.0;
+                if (hi < 0.0 || lo > (double)(ny - 1)) continue;
+                y0 = lo > 0.0 ? (int64_t)lo : 0;
+                y1 = hi < (double)(ny - 1) ? (int64_t)hi : ny - 1;
+            }
+            double* row = out + x * ny;
+            for (int64_t y = y0; y <= y1; y++) {
+                const double d = fabs(a - ((double)y - cy[k]) * c[k]);
+                double t = 1.0 - d / hw;
+                t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+                row[y] -= depth * t;
+            }
+        }
+    }
+    return 0;
+}

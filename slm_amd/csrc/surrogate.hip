@@ -1,0 +1,614 @@
+// surrogate.hip -- kernels of the BF16 convolutional surrogate of the fields.
+//
+// A U-Net learns log(1+slt), log(1+slc) and a channel mask from the DTM, on
+// fields the tracer produced; it has no counterpart in the reference.
+// Activations are NHWC bf16 (channels contiguous, a pixel's channels at
+// base + pixel*ld so concatenations are channel slices of one buffer);
+// weights are bf16 [Cout][taps][Cin] with fp32 master copies; every product
+// accumulates in fp32 on MFMA (v_mfma_f32_32x32x16_bf16).
+//
+//   conv (3x3 pad 1, or 1x1)  implicit GEMM: M = pixels, N = Cout, K = taps*Cin;
+//                             bias and ReLU fused into the epilogue
+//   conv backward-data        the same kernel on dY with weights flipped and
+//                             transposed (k_wflip)
+//   conv backward-weight      dW = dY^T X: pixels are the reduction; tiles of
+//                             dY and im2col(X) staged transposed in LDS
+//   maxpool 2x2, nearest upsample 2x, ReLU mask, add, loss, Adam
+//
+// MFMA 32x32x16 operand layout (gfx950, checked on the device): lane l holds
+// A[l%32][8*(l/32) .. +8], B[8*(l/32) .. +8][l%32]; the 16 results of lane l
+// are D[(v/4)*8 + 4*(l/32) + v%4][l%32], v = 0..15.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "common.hpp"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float b2f(bf16 v) { return (float)v; }
+__device__ __forceinline__ bf16 f2b(float v) { return (bf16)v; }   // round to nearest even
+
+// ------------------------------------------------------------------ convolution
+
+// One wave computes a 32-pixel x 32-channel tile; a block has WM x WN waves.
+template <int TAPS, bool RELU, int WM, int WN>
+__global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ w,
+                                              const float* __restrict__ bias, bf16* __restrict__ y, int ldy,
+                                              int M, int H, int W, int Cin, int Cout) {
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pm = blockIdx.x * (WM * 32) + (wid % WM) * 32;
+    const int pc = blockIdx.y * (WN * 32) + (wid / WM) * 32;
+    if (pm >= M || pc >= Cout) return;             // wave-uniform
+    const int r = lane & 31, kg = (lane >> 5) * 8;
+    const int p = pm + r;
+    const int pw = p % W, ph = (p / W) % H, pn = p / (W * H);
+    const int co = pc + r;
+    const bf16* wrow = w + (size_t)co * TAPS * Cin;
+    f32x16 acc = {};
+    const bf16x8 zero = {};
+#pragma unroll 1
+    for (int tap = 0; tap < TAPS; tap++) {
+        const int hh = ph + (TAPS == 9 ? tap / 3 - 1 : 0), ww = pw + (TAPS == 9 ? tap % 3 - 1 : 0);
+        const bool ok = hh >= 0 && hh < H && ww >= 0 && ww < W;
+        const bf16* xp = x + (size_t)((pn * H + (ok ? hh : 0)) * W + (ok ? ww : 0)) * ldx + kg;
+        const bf16* wp = wrow + tap * Cin + kg;
+
+        for (int c0 = 0; c0 < Cin; c0 += 16) {
+            const bf16x8 a = ok ? *(const bf16x8*)(xp + c0) : zero;
+            const bf16x8 b = *(const bf16x8*)(wp + c0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+    }
+    const float bc = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+        float val = acc[v] + bc;
+        if (RELU) val = fmaxf(val, 0.0f);
+        y[(size_t)(pm + rr) * ldy + co] = f2b(val);
+    }
+}
+
+// wt[ci][tap'][co] = w[co][8 - tap'][ci] (3x3) or w[co][0][ci] (1x1): the
+// weights of the backward-data convolution.
+__global__ void k_wflip(const bf16* __restrict__ w, bf16* __restrict__ wt, int taps, int Cin, int Cout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = Cout * taps * Cin;
+    if (i >= n) return;
+    const int ci = i % Cin, tap = (i / Cin) % taps, co = i / (Cin * taps);
+    const int tflip = taps - 1 - tap;
+    wt[((size_t)ci * taps + tflip) * Cout + co] = w[i];
+}
+
+// dW[co][tap*Cin+ci] += sum over pixels of dY[p][co] * X[p + off(tap)][ci];
+// block: 64 channels x 64 k, pixels [blockIdx.z*ppb, +ppb) in chunks of 64.
+template <int TAPS>
+__global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ dy,
+                                               int lddy, float* __restrict__ dw, int M, int H, int W, int Cin,
+                                               int Cout, int ppb) {
+    __shared__ __attribute__((aligned(16))) bf16 dyT[64][64 + 8];   // [co][p]
+    __shared__ __attribute__((aligned(16))) bf16 xT[64][64 + 8];    // [k][p]
+    const int K = TAPS * Cin;
+    const int c0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+    const int p_begin = blockIdx.z * ppb, p_end = min(M, p_begin + ppb);
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    const int wc = (wid & 1) * 32, wk = (wid >> 1) * 32;
+    const bool wave_on = (c0 + wc < Cout) && (k0 + wk < K);
+    f32x16 acc = {};
+    for (int pb = p_begin; pb < p_end; pb += 64) {
+        // stage: 64 pixels x 64 channels of dY, 64 pixels x 64 k of im2col(X), transposed
+        for (int v = t; v < 512; v += 256) {
+            const int pl = v >> 3, cv = (v & 7) * 8;
+            const int p = pb + pl;
+            bf16x8 a = {}, b = {};
+            if (p < p_end) {
+                if (c0 + cv < Cout) a = *(const bf16x8*)(dy + (size_t)p * lddy + c0 + cv);
+                const int k = k0 + cv;
+                if (k < K) {
+                    const int tap = k / Cin, ci = k % Cin;
+                    const int pw = p % W, ph = (p / W) % H, pn = p / (W * H);
+                    const int hh = ph + (TAPS == 9 ? tap / 3 - 1 : 0), ww = pw + (TAPS == 9 ? tap % 3 - 1 : 0);
+                    if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                        b = *(const bf16x8*)(x + (size_t)((pn * H + hh) * W + ww) * ldx + ci);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                dyT[cv + j][pl] = a[j];
+                xT[cv + j][pl] = b[j];
+            }
+        }
+        __syncthreads();
+        if (wave_on) {
+            const int r = lane & 31, kg = (lane >> 5) * 8;
+#pragma unroll
+            for (int kk = 0; kk < 64; kk += 16) {
+                const bf16x8 a = *(const bf16x8*)&dyT[wc + r][kk + kg];
+                const bf16x8 b = *(const bf16x8*)&xT[wk + r][kk + kg];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (!wave_on) return;
+    const int kcol = k0 + wk + (lane & 31);
+    if (kcol >= K) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int co = c0 + wc + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+        if (co < Cout) atomicAdd(&dw[(size_t)co * K + kcol], acc[v]);
+    }
+}
+
+// db[co] += sum over pixels of dY[p][co]
+__global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, int lddy, float* __restrict__ db,
+                                               int M, int Cout, int ppb) {
+    const int co = blockIdx.x * 256 + threadIdx.x;
+    if (co >= Cout) return;
+    const int p0 = blockIdx.y * ppb, p1 = min(M, p0 + ppb);
+    float s = 0.0f;
+    for (int p = p0; p < p1; p++) s += b2f(dy[(size_t)p * lddy + co]);
+    atomicAdd(&db[co], s);
+}
+
+// ------------------------------------------------------------------ elementwise
+
+// dY *= (Y > 0): the ReLU backward on the stored (post-ReLU) output
+__global__ void k_relu_bwd(const bf16* __restrict__ y, int ldy, bf16* __restrict__ dy, int lddy, int M, int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * C) return;
+    const size_t p = i / C, c = i % C;
+    if (!(b2f(y[p * ldy + c]) > 0.0f)) dy[p * lddy + c] = f2b(0.0f);
+}
+
+// 2x2 max pool (stride 2): y[n][h][w][c] from x[n][2h..][2w..][c]
+__global__ void k_pool(const bf16* __restrict__ x, int ldx, bf16* __restrict__ y, int ldy, int N, int H, int W,
+                       int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    if (i >= (size_t)N * Ho * Wo * C) return;
+    const int c = i % C;
+    const size_t q = i / C;
+    const int wo = q % Wo, ho = (q / Wo) % Ho, n = q / ((size_t)Wo * Ho);
+    float m = -INFINITY;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            m = fmaxf(m, b2f(x[((size_t)(n * H + 2 * ho + a) * W + 2 * wo + b) * ldx + c]));
+    y[q * ldy + c] = f2b(m);
+}
+
+// gradient to the first maximum of each 2x2 window (row-major), others 0
+__global__ void k_pool_bwd(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ y, int ldy,
+                           const bf16* __restrict__ dy, int lddy, bf16* __restrict__ dx, int lddx, int N, int H,
+                           int W, int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    if (i >= (size_t)N * Ho * Wo * C) return;
+    const int c = i % C;
+    const size_t q = i / C;
+    const int wo = q % Wo, ho = (q / Wo) % Ho, n = q / ((size_t)Wo * Ho);
+    const float m = b2f(y[q * ldy + c]);
+    const bf16 g = dy[q * lddy + c];
+    bool done = false;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const size_t o = (size_t)(n * H + 2 * ho + a) * W + 2 * wo + b;
+            const bool hit = !done && b2f(x[o * ldx + c]) == m;
+            dx[o * lddx + c] = hit ? g : f2b(0.0f);
+            done = done || hit;
+        }
+}
+
+// nearest 2x upsample: y[n][h][w][c] = x[n][h/2][w/2][c]  (y is H x W)
+__global__ void k_up(const bf16* __restrict__ x, int ldx, bf16* __restrict__ y, int ldy, int N, int H, int W, int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * H * W * C) return;
+    const int c = i % C;
+    const size_t q = i / C;
+    const int w = q % W, h = (q / W) % H, n = q / ((size_t)W * H);
+    y[q * ldy + c] = x[((size_t)(n * (H / 2) + h / 2) * (W / 2) + w / 2) * ldx + c];
+}
+
+// dx[n][h][w][c] = sum of the 4 dy of its 2x2 block (dy is 2H x 2W)
+__global__ void k_up_bwd(const bf16* __restrict__ dy, int lddy, bf16* __restrict__ dx, int lddx, int N, int H, int W,
+                         int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)N * H * W * C) return;
+    const int c = i % C;
+    const size_t q = i / C;
+    const int w = q % W, h = (q / W) % H, n = q / ((size_t)W * H);
+    float s = 0.0f;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            s += b2f(dy[((size_t)(n * 2 * H + 2 * h + a) * (2 * W) + 2 * w + b) * lddy + c]);
+    dx[q * lddx + c] = f2b(s);
+}
+
+// a += b (channel slices)
+__global__ void k_add(bf16* __restrict__ a, int lda, const bf16* __restrict__ b, int ldb, int M, int C) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * C) return;
+    const size_t p = i / C, c = i % C;
+    a[p * lda + c] = f2b(b2f(a[p * lda + c]) + b2f(b[p * ldb + c]));
+}
+
+// Loss over valid pixels: squared error on channels 0, 1 (log1p slt, slc) and
+// binary cross-entropy with logits on channel 2 (channel mask).  Writes the
+// gradient (scaled by 1/n_valid) into dhead and adds the summed loss terms
+// to out[0..2].
+__global__ __launch_bounds__(256) void k_loss(const bf16* __restrict__ head, int ldh, const float* __restrict__ target,
+                                              const uint8_t* __restrict__ valid, bf16* __restrict__ dhead, int lddh,
+                                              int M, float inv_n, float w_bce, float* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    float l0 = 0.f, l1 = 0.f, l2 = 0.f;
+    if (p < M) {
+        const bool ok = valid[p] != 0;
+        const float a = b2f(head[(size_t)p * ldh + 0]), b = b2f(head[(size_t)p * ldh + 1]);
+        const float z = b2f(head[(size_t)p * ldh + 2]);
+        const float ta = target[3 * (size_t)p], tb = target[3 * (size_t)p + 1], tc = target[3 * (size_t)p + 2];
+        float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+        if (ok) {
+            l0 = (a - ta) * (a - ta);
+            l1 = (b - tb) * (b - tb);
+            // BCE with logits, stable: max(z,0) - z*t + log(1+exp(-|z|))
+            l2 = fmaxf(z, 0.f) - z * tc + log1pf(expf(-fabsf(z)));
+            g0 = 2.f * (a - ta) * inv_n;
+            g1 = 2.f * (b - tb) * inv_n;
+            g2 = w_bce * (1.f / (1.f + expf(-z)) - tc) * inv_n;
+        }
+        dhead[(size_t)p * lddh + 0] = f2b(g0);
+        dhead[(size_t)p * lddh + 1] = f2b(g1);
+        dhead[(size_t)p * lddh + 2] = f2b(g2);
+        for (int c = 3; c < lddh && c < 32; c++) dhead[(size_t)p * lddh + c] = f2b(0.f);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        l0 += __shfl_down(l0, off);
+        l1 += __shfl_down(l1, off);
+        l2 += __shfl_down(l2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], l0);
+        atomicAdd(&out[1], l1);
+        atomicAdd(&out[2], l2);
+    }
+}
+
+// Adam (Kingma & Ba 2015) on fp32 master weights, bf16 copy for the kernels.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, bf16* __restrict__ pb, int n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (pb) pb[i] = f2b(pi);
+}
+
+// ------------------------------------------------------------------ tiles
+//
+// A tile is an S x S window of a grid (the tracer's padded grid, x-major:
+// pixel (i, j) at i*ny + j) whose top-left corner tiles[2t], tiles[2t+1] may
+// lie outside the grid; pixels outside the grid or masked are invalid.
+
+__device__ __forceinline__ bool tile_pixel(const int* tiles, int t, int S, int q, int nx, int ny, size_t& idx) {
+    const int i = tiles[2 * t] + q / S, j = tiles[2 * t + 1] + q % S;
+    if (i < 0 || j < 0 || i >= nx || j >= ny) return false;
+    idx = (size_t)i * ny + j;
+    return true;
+}
+
+// mean[t] = mean of h over the tile's valid pixels (0 when none)
+__global__ __launch_bounds__(256) void k_tile_mean(const float* __restrict__ h, const uint8_t* __restrict__ mask,
+                                                   int nx, int ny, const int* __restrict__ tiles, int S,
+                                                   float* __restrict__ mean) {
+    __shared__ float ssum[4];
+    __shared__ int scnt[4];
+    const int t = blockIdx.x;
+    float s = 0.f;
+    int c = 0;
+    for (int q = threadIdx.x; q < S * S; q += 256) {
+        size_t idx;
+        if (tile_pixel(tiles, t, S, q, nx, ny, idx) && !mask[idx]) {
+            s += h[idx];
+            c++;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_down(s, off);
+        c += __shfl_down(c, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        ssum[threadIdx.x >> 6] = s;
+        scnt[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float st = ssum[0] + ssum[1] + ssum[2] + ssum[3];
+        const int ct = scnt[0] + scnt[1] + scnt[2] + scnt[3];
+        mean[t] = ct ? st / (float)ct : 0.f;
+    }
+}
+
+// Network input, 16 bf16 channels per pixel: (h - mean)/h_scale, u, v, valid, 0...
+__global__ void k_tile_inputs(const float* __restrict__ h, const float* __restrict__ uv,
+                              const uint8_t* __restrict__ mask, int nx, int ny, const int* __restrict__ tiles,
+                              int n_tiles, int S, const float* __restrict__ mean, float inv_scale,
+                              bf16* __restrict__ x) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n_tiles * S * S) return;
+    const int t = (int)(i / ((size_t)S * S)), q = (int)(i % ((size_t)S * S));
+    size_t idx;
+    bf16x8 lo = {}, hi = {};
+    if (tile_pixel(tiles, t, S, q, nx, ny, idx) && !mask[idx]) {
+        lo[0] = f2b((h[idx] - mean[t]) * inv_scale);
+        lo[1] = f2b(uv[2 * idx]);
+        lo[2] = f2b(uv[2 * idx + 1]);
+        lo[3] = f2b(1.0f);
+    }
+    *(bf16x8*)(x + i * 16) = lo;
+    *(bf16x8*)(x + i * 16 + 8) = hi;
+}
+
+// Training targets from the raw downstream fields: log1p(slt/d^2),
+// log1p(slc/d^2), [slc/d^2 >= threshold]; valid = in grid and not masked.
+__global__ void k_tile_targets(const uint32_t* __restrict__ slc, const uint32_t* __restrict__ slt,
+                               const uint8_t* __restrict__ mask, int nx, int ny, const int* __restrict__ tiles,
+                               int n_tiles, int S, float inv_d2, float threshold, float* __restrict__ target,
+                               uint8_t* __restrict__ valid) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n_tiles * S * S) return;
+    const int t = (int)(i / ((size_t)S * S)), q = (int)(i % ((size_t)S * S));
+    size_t idx;
+    float a = 0.f, b = 0.f, c = 0.f;
+    uint8_t ok = 0;
+    if (tile_pixel(tiles, t, S, q, nx, ny, idx) && !mask[idx]) {
+        const float dens = (float)slc[idx] * inv_d2;
+        a = log1pf((float)slt[idx] * inv_d2);
+        b = log1pf(dens);
+        c = dens >= threshold ? 1.f : 0.f;
+        ok = 1;
+    }
+    target[3 * i] = a;
+    target[3 * i + 1] = b;
+    target[3 * i + 2] = c;
+    valid[i] = ok;
+}
+
+// Stitch: the core [halo, S-halo)^2 of each tile goes to out[k][pixel]
+// (planar float32, k = 0..2: head channels 0..2) where it lies in the grid.
+__global__ void k_tile_stitch(const bf16* __restrict__ head, int ldh, const int* __restrict__ tiles, int n_tiles,
+                              int S, int halo, int nx, int ny, float* __restrict__ out) {
+    const int C = S - 2 * halo;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n_tiles * C * C) return;
+    const int t = (int)(i / ((size_t)C * C)), q = (int)(i % ((size_t)C * C));
+    const int a = halo + q / C, b = halo + q % C;
+    const int gi = tiles[2 * t] + a, gj = tiles[2 * t + 1] + b;
+    if (gi < 0 || gj < 0 || gi >= nx || gj >= ny) return;
+    const size_t src = ((size_t)t * S * S + (size_t)a * S + b) * ldh;
+    const size_t dst = (size_t)gi * ny + gj, nxy = (size_t)nx * ny;
+    out[dst] = b2f(head[src]);
+    out[nxy + dst] = b2f(head[src + 1]);
+    out[2 * nxy + dst] = b2f(head[src + 2]);
+}
+
+// Traced raw fields -> the surrogate's planar fields over a grid: out[0] =
+// log1p(slt/d^2), out[1] = log1p(slc/d^2), out[2] = +1 where slc/d^2 >=
+// threshold else -1 (a channel "logit"); masked pixels 0, 0, -1.
+__global__ void k_fields_log(const uint32_t* __restrict__ slc, const uint32_t* __restrict__ slt,
+                             const uint8_t* __restrict__ mask, size_t nxy, float inv_d2, float threshold,
+                             float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxy) return;
+    float a = 0.f, b = 0.f, c = -1.f;
+    if (!mask[i]) {
+        const float dens = (float)slc[i] * inv_d2;
+        a = log1pf((float)slt[i] * inv_d2);
+        b = log1pf(dens);
+        c = dens >= threshold ? 1.f : -1.f;
+    }
+    out[i] = a;
+    out[nxy + i] = b;
+    out[2 * nxy + i] = c;
+}
+
+inline dim3 g1(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+int check_conv(int M, int Cin, int Cout, int ldx, int ldy, int taps) {
+    if (taps != 1 && taps != 9) return slm_fail(SLM_E_INVALID_ARG, "taps %d is not 1 or 9", taps);
+    if (M <= 0 || M % 32) return slm_fail(SLM_E_INVALID_ARG, "pixel count %d not a positive multiple of 32", M);
+    if (Cin <= 0 || Cin % 16) return slm_fail(SLM_E_INVALID_ARG, "Cin %d not a positive multiple of 16", Cin);
+    if (Cout <= 0 || Cout % 32) return slm_fail(SLM_E_INVALID_ARG, "Cout %d not a positive multiple of 32", Cout);
+    if (ldx < Cin || ldx % 8 || ldy < Cout) return slm_fail(SLM_E_INVALID_ARG, "bad strides %d %d", ldx, ldy);
+    return SLM_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" int slm_nn_conv(const void* x, int ldx, const void* w, const float* bias, void* y, int ldy, int N, int H,
+                           int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags) {
+    const int M = N * H * W;
+    int rc = check_conv(M, Cin, Cout, ldx, ldy, taps);
+    if (rc) return rc;
+    if (!x || !w || !y) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (((uintptr_t)x) % 16) return slm_fail(SLM_E_INVALID_ARG, "x not 16-byte aligned");
+    hipStream_t s = (hipStream_t)stream;
+    const bf16* xb = (const bf16*)x;
+    const bf16* wb = (const bf16*)w;
+    bf16* yb = (bf16*)y;
+    if (Cout >= 64) {
+        const dim3 grid((M + 63) / 64, (Cout + 63) / 64);
+#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 2, 2>), grid, dim3(256), 0, s, xb, ldx, wb, bias, yb, ldy, M, H, W, Cin, Cout)
+        if (taps == 9) { if (relu) CONV_L(9, true); else CONV_L(9, false); }
+        else { if (relu) CONV_L(1, true); else CONV_L(1, false); }
+#undef CONV_L
+    } else {
+        const dim3 grid((M + 127) / 128, (Cout + 31) / 32);
+#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 4, 1>), grid, dim3(256), 0, s, xb, ldx, wb, bias, yb, ldy, M, H, W, Cin, Cout)
+        if (taps == 9) { if (relu) CONV_L(9, true); else CONV_L(9, false); }
+        else { if (relu) CONV_L(1, true); else CONV_L(1, false); }
+#undef CONV_L
+    }
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_wflip(const void* w, void* wt, int taps, int Cin, int Cout, void* stream, uint32_t flags) {
+    if (!w || !wt || (taps != 1 && taps != 9) || Cin <= 0 || Cout <= 0)
+        return slm_fail(SLM_E_INVALID_ARG, "bad wflip arguments");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_wflip, g1((size_t)Cout * taps * Cin), dim3(256), 0, s, (const bf16*)w, (bf16*)wt, taps, Cin,
+                       Cout);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int lddy, float* dw, float* db, int N, int H,
+                                 int W, int Cin, int Cout, int taps, void* stream, uint32_t flags) {
+    const int M = N * H * W;
+    int rc = check_conv(M, Cin, Cout, ldx, lddy, taps);
+    if (rc) return rc;
+    if (!x || !dy || !dw) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    const int K = taps * Cin;
+    const int tiles = ((Cout + 63) / 64) * ((K + 63) / 64);
+    // enough pixel splits to fill the chip (~1024 blocks), at least 64 pixels each
+    int splits = (1024 + tiles - 1) / tiles;
+    int ppb = (M + splits - 1) / splits;
+    ppb = ((ppb + 63) / 64) * 64;
+    splits = (M + ppb - 1) / ppb;
+    const dim3 grid((Cout + 63) / 64, (K + 63) / 64, splits);
+    if (taps == 9)
+        hipLaunchKernelGGL(k_wgrad<9>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H, W,
+                           Cin, Cout, ppb);
+    else
+        hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H, W,
+                           Cin, Cout, ppb);
+    if (db) {
+        const int bsplits = 256, bppb = (M + bsplits - 1) / bsplits;
+        hipLaunchKernelGGL(k_bgrad, dim3((Cout + 255) / 256, bsplits), dim3(256), 0, s, (const bf16*)dy, lddy, db, M,
+                           Cout, bppb);
+    }
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_relu_bwd(const void* y, int ldy, void* dy, int lddy, int M, int C, void* stream, uint32_t flags) {
+    if (!y || !dy || M < 0 || C <= 0) return slm_fail(SLM_E_INVALID_ARG, "bad relu_bwd arguments");
+    hipLaunchKernelGGL(k_relu_bwd, g1((size_t)M * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)y, ldy, (bf16*)dy,
+                       lddy, M, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_pool(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
+                           uint32_t flags) {
+    if (!x || !y || H % 2 || W % 2) return slm_fail(SLM_E_INVALID_ARG, "bad pool arguments");
+    hipLaunchKernelGGL(k_pool, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream,
+                       (const bf16*)x, ldx, (bf16*)y, ldy, N, H, W, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_pool_bwd(const void* x, int ldx, const void* y, int ldy, const void* dy, int lddy, void* dx,
+                               int lddx, int N, int H, int W, int C, void* stream, uint32_t flags) {
+    if (!x || !y || !dy || !dx || H % 2 || W % 2) return slm_fail(SLM_E_INVALID_ARG, "bad pool_bwd arguments");
+    hipLaunchKernelGGL(k_pool_bwd, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream,
+                       (const bf16*)x, ldx, (const bf16*)y, ldy, (const bf16*)dy, lddy, (bf16*)dx, lddx, N, H, W, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_up(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
+                         uint32_t flags) {
+    if (!x || !y || H % 2 || W % 2) return slm_fail(SLM_E_INVALID_ARG, "bad upsample arguments");
+    hipLaunchKernelGGL(k_up, g1((size_t)N * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, ldx,
+                       (bf16*)y, ldy, N, H, W, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_up_bwd(const void* dy, int lddy, void* dx, int lddx, int N, int H, int W, int C, void* stream,
+                             uint32_t flags) {
+    if (!dy || !dx) return slm_fail(SLM_E_INVALID_ARG, "bad upsample backward arguments");
+    hipLaunchKernelGGL(k_up_bwd, g1((size_t)N * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)dy, lddy,
+                       (bf16*)dx, lddx, N, H, W, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C, void* stream, uint32_t flags) {
+    if (!a || !b) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipLaunchKernelGGL(k_add, g1((size_t)M * C), dim3(256), 0, (hipStream_t)stream, (bf16*)a, lda, (const bf16*)b, ldb,
+                       M, C);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead, int lddh,
+                           int M, float inv_n, float w_bce, float* out, void* stream, uint32_t flags) {
+    if (!head || !target || !valid || !dhead || !out || ldh < 3 || lddh < 3)
+        return slm_fail(SLM_E_INVALID_ARG, "bad loss arguments");
+    hipLaunchKernelGGL(k_loss, g1((size_t)M), dim3(256), 0, (hipStream_t)stream, (const bf16*)head, ldh, target, valid,
+                       (bf16*)dhead, lddh, M, inv_n, w_bce, out);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_adam(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1, float b2,
+                           float eps, int step, void* stream, uint32_t flags) {
+    if (!p || !g || !m || !v || n <= 0 || step <= 0) return slm_fail(SLM_E_INVALID_ARG, "bad adam arguments");
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, g1((size_t)n), dim3(256), 0, (hipStream_t)stream, p, g, m, v, (bf16*)pb, n, lr, b1, b2,
+                       eps, c1, c2);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_tile_inputs(const float* h, const float* uv, const uint8_t* mask, int nx, int ny,
+                                  const int* tiles, int n_tiles, int S, float h_scale, float* mean_ws, void* x,
+                                  void* stream, uint32_t flags) {
+    if (!h || !uv || !mask || !tiles || !mean_ws || !x || nx <= 0 || ny <= 0 || n_tiles < 0 || S <= 0 ||
+        !(h_scale > 0.f))
+        return slm_fail(SLM_E_INVALID_ARG, "bad tile_inputs arguments");
+    if (((uintptr_t)x) % 16) return slm_fail(SLM_E_INVALID_ARG, "x not 16-byte aligned");
+    if (!n_tiles) return slm_finish(stream, flags);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tile_mean, dim3(n_tiles), dim3(256), 0, s, h, mask, nx, ny, tiles, S, mean_ws);
+    hipLaunchKernelGGL(k_tile_inputs, g1((size_t)n_tiles * S * S), dim3(256), 0, s, h, uv, mask, nx, ny, tiles,
+                       n_tiles, S, (const float*)mean_ws, 1.0f / h_scale, (bf16*)x);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_tile_targets(const uint32_t* slc, const uint32_t* slt, const uint8_t* mask, int nx, int ny,
+                                   const int* tiles, int n_tiles, int S, int density, float threshold,
+                                   float* target, uint8_t* valid, void* stream, uint32_t flags) {
+    if (!slc || !slt || !mask || !tiles || !target || !valid || nx <= 0 || ny <= 0 || n_tiles < 0 || S <= 0 ||
+        density <= 0)
+        return slm_fail(SLM_E_INVALID_ARG, "bad tile_targets arguments");
+    if (!n_tiles) return slm_finish(stream, flags);
+    hipLaunchKernelGGL(k_tile_targets, g1((size_t)n_tiles * S * S), dim3(256), 0, (hipStream_t)stream, slc, slt,
+                       mask, nx, ny, tiles, n_tiles, S, 1.0f / (float)(density * density), threshold, target, valid);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_tile_stitch(const void* head, int ldh, const int* tiles, int n_tiles, int S, int halo, int nx,
+                                  int ny, float* out, void* stream, uint32_t flags) {
+    if (!head || !tiles || !out || ldh < 3 || n_tiles < 0 || S <= 0 || halo < 0 || 2 * halo >= S || nx <= 0 ||
+        ny <= 0)
+        return slm_fail(SLM_E_INVALID_ARG, "bad tile_stitch arguments");
+    if (!n_tiles) return slm_finish(stream, flags);
+    const int C = S - 2 * halo;
+    hipLaunchKernelGGL(k_tile_stitch, g1((size_t)n_tiles * C * C), dim3(256), 0, (hipStream_t)stream,
+                       (const bf16*)head, ldh, tiles, n_tiles, S, halo, nx, ny, out);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_fields_log(const uint32_t* slc, const uint32_t* slt, const uint8_t* mask, size_t nxy,
+                                 int density, float threshold, float* out, void* stream, uint32_t flags) {
+    if (!slc || !slt || !mask || !out || density <= 0) return slm_fail(SLM_E_INVALID_ARG, "bad fields_log arguments");
+    if (!nxy) return slm_finish(stream, flags);
+    hipLaunchKernelGGL(k_fields_log, g1(nxy), dim3(256), 0, (hipStream_t)stream, slc, slt, mask, nxy,
+                       1.0f / (float)(density * density), threshold, out);
+    return slm_finish(stream, flags);
+}

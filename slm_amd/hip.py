@@ -26,6 +26,7 @@ _ERR_NAMES = {-1: "SLM_E_INVALID_ARG", -2: "SLM_E_HIP", -3: "SLM_E_OOM", -4: "SL
 # Exported symbols and their prototypes: (restype, argtypes)
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
+_I = ctypes.c_int
 _PROTOS = {
     "slm_abi_version": (_U32, []),
     "slm_last_error": (ctypes.c_char_p, []),
@@ -46,6 +47,22 @@ _PROTOS = {
     "slm_kde_histogram": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_map_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
+    "slm_nn_conv": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_wflip": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _U32]),
+    "slm_nn_conv_wgrad": (ctypes.c_int, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_relu_bwd": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _P, _U32]),
+    "slm_nn_pool": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_pool_bwd": (ctypes.c_int, [_P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_up": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_up_bwd": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_add": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _P, _U32]),
+    "slm_nn_loss": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P, _U32]),
+    "slm_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, _I, _P, _U32]),
+    "slm_nn_tile_inputs": (ctypes.c_int, [_P, _P, _P, _I, _I, _P, _I, _I, ctypes.c_float, _P, _P, _P, _U32]),
+    "slm_nn_tile_targets": (ctypes.c_int, [_P, _P, _P, _I, _I, _P, _I, _I, _I, ctypes.c_float, _P, _P, _P, _U32]),
+    "slm_nn_tile_stitch": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _U32]),
+    "slm_nn_fields_log": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t, _I, ctypes.c_float, _P, _P, _U32]),
     "slm_trace_fields_batch_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmFieldsJob), _U32]),
     "slm_trace_fields_batch": (ctypes.c_int, [ctypes.POINTER(SlmFieldsJob), _U32, ctypes.POINTER(SlmBatchOpts), _P,
                                               _P, ctypes.c_size_t, _P, _U32]),

@@ -26,6 +26,9 @@ def _hostlib():
         lib = ctypes.CDLL(path)
         lib.slm_fill_sinks.restype = ctypes.c_int
         lib.slm_fill_sinks.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double]
+        lib.slm_v_valleys.restype = ctypes.c_int
+        lib.slm_v_valleys.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int] + \
+            [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double]
         _HOSTLIB = lib
     return _HOSTLIB
 
@@ -58,15 +61,19 @@ def spectral_noise(nx: int, ny: int, beta: float, rng: np.random.Generator) -> n
 
 def v_valleys(nx: int, ny: int, pixel_size: float, n: int, depth: float, half_width_px: float,
               rng: np.random.Generator) -> np.ndarray:
-    """Sum of n straight V-shaped incisions (depth in m) across the grid."""
-    x = np.arange(nx)[:, None].astype(np.float64)
-    y = np.arange(ny)[None, :].astype(np.float64)
+    """Sum of n straight V-shaped incisions (depth in m) across the grid:
+    -depth * clip(1 - dist/half_width_px, 0, 1) per valley, with the line's
+    angle and centre drawn from rng (angle, then centre x, then centre y)."""
+    ang, cx, cy = np.zeros(n), np.zeros(n), np.zeros(n)
+    for k in range(n):
+        ang[k] = rng.uniform(0.0, np.pi)
+        cx[k], cy[k] = rng.uniform(0.2, 0.8) * nx, rng.uniform(0.2, 0.8) * ny
+    s, c = np.sin(ang), np.cos(ang)
     out = np.zeros((nx, ny))
-    for _ in range(n):
-        ang = rng.uniform(0.0, np.pi)
-        cx, cy = rng.uniform(0.2, 0.8) * nx, rng.uniform(0.2, 0.8) * ny
-        d = np.abs((x - cx) * np.sin(ang) - (y - cy) * np.cos(ang))
-        out -= depth * np.clip(1.0 - d / half_width_px, 0.0, 1.0)
+    rc = _hostlib().slm_v_valleys(out.ctypes.data, nx, ny, n, cx.ctypes.data, cy.ctypes.data, s.ctypes.data,
+                                  c.ctypes.data, float(depth), float(half_width_px))
+    if rc != 0:
+        raise ValueError("slm_v_valleys failed")
     return out
 
 
@@ -88,7 +95,7 @@ def cones_with_pits(nx: int, ny: int, pixel_size: float, relief: float, n_pits: 
 def make_dtm(n: int | tuple, pixel_size: float = 1.0, relief: float = 20.0, tilt: float = 0.05,
              beta: float = 2.0, seed: int = 0, nan_border: int = 2, n_valleys: int = 0,
              valley_depth: float | None = None, cone: bool = False, n_pits: int = 0,
-             fill: bool = True) -> np.ndarray:
+             fill: bool = True, valley_half_width_px: float | None = None) -> np.ndarray:
     """Seeded synthetic DTM (float32 [nx, ny], metres, NaN border)."""
     nx, ny = (n, n) if np.isscalar(n) else n
     rng = np.random.default_rng(seed)
@@ -98,7 +105,8 @@ def make_dtm(n: int | tuple, pixel_size: float = 1.0, relief: float = 20.0, tilt
     h = h + tilt * (0.8 * xs + 0.6 * ys)
     if n_valleys:
         depth = valley_depth if valley_depth is not None else 0.5 * relief
-        h = h + v_valleys(nx, ny, pixel_size, n_valleys, depth, max(8.0, 0.03 * max(nx, ny)), rng)
+        hw = valley_half_width_px if valley_half_width_px is not None else max(8.0, 0.03 * max(nx, ny))
+        h = h + v_valleys(nx, ny, pixel_size, n_valleys, depth, hw, rng)
     if cone:
         h = 0.3 * h + cones_with_pits(nx, ny, pixel_size, relief, n_pits, rng)
     if nan_border:

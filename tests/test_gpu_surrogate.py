@@ -1,0 +1,240 @@
+"""GPU tests of the surrogate's kernels and network (slm_nn_*).
+
+Each kernel is compared with the same operation on the CPU in float32 on the
+same bf16 values (products accumulate in fp32 on both sides, in different
+orders, so bf16 outputs may differ in the last place); the whole network's
+parameter gradients are compared with the CPU oracle
+(oracle/surrogate_ref.py) on two 32x32 tiles; and a short training run must
+lower the loss well below that of the best constant predictor."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ULP = 2.0 ** -7          # bf16 relative spacing (8 significant bits)
+
+
+def _bf(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16)
+
+
+def _close_bf16(got, want, frac=0.99):
+    """bf16 results: at least frac exactly equal, all within 2 ulps (relative to
+    the larger of |want| and a small floor)."""
+    got = np.asarray(got, np.float32)
+    want = np.asarray(want, np.float32)
+    scale = np.maximum(np.abs(want), 1e-3 * max(1e-6, float(np.abs(want).max())))
+    rel = np.abs(got - want) / scale
+    assert rel.max() <= 2 * ULP + 1e-7, rel.max()
+    assert (got == want).mean() >= frac, (got == want).mean()
+
+
+@pytest.mark.parametrize("taps,relu,cin,cout,ld_extra", [(9, True, 16, 32, 0), (9, False, 64, 64, 32),
+                                                       (1, False, 32, 32, 0), (9, True, 96, 128, 0)])
+def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra):
+    import torch
+    import torch.nn.functional as F
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(taps * 100 + cin)
+    N, H, W = 2, 16, 24
+    x = rs.standard_normal((N, H, W, cin + ld_extra)).astype(np.float32)
+    w = (rs.standard_normal((cout, taps, cin)) / np.sqrt(cin * taps)).astype(np.float32)
+    b = rs.standard_normal(cout).astype(np.float32) * 0.1
+    xb, wb = _bf(x), _bf(w)
+    xg = xb.to(gpu).reshape(N * H * W, cin + ld_extra)
+    y = torch.empty((N * H * W, cout), dtype=torch.bfloat16, device=gpu)
+    S.conv(xg, cin + ld_extra, wb.to(gpu).reshape(-1), torch.from_numpy(b).to(gpu), y, cout, N, H, W, cin, cout,
+           taps, relu)
+    torch.cuda.synchronize()
+    k = 3 if taps == 9 else 1
+    xr = xb.float()[..., :cin].permute(0, 3, 1, 2)
+    wr = wb.float().reshape(cout, k, k, cin).permute(0, 3, 1, 2)
+    ref = F.conv2d(xr, wr, torch.from_numpy(b), padding=1 if taps == 9 else 0)
+    if relu:
+        ref = F.relu(ref)
+    ref = ref.to(torch.bfloat16).float().permute(0, 2, 3, 1).reshape(N * H * W, cout)
+    _close_bf16(y.float().cpu().numpy(), ref.numpy())
+
+
+@pytest.mark.parametrize("taps,cin,cout", [(9, 16, 32), (9, 64, 128), (1, 32, 32), (9, 96, 32)])
+def test_conv_backward(gpu, taps, cin, cout):
+    """Backward-weight (fp32) and backward-data (flipped weights through the
+    forward kernel) against torch autograd on the CPU."""
+    import torch
+    import torch.nn.functional as F
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(7 + cin)
+    N, H, W = 2, 16, 16
+    x = _bf(rs.standard_normal((N, H, W, cin)))
+    w = _bf(rs.standard_normal((cout, taps, cin)) / np.sqrt(cin * taps))
+    dy = _bf(rs.standard_normal((N, H, W, cout)))
+    k = 3 if taps == 9 else 1
+    xr = x.float().permute(0, 3, 1, 2).requires_grad_(True)
+    wr = w.float().reshape(cout, k, k, cin).permute(0, 3, 1, 2).contiguous().requires_grad_(True)
+    yr = F.conv2d(xr, wr, padding=1 if taps == 9 else 0)
+    yr.backward(dy.float().permute(0, 3, 1, 2))
+    dw_ref = wr.grad.permute(0, 2, 3, 1).reshape(cout, taps, cin).numpy()
+    dx_ref = xr.grad.to(torch.bfloat16).float().permute(0, 2, 3, 1).reshape(-1, cin).numpy()
+    M = N * H * W
+    xg, wg, dyg = x.to(gpu).reshape(M, cin), w.to(gpu).reshape(-1), dy.to(gpu).reshape(M, cout)
+    dw = torch.zeros(cout * taps * cin, dtype=torch.float32, device=gpu)
+    db = torch.zeros(cout, dtype=torch.float32, device=gpu)
+    S.wgrad(xg, cin, dyg, cout, dw, db, N, H, W, cin, cout, taps)
+    if cin % 32 == 0:          # backward-data has Cin as its output channels (the network skips it for Cin 16)
+        wt = torch.empty_like(wg)
+        S.wflip(wg, wt, taps, cin, cout)
+        dx = torch.empty((M, cin), dtype=torch.bfloat16, device=gpu)
+        S.conv(dyg, cout, wt, None, dx, cin, N, H, W, cout, cin, taps, False)
+    torch.cuda.synchronize()
+    got = dw.cpu().numpy().reshape(cout, taps, cin)
+    assert np.abs(got - dw_ref).max() <= 1e-4 * np.abs(dw_ref).max() + 1e-5
+    db_ref = dy.float().sum(dim=(0, 1, 2)).numpy()
+    assert np.allclose(db.cpu().numpy(), db_ref, rtol=1e-5, atol=1e-4)
+    if cin % 32 == 0:
+        _close_bf16(dx.float().cpu().numpy(), dx_ref, frac=0.97)
+
+
+def test_pool_up_add_loss_adam(gpu):
+    import torch
+    import torch.nn.functional as F
+    from slm_amd import hip
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(3)
+    N, H, W, C = 2, 8, 12, 32
+    x = _bf(rs.standard_normal((N, H, W, C)))
+    xg = x.to(gpu).reshape(-1, C)
+    y = torch.empty((N * H * W // 4, C), dtype=torch.bfloat16, device=gpu)
+    S.pool(xg, C, y, C, N, H, W, C)
+    ref = F.max_pool2d(x.float().permute(0, 3, 1, 2), 2).permute(0, 2, 3, 1).reshape(-1, C)
+    assert torch.equal(y.float().cpu(), ref)
+    dy = _bf(rs.standard_normal((N * H * W // 4, C)))
+    dx = torch.empty((N * H * W, C), dtype=torch.bfloat16, device=gpu)
+    S.pool_bwd(xg, C, y, C, dy.to(gpu), C, dx, C, N, H, W, C)
+    xr = x.float().permute(0, 3, 1, 2).requires_grad_(True)
+    F.max_pool2d(xr, 2).backward(dy.float().reshape(N, H // 2, W // 2, C).permute(0, 3, 1, 2))
+    assert torch.equal(dx.float().cpu(), xr.grad.permute(0, 2, 3, 1).reshape(-1, C))
+    # upsample and its backward
+    u = torch.empty((N * H * W, C), dtype=torch.bfloat16, device=gpu)
+    S.up(y, C, u, C, N, H, W, C)
+    uref = F.interpolate(ref.reshape(N, H // 2, W // 2, C).permute(0, 3, 1, 2), scale_factor=2, mode="nearest")
+    assert torch.equal(u.float().cpu(), uref.permute(0, 2, 3, 1).reshape(-1, C))
+    du = _bf(rs.standard_normal((N * H * W, C)))
+    dd = torch.empty((N * H * W // 4, C), dtype=torch.bfloat16, device=gpu)
+    S.up_bwd(du.to(gpu), C, dd, C, N, H // 2, W // 2, C)
+    dref = du.float().reshape(N, H // 2, 2, W // 2, 2, C).sum(dim=(2, 4)).reshape(-1, C)
+    _close_bf16(dd.float().cpu().numpy(), dref.to(torch.bfloat16).float().numpy())
+    # loss (sums and gradient) against numpy
+    M = N * H * W
+    head = _bf(rs.standard_normal((M, 32))).to(gpu)
+    tgt = rs.random((M, 3)).astype(np.float32)
+    tgt[:, 2] = (tgt[:, 2] > 0.7).astype(np.float32)
+    valid = (rs.random(M) > 0.2).astype(np.uint8)
+    dh = torch.empty_like(head)
+    out = torch.zeros(3, dtype=torch.float32, device=gpu)
+    tg, vg = torch.from_numpy(tgt).to(gpu), torch.from_numpy(valid).to(gpu)    # held while the kernel runs
+    hip.check(hip.load().slm_nn_loss(S._p(head), 32, S._p(tg), S._p(vg), S._p(dh), 32, M,
+                                     S.ctypes.c_float(1.0 / valid.sum()), S.ctypes.c_float(2.0), S._p(out),
+                                     S._s(), 1))
+    h = head.float().cpu().numpy()
+    v = valid.astype(bool)
+    z = h[:, 2].astype(np.float64)
+    bce = np.maximum(z, 0) - z * tgt[:, 2] + np.log1p(np.exp(-np.abs(z)))
+    want = [((h[:, 0] - tgt[:, 0]) ** 2)[v].sum(), ((h[:, 1] - tgt[:, 1]) ** 2)[v].sum(), bce[v].sum()]
+    assert np.allclose(out.cpu().numpy(), want, rtol=1e-4)
+    g2 = 2.0 * (1 / (1 + np.exp(-z)) - tgt[:, 2]) / v.sum()
+    assert np.allclose(dh.float().cpu().numpy()[v, 2], g2[v], rtol=2 * ULP, atol=1e-6)
+    assert np.all(dh.float().cpu().numpy()[~v] == 0)
+    # Adam step against the closed form
+    n = 1000
+    p = torch.from_numpy(rs.standard_normal(n).astype(np.float32)).to(gpu)
+    g = torch.from_numpy(rs.standard_normal(n).astype(np.float32)).to(gpu)
+    m0 = torch.zeros_like(p)
+    v0 = torch.zeros_like(p)
+    p0 = p.cpu().numpy().copy()
+    hip.check(hip.load().slm_nn_adam(S._p(p), S._p(g), S._p(m0), S._p(v0), None, n, S.ctypes.c_float(1e-3),
+                                     S.ctypes.c_float(0.9), S.ctypes.c_float(0.999), S.ctypes.c_float(1e-8), 1,
+                                     S._s(), 1))
+    gg = g.cpu().numpy()
+    want_p = p0 - 1e-3 * gg / (np.abs(gg) + 1e-8)      # first step: m/c1 = g, v/c2 = g^2
+    assert np.allclose(p.cpu().numpy(), want_p, rtol=1e-5, atol=1e-7)
+
+
+def test_network_gradients_match_oracle(gpu):
+    import torch
+    from oracle.surrogate_ref import RefUNet
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(11)
+    N, H, W = 2, 32, 32
+    params = S.init_params(seed=5)
+    # non-zero biases so every path is exercised
+    params = {k: (w, (rs.standard_normal(b.shape) * 0.05).astype(np.float32)) for k, (w, b) in params.items()}
+    x = np.zeros((N, H, W, S.IN_CH), np.float32)
+    x[..., :4] = rs.standard_normal((N, H, W, 4))
+    tgt = np.stack([rs.random((N, H, W)) * 3, rs.random((N, H, W)) * 3, (rs.random((N, H, W)) > 0.8) * 1.0],
+                   -1).astype(np.float32)
+    valid = rs.random((N, H, W)) > 0.1
+    net = S.UNet(params, device=gpu)
+    xb = _bf(x)
+    head = net.forward(xb.to(gpu).reshape(-1, S.IN_CH), N, H, W)
+    dhead, sums, nv = net.loss(torch.from_numpy(tgt.reshape(-1, 3)).to(gpu),
+                               torch.from_numpy(valid.reshape(-1).astype(np.uint8)).to(gpu))
+    net.backward(dhead)
+    torch.cuda.synchronize()
+    ref = RefUNet(params, S.LAYERS)
+    rh = ref.forward(xb.float().permute(0, 3, 1, 2))
+    loss, parts = ref.loss(rh, tgt, valid)
+    loss.backward()
+    gh = head.float().cpu().numpy().reshape(N, H, W, S.OUT_CH)[..., :3]
+    rhn = rh.detach().permute(0, 2, 3, 1).numpy()[..., :3]
+    assert np.abs(gh - rhn).max() <= 0.05 * np.abs(rhn).max()
+    assert np.allclose(sums.cpu().numpy(), [float(t) for t in parts], rtol=0.02)
+    g_ref = ref.grads()
+    gflat = net.g.cpu().numpy()
+    for name, cin, cout, taps, _ in S.LAYERS:
+        o, nw, ob, nb = net.offsets[name]
+        gw = gflat[o:o + nw].reshape(cout, taps, cin)
+        rw = g_ref[name][0]
+        cos = float((gw * rw).sum() / (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-30))
+        rel = float(np.linalg.norm(gw - rw) / (np.linalg.norm(rw) + 1e-30))
+        print(name, "cos %.5f rel %.4f" % (cos, rel), flush=True)
+        assert cos > 0.995 and rel < 0.1, name
+        gb, rb = gflat[ob:ob + nb], g_ref[name][1]
+        assert np.linalg.norm(gb - rb) <= 0.1 * np.linalg.norm(rb) + 1e-6, name
+
+
+def test_training_beats_constant_predictor(gpu):
+    """A few hundred Adam steps on tiles whose targets are a known function of
+    the input (smoothed height and its threshold): the loss falls and the
+    fitted map beats the per-channel constant that minimises the loss."""
+    import torch
+    from scipy import ndimage
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(4)
+    n, H, W = 32, 32, 32
+    h = ndimage.gaussian_filter(rs.standard_normal((n, H, W)), (0, 3, 3)).astype(np.float32) * 5
+    x = np.zeros((n, H, W, S.IN_CH), np.float32)
+    x[..., 0] = h
+    x[..., 3] = 1.0
+    sm = ndimage.gaussian_filter(h, (0, 2, 2))
+    tgt = np.stack([np.abs(sm), np.maximum(sm, 0) * 2, (sm > 0.5) * 1.0], -1).astype(np.float32)
+    valid = np.ones((n, H, W), bool)
+    net = S.UNet(S.init_params(seed=1), device=gpu)
+    xs = _bf(x).to(gpu).reshape(-1, S.IN_CH)
+    ts = torch.from_numpy(tgt.reshape(-1, 3)).to(gpu)
+    vs = torch.from_numpy(valid.reshape(-1).astype(np.uint8)).to(gpu)
+    losses = []
+    for it in range(300):
+        net.forward(xs, n, H, W)
+        dh, sums, nv = net.loss(ts, vs)
+        net.backward(dh)
+        net.adam(lr=2e-3)
+        losses.append(float(sums.sum().item()) / nv)
+    # the constant predictor: channel means (MSE) and the prevalence logit (BCE)
+    mu = tgt.reshape(-1, 3).mean(0)
+    pc = min(max(mu[2], 1e-6), 1 - 1e-6)
+    const = ((tgt[..., 0] - mu[0]) ** 2).mean() + ((tgt[..., 1] - mu[1]) ** 2).mean() + \
+        -(pc * np.log(pc) + (1 - pc) * np.log(1 - pc))
+    print("loss first %.4f last %.4f constant %.4f" % (losses[0], losses[-1], const), flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.5 * const

@@ -178,3 +178,36 @@ def test_gradient_uv_planar_closed_form():
     assert not m.any()
     n = np.hypot(a, b)
     assert np.allclose(uv[..., 0], a / n, atol=1e-6) and np.allclose(uv[..., 1], b / n, atol=1e-6)
+
+
+def test_v_valleys_c_matches_numpy_expression():
+    """The banded C valley loop gives the bits of the whole-grid numpy
+    expression -depth*clip(1 - |(x-cx) sin - (y-cy) cos| / hw, 0, 1)."""
+    from slm_amd import synth
+    for nx, ny, n, hw in [(120, 77, 4, 9.0), (64, 300, 6, 30.5), (257, 255, 3, 61.44)]:
+        rng = np.random.default_rng(nx + ny)
+        got = synth.v_valleys(nx, ny, 1.0, n, 30.0, hw, np.random.default_rng(nx + ny))
+        x = np.arange(nx)[:, None].astype(np.float64)
+        y = np.arange(ny)[None, :].astype(np.float64)
+        want = np.zeros((nx, ny))
+        for _ in range(n):
+            ang = rng.uniform(0.0, np.pi)
+            cx, cy = rng.uniform(0.2, 0.8) * nx, rng.uniform(0.2, 0.8) * ny
+            d = np.abs((x - cx) * np.sin(ang) - (y - cy) * np.cos(ang))
+            want -= 30.0 * np.clip(1.0 - d / hw, 0.0, 1.0)
+        assert np.array_equal(got, want)
+        assert (got < 0).any()
+
+
+def test_surrogate_tile_cores_partition_grid():
+    from slm_amd import gated
+    for nx, ny, size, halo in [(100, 37, 32, 4), (256, 256, 64, 8), (8190, 8190, 256, 16)]:
+        o = gated.tile_origins(nx, ny, size, halo)
+        c = size - 2 * halo
+        if nx * ny <= 70000:
+            cover = np.zeros((nx, ny), np.int32)
+            for i0, j0 in o:
+                cover[max(0, i0 + halo):i0 + halo + c, max(0, j0 + halo):j0 + halo + c] += 1
+            assert (cover == 1).all()
+        assert len(o) == int(np.ceil(nx / c)) * int(np.ceil(ny / c))
+        assert o.min() == -halo and (o + halo).max() < max(nx, ny)
