@@ -161,6 +161,7 @@ def bench_surrogate(args):
     from slm_amd import surrogate as S
     dev = torch.device("cuda", 0)
     log = lambda m: print(m, file=sys.stderr, flush=True)
+    gated.TRACE["max_length"] = float(args.max_length)
     if args.weights and os.path.exists(args.weights):
         z = np.load(args.weights)
         params = {name: (z[name + ".w"], z[name + ".b"]) for name, *_ in S.LAYERS}
@@ -168,14 +169,21 @@ def bench_surrogate(args):
         train_rep = {"loaded": args.weights}
     else:
         data = gated.training_set(n_dtms=args.train_dtms, n=2048, size=128, log=log, device=dev)
-        const = gated.constant_loss(data)
-        net, hist = gated.train(data, epochs=args.epochs, batch=64, lr=1e-3, log=log, device=dev)
+        const = gated.constant_loss(data, w_len=args.w_len, w_bce=args.w_bce)
+        net, hist = gated.train(data, epochs=args.epochs, batch=args.batch, lr=args.lr, log=log, device=dev,
+                                w_len=args.w_len, w_bce=args.w_bce, cosine=not args.no_cosine)
+        cal = gated.calibrate(net, data)
+        log("channel logit threshold %.2f: F1 %.4f (at 0: %.4f)" % cal)
         params, theta = net.params_numpy(), data.theta
-        train_rep = {"tiles": data.n_tiles, "tile": data.size, "epochs": args.epochs, "batch": 64, "lr": 1e-3,
+        train_rep = {"tiles": data.n_tiles, "tile": data.size, "epochs": args.epochs, "batch": args.batch,
+                     "lr": args.lr, "lr_schedule": "constant" if args.no_cosine else "warmup 200 + cosine",
+                     "loss_weights": {"length": args.w_len, "density": 1.0, "channel": args.w_bce},
                      "theta": theta, "constant_loss": const, "loss_first_epoch": hist["loss"][0],
                      "loss_last_epoch": hist["loss"][-1], "train_s": hist["train_ms"] / 1e3,
                      "tiles_per_s": hist["tiles_per_s"], "train_tflops": hist["tflops"],
-                     "data_generate_s": data.timings["generate_s"], "data_trace_s": data.timings["trace_s"]}
+                     "data_generate_s": data.timings["generate_s"], "data_trace_s": data.timings["trace_s"],
+                     "channel_logit_threshold": cal[0], "train_channel_f1": cal[1],
+                     "train_channel_f1_uncalibrated": cal[2]}
         del data, net
         if args.save_weights:
             np.savez(args.save_weights, theta=theta,
@@ -205,7 +213,15 @@ def bench_surrogate(args):
         (plain, _), ms = timed(lambda: gated.plain_map(grid, theta))
         plain_ms.append(ms)
     plain = plain.clone()
-    gkw = dict(size=256, halo=16, batch=64, check_frac=0.02, rng_seed=2, max_err=0.05, min_f1=0.95)
+    # the metric's noise floor: a second plain map whose seeds draw other jitter
+    grid._seeds = grid.seeds().flip(0).contiguous()
+    plain2, _ = gated.plain_map(grid, theta)
+    floor = gated.compare(plain2, plain, grid.mask, theta, (grid.nx, grid.ny))
+    del plain2
+    grid._seeds = None
+    size, halo = args.map_tile, args.map_halo
+    batch = max(1, (64 * 256 * 256) // (size * size))
+    gkw = dict(size=size, halo=halo, batch=batch, check_frac=0.02, rng_seed=2, max_err=0.05, min_f1=0.95)
     for _ in range(args.warmup):
         gated.gated_map(inf, grid, theta, **gkw)
     gated_ms = []
@@ -213,15 +229,15 @@ def bench_surrogate(args):
         (res, ms) = timed(lambda: gated.gated_map(inf, grid, theta, **gkw))
         gated_ms.append(ms)
     out, rep = res
-    (pred, _), pred_ms = timed(lambda: gated.predict(inf, grid, 256, 16, 64))
+    (pred, _), pred_ms = timed(lambda: gated.predict(inf, grid, size, halo, batch))
     fwd = []
     for _ in range(3):
-        _, ms = timed(lambda: inf.forward(inf.A["x"], 64, 256, 256))
+        _, ms = timed(lambda: inf.forward(inf.A["x"], batch, size, size))
         fwd.append(ms)
     fwd_ms = float(np.median(fwd))
-    fwd_tflops = inf.flops_per_pixel() * 64 * 256 * 256 / (fwd_ms / 1e3) / 1e12
-    m_gated = gated.compare(out, plain, grid.mask)
-    m_pred = gated.compare(pred, plain, grid.mask)
+    fwd_tflops = inf.flops_per_pixel() * batch * size * size / (fwd_ms / 1e3) / 1e12
+    m_gated = gated.compare(out, plain, grid.mask, theta, (grid.nx, grid.ny))
+    m_pred = gated.compare(pred, plain, grid.mask, theta, (grid.nx, grid.ny))
     tp, tg = float(np.median(plain_ms)), float(np.median(gated_ms))
     peak = 2500.0
     res = {
@@ -238,20 +254,21 @@ def bench_surrogate(args):
         "dtype": "bf16",
         "data": "synthetic (seeded DTMs, surrogate trained on their traces; random-init weights trained here)",
         "config": {"workload": "surrogate_gated_%d" % args.map_n, "map_px": grid.nx * grid.ny,
-                   "tile": 256, "halo": 16, "check_frac": 0.02, "check_rng_seed": 2,
+                   "tile": size, "halo": halo, "check_frac": 0.02, "check_rng_seed": 2,
                    "tolerance": {"median_rel_err": 0.05, "channel_f1": 0.95},
                    "trace": gated.TRACE, "directions": 1, "train_dtms": args.train_dtms},
         "surrogate": {"train": train_rep,
                       "map": {"plain_ms": tp, "gated_ms": tg, "predict_only_ms": pred_ms,
                               "plain_steps": int(steps.item()), "host_generation_s": round(gen_s, 1),
                               "gated_vs_plain": m_gated, "prediction_vs_plain": m_pred,
+                              "plain_vs_plain_other_jitter": floor,
                               "tiles": rep["tiles"], "checked": rep["checked"], "failed": rep["failed"],
                               "fallback": rep["fallback"], "check_steps": rep["check_steps"],
                               "checks": [{k: c[k] for k in ("median_rel_err", "f1", "channel_px", "ok")}
                                          for c in rep["checks"]]}},
         "roofline": {"bound": "mfma", "achieved": fwd_tflops, "peak": peak, "unit": "TFLOP/s",
                      "frac": fwd_tflops / peak, "traffic": None, "kernel": "network forward (k_conv)",
-                     "flop_per_px": inf.flops_per_pixel(), "px_per_launch": 64 * 256 * 256,
+                     "flop_per_px": inf.flops_per_pixel(), "px_per_launch": batch * size * size,
                      "mean_forward_ms": fwd_ms},
         "cpu_baseline": None,
     }
@@ -270,7 +287,15 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batch_c3: skip the autotuner (library defaults)")
     ap.add_argument("--train-dtms", type=int, default=32, help="surrogate: traced 2048^2 DTMs for training")
     ap.add_argument("--epochs", type=int, default=20, help="surrogate: training epochs")
+    ap.add_argument("--batch", type=int, default=32, help="surrogate: training batch (tiles)")
+    ap.add_argument("--lr", type=float, default=1e-3, help="surrogate: Adam peak learning rate")
+    ap.add_argument("--no-cosine", action="store_true", help="surrogate: constant learning rate after warm-up")
+    ap.add_argument("--w-len", type=float, default=0.25, help="surrogate: loss weight of log1p(slt)")
+    ap.add_argument("--w-bce", type=float, default=5.0, help="surrogate: loss weight of the channel term")
+    ap.add_argument("--max-length", type=float, default=100.0, help="surrogate: streamline cap (m) of all tracing")
     ap.add_argument("--map-n", type=int, default=8192, help="surrogate: side of the gated map's DTM")
+    ap.add_argument("--map-tile", type=int, default=256, help="surrogate: inference tile side")
+    ap.add_argument("--map-halo", type=int, default=16, help="surrogate: inference tile overlap per side")
     ap.add_argument("--map-seed", type=int, default=2, help="surrogate: seed of the gated map's DTM")
     ap.add_argument("--weights", default="", help="surrogate: load trained weights (.npz) instead of training")
     ap.add_argument("--save-weights", default="", help="surrogate: save the trained weights (.npz)")

@@ -47,6 +47,8 @@ extern "C" {
                                     uv quads built by a previous slm_trace_fields call
                                     with the same seeds, n_seed_points, seeds_chunk_offset,
                                     grid and uv: reuse them */
+#define SLM_FLAG_NN_DIRECT 16u     /* slm_nn_conv: use the direct kernel, not the LDS-tiled one
+                                      (test hook; both compute the same products) */
 #define SLM_FLAG_FORCE_FALLBACK 8u /* test hook: reject the packed 64-bit accumulation so the
                                       32-bit fallback path runs (results are identical) */
 
@@ -480,6 +482,14 @@ int slm_fix_left_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t
 /* y = conv(x, w) + bias, then ReLU when relu != 0 (bias may be NULL). */
 int slm_nn_conv(const void* x, int ldx, const void* w, const float* bias, void* y, int ldy,
                 int N, int H, int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags);
+/* The same convolution over the concatenation [up2(x2) | x]: input channels
+ * [0, c_up) are read from x2, an N x H/2 x W/2 image, at (h/2, w/2) (a fused
+ * nearest 2x upsample); channels [c_up, Cin) from x at channel index c
+ * (x's first c_up channels are not read).  c_up a positive multiple of 32;
+ * H and W even. */
+int slm_nn_conv_up(const void* x, int ldx, const void* x2, int ldx2, int c_up, const void* w, const float* bias,
+                   void* y, int ldy, int N, int H, int W, int Cin, int Cout, int taps, int relu, void* stream,
+                   uint32_t flags);
 /* wt[ci][taps-1-t][co] = w[co][t][ci]: weights of the backward-data
  * convolution (slm_nn_conv on dY with Cin and Cout exchanged). */
 int slm_nn_wflip(const void* w, void* wt, int taps, int Cin, int Cout, void* stream, uint32_t flags);
@@ -502,11 +512,13 @@ int slm_nn_up_bwd(const void* dy, int lddy, void* dx, int lddx, int N, int H, in
                   uint32_t flags);
 /* a += b over M pixels x C channels. */
 int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C, void* stream, uint32_t flags);
-/* Loss over valid pixels: squared error on head channels 0 and 1, binary
- * cross-entropy with logits (weight w_bce) on channel 2; dhead = gradient
- * of the total times inv_n (channels >= 3 zero); out[0..2] += the summed terms. */
+/* Loss over valid pixels: squared error on head channels 0 (weight w_len)
+ * and 1, binary cross-entropy with logits (weight w_bce) on channel 2;
+ * dhead = gradient of the weighted total times inv_n (channels >= 3 zero);
+ * out[0..2] += the unweighted summed terms. */
 int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead,
-                int lddh, int M, float inv_n, float w_bce, float* out, void* stream, uint32_t flags);
+                int lddh, int M, float inv_n, float w_len, float w_bce, float* out, void* stream,
+                uint32_t flags);
 /* One Adam step (bias-corrected, step >= 1) on fp32 p with moments m, v;
  * writes the bf16 copy pb when not NULL. */
 int slm_nn_adam(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1,
@@ -517,7 +529,9 @@ int slm_nn_adam(float* p, const float* g, float* m, float* v, void* pb, int n, f
  * pixels outside it count as masked).  Tile t's pixels are rows
  * t*S*S .. (t+1)*S*S - 1 of the NHWC buffers. */
 /* Network input, 16 bf16 channels: (h - tile mean of valid h) / h_scale, u,
- * v, valid (1/0), then zeros; mean_ws holds n_tiles floats of workspace. */
+ * v, valid (1/0), div(uv), 10 x Laplacian(h), 5 x |grad h| (central
+ * differences in fp32, per pixel; 0 where a 4-neighbour is masked or off the
+ * grid), then zeros; mean_ws holds n_tiles floats of workspace. */
 int slm_nn_tile_inputs(const float* h, const float* uv, const uint8_t* mask, int nx, int ny, const int* tiles,
                        int n_tiles, int S, float h_scale, float* mean_ws, void* x, void* stream, uint32_t flags);
 /* Training targets from the raw downstream fields (uint32): log1p(slt/d^2),

@@ -51,7 +51,7 @@ class RefUNet:
         d0 = self.conv("d0b", self.conv("d0a", torch.cat([up(d1), e0], 1)))
         return self.conv("head", d0)
 
-    def loss(self, head, target, valid, w_bce=1.0):
+    def loss(self, head, target, valid, w_bce=1.0, w_len=1.0):
         """head [N, 32, H, W]; target [N, H, W, 3]; valid [N, H, W] bool."""
         v = torch.from_numpy(valid)
         t = torch.from_numpy(target)
@@ -60,7 +60,7 @@ class RefUNet:
         l0 = ((a - t[..., 0][v]) ** 2).sum()
         l1 = ((b - t[..., 1][v]) ** 2).sum()
         l2 = F.binary_cross_entropy_with_logits(z, t[..., 2][v], reduction="sum")
-        return (l0 + l1 + w_bce * l2) / n, (l0, l1, l2)
+        return (w_len * l0 + l1 + w_bce * l2) / n, (l0, l1, l2)
 
     def grads(self):
         """{name: (dW [cout, taps, cin], db [cout])} as fp32 numpy."""

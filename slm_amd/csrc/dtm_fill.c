@@ -82,6 +82,9 @@ int slm_fill_sinks(double* h, int64_t nx, int64_t ny, double eps) {
     return 0;
 }
 
+int slm_v_valley_segments(double* out, int64_t nx, int64_t ny, int n, const double* cx, const double* cy,
+                          const double* s, const double* c, double depth, double hw, double half_len);
+
 /* V-shaped incisions for the synthetic DTM generator: for each valley k,
  * out[x][y] -= depth * clip(1 - |(x - cx)*s - (y - cy)*c| / hw, 0, 1), valleys
  * applied in order.  Only the band of columns where the distance can be
@@ -90,6 +93,14 @@ int slm_fill_sinks(double* h, int64_t nx, int64_t ny, double eps) {
  * out: float64 [nx*ny] C-order (x major), accumulated in place. */
 int slm_v_valleys(double* out, int64_t nx, int64_t ny, int n, const double* cx, const double* cy,
                   const double* s, const double* c, double depth, double hw) {
+    return slm_v_valley_segments(out, nx, ny, n, cx, cy, s, c, depth, hw, 0.0);
+}
+
+/* As slm_v_valleys, for segments: with half_len > 0 each valley also fades
+ * out along its axis, its depth times clip((half_len - |a|) / hw, 0, 1)
+ * where a = (x - cx)*c + (y - cy)*s is the position along the valley. */
+int slm_v_valley_segments(double* out, int64_t nx, int64_t ny, int n, const double* cx, const double* cy,
+                          const double* s, const double* c, double depth, double hw, double half_len) {
     if (!out || nx <= 0 || ny <= 0 || hw <= 0.0) return -1;
     for (int k = 0; k < n; k++) {
         for (int64_t x = 0; x < nx; x++) {
@@ -107,6 +118,12 @@ int slm_v_valleys(double* out, int64_t nx, int64_t ny, int n, const double* cx, 
                 const double d = fabs(a - ((double)y - cy[k]) * c[k]);
                 double t = 1.0 - d / hw;
                 t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+                if (half_len > 0.0) {
+                    const double along = fabs(((double)x - cx[k]) * c[k] + ((double)y - cy[k]) * s[k]);
+                    double f = (half_len - along) / hw;
+                    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+                    t *= f;
+                }
                 row[y] -= depth * t;
             }
         }

@@ -35,8 +35,11 @@ __device__ __forceinline__ bf16 f2b(float v) { return (bf16)v; }   // round to n
 // ------------------------------------------------------------------ convolution
 
 // One wave computes a 32-pixel x 32-channel tile; a block has WM x WN waves.
+// Input channels [0, c_up) come from x2, a half-resolution image read at
+// (h/2, w/2) (a fused nearest 2x upsample); the rest from x (c_up = 0: x only).
 template <int TAPS, bool RELU, int WM, int WN>
-__global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ w,
+__global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ x2,
+                                              int ldx2, int c_up, const bf16* __restrict__ w,
                                               const float* __restrict__ bias, bf16* __restrict__ y, int ldy,
                                               int M, int H, int W, int Cin, int Cout) {
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -55,10 +58,12 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ld
         const int hh = ph + (TAPS == 9 ? tap / 3 - 1 : 0), ww = pw + (TAPS == 9 ? tap % 3 - 1 : 0);
         const bool ok = hh >= 0 && hh < H && ww >= 0 && ww < W;
         const bf16* xp = x + (size_t)((pn * H + (ok ? hh : 0)) * W + (ok ? ww : 0)) * ldx + kg;
+        const bf16* xp2 =
+            x2 + (size_t)((pn * (H / 2) + (ok ? hh : 0) / 2) * (W / 2) + (ok ? ww : 0) / 2) * ldx2 + kg;
         const bf16* wp = wrow + tap * Cin + kg;
 
         for (int c0 = 0; c0 < Cin; c0 += 16) {
-            const bf16x8 a = ok ? *(const bf16x8*)(xp + c0) : zero;
+            const bf16x8 a = ok ? *(const bf16x8*)(c0 < c_up ? xp2 + c0 : xp + c0) : zero;
             const bf16x8 b = *(const bf16x8*)(wp + c0);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
@@ -70,6 +75,109 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ld
         float val = acc[v] + bc;
         if (RELU) val = fmaxf(val, 0.0f);
         y[(size_t)(pm + rr) * ldy + co] = f2b(val);
+    }
+}
+
+// 3x3 convolution with LDS-staged tiles.  A block computes a BH x BW pixel
+// tile of one image (BW = 32 or 16 columns) for CT output channels.  For
+// each chunk of KC input channels it stages the (BH+2) x (BW+2) input halo
+// and the CT x 9 x KC weights in LDS (rows padded by 8 bf16 against bank
+// conflicts); then every tap reads its shifted A fragments from the halo, so
+// each input element comes from L2 once per chunk instead of nine times.
+// Wave w owns pixel groups w*RW .. w*RW+RW-1 (a group is 32 pixels: one
+// 32-wide row, or two 16-wide rows) and all CT channels: RW x CT/32
+// accumulator tiles of 32x32.
+// Input channels [0, c_up) come from x2 at half resolution, as in k_conv.
+template <int BW, int RW, int CT, int KC, bool RELU>
+__global__ __launch_bounds__(256, 2) void k_conv_lds(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ x2,
+                                                     int ldx2, int c_up, const bf16* __restrict__ w,
+                                                     const float* __restrict__ bias, bf16* __restrict__ y, int ldy,
+                                                     int H, int W, int Cin, int Cout) {
+    constexpr int GR = 32 / BW;                 // rows per pixel group
+    constexpr int BH = 4 * RW * GR;             // block tile rows
+    constexpr int XP = KC + 8;                  // halo pixel stride (elements)
+    constexpr int WP = 9 * KC + 8;              // weight row stride (elements)
+    constexpr int NJ = CT / 32;
+    __shared__ __attribute__((aligned(16))) bf16 xs[(BH + 2) * (BW + 2) * XP];
+    __shared__ __attribute__((aligned(16))) bf16 ws[CT * WP];
+    const int tiles_w = W / BW, tiles_h = H / BH;
+    const int tw = blockIdx.x % tiles_w, th = (blockIdx.x / tiles_w) % tiles_h, n = blockIdx.x / (tiles_w * tiles_h);
+    const int h0 = th * BH, w0 = tw * BW, co0 = blockIdx.y * CT;
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    const int r = lane & 31, kg = (lane >> 5) * 8;
+    f32x16 acc[RW][NJ];
+#pragma unroll
+    for (int i = 0; i < RW; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) acc[i][j] = f32x16{};
+    // this lane's pixel (row, column) in the block tile, per group
+    int prow[RW], pcol[RW];
+#pragma unroll
+    for (int i = 0; i < RW; i++) {
+        const int g = wid * RW + i;
+        prow[i] = g * GR + r / BW;
+        pcol[i] = r % BW;
+    }
+    const bf16x8 zero = {};
+    for (int c0 = 0; c0 < Cin; c0 += KC) {
+        __syncthreads();
+        constexpr int XV = (BH + 2) * (BW + 2) * (KC / 8);
+        for (int v = t; v < XV; v += 256) {
+            const int px = v / (KC / 8), part = v % (KC / 8);
+            const int hr = px / (BW + 2), hc = px % (BW + 2);
+            const int gh = h0 + hr - 1, gw = w0 + hc - 1;
+            bf16x8 val = zero;
+            if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
+                const int c = c0 + part * 8;
+                val = c < c_up ? *(const bf16x8*)(x2 + ((size_t)(n * (H / 2) + gh / 2) * (W / 2) + gw / 2) * ldx2 + c)
+                               : *(const bf16x8*)(x + ((size_t)(n * H + gh) * W + gw) * ldx + c);
+            }
+            *(bf16x8*)&xs[px * XP + part * 8] = val;
+        }
+        constexpr int WV = CT * 9 * (KC / 8);
+        for (int v = t; v < WV; v += 256) {
+            const int co = v / (9 * KC / 8), rem = v % (9 * KC / 8);
+            const int tap = rem / (KC / 8), part = rem % (KC / 8);
+            *(bf16x8*)&ws[co * WP + tap * KC + part * 8] =
+                *(const bf16x8*)(w + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 + part * 8);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 16) {
+                bf16x8 b[NJ];
+#pragma unroll
+                for (int j = 0; j < NJ; j++) b[j] = *(const bf16x8*)&ws[(j * 32 + r) * WP + tap * KC + kk + kg];
+#pragma unroll
+                for (int i = 0; i < RW; i++) {
+                    const bf16x8 a =
+                        *(const bf16x8*)&xs[((prow[i] + dy) * (BW + 2) + pcol[i] + dx) * XP + kk + kg];
+#pragma unroll
+                    for (int j = 0; j < NJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[j], acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // epilogue: D row = pixel within the group, column = channel
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const int co = co0 + j * 32 + r;
+        const float bc = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < RW; i++) {
+            const int g = wid * RW + i;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int rr = (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+                const int row = g * GR + rr / BW, col = rr % BW;
+                float val = acc[i][j][v] + bc;
+                if (RELU) val = fmaxf(val, 0.0f);
+                y[((size_t)(n * H + h0 + row) * W + w0 + col) * ldy + co] = f2b(val);
+            }
+        }
     }
 }
 
@@ -204,13 +312,15 @@ __global__ void k_pool_bwd(const bf16* __restrict__ x, int ldx, const bf16* __re
 }
 
 // nearest 2x upsample: y[n][h][w][c] = x[n][h/2][w/2][c]  (y is H x W)
+// (8 channels per thread: C, ldx, ldy multiples of 8, 16-byte aligned buffers)
 __global__ void k_up(const bf16* __restrict__ x, int ldx, bf16* __restrict__ y, int ldy, int N, int H, int W, int C) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)N * H * W * C) return;
-    const int c = i % C;
-    const size_t q = i / C;
+    const int C8 = C / 8;
+    if (i >= (size_t)N * H * W * C8) return;
+    const int c = (int)(i % C8) * 8;
+    const size_t q = i / C8;
     const int w = q % W, h = (q / W) % H, n = q / ((size_t)W * H);
-    y[q * ldy + c] = x[((size_t)(n * (H / 2) + h / 2) * (W / 2) + w / 2) * ldx + c];
+    *(bf16x8*)(y + q * ldy + c) = *(const bf16x8*)(x + ((size_t)(n * (H / 2) + h / 2) * (W / 2) + w / 2) * ldx + c);
 }
 
 // dx[n][h][w][c] = sum of the 4 dy of its 2x2 block (dy is 2H x 2W)
@@ -236,13 +346,13 @@ __global__ void k_add(bf16* __restrict__ a, int lda, const bf16* __restrict__ b,
     a[p * lda + c] = f2b(b2f(a[p * lda + c]) + b2f(b[p * ldb + c]));
 }
 
-// Loss over valid pixels: squared error on channels 0, 1 (log1p slt, slc) and
-// binary cross-entropy with logits on channel 2 (channel mask).  Writes the
-// gradient (scaled by 1/n_valid) into dhead and adds the summed loss terms
-// to out[0..2].
+// Loss over valid pixels: w_len * squared error on channel 0 (log1p slt),
+// squared error on channel 1 (log1p slc) and w_bce * binary cross-entropy
+// with logits on channel 2 (channel mask).  Writes the gradient (scaled by
+// 1/n_valid) into dhead and adds the unweighted summed terms to out[0..2].
 __global__ __launch_bounds__(256) void k_loss(const bf16* __restrict__ head, int ldh, const float* __restrict__ target,
                                               const uint8_t* __restrict__ valid, bf16* __restrict__ dhead, int lddh,
-                                              int M, float inv_n, float w_bce, float* __restrict__ out) {
+                                              int M, float inv_n, float w_len, float w_bce, float* __restrict__ out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     float l0 = 0.f, l1 = 0.f, l2 = 0.f;
     if (p < M) {
@@ -256,7 +366,7 @@ __global__ __launch_bounds__(256) void k_loss(const bf16* __restrict__ head, int
             l1 = (b - tb) * (b - tb);
             // BCE with logits, stable: max(z,0) - z*t + log(1+exp(-|z|))
             l2 = fmaxf(z, 0.f) - z * tc + log1pf(expf(-fabsf(z)));
-            g0 = 2.f * (a - ta) * inv_n;
+            g0 = w_len * 2.f * (a - ta) * inv_n;
             g1 = 2.f * (b - tb) * inv_n;
             g2 = w_bce * (1.f / (1.f + expf(-z)) - tc) * inv_n;
         }
@@ -338,7 +448,13 @@ __global__ __launch_bounds__(256) void k_tile_mean(const float* __restrict__ h, 
     }
 }
 
-// Network input, 16 bf16 channels per pixel: (h - mean)/h_scale, u, v, valid, 0...
+// Network input, 16 bf16 channels per pixel: (h - mean)/h_scale, u, v,
+// valid, then three second-order features computed in fp32 before the bf16
+// rounding (which would leave h too coarse to difference): the divergence
+// of uv (central differences, per pixel), 10 x the 5-point Laplacian of h
+// (metres per pixel^2) and 5 x the slope |grad h| (metres per pixel); the
+// three are 0 where a 4-neighbour is masked or outside the grid.  Channels
+// 7..15 are 0.
 __global__ void k_tile_inputs(const float* __restrict__ h, const float* __restrict__ uv,
                               const uint8_t* __restrict__ mask, int nx, int ny, const int* __restrict__ tiles,
                               int n_tiles, int S, const float* __restrict__ mean, float inv_scale,
@@ -353,6 +469,18 @@ __global__ void k_tile_inputs(const float* __restrict__ h, const float* __restri
         lo[1] = f2b(uv[2 * idx]);
         lo[2] = f2b(uv[2 * idx + 1]);
         lo[3] = f2b(1.0f);
+        const int gi = (int)(idx / ny), gj = (int)(idx % ny);
+        if (gi > 0 && gj > 0 && gi + 1 < nx && gj + 1 < ny) {
+            const size_t n0 = idx - ny, n1 = idx + ny, n2 = idx - 1, n3 = idx + 1;
+            if (!mask[n0] && !mask[n1] && !mask[n2] && !mask[n3]) {
+                const float div = 0.5f * (uv[2 * n1] - uv[2 * n0]) + 0.5f * (uv[2 * n3 + 1] - uv[2 * n2 + 1]);
+                const float lap = h[n0] + h[n1] + h[n2] + h[n3] - 4.0f * h[idx];
+                const float gx = 0.5f * (h[n1] - h[n0]), gy = 0.5f * (h[n3] - h[n2]);
+                lo[4] = f2b(div);
+                lo[5] = f2b(10.0f * lap);
+                lo[6] = f2b(5.0f * sqrtf(gx * gx + gy * gy));
+            }
+        }
     }
     *(bf16x8*)(x + i * 16) = lo;
     *(bf16x8*)(x + i * 16 + 8) = hi;
@@ -436,31 +564,82 @@ int check_conv(int M, int Cin, int Cout, int ldx, int ldy, int taps) {
 
 // ------------------------------------------------------------------ C ABI
 
-extern "C" int slm_nn_conv(const void* x, int ldx, const void* w, const float* bias, void* y, int ldy, int N, int H,
-                           int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags) {
+namespace {
+
+int conv_impl(const void* x, int ldx, const void* x2, int ldx2, int c_up, const void* w, const float* bias, void* y,
+              int ldy, int N, int H, int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags) {
     const int M = N * H * W;
     int rc = check_conv(M, Cin, Cout, ldx, ldy, taps);
     if (rc) return rc;
     if (!x || !w || !y) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     if (((uintptr_t)x) % 16) return slm_fail(SLM_E_INVALID_ARG, "x not 16-byte aligned");
+    if (((uintptr_t)w) % 16) return slm_fail(SLM_E_INVALID_ARG, "w not 16-byte aligned");
+    if (c_up) {
+        if (!x2 || c_up < 0 || c_up > Cin || c_up % 32 || ldx2 < c_up || ldx2 % 8 || ((uintptr_t)x2) % 16 || H % 2 ||
+            W % 2)
+            return slm_fail(SLM_E_INVALID_ARG, "bad upsampled source (c_up %d, ldx2 %d)", c_up, ldx2);
+    } else {
+        x2 = x;
+        ldx2 = ldx;
+    }
     hipStream_t s = (hipStream_t)stream;
+    const bf16* x2b = (const bf16*)x2;
     const bf16* xb = (const bf16*)x;
     const bf16* wb = (const bf16*)w;
     bf16* yb = (bf16*)y;
+    // LDS-tiled path: 3x3, rows of 32 (or exactly 16) columns, whole block tiles
+    const int bw = (W % 32 == 0) ? 32 : (W == 16 ? 16 : 0);
+    const int ct = (Cout % 64 == 0) ? 64 : 32;
+    const int rw = (bw == 32 && ct == 32) ? 4 : 2;
+    const int bh = bw ? 4 * rw * (32 / bw) : 0;
+    if (taps == 9 && bw && H % bh == 0 && !(flags & SLM_FLAG_NN_DIRECT)) {
+        const dim3 grid((unsigned)(N * (H / bh) * (W / bw)), (unsigned)(Cout / ct));
+        const bool k32 = Cin % 32 == 0;
+#define LDS_L(BW, RW, CT, KC)                                                                                      \
+    do {                                                                                                           \
+        if (relu) hipLaunchKernelGGL((k_conv_lds<BW, RW, CT, KC, true>), grid, dim3(256), 0, s, xb, ldx, x2b,      \
+                                     ldx2, c_up, wb, bias, yb, ldy, H, W, Cin, Cout);                              \
+        else hipLaunchKernelGGL((k_conv_lds<BW, RW, CT, KC, false>), grid, dim3(256), 0, s, xb, ldx, x2b, ldx2,     \
+                                c_up, wb, bias, yb, ldy, H, W, Cin, Cout);                                         \
+    } while (0)
+        if (bw == 32) {
+            if (ct == 64) { if (k32) LDS_L(32, 2, 64, 32); else LDS_L(32, 2, 64, 16); }
+            else { if (k32) LDS_L(32, 4, 32, 32); else LDS_L(32, 4, 32, 16); }
+        } else {
+            if (ct == 64) { if (k32) LDS_L(16, 2, 64, 32); else LDS_L(16, 2, 64, 16); }
+            else { if (k32) LDS_L(16, 2, 32, 32); else LDS_L(16, 2, 32, 16); }
+        }
+#undef LDS_L
+        return slm_finish(stream, flags);
+    }
     if (Cout >= 64) {
         const dim3 grid((M + 63) / 64, (Cout + 63) / 64);
-#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 2, 2>), grid, dim3(256), 0, s, xb, ldx, wb, bias, yb, ldy, M, H, W, Cin, Cout)
+#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 2, 2>), grid, dim3(256), 0, s, xb, ldx, x2b, ldx2, c_up, wb, bias, yb, ldy, M, H, W, Cin, Cout)
         if (taps == 9) { if (relu) CONV_L(9, true); else CONV_L(9, false); }
         else { if (relu) CONV_L(1, true); else CONV_L(1, false); }
 #undef CONV_L
     } else {
         const dim3 grid((M + 127) / 128, (Cout + 31) / 32);
-#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 4, 1>), grid, dim3(256), 0, s, xb, ldx, wb, bias, yb, ldy, M, H, W, Cin, Cout)
+#define CONV_L(T, R) hipLaunchKernelGGL((k_conv<T, R, 4, 1>), grid, dim3(256), 0, s, xb, ldx, x2b, ldx2, c_up, wb, bias, yb, ldy, M, H, W, Cin, Cout)
         if (taps == 9) { if (relu) CONV_L(9, true); else CONV_L(9, false); }
         else { if (relu) CONV_L(1, true); else CONV_L(1, false); }
 #undef CONV_L
     }
     return slm_finish(stream, flags);
+}
+
+}  // namespace
+
+extern "C" int slm_nn_conv(const void* x, int ldx, const void* w, const float* bias, void* y, int ldy, int N, int H,
+                           int W, int Cin, int Cout, int taps, int relu, void* stream, uint32_t flags) {
+    return conv_impl(x, ldx, nullptr, 0, 0, w, bias, y, ldy, N, H, W, Cin, Cout, taps, relu, stream, flags);
+}
+
+extern "C" int slm_nn_conv_up(const void* x, int ldx, const void* x2, int ldx2, int c_up, const void* w,
+                              const float* bias, void* y, int ldy, int N, int H, int W, int Cin, int Cout, int taps,
+                              int relu, void* stream, uint32_t flags) {
+    if (c_up <= 0) return slm_fail(SLM_E_INVALID_ARG, "c_up must be positive");
+    return conv_impl(x, ldx, x2, ldx2, c_up, w, bias, y, ldy, N, H, W, Cin, Cout, taps, relu, stream, flags);
 }
 
 extern "C" int slm_nn_wflip(const void* w, void* wt, int taps, int Cin, int Cout, void* stream, uint32_t flags) {
@@ -526,8 +705,9 @@ extern "C" int slm_nn_pool_bwd(const void* x, int ldx, const void* y, int ldy, c
 
 extern "C" int slm_nn_up(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
                          uint32_t flags) {
-    if (!x || !y || H % 2 || W % 2) return slm_fail(SLM_E_INVALID_ARG, "bad upsample arguments");
-    hipLaunchKernelGGL(k_up, g1((size_t)N * H * W * C), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, ldx,
+    if (!x || !y || H % 2 || W % 2 || C % 8 || ldx % 8 || ldy % 8 || ((uintptr_t)x) % 16 || ((uintptr_t)y) % 16)
+        return slm_fail(SLM_E_INVALID_ARG, "bad upsample arguments");
+    hipLaunchKernelGGL(k_up, g1((size_t)N * H * W * (C / 8)), dim3(256), 0, (hipStream_t)stream, (const bf16*)x, ldx,
                        (bf16*)y, ldy, N, H, W, C);
     return slm_finish(stream, flags);
 }
@@ -548,11 +728,11 @@ extern "C" int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C
 }
 
 extern "C" int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead, int lddh,
-                           int M, float inv_n, float w_bce, float* out, void* stream, uint32_t flags) {
+                           int M, float inv_n, float w_len, float w_bce, float* out, void* stream, uint32_t flags) {
     if (!head || !target || !valid || !dhead || !out || ldh < 3 || lddh < 3)
         return slm_fail(SLM_E_INVALID_ARG, "bad loss arguments");
     hipLaunchKernelGGL(k_loss, g1((size_t)M), dim3(256), 0, (hipStream_t)stream, (const bf16*)head, ldh, target, valid,
-                       (bf16*)dhead, lddh, M, inv_n, w_bce, out);
+                       (bf16*)dhead, lddh, M, inv_n, w_len, w_bce, out);
     return slm_finish(stream, flags);
 }
 

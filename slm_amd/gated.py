@@ -46,14 +46,21 @@ TRACE = {"subpixel_seed_point_density": 1, "jitter_magnitude": 1.0, "max_length"
 H_SCALE = 10.0          # metres per unit of the height input channel
 
 
+# stationary noise scale: about 60 m of relief on a 2048^2 grid
+NOISE_SCALE = 60.0 / 21800.0
+
+
 def family_dtm(n: int, seed: int) -> np.ndarray:
-    """n x n DTM at 1 m: spectral noise whose relief grows with n (60 m per
-    2048 px, so slopes are alike at every n), 5% tilt, V-valleys 30 m deep and
-    61 px wide at a density of 3 per 2048^2, sinks filled, 2-px NaN border."""
+    """n x n DTM at 1 m with the same local statistics at every n: stationary
+    spectral noise (amplitude ~ k^-2, about 60 m of relief per 2048^2
+    window), 5% tilt, V-valley segments 30 m deep, 61 px half width and
+    2400 px long, centred uniformly over the grid at 3 per 2048^2, sinks
+    filled, 2-px NaN border."""
     scale = n / 2048.0
-    return synth.make_dtm(n, pixel_size=1.0, relief=60.0 * scale, tilt=0.05, seed=seed, nan_border=2,
+    return synth.make_dtm(n, pixel_size=1.0, tilt=0.05, seed=seed, nan_border=2,
                           n_valleys=max(1, int(round(3 * scale * scale))), valley_depth=30.0,
-                          valley_half_width_px=61.44)
+                          valley_half_width_px=61.44, noise_scale=NOISE_SCALE, valley_half_length_px=1200.0,
+                          valley_centre_range=(0.0, 1.0))
 
 
 def _dev(device=None):
@@ -222,24 +229,35 @@ def training_set(n_dtms=32, n=2048, size=128, seed0=1000, theta_quantile=0.97, t
     return TrainingSet(x, target, valid, n_valid, size, theta, timings)
 
 
-def constant_loss(data):
+def constant_loss(data, w_len=0.25, w_bce=5.0):
     """Loss per valid pixel of the best constant prediction (channel means for
-    the squared errors, the prevalence logit for the cross-entropy)."""
+    the squared errors, the prevalence logit for the cross-entropy), with the
+    weights of train()."""
     import torch
     v = data.valid.bool()
     t = data.target[v].double()
-    var = t[:, :2].var(0, unbiased=False).sum().item()
+    var = t[:, :2].var(0, unbiased=False).cpu().numpy()
     p = min(max(t[:, 2].mean().item(), 1e-9), 1 - 1e-9)
-    return var - (p * math.log(p) + (1 - p) * math.log(1 - p))
+    return w_len * var[0] + var[1] - w_bce * (p * math.log(p) + (1 - p) * math.log(1 - p))
 
 
-def train(data, epochs=20, batch=64, lr=1e-3, seed=0, device=None, log=None):
+def train(data, epochs=20, batch=32, lr=1e-3, seed=0, device=None, log=None, w_len=0.25, w_bce=5.0,
+          warmup=200, cosine=True):
     """Adam over shuffled batches of tiles; returns (net, history) where
-    history holds the mean loss per valid pixel of each epoch and timings."""
+    history holds the mean loss per valid pixel of each epoch (weighted:
+    w_len on the length term, w_bce on the channel term) and timings.  The
+    learning rate rises linearly to lr over warmup steps, then (cosine)
+    decays to 0 along half a cosine over the remaining steps."""
     import torch
     dev = _dev(device)
     size = data.size
-    net = S.UNet(S.init_params(seed), device=dev)
+    params = S.init_params(seed)
+    # head biases at the constant predictor: target means and the prevalence logit
+    v = data.valid.bool()
+    mu = data.target[v].double().mean(0).cpu().numpy()
+    prev = min(max(float(mu[2]), 1e-6), 1 - 1e-6)
+    params["head"][1][:3] = [mu[0], mu[1], math.log(prev / (1 - prev))]
+    net = S.UNet(params, device=dev)
     net._alloc(batch, size, size)
     xv = data.x.view(data.n_tiles, size * size * S.IN_CH)
     tv = data.target.view(data.n_tiles, size * size * 3)
@@ -252,6 +270,8 @@ def train(data, epochs=20, batch=64, lr=1e-3, seed=0, device=None, log=None):
     n_batches = data.n_tiles // batch
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     totals = []
+    total_steps = epochs * n_batches
+    step = 0
     ev[0].record()
     for ep in range(epochs):
         perm = rng.permutation(data.n_tiles)
@@ -265,24 +285,62 @@ def train(data, epochs=20, batch=64, lr=1e-3, seed=0, device=None, log=None):
             nv = int(data.n_valid[perm[b * batch:(b + 1) * batch]].sum())
             tot += nv
             net.forward(net.A["x"], batch, size, size)
-            dh, _, _ = net.loss(tb.view(-1, 3), vb.view(-1), n_valid=nv, out=sums[ep])
+            dh, _, _ = net.loss(tb.view(-1, 3), vb.view(-1), n_valid=nv, out=sums[ep], w_len=w_len, w_bce=w_bce)
             net.backward(dh)
-            net.adam(lr=lr)
+            f = min(1.0, (step + 1) / max(1, warmup))
+            if cosine and step >= warmup:
+                f *= 0.5 * (1.0 + math.cos(math.pi * (step - warmup) / max(1, total_steps - warmup)))
+            net.adam(lr=lr * f)
+            step += 1
         totals.append(tot)
         if log:
             s = sums[ep].cpu().numpy()
-            log("epoch %d: loss %.4f (sse %.4f %.4f, bce %.4f)" % (ep, s.sum() / tot, s[0] / tot, s[1] / tot,
-                                                                   s[2] / tot))
+            log("epoch %d: loss %.4f (sse %.4f %.4f, bce %.4f)" % (ep, (w_len * s[0] + s[1] + w_bce * s[2]) / tot,
+                                                                   s[0] / tot, s[1] / tot, s[2] / tot))
     ev[1].record()
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[1])
-    s = sums.cpu().numpy()
+    s = sums.cpu().numpy().astype(np.float64) * np.array([w_len, 1.0, w_bce])
     hist = {"loss": [float(s[e].sum() / totals[e]) for e in range(epochs)],
             "train_ms": ms, "steps": epochs * n_batches, "batch": batch,
             "tiles_per_s": epochs * n_batches * batch / (ms / 1e3),
             # forward + backward-data + backward-weight: three passes of the forward FLOP
             "tflops": 3 * net.flops_per_pixel() * size * size * batch * epochs * n_batches / (ms / 1e3) / 1e12}
     return net, hist
+
+
+def calibrate(net, data, n_tiles=1024, batch=64):
+    """Shift the channel logit so that 0 is the threshold with the best
+    channel-pixel F1 on (up to) n_tiles training tiles: the head's channel
+    bias absorbs the threshold.  Returns (threshold, F1 at it, F1 at 0)."""
+    import torch
+    size = data.size
+    n = min(n_tiles, data.n_tiles) // batch * batch
+    logits, truth = [], []
+    xv = data.x.view(data.n_tiles, size * size * S.IN_CH)
+    for b0 in range(0, n, batch):
+        net._alloc(batch, size, size)
+        net.A["x"].view(batch, -1).copy_(xv[b0:b0 + batch])
+        head = net.forward(net.A["x"], batch, size, size)
+        sl = slice(b0 * size * size, (b0 + batch) * size * size)
+        v = data.valid[sl].bool()
+        logits.append(head[:, 2].float()[v])
+        truth.append(data.target[sl, 2][v] > 0.5)
+    z, t = torch.cat(logits), torch.cat(truth)
+    thr = torch.arange(-4.0, 4.0001, 0.05, device=z.device)
+    pc = z[None, :] > thr[:, None] if z.numel() * thr.numel() < 2 ** 31 else None
+    f1s = []
+    for k in range(thr.numel()):
+        p = pc[k] if pc is not None else z > thr[k]
+        tp = (p & t).sum()
+        f1s.append((2 * tp / (2 * tp + (p & ~t).sum() + (~p & t).sum()).clamp_min(1)).item())
+    k = int(np.argmax(f1s))
+    best = float(thr[k].item())
+    f0 = f1s[int(np.argmin(np.abs(thr.cpu().numpy())))]
+    _, _, ob, _ = net.offsets["head"]
+    net.p[ob + 2] -= best
+    net.pb[ob + 2] = net.p[ob + 2].to(net.pb.dtype)
+    return best, f1s[k], f0
 
 
 # ------------------------------------------------------------------ maps
@@ -388,18 +446,40 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
     return out, report
 
 
-def compare(fields, ref, mask):
+def _f1(pc, tc):
+    tp = int((pc & tc).sum().item())
+    fp = int((pc & ~tc).sum().item())
+    fn = int((~pc & tc).sum().item())
+    return 2 * tp / max(1, 2 * tp + fp + fn), tp + fn, tp + fp
+
+
+def compare(fields, ref, mask, theta=None, shape=None):
     """Median relative error of log-density and channel-pixel F1 of fields
-    against ref over the valid pixels (mask == 0)."""
+    against ref over the valid pixels (mask == 0); given theta, also the F1
+    of channels read from the log-density (density >= theta); given the grid
+    shape, also the F1 with one pixel of tolerance."""
     v = mask.reshape(-1) == 0
     r1 = ref[1][v]
     rel = (fields[1][v] - r1).abs() / r1.clamp_min(1e-6)
     rel0 = (fields[0][v] - ref[0][v]).abs() / ref[0][v].clamp_min(1e-6)
-    pc, tc = fields[2][v] > 0, ref[2][v] > 0
-    tp = int((pc & tc).sum().item())
-    fp = int((pc & ~tc).sum().item())
-    fn = int((~pc & tc).sum().item())
-    return {"median_rel_err_log_density": float(rel.median().item()),
-            "median_rel_err_log_length": float(rel0.median().item()),
-            "channel_f1": 2 * tp / max(1, 2 * tp + fp + fn),
-            "channel_px_ref": tp + fn, "channel_px_pred": tp + fp}
+    tc = ref[2][v] > 0
+    f1, n_ref, n_pred = _f1(fields[2][v] > 0, tc)
+    out = {"median_rel_err_log_density": float(rel.median().item()),
+           "median_rel_err_log_length": float(rel0.median().item()),
+           "channel_f1": f1, "channel_px_ref": n_ref, "channel_px_pred": n_pred}
+    if shape is not None:
+        # F1 with one pixel of tolerance: a predicted channel pixel counts when
+        # a reference channel pixel is among its 8 neighbours, and vice versa
+        import torch
+        pm = (fields[2] > 0).view(1, 1, *shape).float()
+        tm = (ref[2] > 0).view(1, 1, *shape).float()
+        vm = v.view(*shape)
+        pd = torch.nn.functional.max_pool2d(pm, 3, 1, 1)[0, 0] > 0
+        td = torch.nn.functional.max_pool2d(tm, 3, 1, 1)[0, 0] > 0
+        p_ok = int(((pm[0, 0] > 0) & td & vm).sum().item())
+        t_ok = int(((tm[0, 0] > 0) & pd & vm).sum().item())
+        prec, recall = p_ok / max(1, n_pred), t_ok / max(1, n_ref)
+        out["channel_f1_tol1px"] = 2 * prec * recall / max(1e-12, prec + recall)
+    if theta is not None:
+        out["channel_f1_from_density"] = _f1(fields[1][v] >= math.log1p(theta), tc)[0]
+    return out

@@ -48,6 +48,7 @@ _PROTOS = {
     "slm_kde_pdf": (ctypes.c_int, [ctypes.POINTER(SlmKdeParams), _P, _P, _P, _U32]),
     "slm_map_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_nn_conv": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _U32]),
+    "slm_nn_conv_up": (ctypes.c_int, [_P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_wflip": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _U32]),
     "slm_nn_conv_wgrad": (ctypes.c_int, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_relu_bwd": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _P, _U32]),
@@ -56,7 +57,8 @@ _PROTOS = {
     "slm_nn_up": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_up_bwd": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_add": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _P, _U32]),
-    "slm_nn_loss": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P, _U32]),
+    "slm_nn_loss": (ctypes.c_int, [_P, _I, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P,
+                                   _U32]),
     "slm_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, _I, _P, _U32]),
     "slm_nn_tile_inputs": (ctypes.c_int, [_P, _P, _P, _I, _I, _P, _I, _I, ctypes.c_float, _P, _P, _P, _U32]),

@@ -28,7 +28,8 @@ import numpy as np
 
 from . import hip
 
-IN_CH = 16          # input channels: h (normalised), u, v, valid, zeros
+IN_CH = 16          # input channels (the MFMA K granule): N_FEATURES used, then zeros
+N_FEATURES = 7      # h (normalised), u, v, valid, div(uv), Laplacian(h), slope
 OUT_CH = 32         # head channels computed (0..2 used)
 N_OUT = 3
 
@@ -46,6 +47,12 @@ def _s():
 def conv(x, ldx, w, b, y, ldy, N, H, W, cin, cout, taps, relu):
     hip.check(hip.load().slm_nn_conv(_p(x), ldx, _p(w), _p(b) if b is not None else None, _p(y), ldy, N, H, W, cin,
                                      cout, taps, 1 if relu else 0, _s(), 0))
+
+
+def conv_up(x, ldx, x2, ldx2, c_up, w, b, y, ldy, N, H, W, cin, cout, taps, relu):
+    """conv over [up2(x2) channels 0..c_up | x channels c_up..cin]."""
+    hip.check(hip.load().slm_nn_conv_up(_p(x), ldx, _p(x2), ldx2, c_up, _p(w), _p(b) if b is not None else None,
+                                        _p(y), ldy, N, H, W, cin, cout, taps, 1 if relu else 0, _s(), 0))
 
 
 def wflip(w, wt, taps, cin, cout):
@@ -105,7 +112,7 @@ def init_params(seed=0):
         fan_in = cin * taps
         w = (rs.standard_normal((cout, taps, cin)) * math.sqrt(2.0 / fan_in)).astype(np.float32)
         if name == "e0a":
-            w[:, :, 4:] = 0.0          # zero input channels carry nothing
+            w[:, :, N_FEATURES:] = 0.0     # the unused input channels carry nothing
         if name == "head":
             w[N_OUT:] = 0.0
             w *= 0.5
@@ -236,14 +243,21 @@ class UNet:
         pool(cat2_e2, 384, A["p2"], 128, N, H4, W4, 128)
         c("bna", A["p2"], 128, A["bna"], 256, N, H8, W8)
         c("bnb", A["bna"], 256, A["bnb"], 256, N, H8, W8)
-        up(A["bnb"], 256, A["cat2"], 384, N, H4, W4, 256)
-        c("d2a", A["cat2"], 384, A["d2a"], 128, N, H4, W4)
+        # decoder inputs [up2(coarse) | skip]: in inference the upsample is
+        # fused into the conv's input staging (same values, no copy); training
+        # keeps the concatenation, which the weight gradients read
+        def cat_conv(name, coarse, ldc, cat, ldcat, yout, ldy, n, h, w):
+            if self.inference:
+                conv_up(cat, ldcat, coarse, ldc, ldc, self.w(name), self.b(name), yout, ldy, n, h, w,
+                        *self._io(name))
+            else:
+                up(coarse, ldc, cat, ldcat, n, h, w, ldc)
+                c(name, cat, ldcat, yout, ldy, n, h, w)
+        cat_conv("d2a", A["bnb"], 256, A["cat2"], 384, A["d2a"], 128, N, H4, W4)
         c("d2b", A["d2a"], 128, A["d2b"], 128, N, H4, W4)
-        up(A["d2b"], 128, A["cat1"], 192, N, H2, W2, 128)
-        c("d1a", A["cat1"], 192, A["d1a"], 64, N, H2, W2)
+        cat_conv("d1a", A["d2b"], 128, A["cat1"], 192, A["d1a"], 64, N, H2, W2)
         c("d1b", A["d1a"], 64, A["d1b"], 64, N, H2, W2)
-        up(A["d1b"], 64, A["cat0"], 96, N, H, W, 64)
-        c("d0a", A["cat0"], 96, A["d0a"], 32, N, H, W)
+        cat_conv("d0a", A["d1b"], 64, A["cat0"], 96, A["d0a"], 32, N, H, W)
         c("d0b", A["d0a"], 32, A["d0b"], 32, N, H, W)
         c("head", A["d0b"], 32, A["head"], OUT_CH, N, H, W)
         return A["head"]
@@ -304,7 +318,7 @@ class UNet:
         cb("e0b", A["e0a"], 32, cat0_e0, 96, gcat0_e0, 96, G["e0a"], 32, N, H, W)
         cb("e0a", A["x"], IN_CH, A["e0a"], 32, G["e0a"], 32, None, 0, N, H, W, need_dx=False)
 
-    def loss(self, target, valid, w_bce=1.0, n_valid=None, out=None):
+    def loss(self, target, valid, w_bce=1.0, n_valid=None, out=None, w_len=1.0):
         """Loss of the last forward against target float32 [N*H*W, 3] and
         valid uint8 [N*H*W]; returns (dhead, [sse0, sse1, bce] sums, n_valid).
         Given n_valid (the number of valid pixels) the call does not wait for
@@ -319,7 +333,8 @@ class UNet:
             out = t.zeros(3, dtype=t.float32, device=self.dev)
         dhead = self.G["head"]
         hip.check(hip.load().slm_nn_loss(_p(self.A["head"]), OUT_CH, _p(target), _p(valid), _p(dhead), OUT_CH, M,
-                                         ctypes.c_float(1.0 / n_valid), ctypes.c_float(w_bce), _p(out), _s(), 0))
+                                         ctypes.c_float(1.0 / n_valid), ctypes.c_float(w_len), ctypes.c_float(w_bce),
+                                         _p(out), _s(), 0))
         return dhead, out, n_valid
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

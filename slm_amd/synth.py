@@ -26,9 +26,9 @@ def _hostlib():
         lib = ctypes.CDLL(path)
         lib.slm_fill_sinks.restype = ctypes.c_int
         lib.slm_fill_sinks.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_double]
-        lib.slm_v_valleys.restype = ctypes.c_int
-        lib.slm_v_valleys.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int] + \
-            [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double]
+        lib.slm_v_valley_segments.restype = ctypes.c_int
+        lib.slm_v_valley_segments.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_int] + \
+            [ctypes.c_void_p] * 4 + [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _HOSTLIB = lib
     return _HOSTLIB
 
@@ -42,9 +42,12 @@ def fill_sinks(h: np.ndarray, eps: float = 1e-3) -> np.ndarray:
     return a
 
 
-def spectral_noise(nx: int, ny: int, beta: float, rng: np.random.Generator) -> np.ndarray:
+def spectral_noise(nx: int, ny: int, beta: float, rng: np.random.Generator, stationary: bool = False) -> np.ndarray:
     """Gaussian random field whose amplitude spectrum falls as |k|^-beta
-    (power spectrum |k|^-2beta), rescaled to the range [0, 1]."""
+    (power spectrum |k|^-2beta), rescaled to the range [0, 1].  stationary:
+    not rescaled; the amplitude at each wavenumber (cycles per pixel) is
+    then the same for every grid size, so windows of a large grid have the
+    statistics of a small grid's (the large grid adds longer wavelengths)."""
     white = rng.standard_normal((nx, ny))
     f = np.fft.rfft2(white)
     kx = np.fft.fftfreq(nx)[:, None]
@@ -54,24 +57,31 @@ def spectral_noise(nx: int, ny: int, beta: float, rng: np.random.Generator) -> n
     f *= k ** (-beta)
     f[0, 0] = 0.0
     z = np.fft.irfft2(f, s=(nx, ny))
+    if stationary:
+        return z
     z -= z.min()
     z /= max(z.max(), 1e-30)
     return z
 
 
 def v_valleys(nx: int, ny: int, pixel_size: float, n: int, depth: float, half_width_px: float,
-              rng: np.random.Generator) -> np.ndarray:
+              rng: np.random.Generator, half_length_px: float = 0.0, centre_range=(0.2, 0.8)) -> np.ndarray:
     """Sum of n straight V-shaped incisions (depth in m) across the grid:
     -depth * clip(1 - dist/half_width_px, 0, 1) per valley, with the line's
-    angle and centre drawn from rng (angle, then centre x, then centre y)."""
+    angle and centre drawn from rng (angle, then centre x, then centre y,
+    centres uniform over centre_range of each side).  half_length_px > 0
+    makes each valley a segment of that half length whose depth fades out
+    over half_width_px at its ends."""
     ang, cx, cy = np.zeros(n), np.zeros(n), np.zeros(n)
+    lo, hi = centre_range
     for k in range(n):
         ang[k] = rng.uniform(0.0, np.pi)
-        cx[k], cy[k] = rng.uniform(0.2, 0.8) * nx, rng.uniform(0.2, 0.8) * ny
+        cx[k], cy[k] = rng.uniform(lo, hi) * nx, rng.uniform(lo, hi) * ny
     s, c = np.sin(ang), np.cos(ang)
     out = np.zeros((nx, ny))
-    rc = _hostlib().slm_v_valleys(out.ctypes.data, nx, ny, n, cx.ctypes.data, cy.ctypes.data, s.ctypes.data,
-                                  c.ctypes.data, float(depth), float(half_width_px))
+    rc = _hostlib().slm_v_valley_segments(out.ctypes.data, nx, ny, n, cx.ctypes.data, cy.ctypes.data,
+                                          s.ctypes.data, c.ctypes.data, float(depth), float(half_width_px),
+                                          float(half_length_px))
     if rc != 0:
         raise ValueError("slm_v_valleys failed")
     return out
@@ -95,18 +105,25 @@ def cones_with_pits(nx: int, ny: int, pixel_size: float, relief: float, n_pits: 
 def make_dtm(n: int | tuple, pixel_size: float = 1.0, relief: float = 20.0, tilt: float = 0.05,
              beta: float = 2.0, seed: int = 0, nan_border: int = 2, n_valleys: int = 0,
              valley_depth: float | None = None, cone: bool = False, n_pits: int = 0,
-             fill: bool = True, valley_half_width_px: float | None = None) -> np.ndarray:
-    """Seeded synthetic DTM (float32 [nx, ny], metres, NaN border)."""
+             fill: bool = True, valley_half_width_px: float | None = None,
+             noise_scale: float | None = None, valley_half_length_px: float = 0.0,
+             valley_centre_range=(0.2, 0.8)) -> np.ndarray:
+    """Seeded synthetic DTM (float32 [nx, ny], metres, NaN border).  With
+    noise_scale the noise is stationary (spectral_noise) times noise_scale
+    metres instead of rescaled to relief."""
     nx, ny = (n, n) if np.isscalar(n) else n
     rng = np.random.default_rng(seed)
-    h = relief * spectral_noise(nx, ny, beta, rng)
+    if noise_scale is not None:
+        h = noise_scale * spectral_noise(nx, ny, beta, rng, stationary=True)
+    else:
+        h = relief * spectral_noise(nx, ny, beta, rng)
     xs = np.arange(nx)[:, None] * pixel_size
     ys = np.arange(ny)[None, :] * pixel_size
     h = h + tilt * (0.8 * xs + 0.6 * ys)
     if n_valleys:
         depth = valley_depth if valley_depth is not None else 0.5 * relief
         hw = valley_half_width_px if valley_half_width_px is not None else max(8.0, 0.03 * max(nx, ny))
-        h = h + v_valleys(nx, ny, pixel_size, n_valleys, depth, hw, rng)
+        h = h + v_valleys(nx, ny, pixel_size, n_valleys, depth, hw, rng, valley_half_length_px, valley_centre_range)
     if cone:
         h = 0.3 * h + cones_with_pits(nx, ny, pixel_size, relief, n_pits, rng)
     if nan_border:

@@ -63,7 +63,18 @@ def test_tile_kernels_match_numpy(gpu):
         assert np.allclose(xs[t, ..., 1], np.where(ok, uv[iic, jjc, 0], 0), rtol=2 ** -8, atol=0)
         assert np.allclose(xs[t, ..., 2], np.where(ok, uv[iic, jjc, 1], 0), rtol=2 ** -8, atol=0)
         assert np.array_equal(xs[t, ..., 3], ok.astype(np.float32))
-        assert not xs[t, ..., 4:].any()
+        # second-order features where all four neighbours are valid
+        inner = ok & (iic > 0) & (jjc > 0) & (iic < nx - 1) & (jjc < ny - 1)
+        m = lambda a, b: mask[np.clip(a, 0, nx - 1), np.clip(b, 0, ny - 1)] == 0
+        inner &= m(iic - 1, jjc) & m(iic + 1, jjc) & m(iic, jjc - 1) & m(iic, jjc + 1)
+        a_, b_ = np.clip(iic, 1, nx - 2), np.clip(jjc, 1, ny - 2)
+        hh = h.astype(np.float64)
+        div = 0.5 * (uv[a_ + 1, b_, 0] - uv[a_ - 1, b_, 0]) + 0.5 * (uv[a_, b_ + 1, 1] - uv[a_, b_ - 1, 1])
+        lap = hh[a_ - 1, b_] + hh[a_ + 1, b_] + hh[a_, b_ - 1] + hh[a_, b_ + 1] - 4 * hh[a_, b_]
+        slope = np.hypot(0.5 * (hh[a_ + 1, b_] - hh[a_ - 1, b_]), 0.5 * (hh[a_, b_ + 1] - hh[a_, b_ - 1]))
+        for ch, want in ((4, div), (5, 10 * lap), (6, 5 * slope)):
+            assert np.allclose(xs[t, ..., ch], np.where(inner, want, 0), rtol=2 ** -7, atol=1e-4), (t, ch)
+        assert not xs[t, ..., 7:].any()
         assert np.array_equal(vs[t], ok.astype(np.uint8))
         assert np.allclose(ts[t, ..., 0], np.where(ok, np.log1p(ct[iic, jjc]), 0), rtol=1e-6)
         assert np.allclose(ts[t, ..., 1], np.where(ok, np.log1p(c[iic, jjc]), 0), rtol=1e-6)

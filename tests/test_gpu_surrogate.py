@@ -30,14 +30,19 @@ def _close_bf16(got, want, frac=0.99):
     assert (got == want).mean() >= frac, (got == want).mean()
 
 
-@pytest.mark.parametrize("taps,relu,cin,cout,ld_extra", [(9, True, 16, 32, 0), (9, False, 64, 64, 32),
-                                                       (1, False, 32, 32, 0), (9, True, 96, 128, 0)])
-def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra):
+# (H, W) = (16, 24) runs the direct kernel; (32, 64), (16, 16) and (32, 32)
+# the LDS-tiled kernel (32- and 16-column block tiles)
+@pytest.mark.parametrize("taps,relu,cin,cout,ld_extra,H,W", [
+    (9, True, 16, 32, 0, 16, 24), (9, False, 64, 64, 32, 16, 24), (1, False, 32, 32, 0, 16, 24),
+    (9, True, 96, 128, 0, 16, 24), (9, True, 16, 32, 0, 32, 64), (9, False, 64, 64, 32, 32, 64),
+    (9, True, 96, 32, 0, 32, 64), (9, True, 384, 128, 0, 16, 16), (9, False, 32, 64, 16, 16, 16),
+    (9, True, 128, 256, 0, 32, 32)])
+def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra, H, W):
     import torch
     import torch.nn.functional as F
     from slm_amd import surrogate as S
     rs = np.random.default_rng(taps * 100 + cin)
-    N, H, W = 2, 16, 24
+    N = 2
     x = rs.standard_normal((N, H, W, cin + ld_extra)).astype(np.float32)
     w = (rs.standard_normal((cout, taps, cin)) / np.sqrt(cin * taps)).astype(np.float32)
     b = rs.standard_normal(cout).astype(np.float32) * 0.1
@@ -55,6 +60,13 @@ def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra):
         ref = F.relu(ref)
     ref = ref.to(torch.bfloat16).float().permute(0, 2, 3, 1).reshape(N * H * W, cout)
     _close_bf16(y.float().cpu().numpy(), ref.numpy())
+    # the direct kernel computes the same products (summed in another order)
+    from slm_amd import hip
+    y2 = torch.empty_like(y)
+    wg, bg = wb.to(gpu).reshape(-1), torch.from_numpy(b).to(gpu)       # held while the kernel runs
+    hip.check(hip.load().slm_nn_conv(S._p(xg), cin + ld_extra, S._p(wg), S._p(bg), S._p(y2), cout, N, H, W, cin,
+                                     cout, taps, 1 if relu else 0, S._s(), 1 | 16))
+    _close_bf16(y2.float().cpu().numpy(), y.float().cpu().numpy(), frac=0.98)
 
 
 @pytest.mark.parametrize("taps,cin,cout", [(9, 16, 32), (9, 64, 128), (1, 32, 32), (9, 96, 32)])
@@ -134,8 +146,8 @@ def test_pool_up_add_loss_adam(gpu):
     out = torch.zeros(3, dtype=torch.float32, device=gpu)
     tg, vg = torch.from_numpy(tgt).to(gpu), torch.from_numpy(valid).to(gpu)    # held while the kernel runs
     hip.check(hip.load().slm_nn_loss(S._p(head), 32, S._p(tg), S._p(vg), S._p(dh), 32, M,
-                                     S.ctypes.c_float(1.0 / valid.sum()), S.ctypes.c_float(2.0), S._p(out),
-                                     S._s(), 1))
+                                     S.ctypes.c_float(1.0 / valid.sum()), S.ctypes.c_float(0.5),
+                                     S.ctypes.c_float(2.0), S._p(out), S._s(), 1))
     h = head.float().cpu().numpy()
     v = valid.astype(bool)
     z = h[:, 2].astype(np.float64)
@@ -144,6 +156,8 @@ def test_pool_up_add_loss_adam(gpu):
     assert np.allclose(out.cpu().numpy(), want, rtol=1e-4)
     g2 = 2.0 * (1 / (1 + np.exp(-z)) - tgt[:, 2]) / v.sum()
     assert np.allclose(dh.float().cpu().numpy()[v, 2], g2[v], rtol=2 * ULP, atol=1e-6)
+    g0 = 0.5 * 2.0 * (h[:, 0] - tgt[:, 0]) / v.sum()
+    assert np.allclose(dh.float().cpu().numpy()[v, 0], g0[v], rtol=2 * ULP, atol=1e-6)
     assert np.all(dh.float().cpu().numpy()[~v] == 0)
     # Adam step against the closed form
     n = 1000
@@ -178,17 +192,17 @@ def test_network_gradients_match_oracle(gpu):
     xb = _bf(x)
     head = net.forward(xb.to(gpu).reshape(-1, S.IN_CH), N, H, W)
     dhead, sums, nv = net.loss(torch.from_numpy(tgt.reshape(-1, 3)).to(gpu),
-                               torch.from_numpy(valid.reshape(-1).astype(np.uint8)).to(gpu))
+                               torch.from_numpy(valid.reshape(-1).astype(np.uint8)).to(gpu), w_bce=3.0, w_len=0.25)
     net.backward(dhead)
     torch.cuda.synchronize()
     ref = RefUNet(params, S.LAYERS)
     rh = ref.forward(xb.float().permute(0, 3, 1, 2))
-    loss, parts = ref.loss(rh, tgt, valid)
+    loss, parts = ref.loss(rh, tgt, valid, w_bce=3.0, w_len=0.25)
     loss.backward()
     gh = head.float().cpu().numpy().reshape(N, H, W, S.OUT_CH)[..., :3]
     rhn = rh.detach().permute(0, 2, 3, 1).numpy()[..., :3]
     assert np.abs(gh - rhn).max() <= 0.05 * np.abs(rhn).max()
-    assert np.allclose(sums.cpu().numpy(), [float(t) for t in parts], rtol=0.02)
+    assert np.allclose(sums.cpu().numpy(), [float(t.detach()) for t in parts], rtol=0.02)
     g_ref = ref.grads()
     gflat = net.g.cpu().numpy()
     for name, cin, cout, taps, _ in S.LAYERS:
@@ -201,6 +215,21 @@ def test_network_gradients_match_oracle(gpu):
         assert cos > 0.995 and rel < 0.1, name
         gb, rb = gflat[ob:ob + nb], g_ref[name][1]
         assert np.linalg.norm(gb - rb) <= 0.1 * np.linalg.norm(rb) + 1e-6, name
+
+
+@pytest.mark.parametrize("N,H,W", [(2, 64, 64), (1, 256, 128), (4, 32, 32)])
+def test_inference_forward_fuses_upsample_bitwise(gpu, N, H, W):
+    """The inference network (upsample fused into the decoder convs) gives the
+    bits of the training network (explicit upsample into the concatenation)."""
+    import torch
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(N * H)
+    params = S.init_params(seed=9)
+    params = {k: (w, (rs.standard_normal(b.shape) * 0.05).astype(np.float32)) for k, (w, b) in params.items()}
+    x = _bf(rs.standard_normal((N * H * W, S.IN_CH))).to(gpu)
+    a = S.UNet(params, device=gpu).forward(x, N, H, W).clone()
+    b = S.UNet(params, device=gpu, inference=True).forward(x, N, H, W)
+    assert torch.equal(a, b)
 
 
 def test_training_beats_constant_predictor(gpu):
