@@ -174,6 +174,11 @@ def bench_surrogate(args):
                                 w_len=args.w_len, w_bce=args.w_bce, cosine=not args.no_cosine)
         cal = gated.calibrate(net, data)
         log("channel logit threshold %.2f: F1 %.4f (at 0: %.4f)" % cal)
+        train_scores = gated.tile_scores(net, data, n_tiles=1024)
+        hold = gated.training_set(n_dtms=2, n=2048, size=128, seed0=90000, theta=data.theta, device=dev)
+        hold_scores = gated.tile_scores(net, hold)
+        del hold
+        log("tiles: train %s, held-out DTMs %s" % (train_scores, hold_scores))
         params, theta = net.params_numpy(), data.theta
         train_rep = {"tiles": data.n_tiles, "tile": data.size, "epochs": args.epochs, "batch": args.batch,
                      "lr": args.lr, "lr_schedule": "constant" if args.no_cosine else "warmup 200 + cosine",
@@ -183,6 +188,7 @@ def bench_surrogate(args):
                      "tiles_per_s": hist["tiles_per_s"], "train_tflops": hist["tflops"],
                      "data_generate_s": data.timings["generate_s"], "data_trace_s": data.timings["trace_s"],
                      "channel_logit_threshold": cal[0], "train_channel_f1": cal[1],
+                     "train_tiles": train_scores, "heldout_tiles": hold_scores,
                      "train_channel_f1_uncalibrated": cal[2]}
         del data, net
         if args.save_weights:

@@ -186,9 +186,12 @@ class TrainingSet:
         self.timings = timings
 
 
-def training_set(n_dtms=32, n=2048, size=128, seed0=1000, theta_quantile=0.97, threads=8, device=None, log=None):
+def training_set(n_dtms=32, n=2048, size=128, seed0=1000, theta_quantile=0.97, threads=8, device=None, log=None,
+                 theta=None):
     """Trace n_dtms seeded n x n family DTMs (downstream) and cut each grid
-    into (n/size)^2 tiles of size^2, as the training data of the surrogate."""
+    into (n/size)^2 tiles of size^2, as the training data of the surrogate.
+    theta: the channel threshold (default: the theta_quantile of the pooled
+    density)."""
     import torch
     dev = _dev(device)
     t0 = time.perf_counter()
@@ -216,8 +219,10 @@ def training_set(n_dtms=32, n=2048, size=128, seed0=1000, theta_quantile=0.97, t
     torch.cuda.synchronize()
     t_trace = time.perf_counter() - t1
     # theta: quantile of the pooled density over valid pixels (a strided sample)
-    dens = torch.cat([slc[g.mask.reshape(-1) == 0].to(torch.float32) / g.density ** 2 for g, slc, _ in raw])
-    theta = float(torch.quantile(dens[:: max(1, dens.numel() // 4000000)], theta_quantile).item())
+    if theta is None:
+        dens = torch.cat([slc[g.mask.reshape(-1) == 0].to(torch.float32) / g.density ** 2 for g, slc, _ in raw])
+        theta = float(torch.quantile(dens[:: max(1, dens.numel() // 4000000)], theta_quantile).item())
+        del dens
     for k, (g, slc, slt) in enumerate(raw):
         sl = slice(k * per * size * size, (k + 1) * per * size * size)
         tile_targets(g, slc, slt, tiles, per, size, theta, target[sl], valid[sl])
@@ -343,6 +348,31 @@ def calibrate(net, data, n_tiles=1024, batch=64):
     return best, f1s[k], f0
 
 
+def tile_scores(net, data, n_tiles=None, batch=64):
+    """Channel F1 (logit > 0) and median relative log-density error of the
+    network on the tiles of a TrainingSet (e.g. held-out DTMs)."""
+    import torch
+    size = data.size
+    n = (data.n_tiles if n_tiles is None else min(n_tiles, data.n_tiles)) // batch * batch
+    xv = data.x.view(data.n_tiles, size * size * S.IN_CH)
+    tp = fp = fn = 0
+    errs = []
+    for b0 in range(0, n, batch):
+        net._alloc(batch, size, size)
+        net.A["x"].view(batch, -1).copy_(xv[b0:b0 + batch])
+        head = net.forward(net.A["x"], batch, size, size)
+        sl = slice(b0 * size * size, (b0 + batch) * size * size)
+        v = data.valid[sl].bool()
+        pc, tc = head[:, 2].float()[v] > 0, data.target[sl, 2][v] > 0.5
+        tp += int((pc & tc).sum().item())
+        fp += int((pc & ~tc).sum().item())
+        fn += int((~pc & tc).sum().item())
+        t1 = data.target[sl, 1][v]
+        errs.append(((head[:, 1].float()[v] - t1).abs() / t1.clamp_min(1e-6))[::16])
+    return {"channel_f1": 2 * tp / max(1, 2 * tp + fp + fn),
+            "median_rel_err_log_density": float(torch.cat(errs).median().item())}
+
+
 # ------------------------------------------------------------------ maps
 
 def tile_origins(nx, ny, size, halo):
@@ -353,19 +383,15 @@ def tile_origins(nx, ny, size, halo):
                      for j in range(int(math.ceil(ny / c)))], np.int32)
 
 
-def predict(net, grid, size=256, halo=16, batch=64, out=None):
-    """Surrogate fields [3, nxy] of a grid, by inference on overlapping tiles."""
+def predict_tiles(net, grid, origins, size, halo, batch, out):
+    """Inference on the tiles at origins (int32 [T, 2]), their cores stitched
+    into out [3, nxy].  Each tile's result depends on that tile alone."""
     import torch
     dev = grid.dev
-    origins = tile_origins(grid.nx, grid.ny, size, halo)
-    n_real = len(origins)
-    pad_n = (-n_real) % batch
+    pad_n = (-len(origins)) % batch
     if pad_n:       # whole batches: extra tiles lie outside the grid (no input, nothing stitched)
-        far = np.full((pad_n, 2), -4 * size, np.int32)
-        origins = np.concatenate([origins, far])
-    tiles = torch.from_numpy(origins).to(dev)
-    if out is None:
-        out = torch.empty((3, grid.nxy), dtype=torch.float32, device=dev)
+        origins = np.concatenate([origins, np.full((pad_n, 2), -4 * size, np.int32)])
+    tiles = torch.from_numpy(np.ascontiguousarray(origins, np.int32)).to(dev)
     net._alloc(batch, size, size)
     mean_ws = torch.empty(batch, dtype=torch.float32, device=dev)
     for b0 in range(0, len(origins), batch):
@@ -373,7 +399,17 @@ def predict(net, grid, size=256, halo=16, batch=64, out=None):
         tile_inputs(grid, tl, batch, size, mean_ws, net.A["x"])
         head = net.forward(net.A["x"], batch, size, size)
         tile_stitch(head, tl, batch, size, halo, grid.nx, grid.ny, out)
-    return out, origins[:n_real]
+    return out
+
+
+def predict(net, grid, size=256, halo=16, batch=64, out=None):
+    """Surrogate fields [3, nxy] of a grid, by inference on overlapping tiles."""
+    import torch
+    origins = tile_origins(grid.nx, grid.ny, size, halo)
+    if out is None:
+        out = torch.empty((3, grid.nxy), dtype=torch.float32, device=grid.dev)
+    predict_tiles(net, grid, origins, size, halo, batch, out)
+    return out, origins
 
 
 def plain_map(grid, theta, steps=None, workspace=None):
@@ -401,18 +437,22 @@ def _tile_check(pred, truth, valid, max_err, min_f1, min_channel_px):
 def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rng_seed=2, max_err=0.05,
               min_f1=0.95, min_channel_px=200, max_fail_frac=0.25, workspace=None):
     """Surrogate map with tracer checks.  A random check_frac of the tiles
-    (numpy default_rng(rng_seed)) is re-traced, each tile's core as a grid of
-    its own with a margin of max_length + 2 px (every streamline that can
-    reach the core starts inside it); a checked tile whose median relative
-    log-density error exceeds max_err, or whose channel F1 is below min_f1
-    (when it has at least min_channel_px traced channel pixels), is replaced
-    by its trace.  When more than max_fail_frac of the checked tiles fail,
-    the map falls back to plain_map.  Returns (fields [3, nxy], report)."""
+    (numpy default_rng(rng_seed)) is predicted and re-traced first, each
+    tile's core as a grid of its own with a margin of max_length + 2 px
+    (every streamline that can reach the core starts inside it).  A checked
+    tile whose median relative log-density error exceeds max_err, or whose
+    channel F1 is below min_f1 (when it has at least min_channel_px traced
+    channel pixels), fails.  When more than max_fail_frac of the checked
+    tiles fail, the map is plain_map and no other tile is predicted;
+    otherwise the other tiles are predicted and each failed checked tile
+    keeps its trace.  Returns (fields [3, nxy], report)."""
     import torch
-    out, origins = predict(net, grid, size, halo, batch)
+    origins = tile_origins(grid.nx, grid.ny, size, halo)
     n_tiles = len(origins)
     k = max(1, int(round(check_frac * n_tiles)))
     chosen = np.sort(np.random.default_rng(rng_seed).choice(n_tiles, k, replace=False))
+    out = torch.empty((3, grid.nxy), dtype=torch.float32, device=grid.dev)
+    predict_tiles(net, grid, origins[chosen], size, halo, batch, out)
     margin = int(math.ceil(grid.max_length_px)) + 2
     c = size - 2 * halo
     jobs, cores = [], []
@@ -443,6 +483,8 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
     if n_fail > max_fail_frac * k:
         out, _ = plain_map(grid, theta, workspace=workspace)
         report["fallback"] = True
+        return out, report
+    predict_tiles(net, grid, np.delete(origins, chosen, axis=0), size, halo, batch, out)
     return out, report
 
 
