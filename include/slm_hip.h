@@ -493,9 +493,15 @@ int slm_nn_conv_up(const void* x, int ldx, const void* x2, int ldx2, int c_up, c
 /* wt[ci][taps-1-t][co] = w[co][t][ci]: weights of the backward-data
  * convolution (slm_nn_conv on dY with Cin and Cout exchanged). */
 int slm_nn_wflip(const void* w, void* wt, int taps, int Cin, int Cout, void* stream, uint32_t flags);
-/* dw[co][t][ci] += sum_p dy[p][co] x[p + off(t)][ci] (fp32), db[co] += sum_p dy[p][co]. */
+/* dw[co][t][ci] += sum_p dy[p][co] x[p + off(t)][ci] (fp32), db[co] += sum_p dy[p][co]
+ * (Cout <= 256; dy and x 16-byte aligned, lddy a multiple of 8). */
 int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int lddy, float* dw, float* db,
                       int N, int H, int W, int Cin, int Cout, int taps, void* stream, uint32_t flags);
+/* Fused: dy[p][c] = 0 where y[p][c] <= 0 (skipped when y is NULL), then
+ * db[c] += sum_p dy[p][c] (skipped when db is NULL).  C a multiple of 32, at
+ * most 256; buffers 16-byte aligned, strides multiples of 8. */
+int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C, void* stream,
+                          uint32_t flags);
 /* dy[p][c] = 0 where y[p][c] <= 0 (ReLU backward on the stored output). */
 int slm_nn_relu_bwd(const void* y, int ldy, void* dy, int lddy, int M, int C, void* stream, uint32_t flags);
 /* 2x2 max pool, stride 2; H, W are the input's (even) sizes. */
@@ -514,8 +520,9 @@ int slm_nn_up_bwd(const void* dy, int lddy, void* dx, int lddx, int N, int H, in
 int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C, void* stream, uint32_t flags);
 /* Loss over valid pixels: squared error on head channels 0 (weight w_len)
  * and 1, binary cross-entropy with logits (weight w_bce) on channel 2;
- * dhead = gradient of the weighted total times inv_n (channels >= 3 zero);
- * out[0..2] += the unweighted summed terms. */
+ * dhead = gradient of the weighted total times inv_n (channels 3 .. min(lddh,
+ * 32)-1 zero; dhead 16-byte aligned, lddh a multiple of 8); out[0..2] += the
+ * unweighted summed terms. */
 int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead,
                 int lddh, int M, float inv_n, float w_len, float w_bce, float* out, void* stream,
                 uint32_t flags);

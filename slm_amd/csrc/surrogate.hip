@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace {
@@ -252,18 +254,152 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, int l
     }
 }
 
-// db[co] += sum over pixels of dY[p][co]
+// dW[co][k] += sum over pixels of dY[p][co] * im2col(X)[p][k], k = tap*Cin + ci.
+// Block tile CT channels x KT k (4 waves of 32x32), pixels [blockIdx.z*ppb,
+// +ppb) in chunks of 128.  Staging: a thread loads an 8-pixel x 8-channel
+// block (eight 16-byte loads), transposes it in registers and stores eight
+// 16-byte rows [channel][8 pixels] to LDS, so the MFMA operands (8 pixels
+// of one channel per lane) are single 16-byte LDS reads.
+template <int TAPS, int CT, int KT>
+__global__ __launch_bounds__(256) void k_wgrad_t(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ dy,
+                                                 int lddy, float* __restrict__ dw, int M, int H, int W, int Cin,
+                                                 int Cout, int ppb) {
+    constexpr int PC = 128, PP = PC + 8;
+    __shared__ __attribute__((aligned(16))) bf16 dyT[CT * PP];
+    __shared__ __attribute__((aligned(16))) bf16 xT[KT * PP];
+    const int K = TAPS * Cin;
+    const int c0 = blockIdx.x * CT, k0 = blockIdx.y * KT;
+    const int p_begin = blockIdx.z * ppb, p_end = min(M, p_begin + ppb);
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    constexpr int WCO = CT / 32;                 // waves along channels
+    const int wc = (wid % WCO) * 32, wk = (wid / WCO) * 32;
+    const bool wave_on = (c0 + wc < Cout) && (k0 + wk < K);
+    constexpr int NB_DY = (PC / 8) * (CT / 8), NB = NB_DY + (PC / 8) * (KT / 8);
+    f32x16 acc = {};
+    const bf16x8 zero = {};
+    for (int pb = p_begin; pb < p_end; pb += PC) {
+        for (int blk = t; blk < NB; blk += 256) {
+            const bool is_dy = blk < NB_DY;
+            const int b2 = is_dy ? blk : blk - NB_DY;
+            const int pg = b2 % (PC / 8), cg = b2 / (PC / 8);
+            bf16x8 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                const int p = pb + pg * 8 + i;
+                v[i] = zero;
+                if (p < p_end) {
+                    if (is_dy) {
+                        if (c0 + cg * 8 < Cout) v[i] = *(const bf16x8*)(dy + (size_t)p * lddy + c0 + cg * 8);
+                    } else {
+                        const int k = k0 + cg * 8;
+                        if (k < K) {
+                            const int tap = k / Cin, ci = k % Cin;
+                            const int pw = p % W, ph = (p / W) % H, pn = p / (W * H);
+                            const int hh = ph + (TAPS == 9 ? tap / 3 - 1 : 0), ww = pw + (TAPS == 9 ? tap % 3 - 1 : 0);
+                            if (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                                v[i] = *(const bf16x8*)(x + (size_t)((pn * H + hh) * W + ww) * ldx + ci);
+                        }
+                    }
+                }
+            }
+            bf16* dst = is_dy ? dyT : xT;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                bf16x8 col;
+#pragma unroll
+                for (int i = 0; i < 8; i++) col[i] = v[i][j];
+                *(bf16x8*)&dst[(cg * 8 + j) * PP + pg * 8] = col;
+            }
+        }
+        __syncthreads();
+        if (wave_on) {
+            const int r = lane & 31, kg = (lane >> 5) * 8;
+#pragma unroll
+            for (int kk = 0; kk < PC; kk += 16) {
+                const bf16x8 a = *(const bf16x8*)&dyT[(wc + r) * PP + kk + kg];
+                const bf16x8 b = *(const bf16x8*)&xT[(wk + r) * PP + kk + kg];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (!wave_on) return;
+    const int kcol = k0 + wk + (lane & 31);
+    if (kcol >= K) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int co = c0 + wc + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
+        if (co < Cout) atomicAdd(&dw[(size_t)co * K + kcol], acc[v]);
+    }
+}
+
+// db[co] += sum over pixels of dY[p][co].  A thread sums 8 channels (one
+// 16-byte load per pixel) over a strided set of pixels; the block's 256/(C/8)
+// pixel lanes are reduced in LDS and each channel gets one atomic per block
+// (the launch keeps the block count small: the atomics share Cout addresses).
 __global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, int lddy, float* __restrict__ db,
-                                               int M, int Cout, int ppb) {
-    const int co = blockIdx.x * 256 + threadIdx.x;
-    if (co >= Cout) return;
-    const int p0 = blockIdx.y * ppb, p1 = min(M, p0 + ppb);
-    float s = 0.0f;
-    for (int p = p0; p < p1; p++) s += b2f(dy[(size_t)p * lddy + co]);
-    atomicAdd(&db[co], s);
+                                               int M, int Cout) {
+    __shared__ float red[256 * 8];
+    const int C8 = Cout / 8;                    // 4 .. 32: divides 256
+    const int t = threadIdx.x, tc = t % C8, tp = t / C8, P = 256 / C8;
+    float acc[8] = {};
+#pragma unroll 4
+    for (size_t p = (size_t)blockIdx.x * P + tp; p < (size_t)M; p += (size_t)gridDim.x * P) {
+        const bf16x8 v = *(const bf16x8*)(dy + p * lddy + tc * 8);
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] += b2f(v[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) red[t * 8 + j] = acc[j];
+    __syncthreads();
+    if (tp == 0) {
+        float sum[8] = {};
+        for (int q = 0; q < P; q++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) sum[j] += red[(q * C8 + tc) * 8 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) atomicAdd(&db[tc * 8 + j], sum[j]);
+    }
 }
 
 // ------------------------------------------------------------------ elementwise
+
+// Fused ReLU backward and bias gradient: dY *= (Y > 0) when y is given (the
+// stored post-ReLU output), then db[c] += sum over pixels of dY[p][c].  A
+// thread handles 8 channels (16-byte accesses) over strided pixels; lanes are
+// reduced in LDS, one atomic per channel per block (few blocks).
+__global__ __launch_bounds__(256) void k_relu_bias(const bf16* __restrict__ y, int ldy, bf16* __restrict__ dy,
+                                                   int lddy, float* __restrict__ db, int M, int C) {
+    __shared__ float red[256 * 8];
+    const int C8 = C / 8;                       // 4 .. 32: divides 256
+    const int t = threadIdx.x, tc = t % C8, tp = t / C8, P = 256 / C8;
+    float acc[8] = {};
+#pragma unroll 2
+    for (size_t p = (size_t)blockIdx.x * P + tp; p < (size_t)M; p += (size_t)gridDim.x * P) {
+        bf16x8 g = *(const bf16x8*)(dy + p * lddy + tc * 8);
+        if (y) {
+            const bf16x8 yv = *(const bf16x8*)(y + p * ldy + tc * 8);
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (!(b2f(yv[j]) > 0.0f)) g[j] = f2b(0.0f);
+            *(bf16x8*)(dy + p * lddy + tc * 8) = g;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[j] += b2f(g[j]);
+    }
+    if (!db) return;                            // uniform
+#pragma unroll
+    for (int j = 0; j < 8; j++) red[t * 8 + j] = acc[j];
+    __syncthreads();
+    if (tp == 0) {
+        float sum[8] = {};
+        for (int q = 0; q < P; q++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) sum[j] += red[(q * C8 + tc) * 8 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++) atomicAdd(&db[tc * 8 + j], sum[j]);
+    }
+}
 
 // dY *= (Y > 0): the ReLU backward on the stored (post-ReLU) output
 __global__ void k_relu_bwd(const bf16* __restrict__ y, int ldy, bf16* __restrict__ dy, int lddy, int M, int C) {
@@ -353,6 +489,7 @@ __global__ void k_add(bf16* __restrict__ a, int lda, const bf16* __restrict__ b,
 __global__ __launch_bounds__(256) void k_loss(const bf16* __restrict__ head, int ldh, const float* __restrict__ target,
                                               const uint8_t* __restrict__ valid, bf16* __restrict__ dhead, int lddh,
                                               int M, float inv_n, float w_len, float w_bce, float* __restrict__ out) {
+    __shared__ float red[4][3];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     float l0 = 0.f, l1 = 0.f, l2 = 0.f;
     if (p < M) {
@@ -370,20 +507,32 @@ __global__ __launch_bounds__(256) void k_loss(const bf16* __restrict__ head, int
             g1 = 2.f * (b - tb) * inv_n;
             g2 = w_bce * (1.f / (1.f + expf(-z)) - tc) * inv_n;
         }
-        dhead[(size_t)p * lddh + 0] = f2b(g0);
-        dhead[(size_t)p * lddh + 1] = f2b(g1);
-        dhead[(size_t)p * lddh + 2] = f2b(g2);
-        for (int c = 3; c < lddh && c < 32; c++) dhead[(size_t)p * lddh + c] = f2b(0.f);
+        // the gradient row: channels 0..2, zeros up to min(lddh, 32), 16-byte stores
+        bf16x8 v = {};
+        v[0] = f2b(g0);
+        v[1] = f2b(g1);
+        v[2] = f2b(g2);
+        bf16* row = dhead + (size_t)p * lddh;
+        *(bf16x8*)row = v;
+        const bf16x8 zero = {};
+        const int c_end = min(lddh, 32);
+        for (int c = 8; c < c_end; c += 8) *(bf16x8*)(row + c) = zero;
     }
     for (int off = 32; off > 0; off >>= 1) {
         l0 += __shfl_down(l0, off);
         l1 += __shfl_down(l1, off);
         l2 += __shfl_down(l2, off);
     }
+    const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&out[0], l0);
-        atomicAdd(&out[1], l1);
-        atomicAdd(&out[2], l2);
+        red[wid][0] = l0;
+        red[wid][1] = l1;
+        red[wid][2] = l2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        atomicAdd(&out[k], red[0][k] + red[1][k] + red[2][k] + red[3][k]);
     }
 }
 
@@ -657,25 +806,46 @@ extern "C" int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int ldd
     int rc = check_conv(M, Cin, Cout, ldx, lddy, taps);
     if (rc) return rc;
     if (!x || !dy || !dw) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (Cout > 256) return slm_fail(SLM_E_INVALID_ARG, "Cout %d above 256", Cout);
+    if (lddy % 8 || ((uintptr_t)dy) % 16 || ((uintptr_t)x) % 16)
+        return slm_fail(SLM_E_INVALID_ARG, "dy/x must be 16-byte aligned with lddy a multiple of 8");
     hipStream_t s = (hipStream_t)stream;
     const int K = taps * Cin;
-    const int tiles = ((Cout + 63) / 64) * ((K + 63) / 64);
-    // enough pixel splits to fill the chip (~1024 blocks), at least 64 pixels each
-    int splits = (1024 + tiles - 1) / tiles;
-    int ppb = (M + splits - 1) / splits;
-    ppb = ((ppb + 63) / 64) * 64;
-    splits = (M + ppb - 1) / ppb;
-    const dim3 grid((Cout + 63) / 64, (K + 63) / 64, splits);
-    if (taps == 9)
-        hipLaunchKernelGGL(k_wgrad<9>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H, W,
-                           Cin, Cout, ppb);
-    else
-        hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H, W,
-                           Cin, Cout, ppb);
+    if (flags & SLM_FLAG_NN_DIRECT) {
+        const int tiles = ((Cout + 63) / 64) * ((K + 63) / 64);
+        // enough pixel splits to fill the chip (~1024 blocks), at least 64 pixels each
+        int splits = (1024 + tiles - 1) / tiles;
+        int ppb = (M + splits - 1) / splits;
+        ppb = ((ppb + 63) / 64) * 64;
+        splits = (M + ppb - 1) / ppb;
+        const dim3 grid((Cout + 63) / 64, (K + 63) / 64, splits);
+        if (taps == 9)
+            hipLaunchKernelGGL(k_wgrad<9>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H,
+                               W, Cin, Cout, ppb);
+        else
+            hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H,
+                               W, Cin, Cout, ppb);
+    } else {
+        // 64 x 64 tiles, or 32 x 128 when Cout is 32; pixel splits of whole 128-pixel
+        // chunks, enough to fill the chip (~1024 blocks)
+        const bool narrow = Cout % 64 != 0;
+        const int ct = narrow ? 32 : 64, kt = narrow ? 128 : 64;
+        const int tiles = (Cout / ct) * ((K + kt - 1) / kt);
+        int splits = (1024 + tiles - 1) / tiles;
+        int ppb = (M + splits - 1) / splits;
+        ppb = ((ppb + 127) / 128) * 128;
+        splits = (M + ppb - 1) / ppb;
+        const dim3 grid(Cout / ct, (K + kt - 1) / kt, splits);
+#define WG_L(T, C, KK) hipLaunchKernelGGL((k_wgrad_t<T, C, KK>), grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H, W, Cin, Cout, ppb)
+        if (taps == 9) { if (narrow) WG_L(9, 32, 128); else WG_L(9, 64, 64); }
+        else { if (narrow) WG_L(1, 32, 128); else WG_L(1, 64, 64); }
+#undef WG_L
+    }
     if (db) {
-        const int bsplits = 256, bppb = (M + bsplits - 1) / bsplits;
-        hipLaunchKernelGGL(k_bgrad, dim3((Cout + 255) / 256, bsplits), dim3(256), 0, s, (const bf16*)dy, lddy, db, M,
-                           Cout, bppb);
+        // few blocks: every block ends in Cout atomics on the same addresses
+        const int P = 256 / (Cout / 8);
+        const int blocks = std::min(256, (M + P - 1) / P);
+        hipLaunchKernelGGL(k_bgrad, dim3(blocks), dim3(256), 0, s, (const bf16*)dy, lddy, db, M, Cout);
     }
     return slm_finish(stream, flags);
 }
@@ -729,8 +899,8 @@ extern "C" int slm_nn_add(void* a, int lda, const void* b, int ldb, int M, int C
 
 extern "C" int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* valid, void* dhead, int lddh,
                            int M, float inv_n, float w_len, float w_bce, float* out, void* stream, uint32_t flags) {
-    if (!head || !target || !valid || !dhead || !out || ldh < 3 || lddh < 3)
-        return slm_fail(SLM_E_INVALID_ARG, "bad loss arguments");
+    if (!head || !target || !valid || !dhead || !out || ldh < 3 || lddh < 8 || lddh % 8 || ((uintptr_t)dhead) % 16)
+        return slm_fail(SLM_E_INVALID_ARG, "bad loss arguments (dhead: 16-byte aligned, lddh a multiple of 8)");
     hipLaunchKernelGGL(k_loss, g1((size_t)M), dim3(256), 0, (hipStream_t)stream, (const bf16*)head, ldh, target, valid,
                        (bf16*)dhead, lddh, M, inv_n, w_len, w_bce, out);
     return slm_finish(stream, flags);
@@ -790,5 +960,18 @@ extern "C" int slm_nn_fields_log(const uint32_t* slc, const uint32_t* slt, const
     if (!nxy) return slm_finish(stream, flags);
     hipLaunchKernelGGL(k_fields_log, g1(nxy), dim3(256), 0, (hipStream_t)stream, slc, slt, mask, nxy,
                        1.0f / (float)(density * density), threshold, out);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C, void* stream,
+                                     uint32_t flags) {
+    if (!dy || M < 0 || C <= 0 || C % 32 || C > 256 || lddy % 8 || ((uintptr_t)dy) % 16 ||
+        (y && (ldy % 8 || ((uintptr_t)y) % 16)))
+        return slm_fail(SLM_E_INVALID_ARG, "bad relu_bias_grad arguments");
+    if (!M || (!y && !db)) return slm_finish(stream, flags);
+    const int P = 256 / (C / 8);
+    const int blocks = std::min(128, (M + P - 1) / P);
+    hipLaunchKernelGGL(k_relu_bias, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const bf16*)y, ldy, (bf16*)dy,
+                       lddy, db, M, C);
     return slm_finish(stream, flags);
 }

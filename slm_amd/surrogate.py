@@ -68,6 +68,12 @@ def relu_bwd(y, ldy, dy, lddy, M, C):
     hip.check(hip.load().slm_nn_relu_bwd(_p(y), ldy, _p(dy), lddy, M, C, _s(), 0))
 
 
+def relu_bias_grad(y, ldy, dy, lddy, db, M, C):
+    """dy *= (y > 0) when y is given, then db += column sums of dy."""
+    hip.check(hip.load().slm_nn_relu_bias_grad(_p(y) if y is not None else None, ldy, _p(dy), lddy,
+                                               _p(db) if db is not None else None, M, C, _s(), 0))
+
+
 def pool(x, ldx, y, ldy, N, H, W, C):
     hip.check(hip.load().slm_nn_pool(_p(x), ldx, _p(y), ldy, N, H, W, C, _s(), 0))
 
@@ -273,9 +279,8 @@ class UNet:
         mask applied here) -> weight/bias grads (+=) and dx (written)."""
         cin, cout, taps, relu = self._io(name)
         M = n * h * w
-        if relu:
-            relu_bwd(yout, ldy, dy, lddy, M, cout)
-        wgrad(xin, ldx, dy, lddy, self.gw(name), self.gb(name), n, h, w, cin, cout, taps)
+        relu_bias_grad(yout if relu else None, ldy, dy, lddy, self.gb(name), M, cout)
+        wgrad(xin, ldx, dy, lddy, self.gw(name), None, n, h, w, cin, cout, taps)
         if need_dx:
             wflip(self.w(name), self.wt[name], taps, cin, cout)
             conv(dy, lddy, self.wt[name], None, dx, lddx, n, h, w, cout, cin, taps, False)

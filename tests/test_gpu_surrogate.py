@@ -69,7 +69,8 @@ def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra, H, W):
     _close_bf16(y2.float().cpu().numpy(), y.float().cpu().numpy(), frac=0.98)
 
 
-@pytest.mark.parametrize("taps,cin,cout", [(9, 16, 32), (9, 64, 128), (1, 32, 32), (9, 96, 32)])
+@pytest.mark.parametrize("taps,cin,cout", [(9, 16, 32), (9, 64, 128), (1, 32, 32), (9, 96, 32), (9, 384, 128),
+                                           (9, 128, 256), (1, 32, 64)])
 def test_conv_backward(gpu, taps, cin, cout):
     """Backward-weight (fp32) and backward-data (flipped weights through the
     forward kernel) against torch autograd on the CPU."""
@@ -101,10 +102,42 @@ def test_conv_backward(gpu, taps, cin, cout):
     torch.cuda.synchronize()
     got = dw.cpu().numpy().reshape(cout, taps, cin)
     assert np.abs(got - dw_ref).max() <= 1e-4 * np.abs(dw_ref).max() + 1e-5
+    # the direct weight-gradient kernel (test hook) sums the same products
+    from slm_amd import hip
+    dw2 = torch.zeros_like(dw)
+    hip.check(hip.load().slm_nn_conv_wgrad(S._p(xg), cin, S._p(dyg), cout, S._p(dw2), None, N, H, W, cin, cout, taps,
+                                           S._s(), 1 | 16))
+    assert np.abs(dw2.cpu().numpy().reshape(cout, taps, cin) - got).max() <= 1e-4 * np.abs(dw_ref).max() + 1e-5
     db_ref = dy.float().sum(dim=(0, 1, 2)).numpy()
     assert np.allclose(db.cpu().numpy(), db_ref, rtol=1e-5, atol=1e-4)
     if cin % 32 == 0:
         _close_bf16(dx.float().cpu().numpy(), dx_ref, frac=0.97)
+
+
+@pytest.mark.parametrize("C,ld", [(32, 32), (64, 96), (256, 256)])
+def test_relu_bias_grad(gpu, C, ld):
+    import torch
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(C)
+    M = 3 * 1024 + 32
+    y = _bf(rs.standard_normal((M, ld))).to(gpu)
+    dy = _bf(rs.standard_normal((M, ld))).to(gpu)
+    dy0 = dy.float().cpu().numpy()
+    db = torch.zeros(C, dtype=torch.float32, device=gpu)
+    S.relu_bias_grad(y, ld, dy, ld, db, M, C)
+    torch.cuda.synchronize()
+    yn = y.float().cpu().numpy()
+    want = np.where(yn[:, :C] > 0, dy0[:, :C], 0.0)
+    got = dy.float().cpu().numpy()
+    assert np.array_equal(got[:, :C], want)
+    assert np.array_equal(got[:, C:], dy0[:, C:])          # channels beyond C untouched
+    assert np.allclose(db.cpu().numpy(), want.sum(0), rtol=1e-5, atol=1e-3)
+    # bias only (no ReLU): dy unchanged
+    db.zero_()
+    S.relu_bias_grad(None, 0, dy, ld, db, M, C)
+    torch.cuda.synchronize()
+    assert np.array_equal(dy.float().cpu().numpy(), got)
+    assert np.allclose(db.cpu().numpy(), got[:, :C].sum(0), rtol=1e-5, atol=1e-3)
 
 
 def test_pool_up_add_loss_adam(gpu):
