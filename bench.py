@@ -428,6 +428,19 @@ def main():
         from slm_amd.pipeline import map_tile
         map_tile(job)  # warm-up
         res, tim = map_tile(job)
+        # the loop and blockage conditioning of the same DTM (the reference's
+        # do_condition), timed alone: the uv stage with it instead of the plain gradient
+        roi_d = torch.from_numpy(np.ascontiguousarray(job.roi, dtype=np.float32)).to(dev)
+        cws = torch.empty(int(hip.load().slm_condition_workspace_size(*job.roi.shape)), dtype=torch.uint8,
+                          device=dev)
+        hip.condition_uv(roi_d, int(job.info.pad_width), float(job.pixel_size), workspace=cws, sync=True)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(5):
+            hip.condition_uv(roi_d, int(job.info.pad_width), float(job.pixel_size), workspace=cws)
+        c1.record()
+        torch.cuda.synchronize()
+        res["condition_uv_ms"] = round(c0.elapsed_time(c1) / 5, 3)
         out["pipeline"] = {"stages": {k: round(v["gpu_ms"], 3) for k, v in tim.items()},
                            "total_ms": round(sum(v["gpu_ms"] for v in tim.values()), 3), "results": res}
     if not args.no_cpu:

@@ -299,6 +299,24 @@ int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8
 int slm_gradient_uv(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size,
                     int do_normalize_speed, float* uv, uint8_t* mask, void* stream, uint32_t flags);
 
+/* DTM conditioning: Preprocess.conditioned_gradient_vector_field
+ * (python/streamlines/preprocess.py:531-600, with find_blockages :647-698,
+ * has_one_diagonal_outflow :390-432, upstream_of_diagonal_outflow :434-473,
+ * find_and_fix_loops :303-350, break_out_of_loop :263-301, check_has_loop
+ * :229-261, fix_blockages :352-388), then the padding and NaN mask of
+ * mask_nan_uv (:602-621): the uv and mask of slm_gradient_uv after the
+ * reference's loop and blockage fixes.  Thresholds are the preprocess JSON's
+ * vecsum_threshold, divergence_threshold, curl_threshold.  maps (optional,
+ * uint8 [3][nx*ny]) receives the blockage codes, the blocked-neighbour codes
+ * and the loop flag of the 2x2 cell at each pixel.  The loop cells of the
+ * last row and column, where the reference reads past its arrays, are never
+ * loops here.  workspace: slm_condition_workspace_size(nx, ny) bytes. */
+size_t slm_condition_workspace_size(uint32_t nx, uint32_t ny);
+int slm_condition_uv(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size,
+                     int do_normalize_speed, double vecsum_threshold, double divergence_threshold,
+                     double curl_threshold, float* uv, uint8_t* mask, uint8_t* maps, void* workspace,
+                     size_t workspace_bytes, void* stream, uint32_t flags);
+
 /*
  * Finishing of the raw fields (fields.py:296-309): from the per-direction raw
  * sums of slm_trace_fields, slc_raw / slt_raw uint32 [2][nxy] (direction k at

@@ -70,6 +70,9 @@ _PROTOS = {
     "slm_trace_fields_batch": (ctypes.c_int, [ctypes.POINTER(SlmFieldsJob), _U32, ctypes.POINTER(SlmBatchOpts), _P,
                                               _P, ctypes.c_size_t, _P, _U32]),
     "slm_gradient_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, _P, _P, _P, _U32]),
+    "slm_condition_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_condition_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),
     "slm_fields_finish_scaled": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _P, _P, _P,
                                                 _U32]),
@@ -335,6 +338,30 @@ def gradient_uv(roi, pad: int, pixel_size: float, do_normalize_speed=True, strea
     check(load().slm_gradient_uv(_ptr(roi), nx, ny, pad, float(np.float32(pixel_size)),
                                  1 if do_normalize_speed else 0, _ptr(uv), _ptr(mask), _stream(stream),
                                  SLM_FLAG_SYNC if sync else 0))
+    return uv, mask.to(torch.bool)
+
+
+def condition_uv(roi, pad: int, pixel_size: float, do_normalize_speed=True, vecsum_threshold=0.95,
+                 divergence_threshold=-0.5, curl_threshold=0.0, maps=False, workspace=None, stream=None, sync=False):
+    """slm_condition_uv: roi float32 [nx, ny] device tensor -> (uv float32
+    [nx+2pad, ny+2pad, 2], mask bool) after the loop and blockage fixes;
+    with maps=True also (blockages, blocked_neighbors, loop cells) uint8 [nx, ny]."""
+    import torch
+    _require_cuda(roi, workspace)
+    nx, ny = roi.shape
+    nxp, nyp = nx + 2 * pad, ny + 2 * pad
+    uv = torch.empty((nxp, nyp, 2), dtype=torch.float32, device=roi.device)
+    mask = torch.empty((nxp, nyp), dtype=torch.uint8, device=roi.device)
+    m = torch.empty((3, nx, ny), dtype=torch.uint8, device=roi.device) if maps else None
+    need = int(load().slm_condition_workspace_size(nx, ny))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=roi.device)
+    check(load().slm_condition_uv(_ptr(roi), nx, ny, pad, float(np.float32(pixel_size)), 1 if do_normalize_speed else 0,
+                                  float(vecsum_threshold), float(divergence_threshold), float(curl_threshold),
+                                  _ptr(uv), _ptr(mask), _ptr(m), _ptr(workspace), workspace.numel(), _stream(stream),
+                                  SLM_FLAG_SYNC if sync else 0))
+    if maps:
+        return uv, mask.to(torch.bool), m
     return uv, mask.to(torch.bool)
 
 

@@ -41,8 +41,11 @@ class _Stages:
         return out
 
 
-def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None):
-    """Run the pipeline on a workloads.Job; returns (results dict, stage timings)."""
+def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None, condition=False):
+    """Run the pipeline on a workloads.Job; returns (results dict, stage timings).
+    condition: the uv stage runs the loop and blockage conditioning
+    (slm_condition_uv, the reference's do_condition) instead of the plain
+    gradient field the job's grids were built with."""
     import torch
     dev = device or torch.device("cuda", torch.cuda.current_device())
     info = job.info
@@ -59,7 +62,10 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     torch.cuda.synchronize()
 
     st.mark("uv")
-    uv_full, mask_full = hip.gradient_uv(roi, pad, float(job.pixel_size))
+    if condition:
+        uv_full, mask_full = hip.condition_uv(roi, pad, float(job.pixel_size))
+    else:
+        uv_full, mask_full = hip.gradient_uv(roi, pad, float(job.pixel_size))
     ox, oy = job.origin
     # the tracer's grids: the bbox crop of the full padded grids (Data, useful.py:37-115)
     uv = uv_full[ox:ox + nxp, oy:oy + nyp].contiguous()

@@ -31,3 +31,12 @@ def test_map_tile_small_256(gpu):
     steps = torch.zeros(1, dtype=torch.int64, device=gpu)
     hip.trace_fields(job.params(1), seeds, mask, uv, slc, slt, steps, sync=True)
     assert int(steps.item()) == res["steps"][0]
+
+
+def test_map_tile_with_conditioning(gpu):
+    """The pipeline runs on the conditioned uv (the reference's default)."""
+    from slm_amd import pipeline, workloads
+    job = workloads.small_256()
+    res, tim = pipeline.map_tile(job, condition=True)
+    assert res["chain_ok"] and res["n_segments"] > 0 and res["hsl_seeds"] > 0
+    assert "uv" in tim
