@@ -573,6 +573,30 @@ int slm_nn_tile_stitch(const void* head, int ldh, const int* tiles, int n_tiles,
 int slm_nn_fields_log(const uint32_t* slc, const uint32_t* slt, const uint8_t* mask, size_t nxy, int density,
                       float threshold, float* out, void* stream, uint32_t flags);
 
+/* ------------------------------------------------------------------------
+ * Mid-slopes and ridges: Mapping.map_midslopes / map_ridges
+ * (python/streamlines/mapping.py:779-817).
+ * ---------------------------------------------------------------------- */
+size_t slm_slopes_workspace_size(uint32_t nx, uint32_t ny);
+/* scipy.ndimage.gaussian_filter of a float32 [nx][ny] grid (reflect borders),
+ * bit for bit: axis 0 then axis 1, float32 between the passes, each output a
+ * float64 sum centre*w[r] + sum_{j=r..1} (left_j + right_j)*w[r-j];
+ * weights: the 2r+1 float64 weights of scipy's _gaussian_kernel1d (device). */
+int slm_gaussian_filter(const float* in, float* out, float* tmp, uint32_t nx, uint32_t ny, const double* weights,
+                        int radius, void* stream, uint32_t flags);
+/* sla float2 [nx*ny] (downstream, upstream): angle = arctan2(dsla, usla) -
+ * pi/4 (float32, arctan2 correctly rounded); mid-slopes where not masked and
+ * |gauss(angle, w_mid)| <= mid_threshold; ridges where not masked and
+ * gauss(angle, w_ridge) <= ridge_level (= float32(-(pi/4) * ridge_threshold)),
+ * dilated twice by the 3x3 square and hole-filled (scipy binary_fill_holes).
+ * Both are thinned (Zhang-Suen, the stand-in for skimage skeletonize) and
+ * their skeletons OR midslope_flag / ridge_flag into mapping.  Synchronises
+ * the stream once per thinning iteration and every 8 hole-filling steps. */
+int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, uint32_t nx, uint32_t ny, const double* w_mid,
+                             int r_mid, float mid_threshold, const double* w_ridge, int r_ridge, float ridge_level,
+                             uint32_t midslope_flag, uint32_t ridge_flag, uint32_t* mapping, void* workspace,
+                             size_t workspace_bytes, void* stream, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -631,6 +631,103 @@ __global__ void k_fix(SlmParams P, const float2* __restrict__ seeds, const uint8
 
 // ------------------------------------------------------------------ host side
 
+// ---------------------------------------------------------- mid-slopes and ridges
+// Mapping.map_midslopes / map_ridges (mapping.py:779-817): the sla angle
+// arctan2(dsla, usla) - pi/4 (float32), smoothed by scipy's gaussian_filter
+// (separable, axis 0 then axis 1, float32 between the passes; each output a
+// float64 sum: centre * w0, then (left + right) * w_j from the outermost pair
+// in; 'reflect' borders), thresholded, for ridges dilated twice by the 3x3
+// square and hole-filled, then thinned; the skeletons flag the mapping grid.
+
+__global__ void k_sla_angle(const float2* __restrict__ sla, uint32_t nxy, float* __restrict__ ang) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxy) return;
+    // np.arctan2 on float32, correctly rounded here (through float64), minus float32(pi/4)
+    const float a = (float)atan2((double)sla[i].x, (double)sla[i].y);
+    ang[i] = a - 0.785398185253143310546875f;
+}
+
+// scipy.ndimage 'reflect' extension of index k into [0, n)
+__device__ __forceinline__ int reflect_idx(int k, int n) {
+    const int p = 2 * n;
+    k %= p;
+    if (k < 0) k += p;
+    return k < n ? k : p - 1 - k;
+}
+
+template <int AXIS>
+__global__ void k_gauss(const float* __restrict__ in, float* __restrict__ out, uint32_t nx, uint32_t ny,
+                        const double* __restrict__ w, int r) {
+#pragma clang fp contract(off)
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    const int n = AXIS == 0 ? (int)nx : (int)ny, pos = AXIS == 0 ? x : y;
+    auto at = [&](int k) -> double {
+        const int kk = reflect_idx(k, n);
+        return (double)in[AXIS == 0 ? (size_t)kk * ny + y : (size_t)x * ny + kk];
+    };
+    double acc = at(pos) * w[r];
+    for (int j = r; j >= 1; j--) acc += (at(pos - j) + at(pos + j)) * w[r - j];
+    out[i] = (float)acc;
+}
+
+__global__ void k_slope_marks(const float* __restrict__ gm, const float* __restrict__ gr,
+                              const uint8_t* __restrict__ mask, uint32_t nxy, float mid_thr, float ridge_level,
+                              uint8_t* __restrict__ mid, uint8_t* __restrict__ ridge) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxy) return;
+    const bool ok = !mask[i];
+    mid[i] = (ok && fabsf(gm[i]) <= mid_thr) ? 1 : 0;
+    ridge[i] = (ok && gr[i] <= ridge_level) ? 1 : 0;
+}
+
+// binary_dilation by the 3x3 square, border value 0
+__global__ void k_dilate8(const uint8_t* __restrict__ a, uint8_t* __restrict__ b, uint32_t nx, uint32_t ny) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    uint8_t v = 0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const int xx = x + dx, yy = y + dy;
+            if (xx >= 0 && yy >= 0 && xx < (int)nx && yy < (int)ny && a[(size_t)xx * ny + yy]) v = 1;
+        }
+    b[i] = v;
+}
+
+// binary_fill_holes: background reached from outside the grid through
+// 4-neighbours (the dilation of scipy's fill with border value 1), one step
+__global__ void k_fill_step(const uint8_t* __restrict__ fat, const uint8_t* __restrict__ rin,
+                            uint8_t* __restrict__ rout, uint32_t nx, uint32_t ny, uint32_t* __restrict__ changed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool grew = false;
+    if (i < nx * ny) {
+        uint8_t r = 0;
+        if (!fat[i]) {
+            const uint32_t x = i / ny, y = i % ny;
+            r = rin[i] || x == 0 || y == 0 || x + 1 == nx || y + 1 == ny || rin[i - ny] || rin[i + ny] ||
+                rin[i - 1] || rin[i + 1];
+        }
+        rout[i] = r;
+        grew = r && !rin[i];
+    }
+    const uint64_t bal = __ballot(grew);
+    if (bal && (threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1))
+        atomicAdd(changed, (uint32_t)__popcll(bal));
+}
+
+__global__ void k_fill_final(uint8_t* __restrict__ fat, const uint8_t* __restrict__ reach, uint32_t nxy) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy) fat[i] = (fat[i] || !reach[i]) ? 1 : 0;
+}
+
+__global__ void k_flag_marks(const uint8_t* __restrict__ a, uint32_t nxy, uint32_t flag,
+                             uint32_t* __restrict__ mapping) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nxy && a[i]) mapping[i] |= flag;
+}
+
 int check_grid(const SlmParams* p) {
     int rc = slm_check_params(p);
     if (rc) return rc;
@@ -666,6 +763,27 @@ inline void zero_u64(u64* a, size_t n, hipStream_t s) {
 }
 
 // Workspace views.
+// Zhang-Suen thinning of the binary image a (b: scratch of the same size) to
+// convergence; one host synchronisation per iteration.
+int zs_thin(uint8_t* a, uint8_t* b, uint32_t nx, uint32_t ny, uint32_t* ctr, hipStream_t s, uint32_t* iters_out) {
+    const dim3 grid((nx * ny + kBlock - 1) / kBlock);
+    uint32_t iters = 0;
+    const uint32_t max_iters = nx + ny + 2;
+    for (;;) {
+        zero_u64((u64*)ctr, 1, s);
+        hipLaunchKernelGGL(k_zs_sub, grid, dim3(kBlock), 0, s, nx, ny, 0, (const uint8_t*)a, b, ctr);
+        hipLaunchKernelGGL(k_zs_sub, grid, dim3(kBlock), 0, s, nx, ny, 1, (const uint8_t*)b, a, ctr);
+        uint32_t changed = 0;
+        HIP_TRY(hipMemcpyAsync(&changed, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        iters++;
+        if (!changed) break;
+        if (iters >= max_iters) return slm_fail(SLM_E_HIP, "thinning did not converge in %u iterations", iters);
+    }
+    if (iters_out) *iters_out = iters;
+    return SLM_OK;
+}
+
 inline u64* ws_a(void* ws) { return (u64*)ws; }
 inline u64* ws_b(void* ws, const SlmParams* p) { return (u64*)ws + p->nxy_padded; }
 inline uint32_t* ws_ctr(void* ws, const SlmParams* p) { return (uint32_t*)((u64*)ws + 2 * (size_t)p->nxy_padded); }
@@ -710,20 +828,7 @@ extern "C" int slm_thin_channels(const SlmParams* p, uint32_t* mapping, void* wo
     uint32_t* ctr = ws_ctr(workspace, p);
     hipLaunchKernelGGL(k_zs_init, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint32_t*)mapping, a);
     uint32_t iters = 0;
-    const uint32_t max_iters = p->nx_padded + p->ny_padded + 2;
-    for (;;) {
-        zero_u64((u64*)ctr, 1, s);
-        hipLaunchKernelGGL(k_zs_sub, pixel_grid(p), dim3(kBlock), 0, s, p->nx_padded, p->ny_padded, 0,
-                           (const uint8_t*)a, b, ctr);
-        hipLaunchKernelGGL(k_zs_sub, pixel_grid(p), dim3(kBlock), 0, s, p->nx_padded, p->ny_padded, 1,
-                           (const uint8_t*)b, a, ctr);
-        uint32_t changed = 0;
-        HIP_TRY(hipMemcpyAsync(&changed, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        iters++;
-        if (!changed) break;
-        if (iters >= max_iters) return slm_fail(SLM_E_HIP, "thinning did not converge in %u iterations", iters);
-    }
+    if ((rc = zs_thin(a, b, p->nx_padded, p->ny_padded, ctr, s, &iters))) return rc;
     hipLaunchKernelGGL(k_zs_final, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint8_t*)a, mapping);
     if (n_iter) *n_iter = iters;
     return slm_finish(stream, flags);
@@ -907,4 +1012,91 @@ extern "C" int slm_fix_left_flanks(const SlmParams* p, const float* seeds_xy, co
                                    const float* uv, const uint32_t* mapping, uint32_t* label, void* workspace,
                                    size_t workspace_bytes, void* stream, uint32_t flags) {
     return fix_flanks(p, seeds_xy, mask, uv, mapping, label, workspace, workspace_bytes, stream, flags, true);
+}
+
+// ---------------------------------------------------------- mid-slopes and ridges
+
+namespace {
+inline size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" size_t slm_slopes_workspace_size(uint32_t nx, uint32_t ny) {
+    const size_t n = (size_t)nx * ny;
+    return 4 * al256(n * 4) + 6 * al256(n) + 512;
+}
+
+extern "C" int slm_gaussian_filter(const float* in, float* out, float* tmp, uint32_t nx, uint32_t ny,
+                                   const double* weights, int radius, void* stream, uint32_t flags) {
+    if (!in || !out || !tmp || !weights || !nx || !ny || radius < 0 || (uint64_t)nx * ny >= 0x80000000ull)
+        return slm_fail(SLM_E_INVALID_ARG, "bad gaussian_filter arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((uint32_t)(((size_t)nx * ny + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_gauss<0>, grid, dim3(kBlock), 0, s, in, tmp, nx, ny, weights, radius);
+    hipLaunchKernelGGL(k_gauss<1>, grid, dim3(kBlock), 0, s, (const float*)tmp, out, nx, ny, weights, radius);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, uint32_t nx, uint32_t ny,
+                                        const double* w_mid, int r_mid, float mid_threshold, const double* w_ridge,
+                                        int r_ridge, float ridge_level, uint32_t midslope_flag, uint32_t ridge_flag,
+                                        uint32_t* mapping, void* workspace, size_t workspace_bytes, void* stream,
+                                        uint32_t flags) {
+    if (!sla || !mask || !mapping || !w_mid || !w_ridge || !workspace || !nx || !ny || r_mid < 0 || r_ridge < 0 ||
+        (uint64_t)nx * ny >= 0x80000000ull)
+        return slm_fail(SLM_E_INVALID_ARG, "bad map_midslopes_ridges arguments");
+    if (workspace_bytes < slm_slopes_workspace_size(nx, ny))
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes, need %zu", workspace_bytes,
+                        slm_slopes_workspace_size(nx, ny));
+    const size_t n = (size_t)nx * ny;
+    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    float* ang = (float*)w;
+    float* tmp = (float*)(w + al256(n * 4));
+    float* gm = (float*)(w + 2 * al256(n * 4));
+    float* gr = (float*)(w + 3 * al256(n * 4));
+    uint8_t* mid = w + 4 * al256(n * 4);
+    uint8_t* ridge = mid + al256(n);
+    uint8_t* fat = ridge + al256(n);
+    uint8_t* scr = fat + al256(n);
+    uint8_t* ra = scr + al256(n);
+    uint8_t* rb = ra + al256(n);
+    uint32_t* ctr = (uint32_t*)(rb + al256(n));
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 grid((uint32_t)((n + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_sla_angle, grid, dim3(kBlock), 0, s, (const float2*)sla, (uint32_t)n, ang);
+    hipLaunchKernelGGL(k_gauss<0>, grid, dim3(kBlock), 0, s, (const float*)ang, tmp, nx, ny, w_mid, r_mid);
+    hipLaunchKernelGGL(k_gauss<1>, grid, dim3(kBlock), 0, s, (const float*)tmp, gm, nx, ny, w_mid, r_mid);
+    hipLaunchKernelGGL(k_gauss<0>, grid, dim3(kBlock), 0, s, (const float*)ang, tmp, nx, ny, w_ridge, r_ridge);
+    hipLaunchKernelGGL(k_gauss<1>, grid, dim3(kBlock), 0, s, (const float*)tmp, gr, nx, ny, w_ridge, r_ridge);
+    hipLaunchKernelGGL(k_slope_marks, grid, dim3(kBlock), 0, s, (const float*)gm, (const float*)gr, mask,
+                       (uint32_t)n, mid_threshold, ridge_level, mid, ridge);
+    // ridges: dilate twice, fill holes
+    hipLaunchKernelGGL(k_dilate8, grid, dim3(kBlock), 0, s, (const uint8_t*)ridge, scr, nx, ny);
+    hipLaunchKernelGGL(k_dilate8, grid, dim3(kBlock), 0, s, (const uint8_t*)scr, fat, nx, ny);
+    zero_u64((u64*)ra, (al256(n)) / 8, s);
+    uint8_t *rin = ra, *rout = rb;
+    const uint32_t max_steps = nx * 2 + ny * 2 + 4;
+    uint32_t steps = 0;
+    for (;;) {
+        zero_u64((u64*)ctr, 1, s);
+        for (int k = 0; k < 8; k++) {            // 8 steps between convergence checks
+            hipLaunchKernelGGL(k_fill_step, grid, dim3(kBlock), 0, s, (const uint8_t*)fat, (const uint8_t*)rin, rout,
+                               nx, ny, ctr);
+            std::swap(rin, rout);
+        }
+        steps += 8;
+        uint32_t grown = 0;
+        HIP_TRY(hipMemcpyAsync(&grown, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!grown) break;
+        if (steps > max_steps) return slm_fail(SLM_E_HIP, "hole filling did not converge in %u steps", steps);
+    }
+    hipLaunchKernelGGL(k_fill_final, grid, dim3(kBlock), 0, s, fat, (const uint8_t*)rin, (uint32_t)n);
+    // skeletons
+    int rc;
+    if ((rc = zs_thin(mid, scr, nx, ny, ctr, s, nullptr))) return rc;
+    if ((rc = zs_thin(fat, scr, nx, ny, ctr, s, nullptr))) return rc;
+    hipLaunchKernelGGL(k_flag_marks, grid, dim3(kBlock), 0, s, (const uint8_t*)mid, (uint32_t)n, midslope_flag,
+                       mapping);
+    hipLaunchKernelGGL(k_flag_marks, grid, dim3(kBlock), 0, s, (const uint8_t*)fat, (uint32_t)n, ridge_flag, mapping);
+    return slm_finish(stream, flags);
 }

@@ -71,6 +71,10 @@ _PROTOS = {
                                               _P, ctypes.c_size_t, _P, _U32]),
     "slm_gradient_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, _P, _P, _P, _U32]),
     "slm_condition_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_slopes_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_gaussian_filter": (ctypes.c_int, [_P, _P, _P, _U32, _U32, _P, ctypes.c_int, _P, _U32]),
+    "slm_map_midslopes_ridges": (ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.c_int, ctypes.c_float, _P, ctypes.c_int,
+                                                ctypes.c_float, _U32, _U32, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_condition_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),
@@ -363,6 +367,52 @@ def condition_uv(roi, pad: int, pixel_size: float, do_normalize_speed=True, vecs
     if maps:
         return uv, mask.to(torch.bool), m
     return uv, mask.to(torch.bool)
+
+
+def gaussian_weights(sigma: float, truncate: float = 4.0):
+    """The weights scipy.ndimage.gaussian_filter uses (float64 numpy, radius):
+    exp(-x^2 / (2 sigma^2)) over |x| <= int(truncate*sigma + 0.5), normalised."""
+    r = int(truncate * float(sigma) + 0.5)
+    x = np.arange(-r, r + 1)
+    w = np.exp(-0.5 / (float(sigma) ** 2) * x ** 2)
+    return w / w.sum(), r
+
+
+def gaussian_filter(x, sigma: float, truncate: float = 4.0, stream=None, sync=False):
+    """slm_gaussian_filter: scipy.ndimage.gaussian_filter of a float32 [nx, ny]
+    device tensor ('reflect' borders), bit for bit."""
+    import torch
+    _require_cuda(x)
+    w, r = gaussian_weights(sigma, truncate)
+    wd = torch.from_numpy(w).to(x.device)
+    out = torch.empty_like(x)
+    tmp = torch.empty_like(x)
+    check(load().slm_gaussian_filter(_ptr(x), _ptr(out), _ptr(tmp), x.shape[0], x.shape[1], _ptr(wd), r,
+                                     _stream(stream), SLM_FLAG_SYNC if sync else 0))
+    return out
+
+
+def map_midslopes_ridges(sla, mask, mapping, midslope_flag, ridge_flag, midslope_filter_sigma=5.0,
+                         midslope_threshold=0.03, ridge_filter_sigma=2.0, ridge_threshold=0.8, stream=None,
+                         sync=False):
+    """slm_map_midslopes_ridges (mapping.py:779-817): sla float32 [nx, ny, 2]
+    (or [nx*ny, 2]), mask bool/uint8 [nx, ny], mapping int32 [nx, ny] updated
+    in place with the mid-slope and ridge skeleton flags."""
+    import torch
+    _require_cuda(sla, mask, mapping)
+    nx, ny = mapping.shape
+    wm, rm = gaussian_weights(midslope_filter_sigma)
+    wr, rr = gaussian_weights(ridge_filter_sigma)
+    wm_d, wr_d = torch.from_numpy(wm).to(sla.device), torch.from_numpy(wr).to(sla.device)
+    m8 = mask.to(torch.uint8).contiguous()
+    ws = torch.empty(int(load().slm_slopes_workspace_size(nx, ny)), dtype=torch.uint8, device=sla.device)
+    level = float(np.float32(-(np.pi / 4) * ridge_threshold))
+    sla_c = sla.contiguous()                  # held while the kernels run
+    check(load().slm_map_midslopes_ridges(_ptr(sla_c), _ptr(m8), nx, ny, _ptr(wm_d), rm,
+                                          float(np.float32(midslope_threshold)), _ptr(wr_d), rr, level,
+                                          int(midslope_flag), int(ridge_flag), _ptr(mapping), _ptr(ws), ws.numel(),
+                                          _stream(stream), SLM_FLAG_SYNC if sync else 0))
+    return mapping
 
 
 def fields_finish_raw(slc_raw, slt_raw, density: int, stream=None, sync=False):

@@ -41,11 +41,16 @@ class _Stages:
         return out
 
 
-def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None, condition=False):
+def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None, condition=False,
+             midslopes=True, midslope_filter_sigma=5.0, midslope_threshold=0.03, ridge_filter_sigma=2.0,
+             ridge_threshold=0.8, return_grids=False):
     """Run the pipeline on a workloads.Job; returns (results dict, stage timings).
     condition: the uv stage runs the loop and blockage conditioning
     (slm_condition_uv, the reference's do_condition) instead of the plain
-    gradient field the job's grids were built with."""
+    gradient field the job's grids were built with.  midslopes: the
+    mid-slope and ridge flags of Mapping.pass1 (mapping.py:286-287, the JSON
+    mapping defaults) are set before the channels are mapped.  return_grids:
+    results["grids"] holds the device grids (uv, mask, sla, slt, slc, mapping)."""
     import torch
     dev = device or torch.device("cuda", torch.cuda.current_device())
     info = job.info
@@ -114,11 +119,20 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
                                channel_quantile).item())
     info.set_thresholds(channel_threshold=np.float32(thr), segmentation_threshold=np.uint32(segmentation_threshold))
     mapping0 = torch.zeros((nxp, nyp), dtype=torch.int32, device=dev)
+    if midslopes:
+        hip.map_midslopes_ridges(sla_f.reshape(nxp, nyp, 2), mask_b, mapping0, int(info.is_midslope),
+                                 int(info.is_ridge), midslope_filter_sigma, midslope_threshold, ridge_filter_sigma,
+                                 ridge_threshold)
     cm = ChannelMapper(info, mask_b, uv, mapping0, slt=slt_dn, slc=slc_dn, fp_contract=fp_contract)
     ok_chain = cm.map_channels_segments()
 
     st.mark("hsl")
-    hs_seeds = pick_seeds(mask=mask_b, map=~cm.mapping, flag=info.is_thinchannel, pad=pad)
+    if midslopes:
+        # lengths.hsl (lengths.py): trace down from the mid-slope pixels
+        hs_seeds = pick_seeds(mask=mask_b, map=cm.mapping, flag=info.is_midslope, pad=pad)
+    else:
+        # without a mid-slope map: from every pixel that is not a thin channel
+        hs_seeds = pick_seeds(mask=mask_b, map=~cm.mapping, flag=info.is_thinchannel, pad=pad)
     info.n_seed_points = np.uint32(hs_seeds.shape[0])
     p = kernel_params(info, 1, "HILLSLOPE_LENGTHS", fp_contract)
     hsl = torch.zeros(hs_seeds.shape[0], dtype=torch.float32, device=dev)
@@ -149,4 +163,9 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     if dist is not None:
         dist.find_modes()
         results["kde_mode_x"] = float(np.asarray(dist.mode_x).reshape(-1)[0])
+    results["midslope_pixels"] = int(((cm.mapping & int(info.is_midslope)) != 0).sum().item())
+    results["ridge_pixels"] = int(((cm.mapping & int(info.is_ridge)) != 0).sum().item())
+    if return_grids:
+        results["grids"] = {"uv": uv, "mask": mask_b, "sla": sla_f, "slt": slt_f, "slc": slc_f,
+                            "mapping": cm.mapping}
     return results, timings

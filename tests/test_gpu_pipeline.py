@@ -19,7 +19,10 @@ def test_map_tile_small_256(gpu):
     print(res, {k: round(v["gpu_ms"], 2) for k, v in tim.items()}, flush=True)
     assert set(tim) == {"uv", "trajectories", "fields", "finish", "mapping", "hsl", "kde"}
     assert res["chain_ok"] and res["n_segments"] > 0 and res["channel_pixels"] > 0
-    assert res["hsl_reached"] > 0.5 * res["hsl_seeds"]
+    # HSL traces start at the mid-slope pixels (lengths.py) and stop, unmeasured,
+    # when they meet another mid-slope line: a share of them reaches a channel
+    assert 0 < res["hsl_seeds"] <= res["midslope_pixels"]
+    assert res["hsl_reached"] > 0.2 * res["hsl_seeds"]
     assert res["kde_mode_x"] is not None and np.isfinite(res["kde_mode_x"])
     # fields steps equal a direct call on the host-prepared inputs
     seeds = torch.from_numpy(job.seeds).to(gpu)

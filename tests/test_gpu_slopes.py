@@ -1,0 +1,69 @@
+"""GPU parity of the mid-slope and ridge mapping (mapping.py:779-817):
+slm_gaussian_filter against scipy.ndimage.gaussian_filter bit for bit
+(sigmas whose radius exceeds the grid included, so 'reflect' folds more than
+once), and slm_map_midslopes_ridges against oracle/slopes_ref.py (scipy's
+gaussian, dilation and hole filling, the oracle's thinning) on smooth sla
+fields and on the sla of a traced DTM."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("shape,sigma", [((64, 50), 2.0), ((40, 33), 5.0), ((9, 12), 3.0), ((300, 257), 1.3)])
+def test_gaussian_filter_bit_exact(gpu, shape, sigma):
+    import torch
+    from scipy import ndimage
+    from slm_amd import hip
+    rs = np.random.default_rng(int(sigma * 10) + shape[0])
+    x = rs.standard_normal(shape).astype(np.float32) * 3
+    got = hip.gaussian_filter(torch.from_numpy(x).to(gpu), sigma, sync=True).cpu().numpy()
+    want = ndimage.gaussian_filter(x, sigma)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _smooth_sla(n, m, seed):
+    from scipy import ndimage
+    rs = np.random.default_rng(seed)
+    d = np.exp(ndimage.gaussian_filter(rs.standard_normal((n, m)), 6) * 8).astype(np.float32) * 30
+    u = np.exp(ndimage.gaussian_filter(rs.standard_normal((n, m)), 6) * 8).astype(np.float32) * 30
+    return np.stack([d, u], -1).astype(np.float32)
+
+
+def _run(gpu, sla, mask):
+    import torch
+    from oracle import slopes_ref
+    from slm_amd import hip, workloads
+    p = workloads.planar(n=8).params()
+    mid_o, ridge_o, gm, gr = slopes_ref.midslopes_ridges(sla, mask, p)
+    mapping = torch.zeros(mask.shape, dtype=torch.int32, device=gpu)
+    hip.map_midslopes_ridges(torch.from_numpy(sla).to(gpu), torch.from_numpy(mask).to(gpu), mapping, 0x800, 0x1000,
+                             sync=True)
+    m = hip.as_u32(mapping)
+    assert np.array_equal((m & 0x800) != 0, mid_o)
+    assert np.array_equal((m & 0x1000) != 0, ridge_o)
+    assert not (m & ~np.uint32(0x1800)).any()
+    return mid_o.sum(), ridge_o.sum()
+
+
+def test_midslopes_ridges_smooth_fields(gpu):
+    sla = _smooth_sla(180, 150, 3)
+    mask = np.zeros((180, 150), bool)
+    mask[:2], mask[-2:], mask[:, :2], mask[:, -2:] = True, True, True, True
+    mask[60:70, 40:90] = True
+    n_mid, n_ridge = _run(gpu, sla, mask)
+    print("midslope px", n_mid, "ridge px", n_ridge, flush=True)
+    assert n_mid > 0 and n_ridge > 0
+
+
+def test_midslopes_ridges_traced_dtm(gpu):
+    """sla of the 256^2 workload traced on the GPU (both directions), as map_tile makes it."""
+    import torch
+    from slm_amd import hip, pipeline, workloads
+    job = workloads.small_256()
+    res, _ = pipeline.map_tile(job, return_grids=True)
+    assert res["midslope_pixels"] > 0
+    sla = res["grids"]["sla"].cpu().numpy().reshape(job.mask.shape + (2,))
+    n_mid, n_ridge = _run(gpu, sla, job.mask)
+    print("midslope px", n_mid, "ridge px", n_ridge, flush=True)
+    assert n_mid > 0
