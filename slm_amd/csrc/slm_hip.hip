@@ -279,6 +279,83 @@ __device__ __forceinline__ bool fields_packed_ok(const FieldsCtrl* ctrl, uint32_
     return !force_fallback && ctrl->sumlow == ctrl->writes;
 }
 
+// Wave-private accumulation table in LDS.  A flushed run (pixel, packed sum)
+// is added into the wave's table instead of global memory; the table is
+// direct-mapped by pixel position (x mod 16, y mod 32, so a 16x32 window of
+// neighbouring pixels never collides).  A run whose slot holds another pixel
+// evicts it: the evicted sum goes to acc with one global 64-bit atomic.  At
+// the end the wave drains its table into acc.  Streamlines of a wave start
+// in one 8x8 tile and converge downhill, so most flushes hit a pixel another
+// lane flushed recently, and the table removes most of the global atomics
+// and every same-address collision among the lanes of one atomic
+// instruction.  All sums are the same integers added in another order.
+//
+// Protocol of one add() (every lane of the wave calls it together; lanes
+// with has = false take no part).  The wave's LDS operations execute in
+// program order, and wavefront-scope fences keep the compiler from moving
+// them across phases:
+//   1. read the slot's tag; a lane whose pixel is there adds its sum (hit);
+//   2. a missing lane swaps the tag from what it read to its own pixel with a
+//      compare-and-swap; the first such lane per slot wins;
+//   3. the winner exchanges the slot's sum for its own (the old sum, which
+//      already holds the hits of phase 1, is the evicted pixel's);
+//   4. a loser whose pixel is the winner's adds into the slot; a loser with
+//      another pixel, and the winner's evicted sum, go to global memory.
+constexpr uint32_t kRunSlots = 512;  // per wave: 4 B tag + 8 B sum each
+constexpr uint32_t kRunEmpty = 0xffffffffu;
+
+__device__ __forceinline__ uint32_t slot_of(uint32_t ix, uint32_t iy) {
+    return ((ix & 15u) << 5) | (iy & 31u);
+}
+
+struct RunTable {
+    uint32_t* tag;
+    unsigned long long* val;
+
+    __device__ __forceinline__ void init(uint32_t lane) {
+        for (uint32_t i = lane; i < kRunSlots; i += 64u) {
+            __hip_atomic_store(&tag[i], kRunEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            __hip_atomic_store(&val[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+
+    __device__ __forceinline__ void add(bool has, uint32_t a, uint32_t s, unsigned long long v,
+                                        unsigned long long* __restrict__ acc) {
+        uint32_t r = kRunEmpty;
+        bool miss = false;
+        if (has) {
+            r = __hip_atomic_load(&tag[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            if (r == a) __hip_atomic_fetch_add(&val[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            else miss = true;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        bool won = false;
+        uint32_t seen = r;
+        if (miss)
+            won = __hip_atomic_compare_exchange_strong(&tag[s], &seen, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        unsigned long long old = 0;
+        if (won) old = __hip_atomic_exchange(&val[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (miss && !won && seen == a)
+            __hip_atomic_fetch_add(&val[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (won && r != kRunEmpty && old) atomicAdd(&acc[r], old);
+        if (miss && !won && seen != a) atomicAdd(&acc[a], v);
+    }
+
+    __device__ __forceinline__ void drain(uint32_t lane, unsigned long long* __restrict__ acc) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        for (uint32_t i = lane; i < kRunSlots; i += 64u) {
+            const uint32_t t = __hip_atomic_load(&tag[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            const unsigned long long v = __hip_atomic_load(&val[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            if (t != kRunEmpty && v) atomicAdd(&acc[t], v);
+        }
+    }
+};
+
 // One persistent wave pulls streamlines from the queue; a lane that finishes
 // takes the next item at once, so divergent lengths do not idle the wave.
 // Work item w: seed ordinal order[w / d^2], sub-pixel start w % d^2 = j*d+i.
@@ -309,25 +386,30 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const float res = P.trajectory_resolution;
     const float combo = P.combo_factor;
+    const uint32_t nxm1 = P.nx_padded - 1u, nym1 = P.ny_padded - 1u;
+
+    __shared__ uint32_t s_tag[PACKED ? 4u * kRunSlots : 1u];
+    __shared__ unsigned long long s_val[PACKED ? 4u * kRunSlots : 1u];
+    RunTable table{s_tag + (threadIdx.x >> 6) * kRunSlots, s_val + (threadIdx.x >> 6) * kRunSlots};
+    if constexpr (PACKED) table.init(lane);
 
     bool active = false;
     bool exhausted = false;
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
     Quad Q;                                        // quad of the current pixel
-    uint32_t run_idx = 0, run_n = 0, run_t = 0;   // current same-pixel run
+    uint32_t run_idx = 0, run_slot = 0, run_n = 0, run_t = 0;   // current same-pixel run
     uint64_t my_steps = 0, my_writes = 0;
 
-    auto flush = [&]() {
-        if (run_n) {
-            if constexpr (PACKED) {
-                atomicAdd(&acc[run_idx], ((unsigned long long)run_t << 32) | (unsigned long long)run_n);
-            } else {
-                atomicAdd(&slt[run_idx], run_t);
-                atomicAdd(&slc[run_idx], run_n);
+    // adds the runs of the lanes with has = true (all lanes call it together)
+    auto emit = [&](bool has, uint32_t a, uint32_t s, uint32_t cn, uint32_t ct) {
+        if constexpr (PACKED) {
+            table.add(has, a, s, ((unsigned long long)ct << 32) | (unsigned long long)cn, acc);
+        } else {
+            if (has) {
+                atomicAdd(&slt[a], ct);
+                atomicAdd(&slc[a], cn);
             }
-            run_n = 0;
-            run_t = 0;
         }
     };
 
@@ -386,16 +468,17 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                         l = 0.f;
                         n = 0;
                         dt = P.dt_max;
-                        const uint32_t idx = array_idx(P, x, y);
+                        const uint32_t ix = pix(x, P.pad_width_pp5, nxm1), iy = pix(y, P.pad_width_pp5, nym1);
+                        const uint32_t idx = iy + P.ny_padded * ix;
                         if (!mask[idx]) {
                             // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
                             Q = load_quad(quad, idx);
                             run_idx = idx;
+                            run_slot = slot_of(ix, iy);
                             run_n = 1u;
                             run_t = slt_inc(0.f);
                             my_writes += 1u;
                             active = (0.f < P.max_length) && (0u < P.max_n_steps);
-                            if (!active) flush();
                         }
                     }
                 }
@@ -403,10 +486,9 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 c_next += take;
             }
         }
-        if (!__any(active)) {
-            if (exhausted) break;
-            continue;
-        }
+        // run that ends this step because the streamline entered another pixel
+        bool f_has = false;
+        uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
         if (active) {
             // compute_step_vec_jittered (computestep.cl:86-106) on quads
             const float2 u1 = jitter_unit<C>(P, interp_quad<C>(P, Q, x, y), rng);
@@ -417,12 +499,12 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
                                           approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
             const float nx = x + d2.x, ny = y + d2.y;
-            const uint32_t nidx = array_idx(P, nx, ny);
+            const uint32_t nix = pix(nx, P.pad_width_pp5, nxm1), niy = pix(ny, P.pad_width_pp5, nym1);
+            const uint32_t nidx = niy + P.ny_padded * nix;
             const uint8_t mk = mask[nidx];
             const Quad Qn = load_quad(quad, nidx);
             if (mk) {
                 active = false;  // end pixel masked: step dropped, loop ends
-                flush();
             } else {
                 // runge_kutta_step_write_sl_data (rungekutta.cl:163-186)
                 const float dl = fast_length<C>(d2.x, d2.y);
@@ -442,17 +524,35 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 my_writes += 1u;
                 // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced
                 if (nidx != run_idx) {
-                    flush();
+                    f_has = true;
+                    f_idx = run_idx;
+                    f_slot = run_slot;
+                    f_n = run_n;
+                    f_t = run_t;
                     run_idx = nidx;
+                    run_slot = slot_of(nix, niy);
+                    run_n = 0;
+                    run_t = 0;
                 }
                 run_n += 1u;
                 run_t += slt_inc(l);
                 active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
-                if (!active) flush();
             }
         }
+        emit(f_has, f_idx, f_slot, f_n, f_t);
+        // runs of streamlines that ended (this step, or at their seed)
+        const bool ended = !active && run_n;
+        if (__any(ended)) {
+            emit(ended, run_idx, run_slot, run_n, run_t);
+            if (ended) {
+                run_n = 0;
+                run_t = 0;
+            }
+        }
+        if (exhausted && !__any(active)) break;
     }
     if constexpr (PACKED) {
+        table.drain(lane, acc);
         // wave reductions, one atomic per wave
         uint64_t v = my_steps, wv = my_writes;
         for (int off = 32; off > 0; off >>= 1) {
@@ -1592,39 +1692,46 @@ template <bool C, bool PACKED>
 __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restrict__ desc, uint32_t n_jobs,
                                                       uint32_t n_work, unsigned long long* __restrict__ steps,
                                                       FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
-                                                      uint32_t chunk, uint32_t n_slices) {
+                                                      uint32_t chunk, uint32_t n_slices,
+                                                      unsigned long long* __restrict__ acc_all) {
     if constexpr (!PACKED) {
         if (fields_packed_ok(ctrl, force_fallback)) return;
     }
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
 
+    // per-wave accumulation table; its tags are pixel indices into the
+    // concatenated accumulator of all jobs (fewer than 2^32 - 1 pixels)
+    __shared__ uint32_t s_tag[PACKED ? 4u * kRunSlots : 1u];
+    __shared__ unsigned long long s_val[PACKED ? 4u * kRunSlots : 1u];
+    RunTable table{s_tag + (threadIdx.x >> 6) * kRunSlots, s_val + (threadIdx.x >> 6) * kRunSlots};
+    if constexpr (PACKED) table.init(lane);
+
     // per-lane job context
     SlmParams P;
     const uint8_t* mask = nullptr;
     const Quad* quad = nullptr;
-    unsigned long long* acc = nullptr;
     uint32_t* slc = nullptr;
     uint32_t* slt = nullptr;
-    uint32_t dt_fixed = 0;
+    uint32_t dt_fixed = 0, pix_begin = 0;
 
     bool active = false, exhausted = false;
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
     Quad Q;
-    uint32_t run_idx = 0, run_n = 0, run_t = 0;
+    uint32_t run_idx = 0, run_slot = 0, run_n = 0, run_t = 0;
     uint64_t my_steps = 0, my_writes = 0;
 
-    auto flush = [&]() {
-        if (run_n) {
-            if constexpr (PACKED) {
-                atomicAdd(&acc[run_idx], ((unsigned long long)run_t << 32) | (unsigned long long)run_n);
-            } else {
-                atomicAdd(&slt[run_idx], run_t);
-                atomicAdd(&slc[run_idx], run_n);
+    // adds the runs of the lanes with has = true (all lanes call it together);
+    // a is the job-local pixel index
+    auto emit = [&](bool has, uint32_t a, uint32_t s, uint32_t cn, uint32_t ct) {
+        if constexpr (PACKED) {
+            table.add(has, pix_begin + a, s, ((unsigned long long)ct << 32) | (unsigned long long)cn, acc_all);
+        } else {
+            if (has) {
+                atomicAdd(&slt[a], ct);
+                atomicAdd(&slc[a], cn);
             }
-            run_n = 0;
-            run_t = 0;
         }
     };
 
@@ -1670,10 +1777,10 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     P = D->P;
                     mask = D->mask;
                     quad = D->quad;
-                    acc = D->acc;
                     slc = D->slc;
                     slt = D->slt;
                     dt_fixed = D->dt_fixed;
+                    pix_begin = D->pix_begin;
                     const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
                     const uint32_t dd = d * d;
                     const uint32_t wl = w - D->work_begin;
@@ -1685,26 +1792,26 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     l = 0.f;
                     n = 0;
                     dt = P.dt_max;
-                    const uint32_t idx = array_idx(P, x, y);
+                    const uint32_t ix = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+                    const uint32_t iy = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
+                    const uint32_t idx = iy + P.ny_padded * ix;
                     if (!mask[idx]) {
                         // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
                         Q = load_quad(quad, idx);
                         run_idx = idx;
+                        run_slot = slot_of(ix, iy);
                         run_n = 1u;
                         run_t = slt_inc(0.f);
                         my_writes += 1u;
                         active = (0.f < P.max_length) && (0u < P.max_n_steps);
-                        if (!active) flush();
                     }
                 }
                 need &= ~__ballot(mine);
                 c_next += take;
             }
         }
-        if (!__any(active)) {
-            if (exhausted) break;
-            continue;
-        }
+        bool f_has = false;
+        uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
         if (active) {
             const float res = P.trajectory_resolution;
             const float combo = P.combo_factor;
@@ -1716,12 +1823,13 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
             const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
                                           approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
             const float nx = x + d2.x, ny = y + d2.y;
-            const uint32_t nidx = array_idx(P, nx, ny);
+            const uint32_t nix = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
+            const uint32_t niy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
+            const uint32_t nidx = niy + P.ny_padded * nix;
             const uint8_t mk = mask[nidx];
             const Quad Qn = load_quad(quad, nidx);
             if (mk) {
                 active = false;
-                flush();
             } else {
                 const float dl = fast_length<C>(d2.x, d2.y);
                 x = nx;
@@ -1739,17 +1847,34 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 my_steps += 1u;
                 my_writes += 1u;
                 if (nidx != run_idx) {
-                    flush();
+                    f_has = true;
+                    f_idx = run_idx;
+                    f_slot = run_slot;
+                    f_n = run_n;
+                    f_t = run_t;
                     run_idx = nidx;
+                    run_slot = slot_of(nix, niy);
+                    run_n = 0;
+                    run_t = 0;
                 }
                 run_n += 1u;
                 run_t += slt_inc(l);
                 active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
-                if (!active) flush();
             }
         }
+        emit(f_has, f_idx, f_slot, f_n, f_t);
+        const bool ended = !active && run_n;
+        if (__any(ended)) {
+            emit(ended, run_idx, run_slot, run_n, run_t);
+            if (ended) {
+                run_n = 0;
+                run_t = 0;
+            }
+        }
+        if (exhausted && !__any(active)) break;
     }
     if constexpr (PACKED) {
+        table.drain(lane, acc_all);
         uint64_t v = my_steps, wv = my_writes;
         for (int off = 32; off > 0; off >>= 1) {
             v += __shfl_down(v, off);
@@ -1950,10 +2075,10 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (n_work) {
         if (C0)
             hipLaunchKernelGGL((k_fields_batch<true, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
-                               n_work, st, ctrl, force, chunk, slices);
+                               n_work, st, ctrl, force, chunk, slices, acc_all);
         else
             hipLaunchKernelGGL((k_fields_batch<false, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, st, ctrl, force, chunk, slices);
+                               n_jobs, n_work, st, ctrl, force, chunk, slices, acc_all);
     }
     hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix, ctrl);
     hipLaunchKernelGGL(k_batch_commit, dim3(grid_for(w.n_pix)), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
@@ -1961,10 +2086,10 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (n_work) {
         if (C0)
             hipLaunchKernelGGL((k_fields_batch<true, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices);
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
         else
             hipLaunchKernelGGL((k_fields_batch<false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices);
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
     }
     return slm_finish(stream, flags);
 }
