@@ -367,7 +367,7 @@ struct RunTable {
 // and GRID_SCALE*ADJUSTED_MAX_ERROR >= sqrt(8)*(1+2^-20) makes
 // fmin(DT_MAX, ADJUSTED_MAX_ERROR*DT_MAX/step_error) == DT_MAX); step_error
 // is then not computed.
-template <bool C, bool PACKED>
+template <bool C, bool PACKED, bool R2>
 __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __restrict__ seeds,
                                                 const SeedRec* __restrict__ order, uint32_t n_work,
                                                 const uint8_t* __restrict__ mask,
@@ -386,6 +386,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
     const float res = P.trajectory_resolution;
     const float combo = P.combo_factor;
+    const float inv_res = 1.0f / res;
     const uint32_t nxm1 = P.nx_padded - 1u, nym1 = P.ny_padded - 1u;
 
     __shared__ uint32_t s_tag[PACKED ? 4u * kRunSlots : 1u];
@@ -398,6 +399,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
     Quad Q;                                        // quad of the current pixel
+    uint32_t cx = 0, cy = 0;                       // and its coordinates
     uint32_t run_idx = 0, run_slot = 0, run_n = 0, run_t = 0;   // current same-pixel run
     uint64_t my_steps = 0, my_writes = 0;
 
@@ -473,11 +475,12 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                         if (!mask[idx]) {
                             // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
                             Q = load_quad(quad, idx);
+                            cx = ix;
+                            cy = iy;
                             run_idx = idx;
                             run_slot = slot_of(ix, iy);
                             run_n = 1u;
                             run_t = slt_inc(0.f);
-                            my_writes += 1u;
                             active = (0.f < P.max_length) && (0u < P.max_n_steps);
                         }
                     }
@@ -491,13 +494,14 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
         uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
         if (active) {
             // compute_step_vec_jittered (computestep.cl:86-106) on quads
-            const float2 u1 = jitter_unit<C>(P, interp_quad<C>(P, Q, x, y), rng);
-            const float2 d1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+            const float2 u1 = jitter_unit<C>(P, interp_quad_px<C>(P, Q, x, y, cx, cy), rng);
+            const float2 d1 = make_float2(approx_r<R2>(u1.x * combo * dt, res, inv_res), approx_r<R2>(u1.y * combo * dt, res, inv_res));
             const float mx = x + d1.x, my = y + d1.y;
-            const Quad Q1 = load_quad(quad, array_idx(P, mx, my));
-            const float2 u2 = jitter_unit<C>(P, interp_quad<C>(P, Q1, mx, my), rng);
-            const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
-                                          approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
+            const uint32_t mix = pix(mx, P.pad_width_pp5, nxm1), miy = pix(my, P.pad_width_pp5, nym1);
+            const Quad Q1 = load_quad(quad, miy + P.ny_padded * mix);
+            const float2 u2 = jitter_unit<C>(P, interp_quad_px<C>(P, Q1, mx, my, mix, miy), rng);
+            const float2 d2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, d1.x), res, inv_res),
+                                          approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, d1.y), res, inv_res));
             const float nx = x + d2.x, ny = y + d2.y;
             const uint32_t nix = pix(nx, P.pad_width_pp5, nxm1), niy = pix(ny, P.pad_width_pp5, nym1);
             const uint32_t nidx = niy + P.ny_padded * nix;
@@ -511,6 +515,8 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 x = nx;
                 y = ny;
                 Q = Qn;
+                cx = nix;
+                cy = niy;
                 const bool stuck = dl < P.integration_halt_threshold;
                 if (!stuck && !dt_fixed) {
                     StepOut o;
@@ -520,8 +526,6 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 }
                 l += dl;
                 n += 1u;
-                my_steps += 1u;
-                my_writes += 1u;
                 // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced
                 if (nidx != run_idx) {
                     f_has = true;
@@ -545,6 +549,9 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
         if (__any(ended)) {
             emit(ended, run_idx, run_slot, run_n, run_t);
             if (ended) {
+                // the streamline's writes: its seed and n accepted steps
+                my_steps += n;
+                my_writes += (uint64_t)n + 1u;
                 run_n = 0;
                 run_t = 0;
             }
@@ -585,6 +592,15 @@ __global__ void k_build_quads(SlmParams P, const float2* __restrict__ uv, Quad* 
 static uint32_t dt_is_fixed(const SlmParams* p) {
     return ((double)p->grid_scale * (double)p->adjusted_max_error >= 2.8284271247461903 * (1.0 + 1.0 / 1048576.0))
                ? 1u : 0u;
+}
+
+// TRAJECTORY_RESOLUTION is a power of two in [1, 2^31]: approximate's division
+// is then an exact scaling (approx_r<true>)
+static bool res_is_pow2(const SlmParams* p) {
+    const float r = p->trajectory_resolution;
+    if (!(r >= 1.0f && r <= 2147483648.0f)) return false;
+    const uint32_t u = (uint32_t)r;
+    return (float)u == r && (u & (u - 1u)) == 0u;
 }
 
 __global__ void k_fields_init(FieldsCtrl* ctrl) {
@@ -700,30 +716,36 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     const uint32_t n_work = n * d * d;
     // persistent grid: as many 256-thread blocks as can be resident at once
     int per_cu = 0;
+    const bool r2 = res_is_pow2(p);
     if (p->fp_contract)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<true, true>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<true, true, true>, 256, 0));
     else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<false, true>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<false, true, true>, 256, 0));
     if (per_cu < 1) per_cu = 1;
     uint32_t grid = (uint32_t)di.cus * (uint32_t)per_cu;
     const uint32_t max_useful = (n_work + 255) / 256;
     if (grid > max_useful) grid = max_useful ? max_useful : 1;
     unsigned long long* st = (unsigned long long*)steps_out;
     const uint32_t fixed = dt_is_fixed(p);
-    if (p->fp_contract)
-        hipLaunchKernelGGL((k_fields<true, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           quad, acc, slc, slt, st, ctrl, force, fixed);
-    else
-        hipLaunchKernelGGL((k_fields<false, true>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
-                           quad, acc, slc, slt, st, ctrl, force, fixed);
+#define SLM_LAUNCH_FIELDS(CC, RR)                                                                            \
+    hipLaunchKernelGGL((k_fields<CC, true, RR>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask, quad, \
+                       acc, slc, slt, st, ctrl, force, fixed)
+    if (p->fp_contract) {
+        if (r2) SLM_LAUNCH_FIELDS(true, true);
+        else SLM_LAUNCH_FIELDS(true, false);
+    } else {
+        if (r2) SLM_LAUNCH_FIELDS(false, true);
+        else SLM_LAUNCH_FIELDS(false, false);
+    }
+#undef SLM_LAUNCH_FIELDS
     hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(nxy)), dim3(256), 0, s, acc, nxy, ctrl);
     hipLaunchKernelGGL(k_fields_commit, dim3(grid_for(nxy)), dim3(256), 0, s, acc, slc, slt, nxy, ctrl,
                        accumulate ? 1u : 0u, force);
     if (p->fp_contract)
-        hipLaunchKernelGGL((k_fields<true, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+        hipLaunchKernelGGL((k_fields<true, false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
                            quad, acc, slc, slt, nullptr, ctrl, force, fixed);
     else
-        hipLaunchKernelGGL((k_fields<false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
+        hipLaunchKernelGGL((k_fields<false, false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
                            quad, acc, slc, slt, nullptr, ctrl, force, fixed);
     return slm_finish(stream, flags);
 }
@@ -1536,19 +1558,19 @@ extern "C" int slm_trace_fields_tiled(const SlmParams* p, const SlmTileOpts* opt
     // fallback: the queue tracer with 32-bit atomics, identity order; exits at once when committed
     int fb_cu = 0;
     if (p->fp_contract)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<true, false>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<true, false, false>, 256, 0));
     else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<false, false>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fb_cu, k_fields<false, false, false>, 256, 0));
     if (fb_cu < 1) fb_cu = 1;
     uint32_t fgrid = (uint32_t)di.cus * (uint32_t)fb_cu;
     const uint32_t max_useful = (n_work + 255) / 256;
     if (fgrid > max_useful) fgrid = max_useful ? max_useful : 1;
     const float2* seeds = (const float2*)seeds_xy;
     if (p->fp_contract)
-        hipLaunchKernelGGL((k_fields<true, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work, mask,
+        hipLaunchKernelGGL((k_fields<true, false, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work, mask,
                            quad, acc, slc, slt, nullptr, &ctrl->f, force, fixed);
     else
-        hipLaunchKernelGGL((k_fields<false, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work,
+        hipLaunchKernelGGL((k_fields<false, false, false>), dim3(fgrid), dim3(256), 0, s, *p, seeds, nullptr, n_work,
                            mask, quad, acc, slc, slt, nullptr, &ctrl->f, force, fixed);
     return slm_finish(stream, flags);
 }
@@ -1688,7 +1710,7 @@ __global__ void k_put_desc(DescPack pack, uint32_t first, uint32_t count, BatchD
     if (i < count) out[first + i] = pack.d[i];
 }
 
-template <bool C, bool PACKED>
+template <bool C, bool PACKED, bool R2>
 __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restrict__ desc, uint32_t n_jobs,
                                                       uint32_t n_work, unsigned long long* __restrict__ steps,
                                                       FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
@@ -1719,6 +1741,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0;
     Quad Q;
+    uint32_t cx = 0, cy = 0;   // pixel of the current position
     uint32_t run_idx = 0, run_slot = 0, run_n = 0, run_t = 0;
     uint64_t my_steps = 0, my_writes = 0;
 
@@ -1798,11 +1821,12 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     if (!mask[idx]) {
                         // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
                         Q = load_quad(quad, idx);
+                        cx = ix;
+                        cy = iy;
                         run_idx = idx;
                         run_slot = slot_of(ix, iy);
                         run_n = 1u;
                         run_t = slt_inc(0.f);
-                        my_writes += 1u;
                         active = (0.f < P.max_length) && (0u < P.max_n_steps);
                     }
                 }
@@ -1815,13 +1839,16 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
         if (active) {
             const float res = P.trajectory_resolution;
             const float combo = P.combo_factor;
-            const float2 u1 = jitter_unit<C>(P, interp_quad<C>(P, Q, x, y), rng);
-            const float2 d1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+            const float inv_res = 1.0f / res;
+            const float2 u1 = jitter_unit<C>(P, interp_quad_px<C>(P, Q, x, y, cx, cy), rng);
+            const float2 d1 = make_float2(approx_r<R2>(u1.x * combo * dt, res, inv_res), approx_r<R2>(u1.y * combo * dt, res, inv_res));
             const float mx = x + d1.x, my = y + d1.y;
-            const Quad Q1 = load_quad(quad, array_idx(P, mx, my));
-            const float2 u2 = jitter_unit<C>(P, interp_quad<C>(P, Q1, mx, my), rng);
-            const float2 d2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, d1.x), res),
-                                          approx(0.5f * mad<C>(u2.y * combo, dt, d1.y), res));
+            const uint32_t mix = pix(mx, P.pad_width_pp5, P.nx_padded - 1u);
+            const uint32_t miy = pix(my, P.pad_width_pp5, P.ny_padded - 1u);
+            const Quad Q1 = load_quad(quad, miy + P.ny_padded * mix);
+            const float2 u2 = jitter_unit<C>(P, interp_quad_px<C>(P, Q1, mx, my, mix, miy), rng);
+            const float2 d2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, d1.x), res, inv_res),
+                                          approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, d1.y), res, inv_res));
             const float nx = x + d2.x, ny = y + d2.y;
             const uint32_t nix = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
             const uint32_t niy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
@@ -1835,6 +1862,8 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 x = nx;
                 y = ny;
                 Q = Qn;
+                cx = nix;
+                cy = niy;
                 const bool stuck = dl < P.integration_halt_threshold;
                 if (!stuck && !dt_fixed) {
                     StepOut o;
@@ -1844,8 +1873,6 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 }
                 l += dl;
                 n += 1u;
-                my_steps += 1u;
-                my_writes += 1u;
                 if (nidx != run_idx) {
                     f_has = true;
                     f_idx = run_idx;
@@ -1867,6 +1894,9 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
         if (__any(ended)) {
             emit(ended, run_idx, run_slot, run_n, run_t);
             if (ended) {
+                // the streamline's writes: its seed and n accepted steps
+                my_steps += n;
+                my_writes += (uint64_t)n + 1u;
                 run_n = 0;
                 run_t = 0;
             }
@@ -2062,10 +2092,12 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
 
     const uint32_t n_work = (uint32_t)w.n_work;
     int per_cu = 0;
+    bool r2 = true;
+    for (uint32_t j = 0; j < n_jobs; j++) r2 = r2 && res_is_pow2(&jobs[j].params);
     if (C0)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<true, true>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<true, true, true>, 256, 0));
     else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<false, true>, 256, 0));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<false, true, true>, 256, 0));
     if (per_cu < 1) per_cu = 1;
     if (blocks_per_cu && (int)blocks_per_cu < per_cu) per_cu = (int)blocks_per_cu;
     uint32_t grid = (uint32_t)di.cus * (uint32_t)per_cu;
@@ -2073,22 +2105,27 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (grid > max_useful) grid = max_useful ? max_useful : 1;
     unsigned long long* st = (unsigned long long*)steps_out;
     if (n_work) {
-        if (C0)
-            hipLaunchKernelGGL((k_fields_batch<true, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
-                               n_work, st, ctrl, force, chunk, slices, acc_all);
-        else
-            hipLaunchKernelGGL((k_fields_batch<false, true>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, st, ctrl, force, chunk, slices, acc_all);
+#define SLM_LAUNCH_BATCH(CC, RR)                                                                     \
+    hipLaunchKernelGGL((k_fields_batch<CC, true, RR>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, \
+                       n_jobs, n_work, st, ctrl, force, chunk, slices, acc_all)
+        if (C0) {
+            if (r2) SLM_LAUNCH_BATCH(true, true);
+            else SLM_LAUNCH_BATCH(true, false);
+        } else {
+            if (r2) SLM_LAUNCH_BATCH(false, true);
+            else SLM_LAUNCH_BATCH(false, false);
+        }
+#undef SLM_LAUNCH_BATCH
     }
     hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix, ctrl);
     hipLaunchKernelGGL(k_batch_commit, dim3(grid_for(w.n_pix)), dim3(256), 0, s, (const BatchDesc*)desc, n_jobs,
                        (size_t)w.n_pix, ctrl, accumulate ? 1u : 0u, force, (const unsigned long long*)acc_all);
     if (n_work) {
         if (C0)
-            hipLaunchKernelGGL((k_fields_batch<true, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
+            hipLaunchKernelGGL((k_fields_batch<true, false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
                                n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
         else
-            hipLaunchKernelGGL((k_fields_batch<false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
+            hipLaunchKernelGGL((k_fields_batch<false, false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
                                n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
     }
     return slm_finish(stream, flags);

@@ -57,19 +57,34 @@ __device__ __forceinline__ float approx(float v, float res) {
     const float c = (float)(int8_t)(i & 0xff);
     return (res == 128.0f) ? c * 0.0078125f : fdiv(c, res);
 }
+// approximate with the branch resolved at compile time.  R2: RES is a power of
+// two (the host checks), so c/RES == c * (1/RES) exactly (c is an integer in
+// [-128, 127] and RES <= 2^31, so nothing underflows) and no division is issued.
+template <bool R2>
+__device__ __forceinline__ float approx_r(float v, float res, float inv_res) {
+    const int32_t i = cvt_i32(v * res);
+    const float c = (float)(int8_t)(i & 0xff);
+    if constexpr (R2) return c * inv_res;
+    else return fdiv(c, res);
+}
 // compress (essentials.cl:71-74)
 __device__ __forceinline__ int8_t compress1(float v, float res) { return (int8_t)(cvt_i32(v * res) & 0xff); }
 
-// lehmer_rand_uint (rng.cl:37-45): ((u64)(uint)(s+1) * 279470273) mod (2^32-5),
-// evaluated by folding 2^32 == 5 (mod 2^32-5) twice; bit-identical to the 64-bit %.
+// lehmer_rand_uint (rng.cl:37-45): ((u64)(uint)(s+1) * 279470273) mod M with
+// M = 2^32-5, bit-identical to the 64-bit %.  With p = a*279470273 = hi*2^32 +
+// lo, 2^32 == 5 (mod M) gives p == 5*hi + lo; 5*hi < 1.4e9 fits 32 bits, and
+// the carry c of lo + 5*hi folds once more as 5c: u = (lo + 5*hi mod 2^32) +
+// 5c stays below 2^32 (c = 1 leaves the sum below 1.4e9) and == p (mod M).
+// One conditional subtraction of M finishes: u + 5 wraps to u - M exactly
+// when u >= M, so min(u, u + 5) is the residue.  One 32x32->64 multiply; the
+// rest is full-rate 32-bit arithmetic.
 __device__ __forceinline__ uint32_t lehmer(uint32_t& s) {
     const uint32_t a = s + 1u;                 // wraps in 32 bits first
-    const uint32_t lo = a * 279470273u;
-    const uint32_t hi = __umulhi(a, 279470273u);
-    uint64_t t = (uint64_t)hi * 5u + lo;       // < 2^32 + 5*2^29
-    t = (t >> 32) * 5u + (uint32_t)t;          // < 2^32 + 5
-    if (t >= 0xfffffffbull) t -= 0xfffffffbull;
-    s = (uint32_t)t;
+    const uint64_t p = (uint64_t)a * 279470273u;
+    const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
+    const uint32_t t = lo + ((hi << 2) + hi);
+    const uint32_t u = t + ((t < lo) ? 5u : 0u);
+    s = min(u, u + 5u);
     return s;
 }
 
@@ -194,11 +209,11 @@ __device__ __forceinline__ Quad load_quad(const Quad* __restrict__ q, uint32_t i
 }
 
 // speed_interpolator (essentials.cl:24-45) on a quad; same arithmetic as
-// speed_interp, so bit-identical.
+// speed_interp, so bit-identical.  interp_quad_px takes the position's pixel
+// (xl, yd) = (pix(x), pix(y)) when the caller already has it.
 template <bool C>
-__device__ __forceinline__ float2 interp_quad(const SlmParams& P, const Quad& Q, float x, float y) {
-    const uint32_t xl = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
-    const uint32_t yd = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
+__device__ __forceinline__ float2 interp_quad_px(const SlmParams& P, const Quad& Q, float x, float y, uint32_t xl,
+                                                 uint32_t yd) {
     const float rx = x - (float)xl + (float)P.pad_width;
     const float lx = 1.0f - rx;
     const float uy = y - (float)yd + (float)P.pad_width;
@@ -208,6 +223,11 @@ __device__ __forceinline__ float2 interp_quad(const SlmParams& P, const Quad& Q,
     const float s0 = mad<C>(dwn0, dy, upp0 * uy), s1 = mad<C>(dwn1, dy, upp1 * uy);
     const float inv = fdiv(1.0f, fsqrt(dot2<C>(s0, s1)));
     return make_float2(s0 * inv, s1 * inv);
+}
+template <bool C>
+__device__ __forceinline__ float2 interp_quad(const SlmParams& P, const Quad& Q, float x, float y) {
+    return interp_quad_px<C>(P, Q, x, y, pix(x, P.pad_width_pp5, P.nx_padded - 1u),
+                             pix(y, P.pad_width_pp5, P.ny_padded - 1u));
 }
 
 // Jitter of compute_step_vec_jittered (computestep.cl:96-98): add the scaled

@@ -63,6 +63,24 @@ def test_fields_bit_exact(gpu, case):
         assert np.array_equal(gslt, oslt), "slt differs at %d cells" % int((gslt != oslt).sum())
 
 
+@pytest.mark.parametrize("res,d,contract", [(100, 2, True), (64, 1, False), (96, 2, False)])
+def test_fields_other_resolutions(gpu, res, d, contract):
+    """TRAJECTORY_RESOLUTION other than 128: a power of two (64) takes the
+    exact-scaling form of approximate, others (100, 96) divide."""
+    from slm_amd import workloads
+    job = workloads.small_256(jitter=1.0, n=128, d=d, resolution=res)
+    seeds, mask, uv = _dev(job, gpu)
+    m = job.mask.astype(np.uint8).reshape(-1)
+    for sign in (1, -1):
+        p = job.params(sign, fp_contract=contract)
+        assert p.trajectory_resolution == res
+        print("case res=%d d=%d contract=%d sign=%+d" % (res, d, contract, sign), flush=True)
+        gslc, gslt, gsteps = _gpu_fields(p, seeds, mask, uv, gpu)
+        oslc, oslt, osteps = oracle.trace_fields(p, job.seeds, m, job.uv.reshape(-1))
+        assert gsteps == osteps
+        assert np.array_equal(gslc, oslc) and np.array_equal(gslt, oslt)
+
+
 @pytest.mark.parametrize("n,d,jitter", [(96, 2, 1.0), (256, 1, 1.0)])
 def test_fields_fallback_path(gpu, n, d, jitter):
     """The 32-bit fallback accumulation (taken when the packed 64-bit check
