@@ -188,15 +188,33 @@ __global__ void k_zs_final(uint32_t nxy, const uint8_t* __restrict__ a, uint32_t
 // map_channel_heads (channelheads.cl:32-93) with channelheads_runge_kutta_step
 // (rungekutta.cl:335-353: no stuck exit, dt always updated).
 template <bool C>
-__global__ void k_heads(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                        const float2* __restrict__ uv, uint32_t* __restrict__ mapping) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, dt = P.dt_max;
-    uint32_t n = 0, rng = 0, idx = array_idx(P, x, y);
-    const uint32_t prev_idx = idx;
-    while (prev_idx == idx && !mask[idx] && n < P.max_n_steps) {
+__global__ __launch_bounds__(256) void k_heads(SlmParams P, const float2* __restrict__ seeds,
+                                               const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                               uint32_t* __restrict__ mapping) {
+    float x = 0.f, y = 0.f, dt = 0.f;
+    uint32_t n = 0, rng = 0, idx = 0, prev_idx = 0;
+    CycleSkip cyc;
+    auto finish = [&]() {
+        if (n >= P.max_n_steps) return;
+        if (!mask[idx]) {
+            if (((~mapping[idx]) & F_THIN) || (mapping[prev_idx] & F_THIN)) atomicAnd(&mapping[idx], ~F_HEAD);
+        }
+    };
+    auto start = [&](uint32_t k) -> bool {
+        const float2 s = seeds[k];
+        x = s.x;
+        y = s.y;
+        dt = P.dt_max;
+        n = 0;
+        idx = prev_idx = array_idx(P, x, y);
+        cyc.reset();
+        return true;
+    };
+    auto step = [&]() -> bool {
+        if (!(prev_idx == idx && !mask[idx] && n < P.max_n_steps)) {
+            finish();
+            return false;
+        }
         StepOut o;
         compute_step<C, false>(P, uv, dt, x, y, rng, o);
         idx = o.idx;
@@ -204,11 +222,12 @@ __global__ void k_heads(SlmParams P, const float2* __restrict__ seeds, const uin
         map_move<C>(P, o, x, y, idx);
         n++;
         dt = next_dt(P, dt, err);
-    }
-    if (n >= P.max_n_steps) return;
-    if (!mask[idx]) {
-        if (((~mapping[idx]) & F_THIN) || (mapping[prev_idx] & F_THIN)) atomicAnd(&mapping[idx], ~F_HEAD);
-    }
+        // a repeated state inside the seed pixel never leaves it: the trace
+        // runs to MAX_N_STEPS and changes nothing
+        if (prev_idx == idx && cyc.check(x, y, dt, 0u, n, P.max_n_steps)) return false;
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 // --------------------------------------------------------- count_downchannels
@@ -497,12 +516,13 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
 
 // segment_runge_kutta_step (rungekutta.cl:435-459); probe = the pixel of
 // compute_step_vec's next point, before the dxy1 retry.
-template <bool C>
+template <bool C, bool R2>
 __device__ __forceinline__ bool segment_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
-                                             float& x, float& y, uint32_t& idx, uint32_t& n, uint32_t& probe) {
+                                             float& x, float& y, uint32_t& idx, uint32_t& n, uint32_t& probe,
+                                             float inv_res) {
     StepOut o;
     uint32_t rng = 0;
-    compute_step<C, false>(P, uv, dt, x, y, rng, o);
+    compute_step<C, false, R2>(P, uv, dt, x, y, rng, o, inv_res);
     probe = o.idx;
     const float err = step_error<C>(P, o);
     const float dl = map_move<C>(P, o, x, y, idx);
@@ -513,19 +533,38 @@ __device__ __forceinline__ bool segment_step(const SlmParams& P, const float2* _
 }
 
 // segment_hillslopes (segment.cl:133-188).
-template <bool C>
-__global__ void k_seg_hs(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                         const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
-                         uint32_t* __restrict__ label) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, dt = P.dt_max;
-    uint32_t n = 0, probe, idx = array_idx(P, x, y);
-    const uint32_t hs = idx;
-    while (!mask[idx] && ((~mapping[idx]) & F_THIN) && n < P.max_n_steps)
-        if (segment_step<C>(P, uv, dt, x, y, idx, n, probe)) break;
-    if (mapping[idx] & F_THIN) label[hs] = label[idx];
+template <bool C, bool R2>
+__global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __restrict__ seeds,
+                                                const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                                const uint32_t* __restrict__ mapping,
+                                                uint32_t* __restrict__ label) {
+    const float inv_res = 1.0f / P.trajectory_resolution;
+    float x = 0.f, y = 0.f, dt = 0.f;
+    uint32_t n = 0, probe = 0, idx = 0, hs = 0;
+    CycleSkip cyc;
+    auto finish = [&]() {
+        if (mapping[idx] & F_THIN) label[hs] = label[idx];
+    };
+    auto start = [&](uint32_t k) -> bool {
+        const float2 s = seeds[k];
+        x = s.x;
+        y = s.y;
+        dt = P.dt_max;
+        n = 0;
+        cyc.reset();
+        idx = hs = array_idx(P, x, y);
+        return true;
+    };
+    auto step = [&]() -> bool {
+        if (!(!mask[idx] && ((~mapping[idx]) & F_THIN) && n < P.max_n_steps) ||
+            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res)) {
+            finish();
+            return false;
+        }
+        cyc.check(x, y, dt, 0u, n, P.max_n_steps);
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 // rotate_left (segment.cl:195-200) in char arithmetic (abs of a char is a
@@ -583,50 +622,87 @@ __global__ void k_sub_edges(SlmParams P, const float2* __restrict__ seeds, const
 }
 
 // subsegment_flanks (segment.cl:360-414).
-template <bool C>
-__global__ void k_sub_flanks(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                             const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
-                             uint32_t* __restrict__ label) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, dt = P.dt_max;
-    uint32_t n = 0, probe, idx = array_idx(P, x, y);
-    const uint32_t hs = idx;
-    while (!mask[idx] && !(mapping[idx] & F_LEFT) && !(mapping[idx] & F_THIN) && n < P.max_n_steps)
-        if (segment_step<C>(P, uv, dt, x, y, idx, n, probe)) break;
-    if (mapping[idx] & F_LEFT) label[hs] |= LFA;
+template <bool C, bool R2>
+__global__ __launch_bounds__(256) void k_sub_flanks(SlmParams P, const float2* __restrict__ seeds,
+                                                    const uint8_t* __restrict__ mask,
+                                                    const float2* __restrict__ uv,
+                                                    const uint32_t* __restrict__ mapping,
+                                                    uint32_t* __restrict__ label) {
+    const float inv_res = 1.0f / P.trajectory_resolution;
+    float x = 0.f, y = 0.f, dt = 0.f;
+    uint32_t n = 0, probe = 0, idx = 0, hs = 0;
+    CycleSkip cyc;
+    auto start = [&](uint32_t k) -> bool {
+        const float2 s = seeds[k];
+        x = s.x;
+        y = s.y;
+        dt = P.dt_max;
+        n = 0;
+        cyc.reset();
+        idx = hs = array_idx(P, x, y);
+        return true;
+    };
+    auto step = [&]() -> bool {
+        if (!(!mask[idx] && !(mapping[idx] & F_LEFT) && !(mapping[idx] & F_THIN) && n < P.max_n_steps) ||
+            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res)) {
+            if (mapping[idx] & F_LEFT) label[hs] |= LFA;
+            return false;
+        }
+        cyc.check(x, y, dt, 0u, n, P.max_n_steps);
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 // fix_right_flanks (segment.cl:441-504) / fix_left_flanks (:530-594); other
 // pixels' labels are read from the snapshot taken before the launch.
-template <bool C, bool LEFT>
-__global__ void k_fix(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                      const float2* __restrict__ uv, const uint32_t* __restrict__ mapping,
-                      const uint32_t* __restrict__ snap, uint32_t* __restrict__ label) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, dt = P.dt_max;
-    uint32_t n = 0, idx = array_idx(P, x, y);
-    const uint32_t hs = idx;
-    uint32_t own = snap[hs];
+template <bool C, bool LEFT, bool R2>
+__global__ __launch_bounds__(256) void k_fix(SlmParams P, const float2* __restrict__ seeds,
+                                             const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                             const uint32_t* __restrict__ mapping,
+                                             const uint32_t* __restrict__ snap, uint32_t* __restrict__ label) {
+    const float inv_res = 1.0f / P.trajectory_resolution;
+    float x = 0.f, y = 0.f, dt = 0.f;
+    uint32_t n = 0, idx = 0, hs = 0, own = 0;
     auto lab = [&](uint32_t q) -> uint32_t { return q == hs ? own : snap[q]; };
-    for (;;) {
+    CycleSkip cyc;
+    auto finish = [&]() {
+        if ((own & ~LFA) != (lab(idx) & ~LFA)) own = lab(idx);
+        if (LEFT) {
+            if (!(lab(idx) & LFA) && !(mapping[idx] & F_THIN)) own &= ~LFA;
+        } else {
+            if (lab(idx) & LFA) own |= LFA;
+        }
+        label[hs] = own;
+    };
+    auto start = [&](uint32_t k) -> bool {
+        const float2 s = seeds[k];
+        x = s.x;
+        y = s.y;
+        dt = P.dt_max;
+        n = 0;
+        cyc.reset();
+        idx = hs = array_idx(P, x, y);
+        own = snap[hs];
+        return true;
+    };
+    auto step = [&]() -> bool {
         const bool side = LEFT ? ((lab(idx) & LFA) != 0u) : ((lab(idx) & LFA) == 0u);
-        if (!(!mask[idx] && side && !(mapping[idx] & F_THIN) && n < P.max_n_steps)) break;
+        if (!(!mask[idx] && side && !(mapping[idx] & F_THIN) && n < P.max_n_steps)) {
+            finish();
+            return false;
+        }
         uint32_t probe;
-        const bool stuck = segment_step<C>(P, uv, dt, x, y, idx, n, probe);
+        const bool stuck = segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res);
         if ((own & ~LFA) != (lab(probe) & ~LFA)) own = lab(probe);
-        if (stuck) break;
-    }
-    if ((own & ~LFA) != (lab(idx) & ~LFA)) own = lab(idx);
-    if (LEFT) {
-        if (!(lab(idx) & LFA) && !(mapping[idx] & F_THIN)) own &= ~LFA;
-    } else {
-        if (lab(idx) & LFA) own |= LFA;
-    }
-    label[hs] = own;
+        if (stuck) {
+            finish();
+            return false;
+        }
+        cyc.check(x, y, dt, own, n, P.max_n_steps);
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 // ------------------------------------------------------------------ host side
@@ -696,30 +772,107 @@ __global__ void k_dilate8(const uint8_t* __restrict__ a, uint8_t* __restrict__ b
     b[i] = v;
 }
 
-// binary_fill_holes: background reached from outside the grid through
-// 4-neighbours (the dilation of scipy's fill with border value 1), one step
-__global__ void k_fill_step(const uint8_t* __restrict__ fat, const uint8_t* __restrict__ rin,
-                            uint8_t* __restrict__ rout, uint32_t nx, uint32_t ny, uint32_t* __restrict__ changed) {
+// binary_fill_holes: the background reached from outside the grid through
+// 4-neighbours (scipy's fill dilates the inverted image from a border of 1s);
+// the rest of the background is a hole.  Bit-packed rows: word x*nwy + w, bit
+// b is pixel (x, 64w + b).  One wave owns a 64x64 tile, one row word per
+// lane, and closes the tile with Kogge-Stone occluded fills (the shift-and-
+// propagate doubling of bitboard flood fills): along a row within the word,
+// across rows with lane shuffles, alternating until nothing grows.  The tile
+// is seeded by the grid border and by its neighbours' current reach bits;
+// launches repeat until no tile changes.  Reach only grows, so tiles may read
+// neighbours that are being updated: the fixpoint, the set of background
+// pixels 4-connected to an open border pixel, does not depend on the order.
+
+// background bits; bits past ny are 0
+__global__ void k_pack_open(const uint8_t* __restrict__ fat, uint32_t nx, uint32_t ny, uint32_t nwy,
+                            u64* __restrict__ open, u64* __restrict__ reach) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool grew = false;
-    if (i < nx * ny) {
-        uint8_t r = 0;
-        if (!fat[i]) {
-            const uint32_t x = i / ny, y = i % ny;
-            r = rin[i] || x == 0 || y == 0 || x + 1 == nx || y + 1 == ny || rin[i - ny] || rin[i + ny] ||
-                rin[i - 1] || rin[i + 1];
-        }
-        rout[i] = r;
-        grew = r && !rin[i];
-    }
-    const uint64_t bal = __ballot(grew);
-    if (bal && (threadIdx.x & 63) == (uint32_t)(__ffsll((long long)bal) - 1))
-        atomicAdd(changed, (uint32_t)__popcll(bal));
+    if (i >= nx * nwy) return;
+    const uint32_t x = i / nwy, w = i % nwy;
+    const uint32_t y0 = w * 64u, y1 = min(ny, y0 + 64u);
+    const uint8_t* row = fat + (size_t)x * ny;
+    u64 o = 0;
+    for (uint32_t y = y0; y < y1; y++)
+        if (!row[y]) o |= 1ull << (y - y0);
+    open[i] = o;
+    reach[i] = 0;
 }
 
-__global__ void k_fill_final(uint8_t* __restrict__ fat, const uint8_t* __restrict__ reach, uint32_t nxy) {
+__device__ __forceinline__ u64 fill_row(u64 x, u64 o) {
+    u64 a = x, g = o;                      // towards higher y
+    a |= g & (a << 1);  g &= g << 1;
+    a |= g & (a << 2);  g &= g << 2;
+    a |= g & (a << 4);  g &= g << 4;
+    a |= g & (a << 8);  g &= g << 8;
+    a |= g & (a << 16); g &= g << 16;
+    a |= g & (a << 32);
+    u64 b = x, h = o;                      // towards lower y
+    b |= h & (b >> 1);  h &= h >> 1;
+    b |= h & (b >> 2);  h &= h >> 2;
+    b |= h & (b >> 4);  h &= h >> 4;
+    b |= h & (b >> 8);  h &= h >> 8;
+    b |= h & (b >> 16); h &= h >> 16;
+    b |= h & (b >> 32);
+    return a | b;
+}
+
+// the same across the 64 lanes (rows) of the wave
+__device__ __forceinline__ u64 fill_col(u64 x, u64 o, uint32_t lane) {
+    u64 a = x, g = o, b = x, h = o;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        u64 as = __shfl_up(a, d, 64), gs = __shfl_up(g, d, 64);
+        if (lane < d) as = gs = 0;
+        a |= g & as;
+        g &= gs;
+        u64 bs = __shfl_down(b, d, 64), hs = __shfl_down(h, d, 64);
+        if (lane + d > 63u) bs = hs = 0;
+        b |= h & bs;
+        h &= hs;
+    }
+    return a | b;
+}
+
+__global__ __launch_bounds__(256) void k_fill_tiles(const u64* __restrict__ open, u64* __restrict__ reach,
+                                                     uint32_t nx, uint32_t ny, uint32_t nwy, uint32_t n_tiles,
+                                                     uint32_t* __restrict__ changed) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (t >= n_tiles) return;                       // whole waves leave together
+    const uint32_t tx = t / nwy, w = t % nwy;
+    const uint32_t x0 = tx * 64u, x = x0 + lane;
+    const bool row_ok = x < nx;
+    const size_t wi = (size_t)x * nwy + w;
+    const u64 o = row_ok ? open[wi] : 0ull;
+    const u64 r_in = row_ok ? reach[wi] : 0ull;
+    u64 seed = 0;
+    if (row_ok) {
+        if (x == 0 || x + 1 == nx) seed = o;        // first and last grid rows
+        if (w == 0) seed |= o & 1ull;               // y = 0
+        if (w + 1 == nwy) seed |= o & (1ull << ((ny - 1u) & 63u));   // y = ny - 1
+        if (w > 0 && (reach[wi - 1] >> 63)) seed |= o & 1ull;
+        if (w + 1 < nwy && (reach[wi + 1] & 1ull)) seed |= o & (1ull << 63);
+        if (lane == 0 && x0 > 0) seed |= o & reach[wi - nwy];
+        if (lane == 63u && x + 1 < nx) seed |= o & reach[wi + nwy];
+    }
+    u64 r = (r_in | seed) & o;
+    while (true) {
+        const u64 prev = r;
+        r = fill_col(fill_row(r, o), o, lane);
+        if (!__any(r != prev)) break;
+    }
+    const bool grew = r != r_in;
+    if (row_ok && grew) reach[wi] = r;
+    if (__any(grew) && lane == 0) atomicAdd(changed, 1u);
+}
+
+__global__ void k_fill_final(uint8_t* __restrict__ fat, const u64* __restrict__ reach, uint32_t nx, uint32_t ny,
+                             uint32_t nwy) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nxy) fat[i] = (fat[i] || !reach[i]) ? 1 : 0;
+    if (i >= nx * ny) return;
+    const uint32_t x = i / ny, y = i % ny;
+    const bool reached = (reach[(size_t)x * nwy + (y >> 6)] >> (y & 63u)) & 1ull;
+    fat[i] = (fat[i] || !reached) ? 1 : 0;
 }
 
 __global__ void k_flag_marks(const uint8_t* __restrict__ a, uint32_t nxy, uint32_t flag,
@@ -756,6 +909,7 @@ inline dim3 seed_grid(const SlmParams* p) {
 }
 inline dim3 pixel_grid(const SlmParams* p) { return dim3((p->nxy_padded + kBlock - 1) / kBlock); }
 inline uint32_t n_work(const SlmParams* p) { return p->n_seed_points - p->seeds_chunk_offset; }
+
 
 inline void zero_u64(u64* a, size_t n, hipStream_t s) {
     const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
@@ -841,7 +995,12 @@ extern "C" int slm_map_channel_heads(const SlmParams* p, const float* seeds_xy, 
     if (!n_work(p)) return slm_finish(stream, flags);
     if (!mask || !uv || !mapping) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
-    LAUNCH_C(k_heads, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping);
+    if (p->fp_contract)
+        hipLaunchKernelGGL(k_heads<true>, persist_grid(k_heads<true>, n_work(p)), dim3(kBlock), 0, s, SEED_ARGS(p),
+                           mask, (const float2*)uv, mapping);
+    else
+        hipLaunchKernelGGL(k_heads<false>, persist_grid(k_heads<false>, n_work(p)), dim3(kBlock), 0, s, SEED_ARGS(p),
+                           mask, (const float2*)uv, mapping);
     return slm_finish(stream, flags);
 }
 
@@ -946,7 +1105,18 @@ extern "C" int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy,
     if (!n_work(p)) return slm_finish(stream, flags);
     if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
-    LAUNCH_C(k_seg_hs, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping, label);
+#define SLM_LAUNCH_SEG(CC, RR)                                                                              \
+    hipLaunchKernelGGL((k_seg_hs<CC, RR>), persist_grid(k_seg_hs<CC, RR>, n_work(p)), dim3(kBlock), 0, s, SEED_ARGS(p), mask, \
+                       (const float2*)uv, mapping, label)
+    const bool r2 = slm_res_is_pow2(p);
+    if (p->fp_contract) {
+        if (r2) SLM_LAUNCH_SEG(true, true);
+        else SLM_LAUNCH_SEG(true, false);
+    } else {
+        if (r2) SLM_LAUNCH_SEG(false, true);
+        else SLM_LAUNCH_SEG(false, false);
+    }
+#undef SLM_LAUNCH_SEG
     return slm_finish(stream, flags);
 }
 
@@ -970,7 +1140,18 @@ extern "C" int slm_subsegment_flanks(const SlmParams* p, const float* seeds_xy, 
     if (!n_work(p)) return slm_finish(stream, flags);
     if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
-    LAUNCH_C(k_sub_flanks, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, mapping, label);
+#define SLM_LAUNCH_SUB(CC, RR)                                                                                 \
+    hipLaunchKernelGGL((k_sub_flanks<CC, RR>), persist_grid(k_sub_flanks<CC, RR>, n_work(p)), dim3(kBlock), 0, s, SEED_ARGS(p), \
+                       mask, (const float2*)uv, mapping, label)
+    const bool r2 = slm_res_is_pow2(p);
+    if (p->fp_contract) {
+        if (r2) SLM_LAUNCH_SUB(true, true);
+        else SLM_LAUNCH_SUB(true, false);
+    } else {
+        if (r2) SLM_LAUNCH_SUB(false, true);
+        else SLM_LAUNCH_SUB(false, false);
+    }
+#undef SLM_LAUNCH_SUB
     return slm_finish(stream, flags);
 }
 
@@ -985,20 +1166,30 @@ static int fix_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* 
     hipStream_t s = (hipStream_t)stream;
     uint32_t* snap = (uint32_t*)ws_a(workspace);
     HIP_TRY(hipMemcpyAsync(snap, label, (size_t)4 * p->nxy_padded, hipMemcpyDeviceToDevice, s));
-    const dim3 g = seed_grid(p);
     const float2* sd = (const float2*)seeds_xy;
     const float2* u = (const float2*)uv;
+#define SLM_LAUNCH_FIX(CC, LL, RR)                                                                          \
+    hipLaunchKernelGGL((k_fix<CC, LL, RR>), persist_grid(k_fix<CC, LL, RR>, n_work(p)), dim3(kBlock), 0, s, *p, sd, mask, u, \
+                       mapping, snap, label)
+    const bool r2 = slm_res_is_pow2(p);
     if (p->fp_contract) {
-        if (left)
-            hipLaunchKernelGGL((k_fix<true, true>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
-        else
-            hipLaunchKernelGGL((k_fix<true, false>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+        if (left) {
+            if (r2) SLM_LAUNCH_FIX(true, true, true);
+            else SLM_LAUNCH_FIX(true, true, false);
+        } else {
+            if (r2) SLM_LAUNCH_FIX(true, false, true);
+            else SLM_LAUNCH_FIX(true, false, false);
+        }
     } else {
-        if (left)
-            hipLaunchKernelGGL((k_fix<false, true>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
-        else
-            hipLaunchKernelGGL((k_fix<false, false>), g, dim3(kBlock), 0, s, *p, sd, mask, u, mapping, snap, label);
+        if (left) {
+            if (r2) SLM_LAUNCH_FIX(false, true, true);
+            else SLM_LAUNCH_FIX(false, true, false);
+        } else {
+            if (r2) SLM_LAUNCH_FIX(false, false, true);
+            else SLM_LAUNCH_FIX(false, false, false);
+        }
     }
+#undef SLM_LAUNCH_FIX
     return slm_finish(stream, flags);
 }
 
@@ -1020,9 +1211,13 @@ namespace {
 inline size_t al256(size_t n) { return (n + 255) & ~(size_t)255; }
 }  // namespace
 
+// bytes of one bit-packed grid (64-pixel words along y)
+static size_t bits_bytes(uint32_t nx, uint32_t ny) { return (size_t)nx * ((ny + 63u) / 64u) * 8u; }
+
 extern "C" size_t slm_slopes_workspace_size(uint32_t nx, uint32_t ny) {
     const size_t n = (size_t)nx * ny;
-    return 4 * al256(n * 4) + 6 * al256(n) + 512;
+    const size_t nb = std::max(al256(n), al256(bits_bytes(nx, ny)));
+    return 4 * al256(n * 4) + 4 * al256(n) + 2 * nb + 512;
 }
 
 extern "C" int slm_gaussian_filter(const float* in, float* out, float* tmp, uint32_t nx, uint32_t ny,
@@ -1057,9 +1252,10 @@ extern "C" int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, u
     uint8_t* ridge = mid + al256(n);
     uint8_t* fat = ridge + al256(n);
     uint8_t* scr = fat + al256(n);
+    const size_t nb = std::max(al256(n), al256(bits_bytes(nx, ny)));
     uint8_t* ra = scr + al256(n);
-    uint8_t* rb = ra + al256(n);
-    uint32_t* ctr = (uint32_t*)(rb + al256(n));
+    uint8_t* rb = ra + nb;
+    uint32_t* ctr = (uint32_t*)(rb + nb);
     hipStream_t s = (hipStream_t)stream;
     const dim3 grid((uint32_t)((n + kBlock - 1) / kBlock));
     hipLaunchKernelGGL(k_sla_angle, grid, dim3(kBlock), 0, s, (const float2*)sla, (uint32_t)n, ang);
@@ -1072,25 +1268,28 @@ extern "C" int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, u
     // ridges: dilate twice, fill holes
     hipLaunchKernelGGL(k_dilate8, grid, dim3(kBlock), 0, s, (const uint8_t*)ridge, scr, nx, ny);
     hipLaunchKernelGGL(k_dilate8, grid, dim3(kBlock), 0, s, (const uint8_t*)scr, fat, nx, ny);
-    zero_u64((u64*)ra, (al256(n)) / 8, s);
-    uint8_t *rin = ra, *rout = rb;
-    const uint32_t max_steps = nx * 2 + ny * 2 + 4;
-    uint32_t steps = 0;
-    for (;;) {
-        zero_u64((u64*)ctr, 1, s);
-        for (int k = 0; k < 8; k++) {            // 8 steps between convergence checks
-            hipLaunchKernelGGL(k_fill_step, grid, dim3(kBlock), 0, s, (const uint8_t*)fat, (const uint8_t*)rin, rout,
-                               nx, ny, ctr);
-            std::swap(rin, rout);
+    {
+        const uint32_t nwy = (ny + 63u) / 64u;
+        const uint32_t n_tiles = ((nx + 63u) / 64u) * nwy;
+        const uint32_t n_words = nx * nwy;
+        u64* open = (u64*)ra;
+        u64* reach = (u64*)rb;
+        hipLaunchKernelGGL(k_pack_open, dim3((n_words + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const uint8_t*)fat,
+                           nx, ny, nwy, open, reach);
+        const uint32_t max_launches = 4u * n_tiles + 16u;
+        for (uint32_t launches = 0;; launches++) {
+            if (launches > max_launches)
+                return slm_fail(SLM_E_HIP, "hole filling did not converge in %u launches", launches);
+            zero_u64((u64*)ctr, 1, s);
+            hipLaunchKernelGGL(k_fill_tiles, dim3((n_tiles + 3u) / 4u), dim3(256), 0, s, (const u64*)open, reach, nx,
+                               ny, nwy, n_tiles, ctr);
+            uint32_t grown = 0;
+            HIP_TRY(hipMemcpyAsync(&grown, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (!grown) break;
         }
-        steps += 8;
-        uint32_t grown = 0;
-        HIP_TRY(hipMemcpyAsync(&grown, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        if (!grown) break;
-        if (steps > max_steps) return slm_fail(SLM_E_HIP, "hole filling did not converge in %u steps", steps);
+        hipLaunchKernelGGL(k_fill_final, grid, dim3(kBlock), 0, s, fat, (const u64*)reach, nx, ny, nwy);
     }
-    hipLaunchKernelGGL(k_fill_final, grid, dim3(kBlock), 0, s, fat, (const uint8_t*)rin, (uint32_t)n);
     // skeletons
     int rc;
     if ((rc = zs_thin(mid, scr, nx, ny, ctr, s, nullptr))) return rc;

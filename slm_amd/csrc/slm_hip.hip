@@ -594,15 +594,6 @@ static uint32_t dt_is_fixed(const SlmParams* p) {
                ? 1u : 0u;
 }
 
-// TRAJECTORY_RESOLUTION is a power of two in [1, 2^31]: approximate's division
-// is then an exact scaling (approx_r<true>)
-static bool res_is_pow2(const SlmParams* p) {
-    const float r = p->trajectory_resolution;
-    if (!(r >= 1.0f && r <= 2147483648.0f)) return false;
-    const uint32_t u = (uint32_t)r;
-    return (float)u == r && (u & (u - 1u)) == 0u;
-}
-
 __global__ void k_fields_init(FieldsCtrl* ctrl) {
     if (threadIdx.x < kQueues) ctrl->queue[threadIdx.x * kQueueStride] = 0u;
     if (threadIdx.x == 0) {
@@ -716,7 +707,7 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
     const uint32_t n_work = n * d * d;
     // persistent grid: as many 256-thread blocks as can be resident at once
     int per_cu = 0;
-    const bool r2 = res_is_pow2(p);
+    const bool r2 = slm_res_is_pow2(p);
     if (p->fp_contract)
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields<true, true, true>, 256, 0));
     else
@@ -903,23 +894,38 @@ extern "C" int slm_trace_paths(const SlmParams* p, const float* seeds_xy, const 
 // hillslope_lengths (lengths.cl:37-116) with lengths_runge_kutta_step
 // (rungekutta.cl:492-516): one lane per mid-slope/ridge seed.
 template <bool C>
-__global__ void k_hsl(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                      const float2* __restrict__ uv, const uint32_t* __restrict__ mapping, uint32_t thin_flag,
-                      uint32_t from_flag, float* __restrict__ length) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, l = 0.f, dt = P.dt_max;
-    uint32_t n = 0, rng = 0;
+__global__ __launch_bounds__(256) void k_hsl(SlmParams P, const float2* __restrict__ seeds,
+                                             const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                             const uint32_t* __restrict__ mapping, uint32_t thin_flag,
+                                             uint32_t from_flag, float* __restrict__ length) {
+    float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
+    uint32_t n = 0, rng = 0, idx = 0, k = 0;
     bool moved_off = false;
-    uint32_t idx = array_idx(P, x, y);
-    while (((~mapping[idx]) & thin_flag) && n < P.max_n_steps) {
+    CycleSkip cyc;
+    auto start = [&](uint32_t kk) -> bool {
+        const float2 s = seeds[kk];
+        k = kk;
+        x = s.x;
+        y = s.y;
+        l = 0.f;
+        dt = P.dt_max;
+        n = 0;
+        moved_off = false;
+        idx = array_idx(P, x, y);
+        cyc.reset();
+        return true;
+    };
+    auto step = [&]() -> bool {
+        if (!(((~mapping[idx]) & thin_flag) && n < P.max_n_steps)) {
+            if (mapping[idx] & thin_flag) length[k] = l * P.pixel_size;
+            return false;
+        }
         StepOut o;
         compute_step<C, false>(P, uv, dt, x, y, rng, o);
         idx = o.idx;
-        if (mask[idx]) return;
+        if (mask[idx]) return false;
         if (!moved_off && ((~mapping[idx]) & from_flag)) moved_off = true;
-        else if (moved_off && (mapping[idx] & from_flag)) return;
+        else if (moved_off && (mapping[idx] & from_flag)) return false;
         const float err = step_error<C>(P, o);
         float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
         float2 nx = o.next;
@@ -930,12 +936,17 @@ __global__ void k_hsl(SlmParams P, const float2* __restrict__ seeds, const uint8
         x = nx.x;
         y = nx.y;
         idx = array_idx(P, x, y);
-        if (dl < P.integration_halt_threshold) return;
+        if (dl < P.integration_halt_threshold) return false;
         dt = next_dt(P, dt, err);
         l += dl;
         n += 1u;
-    }
-    if (mapping[idx] & thin_flag) length[k] = l * P.pixel_size;
+        // A repeated state (position, dt, moved_off) comes back around a cycle
+        // of non-thin pixels: the trace would run to MAX_N_STEPS and end off
+        // the thin pixels, writing nothing.
+        if (cyc.check(x, y, dt, moved_off ? 1u : 0u, n, P.max_n_steps)) return false;
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 extern "C" int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
@@ -948,11 +959,11 @@ extern "C" int slm_hillslope_lengths(const SlmParams* p, const float* seeds_xy, 
     if (!seeds_xy || !mask || !uv || !mapping || !length) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
     if (p->fp_contract)
-        hipLaunchKernelGGL(k_hsl<true>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
-                           (const float2*)uv, mapping, thin_flag, from_flag, length);
+        hipLaunchKernelGGL(k_hsl<true>, persist_grid(k_hsl<true>, n), dim3(256), 0, s, *p, (const float2*)seeds_xy,
+                           mask, (const float2*)uv, mapping, thin_flag, from_flag, length);
     else
-        hipLaunchKernelGGL(k_hsl<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
-                           (const float2*)uv, mapping, thin_flag, from_flag, length);
+        hipLaunchKernelGGL(k_hsl<false>, persist_grid(k_hsl<false>, n), dim3(256), 0, s, *p, (const float2*)seeds_xy,
+                           mask, (const float2*)uv, mapping, thin_flag, from_flag, length);
     return slm_finish(stream, flags);
 }
 
@@ -2093,7 +2104,7 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     const uint32_t n_work = (uint32_t)w.n_work;
     int per_cu = 0;
     bool r2 = true;
-    for (uint32_t j = 0; j < n_jobs; j++) r2 = r2 && res_is_pow2(&jobs[j].params);
+    for (uint32_t j = 0; j < n_jobs; j++) r2 = r2 && slm_res_is_pow2(&jobs[j].params);
     if (C0)
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<true, true, true>, 256, 0));
     else

@@ -135,10 +135,12 @@ struct StepOut {
     uint32_t idx;
 };
 
-// compute_step_vec (computestep.cl:42-57) / compute_step_vec_jittered (:86-106)
-template <bool C, bool JITTER>
+// compute_step_vec (computestep.cl:42-57) / compute_step_vec_jittered (:86-106).
+// R2: TRAJECTORY_RESOLUTION is a power of two and inv_res its reciprocal.
+template <bool C, bool JITTER, bool R2 = false>
 __device__ __forceinline__ void compute_step(const SlmParams& P, const float2* __restrict__ uv,
-                                             float dt, float x, float y, uint32_t& rng, StepOut& o) {
+                                             float dt, float x, float y, uint32_t& rng, StepOut& o,
+                                             float inv_res = 0.0f) {
     const float res = P.trajectory_resolution;
     const float combo = P.combo_factor;
     float2 u1 = speed_interp<C>(P, uv, x, y);
@@ -151,7 +153,7 @@ __device__ __forceinline__ void compute_step(const SlmParams& P, const float2* _
         u1.y = fdiv(u1.y, len);
     }
     o.uv1 = u1;
-    o.dxy1 = make_float2(approx(u1.x * combo * dt, res), approx(u1.y * combo * dt, res));
+    o.dxy1 = make_float2(approx_r<R2>(u1.x * combo * dt, res, inv_res), approx_r<R2>(u1.y * combo * dt, res, inv_res));
     float2 u2 = speed_interp<C>(P, uv, x + o.dxy1.x, y + o.dxy1.y);
     if constexpr (JITTER) {
         const float2 r = rand_vec(rng);
@@ -161,8 +163,8 @@ __device__ __forceinline__ void compute_step(const SlmParams& P, const float2* _
         u2.x = fdiv(u2.x, len);
         u2.y = fdiv(u2.y, len);
     }
-    o.dxy2 = make_float2(approx(0.5f * mad<C>(u2.x * combo, dt, o.dxy1.x), res),
-                         approx(0.5f * mad<C>(u2.y * combo, dt, o.dxy1.y), res));
+    o.dxy2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, o.dxy1.x), res, inv_res),
+                         approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, o.dxy1.y), res, inv_res));
     o.next = make_float2(x + o.dxy2.x, y + o.dxy2.y);
     o.idx = array_idx(P, o.next.x, o.next.y);
 }
@@ -239,6 +241,91 @@ __device__ __forceinline__ float2 jitter_unit(const SlmParams& P, float2 u, uint
     u.y = mad<C>(r.y, P.jitter_magnitude, u.y);
     const float len = fast_length<C>(u.x, u.y);
     return make_float2(fdiv(u.x, len), fdiv(u.y, len));
+}
+
+
+// A deterministic trace whose state (position, dt, and any carried label)
+// repeats is periodic from there on: every exit test of the loop is a function
+// of that state and fails around the whole cycle, so the trace runs on to
+// MAX_N_STEPS.  Brent's algorithm finds the period lam; whole laps are then
+// skipped (n += laps*lam) and the last partial lap is stepped as before, so
+// the trace ends at the same step, pixel and label.  Positions live on the
+// 1/TRAJECTORY_RESOLUTION lattice, so traces caught in pits and flats do cycle.
+struct CycleSkip {
+    float sx, sy, sdt;
+    uint32_t sown, sn, power;
+    bool on;
+    __device__ __forceinline__ void reset() {
+        sx = sy = sdt = -1.0f;
+        sown = 0u;
+        sn = 0u;
+        power = 1u;
+        on = true;
+    }
+    // true when the state repeats; n then jumps over the whole laps to cap
+    __device__ __forceinline__ bool check(float x, float y, float dt, uint32_t own, uint32_t& n, uint32_t cap) {
+        if (!on) return false;
+        if (x == sx && y == sy && dt == sdt && own == sown && n > sn) {
+            const uint32_t lam = n - sn;
+            n += ((cap - n) / lam) * lam;
+            on = false;
+            return true;
+        }
+        if (n - sn >= power) {
+            sx = x;
+            sy = y;
+            sdt = dt;
+            sown = own;
+            sn = n;
+            power <<= 1;
+        }
+        return false;
+    }
+};
+
+// Persistent lane refill for per-seed traces.  One thread per seed leaves a
+// wave running as long as its longest trace, and these traces run from 1 to
+// MAX_N_STEPS steps.  Here a wave takes 64-seed chunks g, g + G, g + 2G, ...
+// (g its index, G the number of waves; no shared counter, so calls on
+// different streams never interact) and a lane whose trace ends starts the
+// next seed of the chunk at once.  start(k) sets a lane up for seed k and
+// returns whether it has a trace to run; step() advances it one iteration
+// and returns false once the trace has finished (and written its result).
+// Every trace is independent, so the order changes no result.
+template <typename Start, typename Step>
+__device__ __forceinline__ void persist_seeds(uint32_t first, uint32_t n_items, Start start, Step step) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lane ? (~0ull >> (64u - lane)) : 0ull;
+    const uint32_t G = gridDim.x * (blockDim.x >> 6);
+    uint32_t chunk = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    uint32_t c_next = chunk < (n_items + 63u) / 64u ? chunk * 64u : n_items;
+    uint32_t c_end = min(n_items, c_next + 64u);
+    bool active = false;
+    while (true) {
+        uint64_t need = __ballot(!active);
+        while (need && c_next < n_items) {
+            if (c_next >= c_end) {
+                chunk += G;
+                if (chunk >= (n_items + 63u) / 64u) {
+                    c_next = n_items;
+                    break;
+                }
+                c_next = chunk * 64u;
+                c_end = min(n_items, c_next + 64u);
+            }
+            const uint32_t take = min((uint32_t)__popcll(need), c_end - c_next);
+            const uint32_t rank = __popcll(need & lt);
+            const bool mine = ((need >> lane) & 1ull) && rank < take;
+            if (mine) active = start(first + c_next + rank);
+            need &= ~__ballot(mine);
+            c_next += take;
+        }
+        if (!__any(active)) {
+            if (c_next >= n_items) break;
+            continue;
+        }
+        if (active) active = step();
+    }
 }
 
 }  // namespace slm

@@ -31,22 +31,38 @@ def _case(kind):
         job, m = workloads.v_valley(n=96)
         m = channels_from_mapping(m, job.info)
         slt = np.zeros(m.shape, np.float32)
-    elif kind == "y":
+    elif kind in ("y", "y100"):
         job, m = workloads.y_valley(n=128, channel_width=3)
         slt = np.random.default_rng(1).random(m.shape).astype(np.float32)
+        if kind == "y100":
+            # a TRAJECTORY_RESOLUTION that is not a power of two: the traces
+            # divide in approximate instead of scaling
+            job.info.trajectory_resolution = np.uint32(100)
+    elif kind == "pits":
+        job, m, slt = _traced_case(_pitted_job())
     else:
         job, m, slt = _traced_case()
     job.info.set_thresholds(segmentation_threshold=np.uint32(20))
     return job, m, slt
 
 
-def _traced_case():
-    """small_256 DTM: downstream fields traced on the GPU, channels where
-    Mapping.map_channels' test holds for the 97th-percentile slt."""
+def _pitted_job():
+    """A cone with unfilled pits: deterministic traces caught in a pit circle
+    on the position lattice until MAX_N_STEPS (the kernels skip whole laps)."""
+    from slm_amd import synth
+    roi = synth.make_dtm(160, pixel_size=1.0, relief=20.0, tilt=0.05, seed=5, nan_border=2, cone=True,
+                         n_pits=12, fill=False)
+    return workloads.job_from_dtm("pits_160", roi, 1.0, {"subpixel_seed_point_density": 1, "jitter_magnitude": 1.0,
+                                                         "max_length": 1000.0, "integrator_step_factor": 0.5})
+
+
+def _traced_case(job=None):
+    """small_256 DTM (or job): downstream fields traced on the GPU, channels
+    where Mapping.map_channels' test holds for the 97th-percentile slt."""
     import torch
     from slm_amd import hip
     from oracle.postprocess_ref import postprocess_raw
-    job = workloads.small_256(jitter=1.0)
+    job = job if job is not None else workloads.small_256(jitter=1.0)
     dev = torch.device("cuda", 0)
     p = job.params(1)
     seeds = torch.from_numpy(job.seeds).to(dev)
@@ -73,7 +89,8 @@ def _load_oracle(om, gm):
         setattr(om, k, None if v is None else _u32(v).reshape(v.shape))
 
 
-@pytest.mark.parametrize("kind,contract", [("v", True), ("y", True), ("y", False), ("traced", True)])
+@pytest.mark.parametrize("kind,contract", [("v", True), ("y", True), ("y", False), ("y100", True), ("traced", True),
+                                           ("pits", True)])
 def test_chain_stepwise_bit_exact(gpu, kind, contract):
     import torch
     from slm_amd.mapping import ChannelMapper

@@ -67,3 +67,44 @@ def test_midslopes_ridges_traced_dtm(gpu):
     n_mid, n_ridge = _run(gpu, sla, job.mask)
     print("midslope px", n_mid, "ridge px", n_ridge, flush=True)
     assert n_mid > 0
+
+
+def _ring_sla(n, m):
+    """sla whose ridge map (arctan2(d, u) - pi/4 <= -0.8 pi/4 after smoothing)
+    holds closed rings (holes to fill), a ring with a gap (no hole), a long
+    square spiral (the outside reaches its centre only along the spiral) and
+    a ring cut by the grid edge."""
+    ridge = np.zeros((n, m), bool)
+    t = 8
+
+    def box(x0, y0, x1, y1):
+        ridge[x0:x1, y0:y0 + t] = ridge[x0:x1, y1 - t:y1] = True
+        ridge[x0:x0 + t, y0:y1] = ridge[x1 - t:x1, y0:y1] = True
+
+    box(10, 10, 90, 120)
+    box(30, 30, 70, 100)                       # nested: holes inside holes
+    box(110, 10, 190, 90)
+    ridge[140:160, 10:18] = False              # gap: the inside is reached, not a hole
+    # square spiral, corridors 6 px wide, walls t px
+    x0, y0, x1, y1 = 100, 130, 196, 320
+    ridge[x0:x0 + t, y0:y1] = True
+    while x1 - x0 > 2 * (t + 6) and y1 - y0 > 2 * (t + 6):
+        ridge[x0:x1, y1 - t:y1] = True
+        ridge[x1 - t:x1, y0:y1] = True
+        ridge[x0 + t + 6:x1, y0:y0 + t] = True
+        x0, y0, x1, y1 = x0 + t + 6, y0 + t + 6, x1 - t - 6, y1 - t - 6
+        ridge[x0:x0 + t, y0:y1] = True
+    box(-20 + 20, 250, 60, 333)                # touches the grid edge
+    d = np.where(ridge, np.float32(0.01), np.float32(1.0)).astype(np.float32)
+    u = np.ones((n, m), np.float32)
+    return np.stack([d, u], -1).astype(np.float32)
+
+
+def test_midslopes_ridges_holes_and_spiral(gpu):
+    """Hole filling of the ridge map on an odd-shaped grid (neither side a
+    multiple of 64) against scipy's binary_fill_holes through the oracle."""
+    sla = _ring_sla(200, 333)
+    mask = np.zeros((200, 333), bool)
+    n_mid, n_ridge = _run(gpu, sla, mask)
+    print("midslope px", n_mid, "ridge px", n_ridge, flush=True)
+    assert n_ridge > 0

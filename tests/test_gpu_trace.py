@@ -229,6 +229,39 @@ def test_hsl_bit_exact(gpu, contract, offset):
     assert (o == -1.0).sum() < 0.1 * o.size   # most reach the channel; the rest keep their value
 
 
+@pytest.mark.parametrize("contract", [True, False])
+def test_hsl_pits_bit_exact(gpu, contract):
+    """Hillslope lengths from every pixel of a cone with unfilled pits, a few
+    scattered thin-channel pixels: many traces circle in a pit until
+    MAX_N_STEPS and must leave their length untouched, as in the oracle."""
+    import torch
+    from slm_amd import hip, synth, workloads
+    from slm_amd.params import kernel_params
+    from slm_amd.useful import pick_seeds
+    roi = synth.make_dtm(128, pixel_size=1.0, relief=20.0, tilt=0.05, seed=7, nan_border=2, cone=True,
+                         n_pits=10, fill=False)
+    job = workloads.job_from_dtm("pits_128", roi, 1.0, {"subpixel_seed_point_density": 1, "jitter_magnitude": 0.0,
+                                                        "max_length": 1000.0, "integrator_step_factor": 0.5})
+    thin, mid = int(job.info.is_thinchannel), int(job.info.is_midslope)
+    mapping = np.where(job.mask, 0, mid).astype(np.uint32)
+    rs = np.random.default_rng(3)
+    pick = rs.random(job.mask.shape) < 0.004
+    mapping[pick & ~job.mask] = thin
+    seeds = pick_seeds(mask=job.mask, map=mapping, flag=mid, pad=2)
+    job.info.n_seed_points = np.uint32(seeds.shape[0])
+    p = kernel_params(job.info, 1, "HILLSLOPE_LENGTHS", contract)
+    init = np.full(seeds.shape[0], -1.0, dtype=np.float32)
+    o = oracle.hillslope_lengths(p, seeds, job.mask.astype(np.uint8).reshape(-1), job.uv.reshape(-1),
+                                 mapping.reshape(-1), thin, mid, length=init.copy())
+    sd, m, uv = torch.from_numpy(seeds).to(gpu), *_dev(job, gpu)[1:]
+    mp = torch.from_numpy(mapping.reshape(-1).view(np.int32)).to(gpu)
+    L = torch.from_numpy(init.copy()).to(gpu)
+    print("pits hsl seeds", seeds.shape[0], "reached", int((o != -1.0).sum()), flush=True)
+    hip.hillslope_lengths(p, sd, m, uv, mp, thin, mid, L, sync=True)
+    assert np.array_equal(L.cpu().numpy().view(np.uint32), o.view(np.uint32))
+    assert 0 < (o != -1.0).sum() < o.size
+
+
 TILED_CASES = [
     # (grid side, d, jitter, fp_contract, (tile, block, unit, cap, halo), max_error)
     (256, 2, 1.0, True, (32, 256, 0, 0, 0), None),
