@@ -426,8 +426,8 @@ def main():
     }
     if args.pipeline:
         from slm_amd.pipeline import map_tile
-        map_tile(job)  # warm-up
-        res, tim = map_tile(job)
+        map_tile(job, fmm=True)  # warm-up
+        res, tim = map_tile(job, fmm=True)
         # the loop and blockage conditioning of the same DTM (the reference's
         # do_condition), timed alone: the uv stage with it instead of the plain gradient
         roi_d = torch.from_numpy(np.ascontiguousarray(job.roi, dtype=np.float32)).to(dev)

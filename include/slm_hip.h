@@ -591,11 +591,45 @@ int slm_gaussian_filter(const float* in, float* out, float* tmp, uint32_t nx, ui
  * dilated twice by the 3x3 square and hole-filled (scipy binary_fill_holes).
  * Both are thinned (Zhang-Suen, the stand-in for skimage skeletonize) and
  * their skeletons OR midslope_flag / ridge_flag into mapping.  Synchronises
- * the stream once per thinning iteration and every 8 hole-filling steps. */
+ * the stream once per thinning iteration and once per hole-filling launch
+ * (bit-packed 64x64 tiles, repeated until no tile changes). */
 int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, uint32_t nx, uint32_t ny, const double* w_mid,
                              int r_mid, float mid_threshold, const double* w_ridge, int r_ridge, float ridge_level,
                              uint32_t midslope_flag, uint32_t ridge_flag, uint32_t* mapping, void* workspace,
                              size_t workspace_bytes, void* stream, uint32_t flags);
+
+/* ----------------------------------------------------------------------
+ * FMM hillslope lengths: Mapping.fmm_lengths + remap_hsl
+ * (python/streamlines/mapping.py:841-900, :953-985).  The reference calls
+ * skfmm.distance once per subsegment; here all subsegments are solved at
+ * once, coloured so that the 4-dilated domains of one colour are disjoint.
+ * Labels are int32 [nx*ny] (flank sign kept); a subsegment is |label|.
+ * ---------------------------------------------------------------------- */
+/* Label pairs (lo << 32 | hi, |label| values, duplicates included) whose
+ * pixels lie within Manhattan distance 2, i.e. whose dilated domains can
+ * share a pixel.  colour[|label|] == 0xff marks labels that are not
+ * subsegments; n_labels = the length of colour.  *count = the number of
+ * pairs found; only the first cap are written. */
+int slm_fmm_pairs(const int32_t* label, uint32_t nx, uint32_t ny, const uint8_t* colour, uint32_t n_labels,
+                  uint64_t* pairs, uint32_t cap, uint32_t* count, void* stream, uint32_t flags);
+size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny);
+/* For every subsegment a (colour[a] < n_colours): phi = -1 on its
+ * thin-channel pixels ((mapping & thin_flag) == thin_flag), +1 elsewhere on
+ * its domain (the pixels with |label| == a and their 4-neighbours); the
+ * first-order eikonal distance from the zero contour (scikit-fmm's
+ * interface initialisation, grid step dx, float64); per subsegment:
+ * sums[a] = sum of round(signed distance * 2^24) (int64 bit pattern) over the
+ * points a channel reaches, cnt[a] their number, nch[a] the channel pixels,
+ * nphi[a] the pixels with |label| == a.  Arrays of n_labels entries.
+ * Synchronises the stream every 8 relaxation sweeps. */
+int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_flag, const uint8_t* colour,
+                uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx, uint64_t* sums,
+                uint32_t* cnt, uint32_t* nch, uint32_t* nphi, void* workspace, size_t workspace_bytes, void* stream,
+                uint32_t flags);
+/* remap_hsl: grid[i] = hsl[|label[i]|] where the signed label is selected
+ * (selected[|l|] bit 0 for l > 0, bit 1 for l < 0), else 0 (float32). */
+int slm_hsl_remap(const int32_t* label, uint32_t n, const int32_t* hsl, const uint8_t* selected, uint32_t n_labels,
+                  float* grid, void* stream, uint32_t flags);
 
 #ifdef __cplusplus
 }

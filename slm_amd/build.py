@@ -1,7 +1,7 @@
 """Build the native libraries in-tree.
 
   slm_amd/libslm_hip.so   hipcc, gfx950 kernels + C ABI (include/slm_hip.h)
-  slm_amd/libslm_host.so  gcc, host helpers (sink filling for synthetic DTMs)
+  slm_amd/libslm_host.so  gcc, host helpers (sink filling for synthetic DTMs, subsegment colouring)
 
 Run ``python -m slm_amd.build``.  Every build recompiles.
 """
@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
-HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip", "prep.hip", "surrogate.hip"]
+HIP_SOURCES = ["slm_hip.hip", "kde.hip", "mapping.hip", "prep.hip", "surrogate.hip", "hsl.hip"]
 
 
 def _run(cmd):
@@ -46,7 +46,8 @@ def build_hip(out=None):
 def build_host(out=None):
     out = out or os.path.join(HERE, "libslm_host.so")
     tmp = out + ".tmp"
-    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", os.path.join(CSRC, "dtm_fill.c"), "-o", tmp, "-lm"])
+    _run(["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-std=c11", os.path.join(CSRC, "dtm_fill.c"),
+          os.path.join(CSRC, "host_graph.c"), "-o", tmp, "-lm"])
     os.replace(tmp, out)
     return out
 

@@ -75,6 +75,11 @@ _PROTOS = {
     "slm_gaussian_filter": (ctypes.c_int, [_P, _P, _P, _U32, _U32, _P, ctypes.c_int, _P, _U32]),
     "slm_map_midslopes_ridges": (ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.c_int, ctypes.c_float, _P, ctypes.c_int,
                                                 ctypes.c_float, _U32, _U32, _P, _P, ctypes.c_size_t, _P, _U32]),
+    "slm_fmm_pairs": (ctypes.c_int, [_P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32]),
+    "slm_fmm_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_fmm_hsl": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, ctypes.c_double, _P, _P, _P, _P, _P,
+                                   ctypes.c_size_t, _P, _U32]),
+    "slm_hsl_remap": (ctypes.c_int, [_P, _U32, _P, _P, _U32, _P, _P, _U32]),
     "slm_condition_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),

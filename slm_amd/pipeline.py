@@ -16,6 +16,7 @@ import time
 import numpy as np
 
 from . import hip
+from . import hsl as fmm_hsl
 from .kde import Univariate_distribution
 from .mapping import ChannelMapper, pick_seeds
 from .params import kernel_params
@@ -43,14 +44,16 @@ class _Stages:
 
 def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None, condition=False,
              midslopes=True, midslope_filter_sigma=5.0, midslope_threshold=0.03, ridge_filter_sigma=2.0,
-             ridge_threshold=0.8, return_grids=False):
+             ridge_threshold=0.8, fmm=False, return_grids=False):
     """Run the pipeline on a workloads.Job; returns (results dict, stage timings).
     condition: the uv stage runs the loop and blockage conditioning
     (slm_condition_uv, the reference's do_condition) instead of the plain
     gradient field the job's grids were built with.  midslopes: the
     mid-slope and ridge flags of Mapping.pass1 (mapping.py:286-287, the JSON
-    mapping defaults) are set before the channels are mapped.  return_grids:
-    results["grids"] holds the device grids (uv, mask, sla, slt, slc, mapping)."""
+    mapping defaults) are set before the channels are mapped.  fmm: also the
+    FMM hillslope lengths of every subsegment (slm_amd.hsl, the reference's
+    live HSL path) and their grid.  return_grids: results["grids"] holds the
+    device grids (uv, mask, sla, slt, slc, mapping, label, fmm_hsl)."""
     import torch
     dev = device or torch.device("cuda", torch.cuda.current_device())
     info = job.info
@@ -147,6 +150,14 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
         logx = torch.log(pos).cpu().numpy().astype(np.float32)
         dist = Univariate_distribution(logx_array=logx, logy_array=logx, n_hist_bins=512, n_pdf_points=256)
         dist.compute_kde_opencl(kernel="gaussian", bandwidth=0.15, fp_contract=fp_contract)
+    fmm_res = fmm_grid = fmm_trace_mean = None
+    if fmm:
+        # the reference's live HSL path (mapping.py:490-494): FMM lengths per
+        # subsegment, remapped onto the grid
+        st.mark("fmm")
+        fmm_res = fmm_hsl.fmm_lengths(cm.label, cm.mapping, mask_b, int(info.is_thinchannel), job.pixel_size)
+        fmm_grid = fmm_hsl.remap_hsl(cm.label, fmm_res)
+        fmm_trace_mean = fmm_hsl.trace_mean_hsl(fmm_grid)
     st.mark("end")
     timings = st.report()
     results = {
@@ -163,9 +174,15 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     if dist is not None:
         dist.find_modes()
         results["kde_mode_x"] = float(np.asarray(dist.mode_x).reshape(-1)[0])
+    if fmm_res is not None:
+        results["fmm_subsegments"] = int(fmm_res.selected.size)
+        results["fmm_kept"] = int((fmm_res.hsl > 0).sum())
+        results["fmm_colours"] = int(fmm_res.n_colours)
+        results["fmm_mean_hsl_m"] = fmm_res.fmm_mean_hsl
+        results["fmm_trace_mean_hsl_m"] = None if fmm_trace_mean is None else float(fmm_trace_mean)
     results["midslope_pixels"] = int(((cm.mapping & int(info.is_midslope)) != 0).sum().item())
     results["ridge_pixels"] = int(((cm.mapping & int(info.is_ridge)) != 0).sum().item())
     if return_grids:
         results["grids"] = {"uv": uv, "mask": mask_b, "sla": sla_f, "slt": slt_f, "slc": slc_f,
-                            "mapping": cm.mapping}
+                            "mapping": cm.mapping, "label": cm.label, "fmm_hsl": fmm_grid}
     return results, timings

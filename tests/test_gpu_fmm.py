@@ -1,0 +1,79 @@
+"""GPU parity of the FMM hillslope lengths (slm_amd.hsl: slm_fmm_pairs,
+slm_fmm_hsl, slm_hsl_remap) against the CPU oracle (oracle/fmm_ref.py).
+
+Per subsegment the counts must match exactly, the fixed-point distance sums
+to within one unit per point (the GPU relaxation and the oracle's fast march
+reach the same discrete solution, but a value can come from a different
+update by an ulp), and the int32 HSL exactly unless twice the mean distance
+lies within 1e-6 of an integer."""
+import numpy as np
+import pytest
+
+from oracle import fmm_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _voronoi_labels(n, m, k, seed):
+    rs = np.random.default_rng(seed)
+    cx, cy = rs.integers(2, n - 2, k), rs.integers(2, m - 2, k)
+    x, y = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    d = (x[..., None] - cx) ** 2 + (y[..., None] - cy) ** 2
+    owner = d.argmin(axis=2)
+    labels = (rs.permutation(k) + 3) * np.where(rs.random(k) < 0.5, -1, 1)
+    label = labels[owner].astype(np.int32)
+    mapping = np.zeros((n, m), np.uint32)
+    # a channel through each cell: its pixels on the centre's row
+    for j in range(k):
+        mapping[(owner == j) & (x == cx[j])] = 2
+    mask = np.zeros((n, m), bool)
+    mask[:2], mask[-2:], mask[:, :2], mask[:, -2:] = True, True, True, True
+    label[mask] = 0
+    return label, mapping, mask
+
+
+def _compare(gpu, label, mapping, mask, dx):
+    import torch
+    from slm_amd import hsl
+    lt = torch.from_numpy(label).to(gpu)
+    mt = torch.from_numpy(mapping.view(np.int32)).to(gpu)
+    kt = torch.from_numpy(mask).to(gpu)
+    res = hsl.fmm_lengths(lt, mt, kt, 2, dx)
+    sel, h_o, stats = fmm_ref.fmm_lengths(label, mapping, mask, 2, dx)
+    print("subsegments", sel.size, "colours", res.n_colours, flush=True)
+    assert np.array_equal(res.selected, sel)
+    for s, hg, ho in zip(sel, res.hsl, h_o):
+        a = abs(int(s))
+        q, cnt, n_phi, n_ch = stats[a]
+        assert (int(res.cnt[a]), int(res.nphi[a]), int(res.nch[a])) == (cnt, n_phi, n_ch), a
+        assert abs(int(res.sums[a]) - q) <= cnt, a
+        if n_ch and n_phi / n_ch > 4.0:
+            h = 2.0 * (q / fmm_ref.FIX) / cnt
+            if abs(h - round(h)) > 1e-6:
+                assert hg == ho, (a, hg, ho)
+        else:
+            assert hg == 0 and ho == 0
+    grid = hsl.remap_hsl(lt, res).cpu().numpy()
+    assert np.array_equal(grid, fmm_ref.remap(label, sel, res.hsl))
+    assert res.fmm_mean_hsl == fmm_ref.fmm_mean_hsl(h_o)
+    return res
+
+
+@pytest.mark.parametrize("shape,k,seed,dx", [((64, 50), 9, 0, 1.0), ((130, 97), 40, 1, 2.0), ((200, 200), 120, 2, 1.0)])
+def test_fmm_voronoi_subsegments(gpu, shape, k, seed, dx):
+    label, mapping, mask = _voronoi_labels(shape[0], shape[1], k, seed)
+    res = _compare(gpu, label, mapping, mask, dx)
+    assert (res.hsl > 0).sum() > 0
+
+
+def test_fmm_traced_segmentation(gpu):
+    """The labels and thin channels of the segmentation chain on the traced
+    256^2 workload (the grids map_tile produces)."""
+    from slm_amd import pipeline, workloads
+    job = workloads.small_256()
+    res, _ = pipeline.map_tile(job, return_grids=True)
+    g = res["grids"]
+    label = g["label"].cpu().numpy().reshape(job.mask.shape).astype(np.int32)
+    mapping = g["mapping"].cpu().numpy().reshape(job.mask.shape).view(np.uint32)
+    r = _compare(gpu, label, mapping, job.mask, job.pixel_size)
+    assert r.selected.size > 10 and (r.hsl > 0).sum() > 0
