@@ -631,6 +631,33 @@ int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_fla
 int slm_hsl_remap(const int32_t* label, uint32_t n, const int32_t* hsl, const uint8_t* selected, uint32_t n_labels,
                   float* grid, void* stream, uint32_t flags);
 
+/* Mapping.map_hsl (mapping.py:987-1020): hsl float32 [nx*ny] rescaled to
+ * uint16 over [min, max]; when dfw > 0, grey_dilation by a dfw x dfw square
+ * (scipy's window and 'reflect' borders) replaces the masked pixels (mask
+ * u8, 1 = masked); the mean over disk(mdr) (skimage rank.mean: float64 sum
+ * of the in-image footprint values / their number, truncated) is rescaled
+ * into out (float32). */
+size_t slm_map_hsl_workspace_size(uint32_t nx, uint32_t ny);
+int slm_map_hsl(const float* hsl, const uint8_t* mask, uint32_t nx, uint32_t ny, int dfw, int mdr, float* out,
+                void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
+/* Mapping.map_aspect (mapping.py:1022-1053): slope float32 [nx*ny] (degrees,
+ * Preprocess.slope_array) with channel pixels ((mapping & is_channel) ==
+ * is_channel) zeroed; when do_median, quantised to uint8 by max/255 and
+ * median-filtered over disk(r_slope); u, v of uv (float2) quantised by 2/255
+ * and median-filtered over disk(r_uv) (skimage rank.median: the smallest
+ * value v with 2 * #(values <= v) >= the in-image footprint count); aspect =
+ * arctan2(v, u) (float64, rounded to float32); mask = slope <
+ * slope_threshold or channel.  Disk radii up to 100 pixels (sliding
+ * 256-bin histograms in LDS). */
+size_t slm_map_aspect_workspace_size(uint32_t nx, uint32_t ny);
+int slm_map_aspect(const float* slope, const float* uv, const uint32_t* mapping, uint32_t is_channel, uint32_t nx,
+                   uint32_t ny, int r_slope, int r_uv, int do_median, float slope_threshold, float* aspect,
+                   uint8_t* mask, void* workspace, size_t workspace_bytes, void* stream, uint32_t flags);
+/* Preprocess.slope_array (preprocess.py:597): degrees(arctan(raw speed)) of
+ * the ROI float32 [nx][ny], 'edge'-padded to [(nx+2 pad)][(ny+2 pad)]. */
+int slm_slope_deg(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size, float* slope,
+                  void* stream, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

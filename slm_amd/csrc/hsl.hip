@@ -46,25 +46,64 @@ __device__ __forceinline__ uint32_t abs_label(int32_t l) {
 // Label pairs whose dilated domains can share a pixel: pixels at Manhattan
 // distance 1 or 2 with different non-zero |label|.  Half of the offsets, so
 // each unordered pixel pair is seen once; duplicates of a label pair remain.
-__global__ void k_fmm_pairs(const int32_t* __restrict__ label, uint32_t nx, uint32_t ny,
-                            const uint8_t* __restrict__ colour, uint32_t n_labels, u64* __restrict__ pairs,
-                            uint32_t cap, uint32_t* __restrict__ count) {
+__global__ __launch_bounds__(256) void k_fmm_pairs(const int32_t* __restrict__ label, uint32_t nx, uint32_t ny,
+                                                   const uint8_t* __restrict__ colour, uint32_t n_labels,
+                                                   u64* __restrict__ pairs, uint32_t cap,
+                                                   uint32_t* __restrict__ count) {
+    // slots are handed out per block: LDS counter, then one global atomic
+    __shared__ uint32_t s_cnt, s_base;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx * ny) return;
-    const uint32_t a = abs_label(label[i]);
-    if (!a || a >= n_labels || colour[a] == 0xffu) return;
-    const int x = (int)(i / ny), y = (int)(i % ny);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t a = 0;
+    int x = 0, y = 0;
+    if (i < nx * ny) {
+        a = abs_label(label[i]);
+        if (a >= n_labels || colour[a] == 0xffu) a = 0;
+        x = (int)(i / ny);
+        y = (int)(i % ny);
+    }
     const int off[6][2] = {{0, 1}, {0, 2}, {1, -1}, {1, 0}, {1, 1}, {2, 0}};
+    u64 key[6];
+    uint32_t slot[6];
+    uint32_t has = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int xx = x + off[k][0], yy = y + off[k][1];
-        if (xx < 0 || yy < 0 || xx >= (int)nx || yy >= (int)ny) continue;
-        const uint32_t b = abs_label(label[(size_t)xx * ny + yy]);
-        if (!b || b == a || b >= n_labels || colour[b] == 0xffu) continue;
-        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-        const uint32_t slot = atomicAdd(count, 1u);
-        if (slot < cap) pairs[slot] = ((u64)lo << 32) | hi;
+        key[k] = 0;
+        slot[k] = 0;
+        bool h = false;
+        if (a) {
+            const int xx = x + off[k][0], yy = y + off[k][1];
+            if (xx >= 0 && yy >= 0 && xx < (int)nx && yy < (int)ny) {
+                const uint32_t b = abs_label(label[(size_t)xx * ny + yy]);
+                if (b && b != a && b < n_labels && colour[b] != 0xffu) {
+                    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                    key[k] = ((u64)lo << 32) | hi;
+                    h = true;
+                }
+            }
+        }
+        const u64 bal = __ballot(h);
+        if (!bal) continue;
+        const uint32_t leader = (uint32_t)__ffsll((long long)bal) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+        base = __shfl(base, (int)leader);
+        if (h) {
+            slot[k] = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            has |= 1u << k;
+        }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_cnt ? atomicAdd(count, s_cnt) : 0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        if ((has >> k) & 1u) {
+            const uint32_t g = s_base + slot[k];
+            if (g < cap) pairs[g] = key[k];
+        }
 }
 
 // problem key (|label| + 0 = none) and phi sign (1 = negative) of colour c
@@ -152,12 +191,26 @@ constexpr int kMaxLocal = 4 * kT;
 __global__ __launch_bounds__(256) void k_fmm_relax_tiles(const uint32_t* __restrict__ dom,
                                                          const uint8_t* __restrict__ fixed, uint32_t nx, uint32_t ny,
                                                          double dx, double* __restrict__ u,
-                                                         uint32_t* __restrict__ changed) {
+                                                         uint32_t* __restrict__ changed,
+                                                         const uint8_t* __restrict__ act_in,
+                                                         uint8_t* __restrict__ act_out) {
     __shared__ double su[kTH * kTH];
     __shared__ uint32_t sk[kTH * kTH];
     __shared__ uint8_t sfix[kTH * kTH];
-    const uint32_t nty = (ny + kT - 1) / kT;
-    const int x0 = (int)(blockIdx.x / nty) * kT - 1, y0 = (int)(blockIdx.x % nty) * kT - 1;
+    const uint32_t nty = (ny + kT - 1) / kT, ntx = (nx + kT - 1) / kT;
+    const int tx = (int)(blockIdx.x / nty), ty = (int)(blockIdx.x % nty);
+    const int x0 = tx * kT - 1, y0 = ty * kT - 1;
+    if (threadIdx.x == 0) act_out[blockIdx.x] = 0;
+    // a tile can only decrease if it or a neighbouring tile changed in the
+    // previous launch (act_in; all tiles on the first launch)
+    {
+        bool need = false;
+        if (threadIdx.x < 9) {
+            const int ax = tx + (int)threadIdx.x / 3 - 1, ay = ty + (int)threadIdx.x % 3 - 1;
+            if (ax >= 0 && ay >= 0 && ax < (int)ntx && ay < (int)nty) need = act_in[(size_t)ax * nty + ay] != 0;
+        }
+        if (!__syncthreads_or(need)) return;
+    }
     bool any_key = false;
     for (int k = threadIdx.x; k < kTH * kTH; k += blockDim.x) {
         const int lx = k / kTH, ly = k % kTH, gx = x0 + lx, gy = y0 + ly;
@@ -212,7 +265,10 @@ __global__ __launch_bounds__(256) void k_fmm_relax_tiles(const uint32_t* __restr
         if (su[c] < __hip_atomic_load(&u[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             __hip_atomic_store(&u[g], su[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    if (threadIdx.x == 0) atomicAdd(changed, 1u);
+    if (threadIdx.x == 0) {
+        act_out[blockIdx.x] = 1;
+        atomicAdd(changed, 1u);
+    }
 }
 
 // Per problem: fixed-point sum of the signed distances of the points a
@@ -273,6 +329,251 @@ __global__ void k_hsl_remap(const int32_t* __restrict__ label, uint32_t n, const
     grid[i] = v;
 }
 
+// ----------------------------------------------------------------- map_hsl
+// Mapping.map_hsl (mapping.py:987-1020): uint16 rescaling, grey_dilation
+// (scipy: flat dfw x dfw window, offsets -(s-1)/2 .. s/2 along each axis,
+// 'reflect' borders) into the masked pixels only, mean over disk(mdr)
+// (skimage rank.mean: float64 sum / in-image count, truncated), rescale.
+// Float32 arithmetic in numpy's order, no contraction.
+
+// order-preserving uint key of a float (min/max through integer atomics)
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fval(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+__global__ void k_minmax(const float* __restrict__ a, size_t n, uint32_t* __restrict__ mm) {
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t k = fkey(a[i]);
+        lo = min(lo, k);
+        hi = max(hi, k);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_down(lo, o));
+        hi = max(hi, (uint32_t)__shfl_down(hi, o));
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+
+__global__ void k_hsl_clip(const float* __restrict__ hsl, size_t n, const uint32_t* __restrict__ mm,
+                           uint16_t* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float mn = fval(mm[0]), mx = fval(mm[1]);
+    const float t = (65535.0f * (hsl[i] - mn)) / (mx - mn);
+    out[i] = (t >= 0.0f && t < 65536.0f) ? (uint16_t)(uint32_t)t : (uint16_t)0;
+}
+
+__device__ __forceinline__ int reflect_idx(int j, int n) {
+    while (j < 0 || j >= n) j = (j < 0) ? (-j - 1) : (2 * n - j - 1);
+    return j;
+}
+
+template <int AXIS>
+__global__ void k_max_axis(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, uint32_t nx, uint32_t ny,
+                           int lo, int hi) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)nx * ny) return;
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    uint16_t m = 0;
+    for (int k = -lo; k <= hi; k++) {
+        const uint16_t v = AXIS == 0 ? in[(size_t)reflect_idx(x + k, (int)nx) * ny + y]
+                                     : in[(size_t)x * ny + reflect_idx(y + k, (int)ny)];
+        m = v > m ? v : m;
+    }
+    out[i] = m;
+}
+
+// hsl_dilated[~mask] = hsl_clipped[~mask]
+__global__ void k_keep_unmasked(uint16_t* __restrict__ dil, const uint16_t* __restrict__ clip,
+                                const uint8_t* __restrict__ mask, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n && !mask[i]) dil[i] = clip[i];
+}
+
+// Prefix sums along y, stored transposed so that neighbouring threads (x)
+// write neighbouring addresses: P[y][x] = sum of in[x][0..y-1], y in 0..ny.
+template <typename T, typename S>
+__global__ void k_row_prefix(const T* __restrict__ in, uint32_t nx, uint32_t ny, S* __restrict__ P) {
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nx) return;
+    S acc = 0;
+    const T* row = in + (size_t)x * ny;
+    P[x] = 0;
+    for (uint32_t y = 0; y < ny; y++) {
+        acc += (S)row[y];
+        P[(size_t)(y + 1) * nx + x] = acc;
+    }
+}
+
+// half-width of disk(r) at row offset dx: the largest w with w^2 <= r^2 - dx^2
+__device__ __forceinline__ int disk_w(int r, int dx) {
+    const int t = r * r - dx * dx;
+    int w = (int)sqrtf((float)t);
+    while (w * w > t) w--;
+    while ((w + 1) * (w + 1) <= t) w++;
+    return w;
+}
+
+__global__ void k_disk_mean(const u64* __restrict__ P, uint32_t nx, uint32_t ny, int r, uint16_t* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)nx * ny) return;
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    u64 sum = 0, pop = 0;
+    for (int dx = -r; dx <= r; dx++) {
+        const int xx = x + dx;
+        if (xx < 0 || xx >= (int)nx) continue;
+        const int w = disk_w(r, dx);
+        const int y0 = max(0, y - w), y1 = min((int)ny - 1, y + w);
+        sum += P[(size_t)(y1 + 1) * nx + xx] - P[(size_t)y0 * nx + xx];
+        pop += (u64)(y1 - y0 + 1);
+    }
+    out[i] = pop ? (uint16_t)(uint32_t)((double)sum / (double)pop) : (uint16_t)0;
+}
+
+__global__ void k_hsl_unclip(const uint16_t* __restrict__ m, size_t n, const uint32_t* __restrict__ mm,
+                             float* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float mn = fval(mm[0]), mx = fval(mm[1]);
+    out[i] = (((float)m[i]) / 65535.0f) * (mx - mn) + mn;
+}
+
+// -------------------------------------------------------------- map_aspect
+// Mapping.map_aspect (mapping.py:1022-1053): channel slopes zeroed, slope
+// quantised to uint8 by sf = max/255 (float32) and median-filtered over
+// disk(r_slope); u and v quantised by 2/255 and median-filtered over
+// disk(r_uv); aspect = arctan2(v, u) (float64, rounded to float32); mask =
+// slope < threshold, or channel.  Median (skimage rank.median): the smallest
+// value v with 2 * #(values <= v) >= in-image count (k_disk_median).
+
+__global__ void k_slope_zero_channels(const float* __restrict__ slope, const uint32_t* __restrict__ mapping,
+                                      uint32_t is_channel, size_t n, float* __restrict__ out) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = ((mapping[i] & is_channel) == is_channel) ? 0.0f : slope[i];
+}
+
+__global__ void k_quant_slope(const float* __restrict__ s, size_t n, const uint32_t* __restrict__ mm,
+                              uint8_t* __restrict__ q) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float sf = fval(mm[1]) / 255.0f;
+    const float t = s[i] / sf;
+    q[i] = (t >= 0.0f && t < 256.0f) ? (uint8_t)(uint32_t)t : (uint8_t)0;
+}
+
+__global__ void k_quant_uv(const float2* __restrict__ uv, size_t n, uint8_t* __restrict__ qu,
+                           uint8_t* __restrict__ qv) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float sf = (float)(2.0 / 255.0);
+    const float tu = (uv[i].x + 1.0f) / sf, tv = (uv[i].y + 1.0f) / sf;
+    qu[i] = (tu >= 0.0f && tu < 256.0f) ? (uint8_t)(uint32_t)tu : (uint8_t)0;
+    qv[i] = (tv >= 0.0f && tv < 256.0f) ? (uint8_t)(uint32_t)tv : (uint8_t)0;
+}
+
+// Median over disk(r) by a sliding histogram (Huang): a thread owns one
+// column y and a run of kRun pixels along x; it builds the histogram of the
+// footprint at the first pixel, then slides it one pixel along x (for every
+// row offset dy: out goes (x - w, y + dy), in comes (x + 1 + w, y + dy)),
+// reading the median from 16 coarse and 256 fine counts.  Neighbouring
+// threads hold neighbouring y, so each footprint load is one row segment.
+// Histograms live in LDS, bin-major (bin b of thread t at b * 64 + t).
+constexpr int kRun = 64;
+constexpr int kMedThreads = 64;
+
+__global__ __launch_bounds__(kMedThreads) void k_disk_median(const uint8_t* __restrict__ img, uint32_t nx,
+                                                             uint32_t ny, int r, uint8_t* __restrict__ out) {
+    __shared__ uint16_t fine[256 * kMedThreads];
+    __shared__ uint16_t coarse[16 * kMedThreads];
+    const uint32_t t = threadIdx.x;
+    const uint32_t y = blockIdx.x * kMedThreads + t;
+    const uint32_t x0 = blockIdx.y * kRun;
+    for (int b = 0; b < 256; b++) fine[b * kMedThreads + t] = 0;
+    for (int c = 0; c < 16; c++) coarse[c * kMedThreads + t] = 0;
+    if (y >= ny || x0 >= nx) return;                     // this thread's histogram is never read by others
+    auto add = [&](uint8_t v, int d) {
+        fine[v * kMedThreads + t] = (uint16_t)(fine[v * kMedThreads + t] + d);
+        coarse[(v >> 4) * kMedThreads + t] = (uint16_t)(coarse[(v >> 4) * kMedThreads + t] + d);
+    };
+    uint32_t pop = 0;
+    for (int dy = -r; dy <= r; dy++) {
+        const int yy = (int)y + dy;
+        if (yy < 0 || yy >= (int)ny) continue;
+        const int w = disk_w(r, dy);
+        for (int xx = max(0, (int)x0 - w); xx <= min((int)nx - 1, (int)x0 + w); xx++) {
+            add(img[(size_t)xx * ny + yy], 1);
+            pop++;
+        }
+    }
+    const uint32_t x1 = min(nx, x0 + (uint32_t)kRun);
+    for (uint32_t x = x0;; x++) {
+        // the smallest v with 2 * #(values <= v) >= pop
+        uint32_t acc = 0;
+        int c = 0;
+        while (c < 15 && 2u * (acc + coarse[c * kMedThreads + t]) < pop) {
+            acc += coarse[c * kMedThreads + t];
+            c++;
+        }
+        int v = c * 16;
+        while (v < 255 && 2u * (acc + fine[v * kMedThreads + t]) < pop) {
+            acc += fine[v * kMedThreads + t];
+            v++;
+        }
+        out[(size_t)x * ny + y] = (uint8_t)v;
+        if (x + 1 >= x1) break;
+        for (int dy = -r; dy <= r; dy++) {
+            const int yy = (int)y + dy;
+            if (yy < 0 || yy >= (int)ny) continue;
+            const int w = disk_w(r, dy);
+            const int xo = (int)x - w, xi = (int)x + 1 + w;
+            if (xo >= 0) {
+                add(img[(size_t)xo * ny + yy], -1);
+                pop--;
+            }
+            if (xi < (int)nx) {
+                add(img[(size_t)xi * ny + yy], 1);
+                pop++;
+            }
+        }
+    }
+}
+
+__global__ void k_aspect(const uint8_t* __restrict__ mu, const uint8_t* __restrict__ mv,
+                         const uint8_t* __restrict__ ms, const uint32_t* __restrict__ mm, int do_median,
+                         const float* __restrict__ slope0, const uint32_t* __restrict__ mapping, uint32_t is_channel,
+                         float slope_threshold, size_t n, float* __restrict__ aspect, uint8_t* __restrict__ mask) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double sf = 2.0 / 255.0;
+    const double ux = sf * (double)mu[i] - 1.0, vy = sf * (double)mv[i] - 1.0;
+    aspect[i] = (float)atan2(vy, ux);
+    float sl;
+    if (do_median) sl = (fval(mm[1]) / 255.0f) * (float)ms[i];
+    else sl = slope0[i];
+    const bool chan = (mapping[i] & is_channel) == is_channel;
+    mask[i] = (sl < slope_threshold || chan) ? 1 : 0;
+}
+
+__global__ void k_fill_mm(uint32_t* __restrict__ mm) {
+    if (threadIdx.x == 0) {
+        mm[0] = 0xffffffffu;
+        mm[1] = 0u;
+    }
+}
+
+__global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 __global__ void k_zero_bytes(uint8_t* __restrict__ p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0;
@@ -304,7 +605,8 @@ extern "C" int slm_fmm_pairs(const int32_t* label, uint32_t nx, uint32_t ny, con
 
 extern "C" size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny) {
     const size_t n = (size_t)nx * ny;
-    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 256 + 256;
+    const size_t n_tiles = (size_t)((nx + 31) / 32) * ((ny + 31) / 32);
+    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 256 + 2 * al256(n_tiles) + 256;
 }
 
 extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_flag, const uint8_t* colour,
@@ -324,6 +626,9 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
     double* u = (double*)(neg + al256(n));
     uint8_t* fixed = (uint8_t*)u + al256(n * 8);
     uint32_t* ctr = (uint32_t*)(fixed + al256(n));
+    const uint32_t n_tiles = ((nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
+    uint8_t* act_a = (uint8_t*)ctr + 256;
+    uint8_t* act_b = act_a + al256(n_tiles);
     hipStream_t s = (hipStream_t)stream;
     zero(sums, (size_t)n_labels * 8, s);
     zero(cnt, (size_t)n_labels * 4, s);
@@ -331,18 +636,19 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
     zero(nphi, (size_t)n_labels * 4, s);
     const dim3 g = grid_for(n);
     // a bound on the launches keeps a bad input from looping
-    const uint32_t n_tiles = ((nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
     const uint32_t max_launches = 2u * (nx + ny) + 64u;
     for (uint32_t c = 0; c < n_colours; c++) {
         hipLaunchKernelGGL(k_fmm_setup, g, dim3(kBlock), 0, s, label, mapping, thin_flag, colour, n_labels, c, nx, ny,
                            dom, neg);
         hipLaunchKernelGGL(k_fmm_init, g, dim3(kBlock), 0, s, (const uint32_t*)dom, (const uint8_t*)neg, nx, ny, dx,
                            u, fixed);
+        hipLaunchKernelGGL(k_fill_bytes, grid_for(n_tiles), dim3(kBlock), 0, s, act_a, (size_t)n_tiles, (uint8_t)1);
         for (uint32_t launches = 0;; launches++) {
             if (launches > max_launches) return slm_fail(SLM_E_HIP, "eikonal relaxation did not converge");
             zero(ctr, sizeof(uint32_t), s);
             hipLaunchKernelGGL(k_fmm_relax_tiles, dim3(n_tiles), dim3(kBlock), 0, s, (const uint32_t*)dom,
-                               (const uint8_t*)fixed, nx, ny, dx, u, ctr);
+                               (const uint8_t*)fixed, nx, ny, dx, u, ctr, (const uint8_t*)act_a, act_b);
+            std::swap(act_a, act_b);
             uint32_t dec = 0;
             HIP_TRY(hipMemcpyAsync(&dec, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
@@ -360,5 +666,92 @@ extern "C" int slm_hsl_remap(const int32_t* label, uint32_t n, const int32_t* hs
     if (!n) return slm_finish(stream, flags);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_hsl_remap, grid_for(n), dim3(kBlock), 0, s, label, n, hsl, selected, n_labels, grid);
+    return slm_finish(stream, flags);
+}
+
+extern "C" size_t slm_map_hsl_workspace_size(uint32_t nx, uint32_t ny) {
+    const size_t n = (size_t)nx * ny;
+    return 3 * al256(n * 2) + al256((size_t)nx * (ny + 1) * 8) + 256 + 256;
+}
+
+extern "C" int slm_map_hsl(const float* hsl, const uint8_t* mask, uint32_t nx, uint32_t ny, int dfw, int mdr,
+                           float* out, void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
+    if (!hsl || !mask || !out || !workspace || !nx || !ny || dfw < 0 || mdr < 0 ||
+        (uint64_t)nx * ny >= 0x80000000ull || dfw > 2 * (int)std::min(nx, ny))
+        return slm_fail(SLM_E_INVALID_ARG, "bad map_hsl arguments");
+    if (workspace_bytes < slm_map_hsl_workspace_size(nx, ny))
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes, need %zu", workspace_bytes,
+                        slm_map_hsl_workspace_size(nx, ny));
+    const size_t n = (size_t)nx * ny;
+    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    uint16_t* clip = (uint16_t*)w;
+    uint16_t* tmp = (uint16_t*)(w + al256(n * 2));
+    uint16_t* dil = (uint16_t*)(w + 2 * al256(n * 2));
+    u64* P = (u64*)(w + 3 * al256(n * 2));
+    uint32_t* mm = (uint32_t*)((uint8_t*)P + al256((size_t)nx * (ny + 1) * 8));
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 g = grid_for(n);
+    hipLaunchKernelGGL(k_fill_mm, dim3(1), dim3(64), 0, s, mm);
+    hipLaunchKernelGGL(k_minmax, dim3((uint32_t)std::min<size_t>((n + kBlock - 1) / kBlock, 1024)), dim3(kBlock), 0, s,
+                       hsl, n, mm);
+    hipLaunchKernelGGL(k_hsl_clip, g, dim3(kBlock), 0, s, hsl, n, (const uint32_t*)mm, clip);
+    const uint16_t* src = clip;
+    if (dfw > 0) {
+        const int lo = (dfw - 1) / 2, hi = dfw / 2;
+        hipLaunchKernelGGL(k_max_axis<0>, g, dim3(kBlock), 0, s, (const uint16_t*)clip, tmp, nx, ny, lo, hi);
+        hipLaunchKernelGGL(k_max_axis<1>, g, dim3(kBlock), 0, s, (const uint16_t*)tmp, dil, nx, ny, lo, hi);
+        hipLaunchKernelGGL(k_keep_unmasked, g, dim3(kBlock), 0, s, dil, (const uint16_t*)clip, mask, n);
+        src = dil;
+    }
+    hipLaunchKernelGGL((k_row_prefix<uint16_t, u64>), dim3((nx + 63) / 64), dim3(64), 0, s, src, nx, ny, P);
+    hipLaunchKernelGGL(k_disk_mean, g, dim3(kBlock), 0, s, (const u64*)P, nx, ny, mdr, tmp);
+    hipLaunchKernelGGL(k_hsl_unclip, g, dim3(kBlock), 0, s, (const uint16_t*)tmp, n, (const uint32_t*)mm, out);
+    return slm_finish(stream, flags);
+}
+
+extern "C" size_t slm_map_aspect_workspace_size(uint32_t nx, uint32_t ny) {
+    const size_t n = (size_t)nx * ny;
+    return al256(n * 4) + 4 * al256(n) + 256 + 256;
+}
+
+extern "C" int slm_map_aspect(const float* slope, const float* uv, const uint32_t* mapping, uint32_t is_channel,
+                              uint32_t nx, uint32_t ny, int r_slope, int r_uv, int do_median, float slope_threshold,
+                              float* aspect, uint8_t* mask, void* workspace, size_t workspace_bytes, void* stream,
+                              uint32_t flags) {
+    if (!slope || !uv || !mapping || !aspect || !mask || !workspace || !nx || !ny || r_slope < 0 || r_uv < 0 ||
+        (uint64_t)nx * ny >= 0x80000000ull || r_slope > 32767 || r_uv > 32767)
+        return slm_fail(SLM_E_INVALID_ARG, "bad map_aspect arguments");
+    if (workspace_bytes < slm_map_aspect_workspace_size(nx, ny))
+        return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes, need %zu", workspace_bytes,
+                        slm_map_aspect_workspace_size(nx, ny));
+    const size_t n = (size_t)nx * ny;
+    uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
+    float* s0 = (float*)w;
+    uint8_t* qs = w + al256(n * 4);
+    uint8_t* qu = qs + al256(n);
+    uint8_t* qv = qu + al256(n);
+    uint8_t* med = qv + al256(n);
+    uint32_t* mm = (uint32_t*)(med + al256(n));
+    uint8_t* ms = med;   // the slope median (do_median) lands in med
+    hipStream_t s = (hipStream_t)stream;
+    const dim3 g = grid_for(n);
+    // footprint counts fit uint16 for r <= 100 ((2r+1)^2 < 65536)
+    if (std::max(r_slope, r_uv) > 100) return slm_fail(SLM_E_INVALID_ARG, "median disk radius above 100 pixels");
+    const dim3 gm((ny + kMedThreads - 1) / kMedThreads, (nx + kRun - 1) / kRun);
+    hipLaunchKernelGGL(k_slope_zero_channels, g, dim3(kBlock), 0, s, slope, mapping, is_channel, n, s0);
+    hipLaunchKernelGGL(k_fill_mm, dim3(1), dim3(64), 0, s, mm);
+    if (do_median) {
+        hipLaunchKernelGGL(k_minmax, dim3((uint32_t)std::min<size_t>((n + kBlock - 1) / kBlock, 1024)), dim3(kBlock), 0,
+                           s, (const float*)s0, n, mm);
+        hipLaunchKernelGGL(k_quant_slope, g, dim3(kBlock), 0, s, (const float*)s0, n, (const uint32_t*)mm, qs);
+        hipLaunchKernelGGL(k_disk_median, gm, dim3(kMedThreads), 0, s, (const uint8_t*)qs, nx, ny, r_slope, med);
+        ms = med;
+    }
+    hipLaunchKernelGGL(k_quant_uv, g, dim3(kBlock), 0, s, (const float2*)uv, n, qu, qv);
+    hipLaunchKernelGGL(k_disk_median, gm, dim3(kMedThreads), 0, s, (const uint8_t*)qu, nx, ny, r_uv, qs);
+    hipLaunchKernelGGL(k_disk_median, gm, dim3(kMedThreads), 0, s, (const uint8_t*)qv, nx, ny, r_uv, qu);
+    hipLaunchKernelGGL(k_aspect, g, dim3(kBlock), 0, s, (const uint8_t*)qs, (const uint8_t*)qu, (const uint8_t*)ms,
+                       (const uint32_t*)mm, do_median, (const float*)s0, mapping, is_channel, slope_threshold, n,
+                       aspect, mask);
     return slm_finish(stream, flags);
 }

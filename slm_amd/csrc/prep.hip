@@ -306,7 +306,41 @@ inline dim3 grid_of(size_t n) { return dim3((uint32_t)((n + kBlock - 1) / kBlock
 
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Preprocess.slope_array (preprocess.py:597): degrees(arctan(raw speed)),
+// 'edge'-padded, float32; raw speed = |grad h| / pixel_size (np.hypot on
+// float32), 1 where it is 0 or not finite (set_speed_field).  The arctan is
+// the float64 one rounded to float32, then the float32 product by
+// float32(180/pi) (numpy's float32 rad2deg); numpy's float32 arctan is a
+// vectorised approximation that can differ by an ulp.
+__global__ void k_slope_deg(const float* __restrict__ roi, uint32_t nx, uint32_t ny, uint32_t pad, float ps,
+                            float* __restrict__ slope) {
+    const uint32_t nxp = nx + 2 * pad, nyp = ny + 2 * pad;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nxp * nyp) return;
+    const uint32_t x = (uint32_t)min(max((int)(i / nyp) - (int)pad, 0), (int)nx - 1);
+    const uint32_t y = (uint32_t)min(max((int)(i % nyp) - (int)pad, 0), (int)ny - 1);
+    float u, v;
+    neg_gradient(roi, nx, ny, x, y, ps, u, v);
+    float sp = hypot32(u, v);
+    if (!isfinite(sp) || sp == 0.0f) sp = 1.0f;
+    const float a = (float)atan((double)sp);
+    slope[i] = a * (float)(180.0 / 3.14159265358979323846);
+}
+
 }  // namespace
+
+extern "C" int slm_slope_deg(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size,
+                             float* slope, void* stream, uint32_t flags) {
+    if (nx < 2 || ny < 2) return slm_fail(SLM_E_INVALID_ARG, "grid %u x %u: np.gradient needs 2 points", nx, ny);
+    if ((uint64_t)(nx + 2ull * pad) * (ny + 2ull * pad) >= 0x80000000ull)
+        return slm_fail(SLM_E_INVALID_ARG, "padded grid above 2^31 pixels");
+    if (!roi || !slope) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (!(pixel_size > 0.0f)) return slm_fail(SLM_E_INVALID_ARG, "pixel_size must be > 0");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)(nx + 2 * pad) * (ny + 2 * pad);
+    hipLaunchKernelGGL(k_slope_deg, grid_of(n), dim3(kBlock), 0, s, roi, nx, ny, pad, pixel_size, slope);
+    return slm_finish(stream, flags);
+}
 
 extern "C" int slm_gradient_uv(const float* roi, uint32_t nx, uint32_t ny, uint32_t pad, float pixel_size,
                                int do_normalize_speed, float* uv, uint8_t* mask, void* stream, uint32_t flags) {

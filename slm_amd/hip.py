@@ -80,6 +80,12 @@ _PROTOS = {
     "slm_fmm_hsl": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, ctypes.c_double, _P, _P, _P, _P, _P,
                                    ctypes.c_size_t, _P, _U32]),
     "slm_hsl_remap": (ctypes.c_int, [_P, _U32, _P, _P, _U32, _P, _P, _U32]),
+    "slm_map_hsl_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_map_hsl": (ctypes.c_int, [_P, _P, _U32, _U32, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, _P, _U32]),
+    "slm_map_aspect_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_map_aspect": (ctypes.c_int, [_P, _P, _P, _U32, _U32, _U32, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_float, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
+    "slm_slope_deg": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, _P, _P, _U32]),
     "slm_condition_uv": (ctypes.c_int, [_P, _U32, _U32, _U32, ctypes.c_float, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),

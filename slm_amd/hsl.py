@@ -167,3 +167,175 @@ def trace_mean_hsl(grid, hsl_averaging_threshold=5.0):
     v = g[~np.isnan(g)]
     v = v[v >= hsl_averaging_threshold]
     return np.mean(v) if v.shape[0] else None
+
+
+# ------------------------------------------------------------ map_hsl / map_aspect
+
+def map_hsl(hsl_grid, mask, hsl_dilation_width=51, hsl_mean_radius=30, pixel_size=1.0, stream=None):
+    """Mapping.map_hsl (mapping.py:987-1020) on the device: hsl_grid float32
+    [nx, ny] (remap_hsl's grid), mask bool (True = masked, the merged active
+    masks).  Returns hsl_smoothed float32 [nx, ny]."""
+    import torch
+    hip._require_cuda(hsl_grid, mask)
+    nx, ny = hsl_grid.shape
+    dfw = int(hsl_dilation_width / pixel_size)
+    mdr = int(hsl_mean_radius / pixel_size)
+    lib = hip.load()
+    ws_bytes = lib.slm_map_hsl_workspace_size(nx, ny)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=hsl_grid.device)
+    m8 = mask.to(torch.uint8).contiguous()
+    out = torch.empty_like(hsl_grid)
+    hip.check(lib.slm_map_hsl(hip._ptr(hsl_grid), hip._ptr(m8), nx, ny, dfw, mdr, hip._ptr(out), hip._ptr(ws),
+                              ws_bytes, hip._stream(stream), hip.SLM_FLAG_SYNC))
+    return out
+
+
+def slope_deg(roi, pad, pixel_size, stream=None):
+    """Preprocess.slope_array (preprocess.py:597) from the ROI DTM (float32
+    device [nx, ny]): degrees, 'edge'-padded by pad."""
+    import torch
+    hip._require_cuda(roi)
+    nx, ny = roi.shape
+    out = torch.empty((nx + 2 * pad, ny + 2 * pad), dtype=torch.float32, device=roi.device)
+    hip.check(hip.load().slm_slope_deg(hip._ptr(roi), nx, ny, int(pad), float(np.float32(pixel_size)),
+                                       hip._ptr(out), hip._stream(stream), hip.SLM_FLAG_SYNC))
+    return out
+
+
+def _int_radius(r, name):
+    if abs(r - round(r)) > 1e-9:
+        raise ValueError("%s = %g pixels: only whole-pixel disk radii are supported" % (name, r))
+    return int(round(r))
+
+
+def map_aspect(slope, uv, mapping, is_channel, aspect_median_filter_radius=4.0, uv_median_radius=50.0,
+               pixel_size=1.0, aspect_slope_threshold=15.0, do_aspect_median_filtering=True, stream=None):
+    """Mapping.map_aspect (mapping.py:1022-1053) on the device: slope float32
+    [nx, ny] (degrees), uv float32 [nx, ny, 2], mapping int32 [nx, ny]
+    (uint32 bits).  Returns (aspect float32 [nx, ny], mask bool [nx, ny]).
+    The disk radii (metres / pixel_size) must be whole pixels."""
+    import torch
+    hip._require_cuda(slope, uv, mapping)
+    nx, ny = slope.shape
+    r_s = _int_radius(aspect_median_filter_radius / pixel_size, "aspect_median_filter_radius")
+    r_uv = _int_radius(uv_median_radius / pixel_size, "uv_median_radius")
+    lib = hip.load()
+    ws_bytes = lib.slm_map_aspect_workspace_size(nx, ny)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=slope.device)
+    aspect = torch.empty((nx, ny), dtype=torch.float32, device=slope.device)
+    m8 = torch.empty((nx, ny), dtype=torch.uint8, device=slope.device)
+    hip.check(lib.slm_map_aspect(hip._ptr(slope), hip._ptr(uv), hip._ptr(mapping), int(is_channel), nx, ny, r_s,
+                                 r_uv, 1 if do_aspect_median_filtering else 0, float(aspect_slope_threshold),
+                                 hip._ptr(aspect), hip._ptr(m8), hip._ptr(ws), ws_bytes, hip._stream(stream),
+                                 hip.SLM_FLAG_SYNC))
+    return aspect, m8.bool()
+
+
+# ------------------------------------------------- HSL by aspect: host statistics
+
+def combine_hsl_aspect(aspect, aspect_mask, hsl_smoothed, mask, pad, n_bins=60, hsl_averaging_threshold=5.0):
+    """Mapping.combine_hsl_aspect (mapping.py:1055-1102) on host arrays (the
+    device grids of map_aspect and map_hsl, copied back): the unmasked
+    (aspect in degrees, HSL >= threshold) pairs sorted by aspect, the mean HSL
+    in n_bins + 1 aspect bins centred on -180, ..., 180 degrees (the two end
+    bins averaged, since they are one direction), and the [hsl, bin centre in
+    radians] array.  mask: the merged active masks (True = masked).
+    Returns (DataFrame of pairs, per-bin means, averages array)."""
+    import pandas as pd
+    a = np.asarray(aspect, np.float32)
+    h = np.asarray(hsl_smoothed, np.float32)
+    inner = np.index_exp[pad:-pad, pad:-pad]
+    m = np.asarray(mask, bool)[inner] | np.asarray(aspect_mask, bool)[inner]
+    asp, hv = a[inner][~m], h[inner][~m]
+    if hv.shape == (0,):
+        # nothing unmasked: the reference falls back to every pixel
+        asp, hv = a[inner].ravel(), h[inner].ravel()
+    asp = np.rad2deg(asp).astype(np.float32)
+    keep = hv >= hsl_averaging_threshold
+    pairs = np.stack((hv[keep], asp[keep]), axis=1)
+    df = pd.DataFrame(columns=["hsl", "aspect"], data=pairs[pairs[:, 1].argsort()])
+    half = 180 / n_bins
+    width = 2 * half
+    edges = np.linspace(-180, 180 + width, n_bins + 2) - half
+    df["groups"] = pd.cut(df["aspect"], edges)
+    means = df.groupby("groups", observed=False)["hsl"].mean()
+    hsl = np.array(means.values, dtype=np.float64)
+    west = (hsl[0] + hsl[-1]) / 2.0
+    hsl[0] = hsl[-1] = west
+    centres = np.deg2rad(edges[:-1] + half)
+    return df, means, np.stack((hsl, centres), axis=1)
+
+
+def hsl_aspect_stats(haa, df, n_hsl_split=4):
+    """Mapping.hsl_aspect_stats (mapping.py:1104-1251) on combine_hsl_aspect's
+    outputs: mean HSL overall, north- and south-facing (aspect >= 0 / <= 0),
+    their disparity and its ratio to the mean split standard deviation, the
+    mean HSL vector (magnitude, azimuth in degrees), and for the N-facing
+    (45 < aspect < 135) and S-facing (-135 < aspect < -45) HSL values the
+    two-sample tests (Kolmogorov-Smirnov, Mann-Whitney, rank-sum, Student and
+    Welch t), the Q-Q and P-P curves and the P-P disparity.  Returns a dict."""
+    from scipy.interpolate import interp1d
+    from scipy.stats import ks_2samp, mannwhitneyu, ranksums, ttest_ind
+    haa = np.array(haa, dtype=np.float64)
+    ok = ~np.isnan(haa[:, 0])
+    hsl, asp = haa[ok, 0], haa[ok, 1]
+    south, north = hsl[asp <= 0.0], hsl[asp >= 0.0]
+    out = {}
+    out["hsl_mean"] = float(np.mean(hsl[1:]) if -asp[0] == asp[-1] else np.mean(hsl))
+    out["hsl_mean_south"] = float(np.mean(south)) if south.size else 0.0
+    out["hsl_mean_north"] = float(np.mean(north)) if north.size else 0.0
+    out["hsl_ns_disparity"] = abs(out["hsl_mean_north"] - out["hsl_mean_south"])
+    out["hsl_ns_disparity_normed"] = out["hsl_ns_disparity"] / out["hsl_mean"]
+    out["hsl_stddev"] = float(np.std(hsl))
+    splits = []
+    for part in np.split(haa[1:, 0], n_hsl_split):
+        v = part[~np.isnan(part)]
+        splits.append(np.std(v) if v.size else 0.0)
+    out["hsl_split_stddev"] = float(np.mean(splits)) if splits else 0.0
+    out["hsl_split_stddev_normed"] = out["hsl_split_stddev"] / out["hsl_mean"]
+    haa[np.isnan(haa[:, 0])] = 0.0
+    vec = haa[:, 0] * np.exp(1j * haa[:, 1])
+    vec = vec[~np.isnan(vec)]
+    mean_vec = np.mean(vec)
+    out["hsl_mean_magnitude"] = float(abs(mean_vec))
+    out["hsl_mean_azimuth"] = float(np.rad2deg(np.angle(mean_vec)))
+    out["hsl_ns_disparity_confidence"] = out["hsl_ns_disparity"] / out["hsl_split_stddev"]
+    hsl_n = df.hsl[(df.aspect > +45) & (df.aspect < +135)]
+    hsl_s = df.hsl[(df.aspect < -45) & (df.aspect > -135)]
+    un, us = hsl_n.unique(), hsl_s.unique()
+    nn, ns = un.shape[0], us.shape[0]
+    if nn > 0 and ns > 0:
+        out["hsl_ns_ks_nm"] = nn * ns
+        out["hsl_ns_ks_nmfactor"] = float(np.sqrt((nn + ns) / (nn * ns)))
+        ks, mw, rsum = ks_2samp(un, us), mannwhitneyu(un, us), ranksums(un, us)
+        tt, welch = ttest_ind(un, us, equal_var=True), ttest_ind(un, us, equal_var=False)
+        out["hsl_ns_ks"] = (float(ks.statistic), float(ks.pvalue))
+        out["hsl_ns_ttest"] = (float(tt.statistic), float(tt.pvalue))
+        out["hsl_ns_mannwhitney"] = (float(mw.statistic) / (nn * ns), float(mw.pvalue))
+        out["hsl_ns_ranksum"] = (float(rsum.statistic), float(rsum.pvalue))
+        out["hsl_ns_welch"] = (float(welch.statistic), float(welch.pvalue))
+        n_min, s_min, n_max, s_max = min(hsl_n), min(hsl_s), max(hsl_n), max(hsl_s)
+        ns_min, ns_max = (n_min + s_min) / 2, (n_max + s_max) / 2
+        pc = np.linspace(0, 100, num=50)
+        pn, ps = np.percentile(hsl_n, pc), np.percentile(hsl_s, pc)
+        pns = (pn + ps) / 2
+        qn, qs = interp1d(pns, pn), interp1d(pns, ps)
+        fn, fs = interp1d(pn, pc), interp1d(ps, pc)
+        s_from_n = fs(np.clip(pn, s_min, s_max))
+        n_from_s = fn(np.clip(ps, n_min, n_max))
+        grid = np.linspace(ns_min, ns_max, num=50)
+        out["hsl_ns_qq_array"] = np.stack((qs(grid), qn(grid)), axis=1).T
+        fx = np.concatenate((pc, n_from_s))
+        fy = np.concatenate((s_from_n, pc))
+        fm = np.concatenate((s_from_n + pc, pc + n_from_s)) / 2
+        fxym = np.stack((fx, fy, fm), axis=1)
+        fxym = fxym[fxym[:, 0].argsort()].T
+        fxm = interp1d(fxym[2], fxym[0])(pc)
+        fym = interp1d(fxym[2], fxym[1])(pc)
+        fmm = interp1d(fxym[2], fxym[2])(pc)
+        pp = np.stack((pc, fxm, fym, fmm), axis=1).T
+        out["hsl_ns_pp_array"] = pp
+        trap = getattr(np, "trapezoid", None) or getattr(np, "trapz")
+        out["hsl_ns_pp_diff"] = float((50 - (trap(pp[1], pp[2]) / 100)) * 2)
+        out["hsl_ns_min"], out["hsl_ns_max"] = float(ns_min), float(ns_max)
+    return out

@@ -150,7 +150,7 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
         logx = torch.log(pos).cpu().numpy().astype(np.float32)
         dist = Univariate_distribution(logx_array=logx, logy_array=logx, n_hist_bins=512, n_pdf_points=256)
         dist.compute_kde_opencl(kernel="gaussian", bandwidth=0.15, fp_contract=fp_contract)
-    fmm_res = fmm_grid = fmm_trace_mean = None
+    fmm_res = fmm_grid = fmm_trace_mean = ha_stats = None
     if fmm:
         # the reference's live HSL path (mapping.py:490-494): FMM lengths per
         # subsegment, remapped onto the grid
@@ -158,6 +158,17 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
         fmm_res = fmm_hsl.fmm_lengths(cm.label, cm.mapping, mask_b, int(info.is_thinchannel), job.pixel_size)
         fmm_grid = fmm_hsl.remap_hsl(cm.label, fmm_res)
         fmm_trace_mean = fmm_hsl.trace_mean_hsl(fmm_grid)
+        # map_hsl, map_aspect, combine_hsl_aspect, hsl_aspect_stats (mapping.py:987-1251)
+        st.mark("aspect")
+        pad_w = int(info.pad_width)
+        slope_full = fmm_hsl.slope_deg(roi, pad_w, job.pixel_size)
+        slope_c = slope_full[ox:ox + nxp, oy:oy + nyp].contiguous()
+        hsl_smoothed = fmm_hsl.map_hsl(fmm_grid, mask_b, 51, 30, job.pixel_size)
+        aspect, aspect_mask = fmm_hsl.map_aspect(slope_c, uv, cm.mapping, int(info.is_channel), 4.0, 50.0,
+                                                 job.pixel_size)
+        df_ha, _, haa = fmm_hsl.combine_hsl_aspect(aspect.cpu().numpy(), aspect_mask.cpu().numpy(),
+                                                   hsl_smoothed.cpu().numpy(), mask_b.cpu().numpy(), pad_w)
+        ha_stats = fmm_hsl.hsl_aspect_stats(haa, df_ha)
     st.mark("end")
     timings = st.report()
     results = {
@@ -180,6 +191,9 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
         results["fmm_colours"] = int(fmm_res.n_colours)
         results["fmm_mean_hsl_m"] = fmm_res.fmm_mean_hsl
         results["fmm_trace_mean_hsl_m"] = None if fmm_trace_mean is None else float(fmm_trace_mean)
+        results["hsl_aspect"] = {k: ha_stats[k] for k in ("hsl_mean", "hsl_mean_north", "hsl_mean_south",
+                                                           "hsl_ns_disparity", "hsl_mean_azimuth",
+                                                           "hsl_ns_disparity_confidence") if k in ha_stats}
     results["midslope_pixels"] = int(((cm.mapping & int(info.is_midslope)) != 0).sum().item())
     results["ridge_pixels"] = int(((cm.mapping & int(info.is_ridge)) != 0).sum().item())
     if return_grids:

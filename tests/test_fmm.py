@@ -100,3 +100,67 @@ def test_colouring_separates_pairs():
     colour, n_col = hsl.colour_labels(n_labels, problems, pairs)
     assert colour[0] == 0xFF and (colour[1:] < n_col).all()
     assert (colour[lo] != colour[hi]).all()
+
+
+def test_rank_filters_closed_forms():
+    """The oracle's skimage rank mean / median restatements on images whose
+    results are known: a constant image; a lone spike (the median removes it,
+    the mean spreads it as spike / footprint size); edges count only in-image
+    pixels; the footprint disk(r) = {x^2 + y^2 <= r^2}."""
+    from oracle import hsl_map_ref as H
+    assert len(H.disk_offsets(1)) == 5 and len(H.disk_offsets(2)) == 13 and len(H.disk_offsets(3)) == 29
+    c = np.full((9, 11), 77, np.uint16)
+    assert np.array_equal(H.rank_mean(c, 3), c)
+    assert np.array_equal(H.rank_median(c.astype(np.uint8), 2), c.astype(np.uint8))
+    sp = np.zeros((15, 15), np.uint16)
+    sp[7, 7] = 1300
+    m = H.rank_mean(sp, 2)
+    assert m[7, 7] == 1300 // 13 and m[7, 9] == 1300 // 13 and m[7, 10] == 0
+    assert not H.rank_median(sp.astype(np.uint8) * 0 + (sp > 0).astype(np.uint8) * 200, 1).any()
+    # corner: disk(1) has 3 in-image pixels at (0, 0)
+    e = np.zeros((5, 5), np.uint16)
+    e[0, 1] = 9
+    assert H.rank_mean(e, 1)[0, 0] == 3
+    # median: smallest v with 2 * #(<= v) >= pop; pop = 5 inside: values 1,2,3,4,5 -> 3
+    g = np.zeros((3, 3), np.uint8)
+    g[1, 1], g[0, 1], g[2, 1], g[1, 0], g[1, 2] = 3, 1, 5, 2, 4
+    assert H.rank_median(g, 1)[1, 1] == 3
+
+
+def _aspect_case(n_bins=60, north_extra=0.0, seed=0):
+    rs = np.random.default_rng(seed)
+    nx = ny = 64
+    aspect = rs.uniform(-np.pi, np.pi, (nx, ny)).astype(np.float32)
+    hsl = (50.0 + rs.normal(0, 5, (nx, ny))).astype(np.float32)
+    hsl = np.where(np.sin(aspect) > 0, hsl + north_extra, hsl).astype(np.float32)
+    return aspect, np.zeros((nx, ny), bool), hsl, np.zeros((nx, ny), bool)
+
+
+def test_combine_hsl_aspect_bins():
+    from slm_amd import hsl as H
+    aspect, amask, h, mask = _aspect_case()
+    df, means, haa = H.combine_hsl_aspect(aspect, amask, h, mask, pad=2, n_bins=60)
+    assert haa.shape == (61, 2)
+    assert np.isclose(np.rad2deg(haa[0, 1]), -180.0) and np.isclose(np.rad2deg(haa[-1, 1]), 180.0)
+    assert haa[0, 0] == haa[-1, 0]                                   # +-180 degrees is one direction
+    # a middle bin's mean by direct binning
+    deg = np.rad2deg(aspect[2:-2, 2:-2]).astype(np.float32).ravel()
+    hv = h[2:-2, 2:-2].ravel()
+    sel = (deg > -3.0) & (deg <= 3.0) & (hv >= 5.0)
+    assert np.isclose(haa[30, 0], hv[sel].astype(np.float64).mean())
+    assert np.all(np.diff(df.aspect.values) >= 0)
+
+
+def test_hsl_aspect_stats_disparity():
+    from slm_amd import hsl as H
+
+    def stats(extra):
+        df, _, haa = H.combine_hsl_aspect(*_aspect_case(north_extra=extra), pad=2)
+        return H.hsl_aspect_stats(haa, df)
+
+    even, skew = stats(0.0), stats(20.0)
+    assert even["hsl_ns_disparity"] < 2.0
+    assert skew["hsl_mean_north"] > skew["hsl_mean_south"] + 15.0
+    assert skew["hsl_ns_disparity_confidence"] > even["hsl_ns_disparity_confidence"]
+    assert 60.0 < skew["hsl_mean_azimuth"] < 120.0                  # the mean vector points north
+    assert skew["hsl_ns_ks"][1] < 1e-6 and skew["hsl_ns_pp_diff"] > 50
