@@ -143,39 +143,82 @@ __global__ void k_zs_init(uint32_t nxy, const uint32_t* __restrict__ mapping, ui
     if (i < nxy) a[i] = (mapping[i] & (F_CHANNEL | F_INTER)) ? 1 : 0;
 }
 
-// One Zhang-Suen subiteration a -> b; *changed += pixels removed.
-__global__ void k_zs_sub(uint32_t nx, uint32_t ny, int sub, const uint8_t* __restrict__ a,
-                         uint8_t* __restrict__ b, uint32_t* __restrict__ changed) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool del = false;
-    if (i < nx * ny) {
-        const uint8_t v = a[i];
-        uint8_t out = v;
-        if (v) {
-            const int x = (int)(i / ny), y = (int)(i % ny);
-            auto g = [&](int xx, int yy) -> int {
-                return (xx < 0 || yy < 0 || xx >= (int)nx || yy >= (int)ny) ? 0 : a[(size_t)xx * ny + yy];
-            };
-            // P2..P9 = N, NE, E, SE, S, SW, W, NW (N = row x-1)
-            const int q[8] = {g(x - 1, y), g(x - 1, y + 1), g(x, y + 1), g(x + 1, y + 1),
-                              g(x + 1, y), g(x + 1, y - 1), g(x, y - 1), g(x - 1, y - 1)};
-            int B = 0, A = 0;
+// The Zhang-Suen deletion test of a set pixel from its 8 neighbours
+// q = P2..P9 = N, NE, E, SE, S, SW, W, NW (N = row x-1), subiteration sub.
+__device__ __forceinline__ bool zs_delete(const int q[8], int sub) {
+    int B = 0, A = 0;
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                B += q[j];
-                A += (q[j] == 0 && q[(j + 1) & 7] == 1);
-            }
-            const int c1 = sub == 0 ? q[0] * q[2] * q[4] : q[0] * q[2] * q[6];
-            const int c2 = sub == 0 ? q[2] * q[4] * q[6] : q[0] * q[4] * q[6];
-            if (B >= 2 && B <= 6 && A == 1 && c1 == 0 && c2 == 0) {
-                out = 0;
-                del = true;
-            }
-        }
-        b[i] = out;
+    for (int j = 0; j < 8; j++) {
+        B += q[j];
+        A += (q[j] == 0 && q[(j + 1) & 7] == 1);
     }
-    const u64 m = __ballot(del);
-    if (m && (threadIdx.x & 63) == 0) atomicAdd(changed, (uint32_t)__popcll(m));
+    const int c1 = sub == 0 ? q[0] * q[2] * q[4] : q[0] * q[2] * q[6];
+    const int c2 = sub == 0 ? q[2] * q[4] * q[6] : q[0] * q[4] * q[6];
+    return B >= 2 && B <= 6 && A == 1 && c1 == 0 && c2 == 0;
+}
+
+// Zhang-Suen iterations in LDS tiles: a 256-thread block loads a 64x64 tile
+// with a 2K-pixel halo (outside the grid reads 0) and runs K iterations (2K
+// subiterations) on it; after s subiterations the outer s pixels of the
+// region are stale, so the tile's own pixels are exact after all 2K.  It
+// writes them to b and ORs into *iter_mask bit j when iteration j deleted one
+// of them.  a -> b, ping-pong across launches.
+constexpr int kZsT = 64;
+constexpr int kZsK = 8;
+constexpr int kZsH = 2 * kZsK;
+constexpr int kZsR = kZsT + 2 * kZsH;
+
+__global__ __launch_bounds__(256) void k_zs_tiles(uint32_t nx, uint32_t ny, const uint8_t* __restrict__ a,
+                                                  uint8_t* __restrict__ b, uint32_t* __restrict__ iter_mask) {
+    __shared__ uint8_t buf[2][kZsR * kZsR];
+    __shared__ uint32_t s_mask;
+    const uint32_t nty = (ny + kZsT - 1) / kZsT;
+    const int gx0 = (int)(blockIdx.x / nty) * kZsT - kZsH, gy0 = (int)(blockIdx.x % nty) * kZsT - kZsH;
+    if (threadIdx.x == 0) s_mask = 0;
+    bool any = false;
+    for (int k = threadIdx.x; k < kZsR * kZsR; k += blockDim.x) {
+        const int gx = gx0 + k / kZsR, gy = gy0 + k % kZsR;
+        uint8_t v = 0;
+        if (gx >= 0 && gy >= 0 && gx < (int)nx && gy < (int)ny) v = a[(size_t)gx * ny + gy];
+        buf[0][k] = v;
+        any |= v != 0;
+    }
+    if (!__syncthreads_or(any)) {
+        // nothing set in the region: the tile's pixels stay 0
+        for (int k = threadIdx.x; k < kZsT * kZsT; k += blockDim.x) {
+            const int gx = gx0 + kZsH + k / kZsT, gy = gy0 + kZsH + k % kZsT;
+            if (gx < (int)nx && gy < (int)ny) b[(size_t)gx * ny + gy] = 0;
+        }
+        return;
+    }
+    int cur = 0;
+    for (int sub = 0; sub < 2 * kZsK; sub++) {
+        bool del_in = false;
+        for (int k = threadIdx.x; k < kZsR * kZsR; k += blockDim.x) {
+            const int lx = k / kZsR, ly = k % kZsR;
+            uint8_t v = buf[cur][k];
+            if (v && lx > 0 && ly > 0 && lx < kZsR - 1 && ly < kZsR - 1) {
+                const uint8_t* c = &buf[cur][k];
+                const int q[8] = {c[-kZsR], c[-kZsR + 1], c[1], c[kZsR + 1], c[kZsR], c[kZsR - 1], c[-1], c[-kZsR - 1]};
+                if (zs_delete(q, sub & 1)) {
+                    v = 0;
+                    if (lx >= kZsH && ly >= kZsH && lx < kZsH + kZsT && ly < kZsH + kZsT &&
+                        gx0 + lx < (int)nx && gy0 + ly < (int)ny)
+                        del_in = true;
+                }
+            }
+            buf[cur ^ 1][k] = v;
+        }
+        if (__syncthreads_or(del_in) && threadIdx.x == 0) s_mask |= 1u << (sub >> 1);
+        cur ^= 1;
+    }
+    for (int k = threadIdx.x; k < kZsT * kZsT; k += blockDim.x) {
+        const int lx = kZsH + k / kZsT, ly = kZsH + k % kZsT;
+        const int gx = gx0 + lx, gy = gy0 + ly;
+        if (gx < (int)nx && gy < (int)ny) b[(size_t)gx * ny + gy] = buf[cur][lx * kZsR + ly];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_mask) atomicOr(iter_mask, s_mask);
 }
 
 __global__ void k_zs_final(uint32_t nxy, const uint8_t* __restrict__ a, uint32_t* __restrict__ mapping) {
@@ -916,24 +959,33 @@ inline void zero_u64(u64* a, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_u64, dim3((uint32_t)std::max<size_t>(blocks, 1)), dim3(kBlock), 0, s, a, n);
 }
 
-// Workspace views.
 // Zhang-Suen thinning of the binary image a (b: scratch of the same size) to
-// convergence; one host synchronisation per iteration.
+// convergence, kZsK iterations per launch (k_zs_tiles); one host
+// synchronisation per launch.  The iteration count is the global loop's: the
+// iterations that deleted a pixel plus the first one that deleted none.
 int zs_thin(uint8_t* a, uint8_t* b, uint32_t nx, uint32_t ny, uint32_t* ctr, hipStream_t s, uint32_t* iters_out) {
-    const dim3 grid((nx * ny + kBlock - 1) / kBlock);
-    uint32_t iters = 0;
+    const uint32_t n_tiles = ((nx + kZsT - 1) / kZsT) * ((ny + kZsT - 1) / kZsT);
     const uint32_t max_iters = nx + ny + 2;
-    for (;;) {
+    uint8_t* src = a;
+    uint8_t* dst = b;
+    uint32_t iters = 0;
+    for (uint32_t done = 0;; done += kZsK) {
         zero_u64((u64*)ctr, 1, s);
-        hipLaunchKernelGGL(k_zs_sub, grid, dim3(kBlock), 0, s, nx, ny, 0, (const uint8_t*)a, b, ctr);
-        hipLaunchKernelGGL(k_zs_sub, grid, dim3(kBlock), 0, s, nx, ny, 1, (const uint8_t*)b, a, ctr);
-        uint32_t changed = 0;
-        HIP_TRY(hipMemcpyAsync(&changed, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_zs_tiles, dim3(n_tiles), dim3(256), 0, s, nx, ny, (const uint8_t*)src, dst, ctr);
+        std::swap(src, dst);
+        uint32_t mask = 0;
+        HIP_TRY(hipMemcpyAsync(&mask, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        iters++;
-        if (!changed) break;
-        if (iters >= max_iters) return slm_fail(SLM_E_HIP, "thinning did not converge in %u iterations", iters);
+        // the deleting iterations of a launch are a prefix (an iteration that
+        // deletes nothing leaves nothing to delete after it)
+        if (mask != (1u << kZsK) - 1u) {
+            iters = done + (mask ? 32u - (uint32_t)__builtin_clz(mask) : 0u) + 1u;
+            break;
+        }
+        if (done + kZsK >= max_iters)
+            return slm_fail(SLM_E_HIP, "thinning did not converge in %u iterations", done + kZsK);
     }
+    if (src != a) HIP_TRY(hipMemcpyAsync(a, src, (size_t)nx * ny, hipMemcpyDeviceToDevice, s));
     if (iters_out) *iters_out = iters;
     return SLM_OK;
 }

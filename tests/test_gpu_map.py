@@ -123,13 +123,19 @@ def test_chain_stepwise_bit_exact(gpu, kind, contract):
     assert gm.n_segments == solo.n_segments and gm.n_segments > 0
 
 
-def test_thinning_bit_exact(gpu):
+@pytest.mark.parametrize("shape,density,grow", [((300, 420), 0.985, 4), ((257, 130), 0.996, 14), ((70, 65), 1.0, 0)])
+def test_thinning_bit_exact(gpu, shape, density, grow):
+    """Bit-exact skeleton and iteration count; thick blobs need more than one
+    launch of the tiled kernel (8 iterations each); an empty image needs one
+    iteration."""
     import torch
     from scipy import ndimage
     from oracle import oracle
     from slm_amd import hip
     rs = np.random.default_rng(7)
-    img = ndimage.binary_dilation(rs.random((300, 420)) > 0.985, iterations=4)
+    img = rs.random(shape) > density
+    if grow:
+        img = ndimage.binary_dilation(img, iterations=grow)
     p = workloads.planar(n=8).params()
     p.nx_padded, p.ny_padded, p.nxy_padded = img.shape[0], img.shape[1], img.size
     m = np.where(img, np.uint32(1), np.uint32(0)).astype(np.uint32).reshape(-1)
