@@ -10,9 +10,14 @@ metric: streamline-steps/s = accepted RK2 steps of all streamlines, both
 directions, counted exactly by the kernel, divided by wall time of the timed
 steps.  ms_per_step is the 2048^2 slc/slt map time.
 
-roofline: dominant kernel k_fields, algorithmic 73 B per step (64 B of uv
-gathers, 1 B mask, 8 B of slc/slt atomics) x steps per launch / mean launch
-time from HIP events on the launch stream; peak 8 TB/s HBM.
+The default path traces both directions in one persistent launch
+(slm_trace_fields_batch, one job per direction); --path queue and --path
+tiled time one launch per direction.
+
+roofline: dominant kernel k_fields_batch, algorithmic 73 B per step (64 B of
+uv gathers, 1 B mask, 8 B of slc/slt atomics) x steps per launch / mean
+launch time from HIP events on the launch stream (the events bracket the
+whole call: seed sort, uv quads and commit kernels included); peak 8 TB/s HBM.
 cpu_baseline: the CPU oracle (C restatement of the reference kernels, OpenMP)
 on a bounded random sample of the same seeds, steps/s.
 """
@@ -306,8 +311,10 @@ def main():
     ap.add_argument("--weights", default="", help="surrogate: load trained weights (.npz) instead of training")
     ap.add_argument("--save-weights", default="", help="surrogate: save the trained weights (.npz)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--path", default="queue", choices=["tiled", "queue"],
-                    help="tiled: LDS-owned tiles (slm_trace_fields_tiled); queue: global work queue")
+    ap.add_argument("--path", default="batch", choices=["batch", "queue", "tiled"],
+                    help="batch: both directions in one persistent launch (slm_trace_fields_batch); "
+                         "queue: one slm_trace_fields launch per direction; "
+                         "tiled: LDS-owned tiles (slm_trace_fields_tiled), one launch per direction")
     ap.add_argument("--tile", type=int, default=0, help="tile side for --path tiled (0: library default)")
     ap.add_argument("--block", type=int, default=0, help="threads per block for --path tiled")
     ap.add_argument("--unit", type=int, default=0, help="states per unit for --path tiled")
@@ -351,13 +358,27 @@ def main():
     slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     steps = torch.zeros(2, dtype=torch.int64, device=dev)
     topts = hip.tile_opts(args.tile, args.block, args.unit, args.cap, args.halo)
+    jobs = None
     if args.path == "tiled":
         ws = torch.empty(hip.tiled_workspace_size(pd, topts), dtype=torch.uint8, device=dev)
+    elif args.path == "batch":
+        jobs = hip.fields_jobs([(pd, seeds, mask, uv, slc[0], slt[0]), (pu, seeds, mask, uv, slc[1], slt[1])])
+        ws = torch.empty(hip.batch_workspace_size(jobs), dtype=torch.uint8, device=dev)
     else:
         ws = torch.empty(hip.fields_workspace_size(pd), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream()
+    launches = 1 if args.path == "batch" else 2
 
     def one_step(keep_order, events=None):
+        if args.path == "batch":
+            # both directions (each job its own slc/slt) in one launch; the
+            # seeds are sorted into tile order for both jobs inside the call
+            if events is not None:
+                events[0][0].record(stream)
+            hip.trace_fields_batch(jobs, steps[0:1], workspace=ws, stream=stream, keep_order=keep_order)
+            if events is not None:
+                events[0][1].record(stream)
+            return
         for i, p in enumerate((pd, pu)):
             if events is not None:
                 events[i][0].record(stream)
@@ -375,22 +396,23 @@ def main():
         one_step(False)
     torch.cuda.synchronize()
     steps.zero_()
-    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(2)] for _ in range(args.steps)]
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(launches)]
+          for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        one_step(False, ev[k])  # each step re-sorts the seeds (downstream) and reuses them upstream
+        one_step(False, ev[k])  # each step re-sorts the seeds (the per-direction paths reuse them upstream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_steps = int(steps.sum().item())
-    launch_ms = [e[i][0].elapsed_time(e[i][1]) for e in ev for i in range(2)]
+    launch_ms = [e[i][0].elapsed_time(e[i][1]) for e in ev for i in range(launches)]
     mean_launch_s = float(np.mean(launch_ms)) / 1e3
-    steps_per_launch = total_steps / (2 * args.steps)
+    steps_per_launch = total_steps / (launches * args.steps)
     achieved = steps_per_launch * BYTES_PER_STEP / mean_launch_s / 1e9
     value = total_steps / wall
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", ("k_tl_process" if args.path == "tiled" else "k_fields") +
-                         "_tile_2048_traffic.txt")
+    kname = {"batch": "k_fields_batch", "queue": "k_fields", "tiled": "k_tl_process"}[args.path]
+    tfile = os.path.join(ROOT, "profiles", kname + "_tile_2048_traffic.txt")
     if args.workload == "tile_2048" and os.path.exists(tfile):
         # measured HBM-side bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
         for line in open(tfile):
@@ -417,7 +439,7 @@ def main():
                    "directions": 2, "map_ms": wall / args.steps * 1e3},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_tl_process (with its per-round binning)" if args.path == "tiled" else "k_fields",
+                     "kernel": "k_tl_process (with its per-round binning)" if args.path == "tiled" else kname,
                      "path": args.path,
                      "tile_opts": {"tile": topts.tile, "block": topts.block, "unit": topts.unit,
                                    "cap": topts.cap_steps, "halo": topts.halo} if args.path == "tiled" else None,

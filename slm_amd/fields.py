@@ -3,8 +3,8 @@
 Drop-in for python/streamlines/fields.py: same class name, constructor
 signature and in-place ``data`` contract.  ``Fields.integrate`` creates the
 seeds (useful.create_seeds), traces both directions on the GPU through
-libslm_hip.so (slm_trace_fields, replacing kernel integrate_fields launched at
-fields.py:233-274), and finishes the fields on the GPU (slm_fields_finish_raw /
+libslm_hip.so (one slm_trace_fields_batch launch, replacing the two
+integrate_fields launches of fields.py:233-274), and finishes the fields on the GPU (slm_fields_finish_raw /
 slm_fields_finish_scaled) exactly as fields.py:296-309 and :141-164 do.  The OpenCL platform/device arguments are accepted and ignored:
 the current HIP device (torch.cuda.current_device()) is used.
 """
@@ -54,7 +54,7 @@ class Fields:
         vprint(self.verbose, "...done")
 
     def gpu_integrate(self):
-        """fields.py:169-311 with both directions traced by slm_trace_fields."""
+        """fields.py:169-311 with both directions traced by one slm_trace_fields_batch launch."""
         import torch
         info = self.info
         dev = self.device
@@ -65,16 +65,18 @@ class Fields:
         uv_d = torch.from_numpy(np.ascontiguousarray(self.data.uv_array, dtype=np.float32).reshape(-1)).to(dev)
         slc_d = torch.empty((2, nxp * nyp), dtype=torch.int32, device=dev)
         slt_d = torch.empty((2, nxp * nyp), dtype=torch.int32, device=dev)
-        steps_d = torch.zeros(2, dtype=torch.int64, device=dev)
-        ws = None
+        steps_d = torch.zeros(1, dtype=torch.int64, device=dev)
+        jobs = []
         for idx, sign in ((0, +1.0), (1, -1.0)):
             info.downup_sign = sign
             p = kernel_params(info, sign, "INTEGRATE_FIELDS", self.fp_contract)
-            ws = hip.trace_fields(p, seeds, mask_d, uv_d, slc_d[idx], slt_d[idx], steps_d[idx:idx + 1],
-                                  workspace=ws, keep_order=(idx == 1))
+            jobs.append((p, seeds, mask_d, uv_d, slc_d[idx], slt_d[idx]))
+        # both directions in one persistent launch; each direction's slc/slt
+        # are bit-identical to a one-direction slm_trace_fields call
+        hip.trace_fields_batch(hip.fields_jobs(jobs), steps_d)
         slt_f, sla_f = hip.fields_finish_raw(slc_d, slt_d, info.subpixel_seed_point_density)
         torch.cuda.synchronize(dev)
-        self.steps = [int(v) for v in steps_d.cpu()]
+        self.steps = int(steps_d.item())
         self._slc_raw, self._slt = slc_d, slt_f
         self.raw_slc = np.ascontiguousarray(hip.as_u32(slc_d).reshape(2, nxp, nyp).transpose(1, 2, 0))
         self.raw_slt = np.ascontiguousarray(hip.as_u32(slt_d).reshape(2, nxp, nyp).transpose(1, 2, 0))

@@ -99,13 +99,12 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     nxy = nxp * nyp
     slc = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
-    steps = torch.zeros(2, dtype=torch.int64, device=dev)
-    ws = None
+    steps = torch.zeros(1, dtype=torch.int64, device=dev)
     info.n_seed_points = np.uint32(job.seeds.shape[0])
-    for k, sign in enumerate((1, -1)):
-        p = kernel_params(info, sign, "INTEGRATE_FIELDS", fp_contract)
-        ws = hip.trace_fields(p, seeds, mask, uv.reshape(-1), slc[k], slt[k], steps[k:k + 1], workspace=ws,
-                              keep_order=(k == 1))
+    # both directions in one persistent launch (no end-of-launch tail between them)
+    jobs = hip.fields_jobs([(kernel_params(info, sign, "INTEGRATE_FIELDS", fp_contract), seeds, mask,
+                             uv.reshape(-1), slc[k], slt[k]) for k, sign in enumerate((1, -1))])
+    hip.trace_fields_batch(jobs, steps)
 
     st.mark("finish")
     slt_f, sla_f = hip.fields_finish_raw(slc, slt, int(info.subpixel_seed_point_density))
@@ -172,7 +171,7 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     st.mark("end")
     timings = st.report()
     results = {
-        "steps": [int(v) for v in steps.cpu()],
+        "steps": int(steps.item()),
         "channel_threshold": thr,
         "chain_ok": bool(ok_chain),
         "n_segments": int(cm.n_segments),

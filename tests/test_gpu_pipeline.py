@@ -24,7 +24,8 @@ def test_map_tile_small_256(gpu):
     assert 0 < res["hsl_seeds"] <= res["midslope_pixels"]
     assert res["hsl_reached"] > 0.2 * res["hsl_seeds"]
     assert res["kde_mode_x"] is not None and np.isfinite(res["kde_mode_x"])
-    # fields steps equal a direct call on the host-prepared inputs
+    # the batched both-directions launch takes as many steps as two direct
+    # one-direction calls on the host-prepared inputs
     seeds = torch.from_numpy(job.seeds).to(gpu)
     mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(gpu)
     uv = torch.from_numpy(job.uv.reshape(-1)).to(gpu)
@@ -32,8 +33,9 @@ def test_map_tile_small_256(gpu):
     slc = torch.empty(nxy, dtype=torch.int32, device=gpu)
     slt = torch.empty_like(slc)
     steps = torch.zeros(1, dtype=torch.int64, device=gpu)
-    hip.trace_fields(job.params(1), seeds, mask, uv, slc, slt, steps, sync=True)
-    assert int(steps.item()) == res["steps"][0]
+    for sign in (1, -1):
+        hip.trace_fields(job.params(sign), seeds, mask, uv, slc, slt, steps, sync=True)
+    assert int(steps.item()) == res["steps"]
 
 
 def test_map_tile_with_conditioning(gpu):
