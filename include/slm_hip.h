@@ -106,6 +106,16 @@ int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws,
                    uint32_t* rng0_out, uint32_t* draws, void* stream, uint32_t flags);
 
 /*
+ * Self-check of the tracer's arithmetic: the short correctly rounded sqrt and
+ * divide forms the kernels use against the IEEE operators, bit for bit, over
+ * every float in their ranges (counts: device uint64 [8], added to; entries
+ * 0, 2, 4, 6 and 7 count mismatches, 1, 3 and 5 the values tested, see
+ * slm_hip.hip).  No counterpart in the
+ * reference; it pins that the short forms change no bits on this device.
+ */
+int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void* stream, uint32_t flags);
+
+/*
  * Jittered streamline fields (slc / slt), one direction.
  * Replaces kernel integrate_fields (opencl/integratefields.cl:60-114 with
  * jittertrajectory.cl:42-88) as launched by Fields.gpu_integrate

@@ -127,6 +127,74 @@ extern "C" int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint3
     return slm_finish(stream, flags);
 }
 
+// ------------------------------------------------------------------ rounding check
+//
+// The short sqrt / divide forms of trace_common.hpp against the operators,
+// bit for bit.  counts (uint64 [8], accumulated):
+//   [0]/[1] sqrt_short(d) != sqrtf(d), of every float d in [2^-96, 2^100]
+//   [2]/[3] div_short(1, b, recip_short(b)) != 1/b, of every b in [2^-48, 2^50]
+//   [4]/[5] div_short(a, b, recip_short(b)) != a/b, n_num numerators per b with
+//           |a| in [2^-70, 8b], either sign (hashed exponent and mantissa)
+//   [6]     sqrt_cr(d) / inv_sqrt_cr(d) differ from the operators, d = any bits,
+//           or sqrt_short(+0) differs from sqrtf(+0)
+//   [7]     div_by_length(x, y) differs from x/len, y/len, x = any bits, y hashed
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_check_rounding(unsigned long long* __restrict__ counts, uint32_t n_num) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+        const uint32_t bits = (uint32_t)i;
+        const float d = __uint_as_float(bits);
+        if (sqrt_short_ok(d)) {
+            c[1]++;
+            if (__float_as_uint(sqrt_short(d)) != __float_as_uint(fsqrt(d))) c[0]++;
+        }
+        if (d >= 0x1p-48f && d <= 0x1p50f) {
+            c[3]++;
+            const float r = recip_short(d);
+            if (__float_as_uint(div_short(1.0f, d, r)) != __float_as_uint(fdiv(1.0f, d))) c[2]++;
+            const uint32_t eb = (bits >> 23) & 0xffu;
+            const uint32_t lo = 127u - 70u, span = eb + 3u - lo + 1u;
+            for (uint32_t j = 0; j < n_num; j++) {
+                const uint32_t h = mix32(bits * 0x9e3779b9u + j * 0x85ebca6bu + 1u);
+                uint32_t nb = ((lo + h % span) << 23) | (mix32(h) & 0x7fffffu);
+                if (__uint_as_float(nb) > 8.0f * d) nb = __float_as_uint(8.0f * d);
+                nb |= h & 0x80000000u;
+                const float a = __uint_as_float(nb);
+                c[5]++;
+                if (__float_as_uint(div_short(a, d, r)) != __float_as_uint(fdiv(a, d))) c[4]++;
+            }
+        }
+        if (__float_as_uint(sqrt_cr(d)) != __float_as_uint(fsqrt(d)) ||
+            (d == 0.0f && __float_as_uint(sqrt_short(fabsf(d))) != __float_as_uint(fsqrt(fabsf(d)))) ||
+            __float_as_uint(inv_sqrt_cr(d)) != __float_as_uint(fdiv(1.0f, fsqrt(d))))
+            c[6]++;
+        const float y = __uint_as_float(mix32(bits ^ 0x5bd1e995u));
+        const float2 q = div_by_length<false>(d, y);
+        const float len = fsqrt(d * d + y * y);
+        if (__float_as_uint(q.x) != __float_as_uint(fdiv(d, len)) || __float_as_uint(q.y) != __float_as_uint(fdiv(y, len)))
+            c[7]++;
+    }
+    for (int k = 0; k < 8; k++)
+        if (c[k]) atomicAdd(&counts[k], c[k]);
+}
+
+extern "C" int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void* stream, uint32_t flags) {
+    if (!counts) return slm_fail(SLM_E_INVALID_ARG, "counts is NULL");
+    hipLaunchKernelGGL(k_check_rounding, dim3(8192), dim3(256), 0, (hipStream_t)stream,
+                       (unsigned long long*)counts, n_numerators);
+    return slm_finish(stream, flags);
+}
+
 // ------------------------------------------------------------------ seed tiling
 //
 // Work order for the fields tracer: seeds are bucketed into 8x8-pixel tiles,
@@ -1747,6 +1815,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
     uint32_t* slc = nullptr;
     uint32_t* slt = nullptr;
     uint32_t dt_fixed = 0, pix_begin = 0;
+    float inv_res = 0.f;   // 1/TRAJECTORY_RESOLUTION of the lane's job (exact when R2)
 
     bool active = false, exhausted = false;
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
@@ -1815,6 +1884,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     slt = D->slt;
                     dt_fixed = D->dt_fixed;
                     pix_begin = D->pix_begin;
+                    inv_res = 1.0f / P.trajectory_resolution;
                     const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
                     const uint32_t dd = d * d;
                     const uint32_t wl = w - D->work_begin;
@@ -1848,16 +1918,17 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
         bool f_has = false;
         uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
         if (active) {
+            // compute_step_vec_jittered (computestep.cl:86-106) on quads, with
+            // the short sqrt / divide forms where their inputs allow
             const float res = P.trajectory_resolution;
             const float combo = P.combo_factor;
-            const float inv_res = 1.0f / res;
-            const float2 u1 = jitter_unit<C>(P, interp_quad_px<C>(P, Q, x, y, cx, cy), rng);
+            const float2 u1 = jitter_unit_cr<C>(P, interp_quad_px_cr<C>(P, Q, x, y, cx, cy), rng);
             const float2 d1 = make_float2(approx_r<R2>(u1.x * combo * dt, res, inv_res), approx_r<R2>(u1.y * combo * dt, res, inv_res));
             const float mx = x + d1.x, my = y + d1.y;
             const uint32_t mix = pix(mx, P.pad_width_pp5, P.nx_padded - 1u);
             const uint32_t miy = pix(my, P.pad_width_pp5, P.ny_padded - 1u);
             const Quad Q1 = load_quad(quad, miy + P.ny_padded * mix);
-            const float2 u2 = jitter_unit<C>(P, interp_quad_px<C>(P, Q1, mx, my, mix, miy), rng);
+            const float2 u2 = jitter_unit_cr<C>(P, interp_quad_px_cr<C>(P, Q1, mx, my, mix, miy), rng);
             const float2 d2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, d1.x), res, inv_res),
                                           approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, d1.y), res, inv_res));
             const float nx = x + d2.x, ny = y + d2.y;
@@ -1869,7 +1940,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
             if (mk) {
                 active = false;
             } else {
-                const float dl = fast_length<C>(d2.x, d2.y);
+                const float dl = fast_length_cr<C>(d2.x, d2.y);
                 x = nx;
                 y = ny;
                 Q = Qn;

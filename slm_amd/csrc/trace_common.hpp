@@ -10,7 +10,8 @@
 //   * division and sqrt are IEEE correctly rounded: plain '/' and
 //     __builtin_sqrtf lower to the full-precision gfx950 sequences (HIP's
 //     __fsqrt_rn is the native approximate sqrt unless
-//     OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used);
+//     OCML_BASIC_ROUNDED_OPERATIONS is defined, so it is not used), or the
+//     short forms below, which give the same bits in their ranges;
 //   * fast_length(v) = sqrt(dot(v,v)); fast_normalize(v) = v*(1/sqrt(dot(v,v)));
 //   * float->uint / float->char conversions saturate and map NaN to 0.
 #pragma once
@@ -29,10 +30,76 @@ __device__ __forceinline__ float mad(float a, float b, float c) {
 __device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
 __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
 
+// Correctly rounded sqrt and divide, short forms.  gfx950 has no correctly
+// rounded f32 sqrt or divide instruction; the compiler expands '/' and
+// __builtin_sqrtf into IEEE sequences:
+//   sqrt(d):  s = v_sqrt_f32(d), then s-1ulp or s+1ulp by the signs of the fma
+//             residuals d - (s-1ulp)*s and d - (s+1ulp)*s, between an input
+//             scaling by 2^32 (d < 2^-96) and a pass-through of +-0 and +inf;
+//   a / b:    r = v_rcp_f32(b) refined by one fma Newton step, q = a*r
+//             corrected twice by fma residuals, between v_div_scale (operand
+//             scaling), v_div_fmas and v_div_fixup (special values).
+// For d in [2^-96, 2^100], and for b in [2^-48, 2^50] with |a| in [2^-70, 8b],
+// the scaling and fix-up steps leave every value unchanged, so the short forms
+// below (the same instructions without them) return the same bits; the
+// refined reciprocal of a divisor can then also be shared by two quotients.
+// slm_check_rounding compares them with the operators on the device over
+// every float in range (and 6.6e9 quotients).  Outside the ranges the
+// operators are used.
+__device__ __forceinline__ bool sqrt_short_ok(float d) { return d >= 0x1p-96f && d <= 0x1p100f; }
+__device__ __forceinline__ float sqrt_short(float d) {
+    const float s = __builtin_amdgcn_sqrtf(d);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rm = __builtin_fmaf(-sm, s, d);
+    const float rp = __builtin_fmaf(-sp, s, d);
+    const float t = (rm <= 0.0f) ? sm : s;
+    return (rp > 0.0f) ? sp : t;
+}
+// refined reciprocal of the divide sequence (an input of div_short, not 1/b)
+__device__ __forceinline__ float recip_short(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0f);
+    return __builtin_fmaf(e, r, r);
+}
+__device__ __forceinline__ float div_short(float a, float b, float rb) {
+    float q = a * rb;
+    float e = __builtin_fmaf(-b, q, a);
+    q = __builtin_fmaf(e, rb, q);
+    e = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(e, rb, q);
+}
+// sqrt(d) and 1/sqrt(d), each rounding as the operators do
+__device__ __forceinline__ float sqrt_cr(float d) {
+    if (__builtin_expect(sqrt_short_ok(d), 1)) return sqrt_short(d);
+    return fsqrt(d);
+}
+__device__ __forceinline__ float inv_sqrt_cr(float d) {
+    if (__builtin_expect(sqrt_short_ok(d), 1)) {
+        const float s = sqrt_short(d);   // in [2^-48, 2^50]
+        return div_short(1.0f, s, recip_short(s));
+    }
+    return fdiv(1.0f, fsqrt(d));
+}
+
 template <bool C>
 __device__ __forceinline__ float dot2(float x, float y) { return mad<C>(x, x, y * y); }
 template <bool C>
 __device__ __forceinline__ float fast_length(float x, float y) { return fsqrt(dot2<C>(x, y)); }
+template <bool C>
+__device__ __forceinline__ float fast_length_cr(float x, float y) { return sqrt_cr(dot2<C>(x, y)); }
+// (x, y) / fast_length(x, y): two IEEE quotients by one rounded length
+template <bool C>
+__device__ __forceinline__ float2 div_by_length(float x, float y) {
+    const float d = dot2<C>(x, y);
+    if (__builtin_expect(sqrt_short_ok(d) && fabsf(x) >= 0x1p-70f && fabsf(y) >= 0x1p-70f, 1)) {
+        const float len = sqrt_short(d);   // in [2^-48, 2^50], and |x|, |y| <= len * (1 + 2^-22)
+        const float r = recip_short(len);
+        return make_float2(div_short(x, len, r), div_short(y, len, r));
+    }
+    const float len = fsqrt(d);
+    return make_float2(fdiv(x, len), fdiv(y, len));
+}
 
 // Pixel coordinate of a position: min(n-1, (uint)max(0, v+PAD_WIDTH_PP5))
 // (essentials.cl:26-27, :57-58).  t is never NaN after fmaxf.
@@ -241,6 +308,30 @@ __device__ __forceinline__ float2 jitter_unit(const SlmParams& P, float2 u, uint
     u.y = mad<C>(r.y, P.jitter_magnitude, u.y);
     const float len = fast_length<C>(u.x, u.y);
     return make_float2(fdiv(u.x, len), fdiv(u.y, len));
+}
+
+// interp_quad_px and jitter_unit with the short sqrt / divide forms where
+// their inputs allow (same bits).  Used by k_fields_batch, where they measure
+// faster; in k_fields the operator forms above measure faster.
+template <bool C>
+__device__ __forceinline__ float2 interp_quad_px_cr(const SlmParams& P, const Quad& Q, float x, float y, uint32_t xl,
+                                                    uint32_t yd) {
+    const float rx = x - (float)xl + (float)P.pad_width;
+    const float lx = 1.0f - rx;
+    const float uy = y - (float)yd + (float)P.pad_width;
+    const float dy = 1.0f - uy;
+    const float dwn0 = mad<C>(Q.lo.x, lx, Q.lo.z * rx), dwn1 = mad<C>(Q.lo.y, lx, Q.lo.w * rx);
+    const float upp0 = mad<C>(Q.hi.x, lx, Q.hi.z * rx), upp1 = mad<C>(Q.hi.y, lx, Q.hi.w * rx);
+    const float s0 = mad<C>(dwn0, dy, upp0 * uy), s1 = mad<C>(dwn1, dy, upp1 * uy);
+    const float inv = inv_sqrt_cr(dot2<C>(s0, s1));
+    return make_float2(s0 * inv, s1 * inv);
+}
+template <bool C>
+__device__ __forceinline__ float2 jitter_unit_cr(const SlmParams& P, float2 u, uint32_t& rng) {
+    const float2 r = rand_vec(rng);
+    u.x = mad<C>(r.x, P.jitter_magnitude, u.x);
+    u.y = mad<C>(r.y, P.jitter_magnitude, u.y);
+    return div_by_length<C>(u.x, u.y);
 }
 
 

@@ -32,6 +32,7 @@ _PROTOS = {
     "slm_last_error": (ctypes.c_char_p, []),
     "slm_device_count": (ctypes.c_int, []),
     "slm_rng_stream": (ctypes.c_int, [_U32, _U32, _U32, _P, _P, _P, _U32]),
+    "slm_check_rounding": (ctypes.c_int, [_P, _U32, _P, _U32]),
     "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _U32]),
@@ -171,6 +172,15 @@ def rng_stream(idx0: int, n: int, n_draws: int, rng0=None, draws=None, stream=No
     _require_cuda(rng0, draws)
     check(load().slm_rng_stream(idx0, n, n_draws, _ptr(rng0), _ptr(draws), _stream(stream),
                                 SLM_FLAG_SYNC if sync else 0))
+
+
+def check_rounding(n_numerators: int = 8, stream=None):
+    """slm_check_rounding: returns the 8 counts as Python ints (entries 0, 2,
+    4, 6, 7 count mismatches of the short sqrt/divide forms)."""
+    import torch
+    counts = torch.zeros(8, dtype=torch.int64, device=torch.device("cuda", torch.cuda.current_device()))
+    check(load().slm_check_rounding(_ptr(counts), n_numerators, _stream(stream), SLM_FLAG_SYNC))
+    return [int(v) for v in counts.cpu()]
 
 
 def fields_workspace_size(p: SlmParams) -> int:

@@ -31,6 +31,17 @@ def _gpu_fields(p, seeds, mask, uv, dev, force_fallback=False):
     return hip.as_u32(slc), hip.as_u32(slt), int(steps.item())
 
 
+def test_short_rounding_forms(gpu):
+    """The short correctly rounded sqrt/divide forms the tracing kernels use
+    return the operators' bits: every float in their ranges, 6.6e9 quotients,
+    and the guarded wrappers over every bit pattern (NaN, inf, zero, denormal)."""
+    from slm_amd import hip
+    c = hip.check_rounding(8)
+    print("rounding check counts", c, flush=True)
+    assert c[1] > 1_600_000_000 and c[3] > 800_000_000 and c[5] == 8 * c[3]
+    assert c[0] == 0 and c[2] == 0 and c[4] == 0 and c[6] == 0 and c[7] == 0
+
+
 CASES = [
     # (grid side, d, jitter, fp_contract[, max_integration_step_error])
     (96, 2, 1.0, True, 1e-6),
