@@ -38,6 +38,14 @@ typedef unsigned long long u64;
 constexpr int kBlock = 256;
 constexpr double kFix = 16777216.0;   // 2^24
 
+// One colour's state in the workspace; a group of colours is solved together,
+// slot g (blockIdx.y) at base + g * stride.
+struct FmmLayout {
+    uint8_t* base;
+    size_t stride, off_neg, off_u, off_fixed, off_act0, off_act1;
+};
+__device__ __forceinline__ uint8_t* slot_base(const FmmLayout& L) { return L.base + blockIdx.y * L.stride; }
+
 __device__ __forceinline__ uint32_t abs_label(int32_t l) {
     // |label|; the most negative int32 (a flank bit with no label) is no label
     return (l == INT32_MIN) ? 0u : (uint32_t)(l < 0 ? -l : l);
@@ -108,10 +116,13 @@ __global__ __launch_bounds__(256) void k_fmm_pairs(const int32_t* __restrict__ l
 
 // problem key (|label| + 0 = none) and phi sign (1 = negative) of colour c
 __global__ void k_fmm_setup(const int32_t* __restrict__ label, const uint32_t* __restrict__ mapping,
-                            uint32_t thin_flag, const uint8_t* __restrict__ colour, uint32_t n_labels, uint32_t c,
-                            uint32_t nx, uint32_t ny, uint32_t* __restrict__ dom, uint8_t* __restrict__ neg) {
+                            uint32_t thin_flag, const uint8_t* __restrict__ colour, uint32_t n_labels, uint32_t c0,
+                            uint32_t nx, uint32_t ny, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nx * ny) return;
+    const uint32_t c = c0 + blockIdx.y;
+    uint32_t* __restrict__ dom = (uint32_t*)slot_base(L);
+    uint8_t* __restrict__ neg = slot_base(L) + L.off_neg;
     const uint32_t a = abs_label(label[i]);
     uint32_t key = 0;
     if (a && a < n_labels && colour[a] == c) {
@@ -141,10 +152,13 @@ __device__ __forceinline__ double fmm_update(double a, double b, double dx) {
     return (a + b + sqrt(2.0 * dx * dx - d * d)) * 0.5;
 }
 
-__global__ void k_fmm_init(const uint32_t* __restrict__ dom, const uint8_t* __restrict__ neg, uint32_t nx,
-                           uint32_t ny, double dx, double* __restrict__ u, uint8_t* __restrict__ fixed) {
+__global__ void k_fmm_init(uint32_t nx, uint32_t ny, double dx, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nx * ny) return;
+    const uint32_t* __restrict__ dom = (const uint32_t*)slot_base(L);
+    const uint8_t* __restrict__ neg = slot_base(L) + L.off_neg;
+    double* __restrict__ u = (double*)(slot_base(L) + L.off_u);
+    uint8_t* __restrict__ fixed = slot_base(L) + L.off_fixed;
     const uint32_t key = dom[i];
     double v = INFINITY;
     uint8_t f = 0;
@@ -188,12 +202,13 @@ constexpr int kT = 32;
 constexpr int kTH = kT + 2;
 constexpr int kMaxLocal = 4 * kT;
 
-__global__ __launch_bounds__(256) void k_fmm_relax_tiles(const uint32_t* __restrict__ dom,
-                                                         const uint8_t* __restrict__ fixed, uint32_t nx, uint32_t ny,
-                                                         double dx, double* __restrict__ u,
-                                                         uint32_t* __restrict__ changed,
-                                                         const uint8_t* __restrict__ act_in,
-                                                         uint8_t* __restrict__ act_out) {
+__global__ __launch_bounds__(256) void k_fmm_relax_tiles(uint32_t nx, uint32_t ny, double dx, FmmLayout L,
+                                                         uint32_t parity, uint32_t* __restrict__ changed) {
+    const uint32_t* __restrict__ dom = (const uint32_t*)slot_base(L);
+    const uint8_t* __restrict__ fixed = slot_base(L) + L.off_fixed;
+    double* __restrict__ u = (double*)(slot_base(L) + L.off_u);
+    const uint8_t* __restrict__ act_in = slot_base(L) + (parity ? L.off_act1 : L.off_act0);
+    uint8_t* __restrict__ act_out = slot_base(L) + (parity ? L.off_act0 : L.off_act1);
     __shared__ double su[kTH * kTH];
     __shared__ uint32_t sk[kTH * kTH];
     __shared__ uint8_t sfix[kTH * kTH];
@@ -274,10 +289,12 @@ __global__ __launch_bounds__(256) void k_fmm_relax_tiles(const uint32_t* __restr
 // Per problem: fixed-point sum of the signed distances of the points a
 // channel reaches, their count, the channel pixels and the own pixels.  A
 // wave whose active lanes share one key adds once.
-__global__ void k_fmm_reduce(const uint32_t* __restrict__ dom, const uint8_t* __restrict__ neg,
-                             const int32_t* __restrict__ label, const double* __restrict__ u, uint32_t nx,
-                             uint32_t ny, u64* __restrict__ sums, uint32_t* __restrict__ cnt,
-                             uint32_t* __restrict__ nch, uint32_t* __restrict__ nphi) {
+__global__ void k_fmm_reduce(FmmLayout L, const int32_t* __restrict__ label, uint32_t nx, uint32_t ny,
+                             u64* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ nch,
+                             uint32_t* __restrict__ nphi) {
+    const uint32_t* __restrict__ dom = (const uint32_t*)slot_base(L);
+    const uint8_t* __restrict__ neg = slot_base(L) + L.off_neg;
+    const double* __restrict__ u = (const double*)(slot_base(L) + L.off_u);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0;
     long long q = 0;
@@ -603,11 +620,21 @@ extern "C" int slm_fmm_pairs(const int32_t* label, uint32_t nx, uint32_t ny, con
     return slm_finish(stream, flags);
 }
 
-extern "C" size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny) {
+namespace {
+// bytes of one colour slot, and the slot layout
+size_t fmm_slot_bytes(uint32_t nx, uint32_t ny) {
     const size_t n = (size_t)nx * ny;
-    const size_t n_tiles = (size_t)((nx + 31) / 32) * ((ny + 31) / 32);
-    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 256 + 2 * al256(n_tiles) + 256;
+    const size_t n_tiles = (size_t)((nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
+    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 2 * al256(n_tiles);
 }
+}  // namespace
+
+extern "C" size_t slm_fmm_workspace_size_k(uint32_t nx, uint32_t ny, uint32_t k) {
+    if (!nx || !ny || !k) return 0;
+    return 512 + (size_t)k * fmm_slot_bytes(nx, ny);
+}
+
+extern "C" size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny) { return slm_fmm_workspace_size_k(nx, ny, 1); }
 
 extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_flag, const uint8_t* colour,
                            uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx,
@@ -620,42 +647,47 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
         return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes, need %zu", workspace_bytes,
                         slm_fmm_workspace_size(nx, ny));
     const size_t n = (size_t)nx * ny;
+    const size_t slot = fmm_slot_bytes(nx, ny);
+    // colours relaxed together: as many slots as the workspace holds
+    const uint32_t group = (uint32_t)std::min<size_t>(std::max<uint32_t>(n_colours, 1u), (workspace_bytes - 512) / slot);
     uint8_t* w = (uint8_t*)(((uintptr_t)workspace + 255) & ~(uintptr_t)255);
-    uint32_t* dom = (uint32_t*)w;
-    uint8_t* neg = w + al256(n * 4);
-    double* u = (double*)(neg + al256(n));
-    uint8_t* fixed = (uint8_t*)u + al256(n * 8);
-    uint32_t* ctr = (uint32_t*)(fixed + al256(n));
+    uint32_t* ctr = (uint32_t*)w;
     const uint32_t n_tiles = ((nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
-    uint8_t* act_a = (uint8_t*)ctr + 256;
-    uint8_t* act_b = act_a + al256(n_tiles);
+    FmmLayout L;
+    L.base = w + 256;
+    L.stride = slot;
+    L.off_neg = al256(n * 4);
+    L.off_u = L.off_neg + al256(n);
+    L.off_fixed = L.off_u + al256(n * 8);
+    L.off_act0 = L.off_fixed + al256(n);
+    L.off_act1 = L.off_act0 + al256(n_tiles);
     hipStream_t s = (hipStream_t)stream;
     zero(sums, (size_t)n_labels * 8, s);
     zero(cnt, (size_t)n_labels * 4, s);
     zero(nch, (size_t)n_labels * 4, s);
     zero(nphi, (size_t)n_labels * 4, s);
-    const dim3 g = grid_for(n);
     // a bound on the launches keeps a bad input from looping
     const uint32_t max_launches = 2u * (nx + ny) + 64u;
-    for (uint32_t c = 0; c < n_colours; c++) {
-        hipLaunchKernelGGL(k_fmm_setup, g, dim3(kBlock), 0, s, label, mapping, thin_flag, colour, n_labels, c, nx, ny,
-                           dom, neg);
-        hipLaunchKernelGGL(k_fmm_init, g, dim3(kBlock), 0, s, (const uint32_t*)dom, (const uint8_t*)neg, nx, ny, dx,
-                           u, fixed);
-        hipLaunchKernelGGL(k_fill_bytes, grid_for(n_tiles), dim3(kBlock), 0, s, act_a, (size_t)n_tiles, (uint8_t)1);
+    for (uint32_t c0 = 0; c0 < n_colours; c0 += group) {
+        const uint32_t k = std::min(group, n_colours - c0);
+        const dim3 g((uint32_t)((n + kBlock - 1) / kBlock), k);
+        hipLaunchKernelGGL(k_fmm_setup, g, dim3(kBlock), 0, s, label, mapping, thin_flag, colour, n_labels, c0, nx,
+                           ny, L);
+        hipLaunchKernelGGL(k_fmm_init, g, dim3(kBlock), 0, s, nx, ny, dx, L);
+        for (uint32_t j = 0; j < k; j++)
+            hipLaunchKernelGGL(k_fill_bytes, grid_for(n_tiles), dim3(kBlock), 0, s, L.base + j * slot + L.off_act0,
+                               (size_t)n_tiles, (uint8_t)1);
         for (uint32_t launches = 0;; launches++) {
             if (launches > max_launches) return slm_fail(SLM_E_HIP, "eikonal relaxation did not converge");
             zero(ctr, sizeof(uint32_t), s);
-            hipLaunchKernelGGL(k_fmm_relax_tiles, dim3(n_tiles), dim3(kBlock), 0, s, (const uint32_t*)dom,
-                               (const uint8_t*)fixed, nx, ny, dx, u, ctr, (const uint8_t*)act_a, act_b);
-            std::swap(act_a, act_b);
+            hipLaunchKernelGGL(k_fmm_relax_tiles, dim3(n_tiles, k), dim3(kBlock), 0, s, nx, ny, dx, L,
+                               launches & 1u, ctr);
             uint32_t dec = 0;
             HIP_TRY(hipMemcpyAsync(&dec, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (!dec) break;
         }
-        hipLaunchKernelGGL(k_fmm_reduce, g, dim3(kBlock), 0, s, (const uint32_t*)dom, (const uint8_t*)neg, label,
-                           (const double*)u, nx, ny, (u64*)sums, cnt, nch, nphi);
+        hipLaunchKernelGGL(k_fmm_reduce, g, dim3(kBlock), 0, s, L, label, nx, ny, (u64*)sums, cnt, nch, nphi);
     }
     return slm_finish(stream, flags);
 }

@@ -78,6 +78,7 @@ _PROTOS = {
                                                 ctypes.c_float, _U32, _U32, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_fmm_pairs": (ctypes.c_int, [_P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32]),
     "slm_fmm_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
+    "slm_fmm_workspace_size_k": (ctypes.c_size_t, [_U32, _U32, _U32]),
     "slm_fmm_hsl": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, ctypes.c_double, _P, _P, _P, _P, _P,
                                    ctypes.c_size_t, _P, _U32]),
     "slm_hsl_remap": (ctypes.c_int, [_P, _U32, _P, _P, _U32, _P, _P, _U32]),

@@ -67,10 +67,11 @@ def colour_labels(n_labels, problems, pairs):
     return colour, n_col
 
 
-def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None):
+def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None):
     """fmm_lengths (mapping.py:841-900) on device tensors: label int32
     [nx, ny] (signed flank labels), mapping int32 [nx, ny] (uint32 bits),
-    mask bool [nx, ny].  Returns an FmmResult (host arrays)."""
+    mask bool [nx, ny].  max_group caps the colours relaxed together (default:
+    all that fit in 8 GiB).  Returns an FmmResult (host arrays)."""
     import torch
     hip._require_cuda(label, mapping, mask)
     dev = label.device
@@ -101,7 +102,10 @@ def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None):
     cnt = torch.empty(n_labels, dtype=torch.int32, device=dev)
     nch = torch.empty(n_labels, dtype=torch.int32, device=dev)
     nphi = torch.empty(n_labels, dtype=torch.int32, device=dev)
-    ws_bytes = lib.slm_fmm_workspace_size(nx, ny)
+    # all colours relaxed together when they fit in 8 GiB (14 B per pixel per colour)
+    one = lib.slm_fmm_workspace_size_k(nx, ny, 1) - 512
+    k = max(1, min(int(n_col), (8 << 30) // max(one, 1), int(max_group) if max_group else 1 << 30))
+    ws_bytes = lib.slm_fmm_workspace_size_k(nx, ny, k)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     hip.check(lib.slm_fmm_hsl(hip._ptr(label), hip._ptr(mapping), int(is_thinchannel), hip._ptr(colour_d), n_labels,
                               n_col, nx, ny, ctypes.c_double(float(pixel_size)), hip._ptr(sums), hip._ptr(cnt),

@@ -66,6 +66,27 @@ def test_fmm_voronoi_subsegments(gpu, shape, k, seed, dx):
     assert (res.hsl > 0).sum() > 0
 
 
+def test_fmm_colour_groups(gpu):
+    """One colour per relaxation group (the minimum workspace) and every
+    colour in one group solve the same problems: identical counts, and sums
+    within the oracle tolerance of each other."""
+    import torch
+    from slm_amd import hsl
+    label, mapping, mask = _voronoi_labels(130, 97, 40, 1)
+    lt = torch.from_numpy(label).to(gpu)
+    mt = torch.from_numpy(mapping.view(np.int32)).to(gpu)
+    kt = torch.from_numpy(mask).to(gpu)
+    one = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=1)
+    two = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=2)
+    allc = hsl.fmm_lengths(lt, mt, kt, 2, 2.0)
+    print("colours", allc.n_colours, flush=True)
+    assert allc.n_colours >= 3
+    for r in (two, allc):
+        assert np.array_equal(r.cnt, one.cnt) and np.array_equal(r.nch, one.nch) and np.array_equal(r.nphi, one.nphi)
+        assert np.all(np.abs(r.sums.astype(np.int64) - one.sums.astype(np.int64)) <= one.cnt.astype(np.int64))
+        assert np.array_equal(r.selected, one.selected)
+
+
 def test_fmm_traced_segmentation(gpu):
     """The labels and thin channels of the segmentation chain on the traced
     256^2 workload (the grids map_tile produces)."""
