@@ -81,16 +81,25 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     mask = mask_b.to(torch.uint8).reshape(-1)
 
     st.mark("trajectories")
+    # the two directions on two streams: 1000 trajectories fill a few waves,
+    # so the launches overlap instead of running one after the other
     lengths, nsteps = [], []
-    for sign in (1, -1):
+    main = torch.cuda.current_stream(dev)
+    side = hip.side_stream(dev)
+    side.wait_stream(main)
+    for sign, stream in ((1, main), (-1, side)):
         info.n_seed_points = np.uint32(n_t)
         p = kernel_params(info, sign, "INTEGRATE_TRAJECTORY", fp_contract)
         traj = torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=dev)
         ns = torch.empty(n_t, dtype=torch.int16, device=dev)
         ln = torch.empty(n_t, dtype=torch.float32, device=dev)
-        hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln)
+        if stream is side:
+            for t in (traj, ns, ln):
+                t.record_stream(side)
+        hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln, stream=stream)
         lengths.append(ln)
         nsteps.append(ns)
+    main.wait_stream(side)
     tl = torch.stack(lengths, 1).cpu().numpy()
     tn = torch.stack(nsteps, 1).cpu().numpy().view(np.uint16)
     stats = compute_stats(tl, tn, info.pixel_size)

@@ -3,7 +3,8 @@
 Drop-in for python/streamlines/trajectories.py: same class name, constructor
 signature and outputs (data.streamline_arrays_list, traj_nsteps_array,
 traj_length_array, traj_stats_df).  Each direction is one slm_trace_paths
-launch (replacing kernel integrate_trajectory, trajectories.py:257-333); no
+launch (replacing kernel integrate_trajectory, trajectories.py:257-333); the
+two directions run on two streams, so their launches overlap; no
 memory chunking is needed on a 288 GB device, and since results are per seed
 they do not depend on chunking anyway.
 """
@@ -67,18 +68,29 @@ class Trajectories:
         traj_nsteps_array = np.zeros([ns, 2], dtype=np.uint16)
         traj_length_array = np.zeros([ns, 2], dtype=np.float32)
         streamline_arrays_list = [[], []]
-        traj_d = torch.zeros((ns, nl, 2), dtype=torch.int8, device=dev)
-        nsteps_d = torch.zeros(ns, dtype=torch.int16, device=dev)
-        length_d = torch.zeros(ns, dtype=torch.float32, device=dev)
         todo = [(self.do_trace_downstream, 0, np.float32(+1.0)), (self.do_trace_upstream, 1, np.float32(-1.0))]
+        # both directions launched before either is copied back, on two
+        # streams so that the two launches overlap
+        main = torch.cuda.current_stream(dev)
+        side = hip.side_stream(dev)
+        side.wait_stream(main)
+        outs = {}
         for do, idx, sign in todo:
             if not do:
                 continue
             info.downup_sign = sign
             info.seeds_chunk_offset = np.uint32(0)
             p = kernel_params(info, sign, "INTEGRATE_TRAJECTORY", self.fp_contract)
-            hip.trace_paths(p, seeds, mask_d, uv_d, traj_d, nsteps_d, length_d)
-            torch.cuda.synchronize(dev)
+            traj_d = torch.zeros((ns, nl, 2), dtype=torch.int8, device=dev)
+            nsteps_d = torch.zeros(ns, dtype=torch.int16, device=dev)
+            length_d = torch.zeros(ns, dtype=torch.float32, device=dev)
+            stream = main if idx == 0 else side
+            if idx == 1:
+                side.wait_stream(main)   # the zero fills above ran on main
+            hip.trace_paths(p, seeds, mask_d, uv_d, traj_d, nsteps_d, length_d, stream=stream)
+            outs[idx] = (traj_d, nsteps_d, length_d)
+        torch.cuda.synchronize(dev)
+        for idx, (traj_d, nsteps_d, length_d) in outs.items():
             nsteps = nsteps_d.cpu().numpy().view(np.uint16).copy()
             traj = traj_d.cpu().numpy()
             traj_nsteps_array[:, idx] = nsteps
