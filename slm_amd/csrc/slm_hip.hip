@@ -489,6 +489,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     while (true) {
         if (!exhausted) {
             uint64_t need = __ballot(!active);
+            if (__popcll(need) < kRefillMin && __any(active)) need = 0;   // batched refills
             while (need) {
                 if (c_next >= c_end) {
                     // The work order is cut into kQueues slices; a wave starts on
@@ -1843,6 +1844,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
     while (true) {
         if (!exhausted) {
             uint64_t need = __ballot(!active);
+            if (__popcll(need) < kRefillMin && __any(active)) need = 0;   // batched refills
             while (need) {
                 if (c_next >= c_end) {
                     uint32_t base = 0xffffffffu, lim = 0;
