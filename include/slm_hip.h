@@ -186,7 +186,7 @@ typedef struct SlmFieldsJob {
 } SlmFieldsJob;
 
 typedef struct SlmBatchOpts {
-    uint32_t chunk;          /* work items per queue grab (0: 256)                  */
+    uint32_t chunk;          /* work items per queue grab (0: 64)                   */
     uint32_t blocks_per_cu;  /* resident 256-thread blocks per CU (0: occupancy max) */
     uint32_t queue_slices;   /* work-list slices / queue heads, 1..8 (0: 8, one per XCD) */
     uint32_t reserved[5];

@@ -62,7 +62,7 @@ def tune_batch(job_array, workspace=None, cache_path=None, repeats=2, verbose=Fa
         workspace = torch.empty(need, dtype=torch.uint8, device=torch.device("cuda", torch.cuda.current_device()))
     hip.trace_fields_batch(job_array, workspace=workspace)   # builds the seed order and quads
     table = []
-    best = {"chunk": 256, "blocks_per_cu": 0, "queue_slices": 8}
+    best = {"chunk": 64, "blocks_per_cu": 0, "queue_slices": 8}
 
     def run(cfg):
         ms = time_batch(job_array, hip.batch_opts(cfg["chunk"], cfg["blocks_per_cu"], cfg["queue_slices"]),

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     };
 
     // wave-private chunk of the work queue: one global atomic per kChunk items
-    constexpr uint32_t kChunk = 256;
+    constexpr uint32_t kChunk = 64;
     uint32_t c_next = 0, c_end = 0;
     while (true) {
         if (!exhausted) {
@@ -2092,7 +2092,7 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
         if (J.params.n_seed_points > J.params.seeds_chunk_offset && !J.seeds_xy)
             return slm_fail(SLM_E_INVALID_ARG, "job %u: NULL seeds", j);
     }
-    uint32_t chunk = 256, blocks_per_cu = 0, slices = kQueues;
+    uint32_t chunk = 64, blocks_per_cu = 0, slices = kQueues;
     if (opts) {
         if (opts->chunk) chunk = opts->chunk;
         if (opts->blocks_per_cu) blocks_per_cu = opts->blocks_per_cu;
