@@ -273,7 +273,8 @@ def bench_surrogate(args):
                               "plain_steps": int(steps.item()), "host_generation_s": round(gen_s, 1),
                               "gated_vs_plain": m_gated, "prediction_vs_plain": m_pred,
                               "plain_vs_plain_other_jitter": floor,
-                              "tiles": rep["tiles"], "checked": rep["checked"], "failed": rep["failed"],
+                              "tiles": rep["tiles"], "checked": rep["checked"], "evaluated": rep["evaluated"],
+                              "failed": rep["failed"],
                               "fallback": rep["fallback"], "check_steps": rep["check_steps"],
                               "checks": [{k: c[k] for k in ("median_rel_err", "f1", "channel_px", "ok")}
                                          for c in rep["checks"]]}},

@@ -445,7 +445,9 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
     channel pixels), fails.  When more than max_fail_frac of the checked
     tiles fail, the map is plain_map and no other tile is predicted;
     otherwise the other tiles are predicted and each failed checked tile
-    keeps its trace.  Returns (fields [3, nxy], report)."""
+    keeps its trace.  The checks stop at the failure that decides the
+    fallback (report["evaluated"] of report["checked"]).  Returns (fields
+    [3, nxy], report)."""
     import torch
     origins = tile_origins(grid.nx, grid.ny, size, halo)
     n_tiles = len(origins)
@@ -477,9 +479,11 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
         checks.append({"core": [i0, i1, j0, j1], "median_rel_err": med, "f1": f1, "channel_px": nch, "ok": ok})
         if not ok:
             n_fail += 1
+            if n_fail > max_fail_frac * k:
+                break   # the map falls back whatever the remaining checks say
             pr.copy_(tr)
-    report = {"tiles": n_tiles, "checked": k, "failed": n_fail, "check_steps": int(steps.item()),
-              "fallback": False, "checks": checks}
+    report = {"tiles": n_tiles, "checked": k, "evaluated": len(checks), "failed": n_fail,
+              "check_steps": int(steps.item()), "fallback": False, "checks": checks}
     if n_fail > max_fail_frac * k:
         out, _ = plain_map(grid, theta, workspace=workspace)
         report["fallback"] = True
