@@ -36,14 +36,98 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_STEP = 73
 HBM_PEAK_GBS = 8000.0
+CALIB_ITERS = 8192          # slm_calibrate_valu rounds: 16 v_fma_f32 each per wave
+CALIB_BLOCKS_PER_CU = 8     # 32 waves per CU, 8 per SIMD
 
 
-def _cores():
+def _profile_values(path):
+    """'key: value ...' lines of a committed profile summary, as floats."""
+    vals = {}
+    if not os.path.exists(path):
+        return vals
+    for line in open(path):
+        if line.startswith("#") or ":" not in line:
+            continue
+        k, v = line.split(":", 1)
+        try:
+            vals[k.strip()] = float(v.split()[0])
+        except (ValueError, IndexError):
+            pass
+    return vals
+
+
+def valu_peak(stream=None, reps=5):
+    """Sustained wave64 VALU issue rate of the whole chip (wave-instructions
+    per second): slm_calibrate_valu, whose waves issue a known number of
+    independent v_fma_f32, timed with HIP events on its stream."""
+    import torch
+    from slm_amd import hip
+    dev = torch.device("cuda", torch.cuda.current_device())
+    cus = torch.cuda.get_device_properties(dev).multi_processor_count
+    n_blocks = cus * CALIB_BLOCKS_PER_CU
+    out = torch.zeros(n_blocks, dtype=torch.float32, device=dev)
+    stream = stream or torch.cuda.current_stream(dev)
+    hip.calibrate_valu(n_blocks, CALIB_ITERS, out, stream=stream, sync=True)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for a, b in ev:
+        a.record(stream)
+        hip.calibrate_valu(n_blocks, CALIB_ITERS, out, stream=stream)
+        b.record(stream)
+    torch.cuda.synchronize(dev)
+    ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
+    instr = n_blocks * 4 * 16 * CALIB_ITERS
+    return instr / (ms / 1e3), {"blocks": n_blocks, "iters": CALIB_ITERS, "wave_instructions": instr, "ms": ms}
+
+
+def _cpu_info():
+    """The host the CPU baseline runs on: logical CPUs (nproc), the affinity
+    mask, the thread share the environment grants (OMP_NUM_THREADS, the
+    process's CPU share on a shared GPU host), the CPU model (lscpu) and
+    whether an OpenCL CPU device (pocl) is installed, which would let the
+    reference's own kernels run there."""
+    import subprocess
+    info = {"nproc": os.cpu_count() or 1}
     try:
-        n = len(os.sched_getaffinity(0))
+        info["affinity"] = len(os.sched_getaffinity(0))
     except Exception:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        info["affinity"] = info["nproc"]
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    info["omp_num_threads"] = int(omp) if omp.isdigit() and int(omp) > 0 else None
+    info["model"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.lower().startswith("model name"):
+                info["model"] = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    icds = []
+    for d in ("/etc/OpenCL/vendors",):
+        try:
+            icds += sorted(f for f in os.listdir(d) if f.endswith(".icd"))
+        except OSError:
+            pass
+    info["opencl_icds"] = icds
+    cl = None
+    try:
+        r = subprocess.run(["clinfo", "-l"], capture_output=True, text=True, timeout=30)
+        cl = r.stdout
+    except Exception:
+        pass
+    info["pocl_device"] = bool(cl and "portable computing language" in cl.lower()) or \
+        any("pocl" in f.lower() for f in icds)
+    info["clinfo"] = " | ".join(l.strip() for l in cl.splitlines() if l.strip())[:300] if cl else None
+    return info
+
+
+def _cores(info):
+    """Threads for the CPU baseline: every CPU of the affinity mask, within
+    the OMP_NUM_THREADS share when the environment sets one."""
+    n = info["affinity"]
+    if info["omp_num_threads"]:
+        n = min(n, info["omp_num_threads"])
+    return max(1, n)
 
 
 def cpu_baseline(job, target_s=12.0):
@@ -51,7 +135,8 @@ def cpu_baseline(job, target_s=12.0):
     random sample of the map, RNG keys as in the full run), both directions;
     n grows until one measurement takes a few seconds."""
     from oracle import oracle
-    cores = _cores()
+    host = _cpu_info()
+    cores = _cores(host)
     m = job.mask.astype(np.uint8).reshape(-1)
     uv = job.uv.reshape(-1)
     n_total = job.seeds.shape[0]
@@ -69,8 +154,61 @@ def cpu_baseline(job, target_s=12.0):
         n *= 4
     return {"value": steps / dt, "unit": "streamline-steps/s", "cores": cores, "kind": "port",
             "sample": "first %d of %d shuffled seeds (x%d sub-seeds, both directions): %d steps in "
-                      "%.1f s; C restatement of the reference kernels, OpenMP" %
-                      (n, n_total, int(job.info.subpixel_seed_point_density) ** 2, steps, dt)}
+                      "%.1f s; C restatement of the reference kernels, OpenMP, %d threads" %
+                      (n, n_total, int(job.info.subpixel_seed_point_density) ** 2, steps, dt, cores),
+            "host": host,
+            "reference_kernels": "not run: no OpenCL CPU device (pocl) on this host" if not host["pocl_device"]
+            else "pocl present; the reference's .cl sources do not travel with the build"}
+
+
+def tracer_roofline(args, kname, steps_per_launch, mean_launch_s, alg_gbs, stream):
+    """Roofline of the fields tracer.  Two resources are compared, each from
+    the kernel's mean launch time measured here:
+      - VALU issue: the VALU wave-instructions one launch issues (the
+        counter SQ_INSTS_VALU per launch from the committed PMC summary,
+        divided by what that counter reads per issued instruction on the
+        calibration kernel) per second, against the chip's sustained issue
+        rate measured now by the calibration kernel;
+      - HBM: measured DRAM-side bytes per launch (FETCH_SIZE + WRITE_SIZE,
+        copy-calibrated, from the same summary) per second, against 8 TB/s.
+    The bound is the resource with the larger fraction.  The algorithmic
+    bytes (73 B per step: 64 B of uv gathers, 1 B mask, 8 B of slc/slt
+    atomics) are reported beside them: they mostly hit L2, so their rate is
+    not an HBM rate."""
+    prof = {}
+    if args.workload == "tile_2048":
+        prof = _profile_values(os.path.join(ROOT, "profiles", kname + "_tile_2048_traffic.txt"))
+    cal = _profile_values(os.path.join(ROOT, "profiles", "valu_calibration.txt"))
+    traffic = prof.get("bytes_per_launch")
+    hbm = {"peak": HBM_PEAK_GBS, "unit": "GB/s", "algorithmic": alg_gbs,
+           "algorithmic_bytes_per_step": BYTES_PER_STEP, "algorithmic_frac": alg_gbs / HBM_PEAK_GBS,
+           "measured": None, "frac": None}
+    if traffic:
+        hbm["measured"] = traffic / mean_launch_s / 1e9
+        hbm["frac"] = hbm["measured"] / HBM_PEAK_GBS
+    out = {"kernel": kname, "path": args.path, "steps_per_launch": steps_per_launch,
+           "mean_launch_ms": mean_launch_s * 1e3, "traffic": traffic, "hbm": hbm,
+           "l2_hit_rate": prof.get("TCC_hit_rate")}
+    valu = prof.get("SQ_INSTS_VALU_per_launch")
+    per_instr = cal.get("SQ_INSTS_VALU_per_issued_instruction")
+    if valu and per_instr:
+        peak, meta = valu_peak(stream)
+        rate = valu / per_instr / mean_launch_s
+        out["valu"] = {"achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instructions/s",
+                       "frac": rate / peak, "instructions_per_launch": valu / per_instr,
+                       "instructions_per_step": valu / per_instr / steps_per_launch, "calibration": meta}
+    if "valu" in out and (hbm["frac"] is None or out["valu"]["frac"] >= hbm["frac"]):
+        v = out["valu"]
+        out.update({"bound": "valu", "achieved": v["achieved"], "peak": v["peak"], "unit": v["unit"],
+                    "frac": v["frac"]})
+    elif hbm["frac"] is not None:
+        out.update({"bound": "hbm", "achieved": hbm["measured"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm["frac"]})
+    else:
+        # no PMC summary for this workload: only the algorithmic rate is known
+        out.update({"bound": "hbm", "achieved": alg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": alg_gbs / HBM_PEAK_GBS, "note": "algorithmic bytes; no measured counters"})
+    return out
 
 
 def bench_batch(args):
@@ -326,6 +464,9 @@ def main():
                          "finishing, channel mapping and segmentation, hillslope lengths, KDE)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales PMC byte counters)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the VALU calibration kernel four times (a known count of v_fma_f32 per "
+                         "wave; calibrates the VALU instruction counter and the issue rate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the tracing path is measured on a single MI355X "
@@ -336,6 +477,10 @@ def main():
     from slm_amd import hip, workloads
 
     dev = torch.device("cuda", 0)
+    if args.valu_calibration:
+        rate, meta = valu_peak(reps=3)
+        print(json.dumps({"valu_calibration": meta, "wave_instructions_per_s": rate}), flush=True)
+        return
     if args.copy_calibration:
         n = args.copy_calibration << 20
         src = torch.ones(n, dtype=torch.uint8, device=dev)
@@ -411,14 +556,8 @@ def main():
     steps_per_launch = total_steps / (launches * args.steps)
     achieved = steps_per_launch * BYTES_PER_STEP / mean_launch_s / 1e9
     value = total_steps / wall
-    traffic = None
     kname = {"batch": "k_fields_batch", "queue": "k_fields", "tiled": "k_tl_process"}[args.path]
-    tfile = os.path.join(ROOT, "profiles", kname + "_tile_2048_traffic.txt")
-    if args.workload == "tile_2048" and os.path.exists(tfile):
-        # measured HBM-side bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-        for line in open(tfile):
-            if line.startswith("bytes_per_launch:"):
-                traffic = float(line.split(":")[1].split()[0])
+    roofline = tracer_roofline(args, kname, steps_per_launch, mean_launch_s, achieved, stream)
     out = {
         "metric": "streamline-steps/s",
         "value": value,
@@ -438,15 +577,12 @@ def main():
                    "streamlines_per_direction": job.n_streamlines(),
                    "jitter": float(job.info.jitter_magnitude), "max_n_steps": int(job.info.max_n_steps),
                    "directions": 2, "map_ms": wall / args.steps * 1e3},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_tl_process (with its per-round binning)" if args.path == "tiled" else kname,
-                     "path": args.path,
-                     "tile_opts": {"tile": topts.tile, "block": topts.block, "unit": topts.unit,
-                                   "cap": topts.cap_steps, "halo": topts.halo} if args.path == "tiled" else None,
-                     "bytes_per_step": BYTES_PER_STEP,
-                     "steps_per_launch": steps_per_launch, "mean_launch_ms": mean_launch_s * 1e3},
+        "roofline": roofline,
     }
+    if args.path == "tiled":
+        roofline["kernel"] = "k_tl_process (with its per-round binning)"
+        roofline["tile_opts"] = {"tile": topts.tile, "block": topts.block, "unit": topts.unit,
+                                 "cap": topts.cap_steps, "halo": topts.halo}
     if args.pipeline:
         from slm_amd.pipeline import map_tile
         map_tile(job, fmm=True)  # warm-up

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define SLM_ABI_VERSION 1u
+#define SLM_ABI_VERSION 2u
 
 #define SLM_OK              0
 #define SLM_E_INVALID_ARG  -1
@@ -81,12 +81,20 @@ typedef struct SlmParams {
     float    downup_sign;                 /* (float)DOWNUP_SIGN: 1 or 2^32 ("-1u")   */
     uint32_t n_seed_points;               /* N_SEED_POINTS                           */
     uint32_t seeds_chunk_offset;          /* SEEDS_CHUNK_OFFSET                      */
-    uint32_t fp_contract;                 /* 1: a*b+c fused as clang's OpenCL default
-                                             FP_CONTRACT ON does; 0: every op rounded */
+    uint32_t fp_contract;                 /* 0: every op rounded (what the reference's
+                                             own runs computed); 1: a*b+c fused where
+                                             clang's OpenCL default FP_CONTRACT ON would */
     float    pixel_size;                  /* PIXEL_SIZE (metres)                     */
     uint32_t interchannel_max_n_steps;    /* INTERCHANNEL_MAX_N_STEPS (connect_channels) */
     uint32_t segmentation_threshold;      /* SEGMENTATION_THRESHOLD (0 when not rendered) */
-    uint32_t reserved[5];
+    uint32_t approx_wrap;                 /* Q3: rounding of the Euler tail's absolute
+                                             position (rungekutta.cl:115, :237), where
+                                             approximate()'s (char) is out of range:
+                                             0 = truncate onto the 1/TRAJECTORY_RESOLUTION
+                                             lattice (the int32 value, no 8-bit wrap; what
+                                             the reference's own runs computed), 1 = keep
+                                             the low byte (8-bit wrap)                 */
+    uint32_t reserved[4];
 } SlmParams;
 
 /* Library identity. */
@@ -114,6 +122,15 @@ int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws,
  * reference; it pins that the short forms change no bits on this device.
  */
 int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void* stream, uint32_t flags);
+
+/*
+ * Measurement hook with no counterpart in the reference: n_blocks blocks of
+ * 256 threads in which every wave issues exactly 16*n_iter independent
+ * v_fma_f32 (16 chains per lane), for timing the sustained VALU issue rate
+ * and for calibrating the VALU instruction counter against a known count.
+ * out: float [n_blocks], written only if a chain sums to an impossible value.
+ */
+int slm_calibrate_valu(uint32_t n_blocks, uint32_t n_iter, float* out, void* stream, uint32_t flags);
 
 /*
  * Jittered streamline fields (slc / slt), one direction.

@@ -77,6 +77,7 @@ class K:
         self.N = int(p.n_seed_points)
         self.OFF = int(p.seeds_chunk_offset)
         self.contract = bool(p.fp_contract)
+        self.wrap = bool(getattr(p, "approx_wrap", 0))
 
     # a*b + c as clang forms it: fused under FP_CONTRACT ON
     def muladd(self, a, b, c):
@@ -281,7 +282,12 @@ def paths(p, seeds, mask2d, uv3d):
                 else:
                     dt = f32(min(tx, ty))
                 x, y = k.muladd(su, dt, x), k.muladd(sv, dt, y)
-                x, y = approximate(k, x), approximate(k, y)
+                if k.wrap:
+                    x, y = approximate(k, x), approximate(k, y)
+                else:
+                    # (char) out of range keeps the int32 truncation: the lattice value
+                    x = f32(f32(np.trunc(np.float64(f32(x * k.RES)))) / k.RES)
+                    y = f32(f32(np.trunc(np.float64(f32(y * k.RES)))) / k.RES)
                 x = f32(min(max(x, f32(-0.5)), k.NXF_MP5))
                 y = f32(min(max(y, f32(-0.5)), k.NYF_MP5))
                 dl = fast_length(k, f32(x - px), f32(y - py))

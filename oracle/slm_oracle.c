@@ -195,8 +195,17 @@ static void trajectory_record(const SlmParams* p, const uint8_t* mask, const flo
             dt = minmag(dtx, dty);
             vec[0] = MAD(c, su, dt, vec[0]);
             vec[1] = MAD(c, sv, dt, vec[1]);
-            vec[0] = approx(p, vec[0]);
-            vec[1] = approx(p, vec[1]);
+            /* approximate(*vec) (rungekutta.cl:115) of the absolute position:
+             * (char) is out of range here; approx_wrap selects the low byte,
+             * otherwise the int32 truncation (the reference's recorded runs) */
+            if (p->approx_wrap) {
+                vec[0] = approx(p, vec[0]);
+                vec[1] = approx(p, vec[1]);
+            } else {
+                const real res = (real)p->trajectory_resolution;
+                vec[0] = (real)cvt_i32(vec[0] * res) / res;
+                vec[1] = (real)cvt_i32(vec[1] * res) / res;
+            }
             vec[0] = r_fmin(r_fmax(vec[0], (real)-0.5), (real)p->nxf_mp5);
             vec[1] = r_fmin(r_fmax(vec[1], (real)-0.5), (real)p->nyf_mp5);
             dl = fast_length(c, vec[0] - prev[0], vec[1] - prev[1]);

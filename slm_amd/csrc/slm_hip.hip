@@ -195,6 +195,32 @@ extern "C" int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void*
     return slm_finish(stream, flags);
 }
 
+// VALU issue-rate calibration: each lane runs n_iter rounds of 16 independent
+// fmaf chains, so each wave issues exactly 16*n_iter v_fma_f32 (plus a few
+// dozen set-up and reduction instructions).  Timing it, and reading the VALU
+// instruction counter over it, gives the chip's sustained wave64 VALU issue
+// rate and what that counter counts per issued instruction.
+__global__ __launch_bounds__(256) void k_calib_valu(float* __restrict__ out, uint32_t n_iter, float a, float b) {
+    float x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = (float)(threadIdx.x + 16u * i);
+    for (uint32_t k = 0; k < n_iter; ++k) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) x[i] = __builtin_fmaf(x[i], a, b);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s += x[i];
+    if (s == 1.25f) out[blockIdx.x] = s;   // keeps the chains live; never true for the bench arguments
+}
+
+extern "C" int slm_calibrate_valu(uint32_t n_blocks, uint32_t n_iter, float* out, void* stream, uint32_t flags) {
+    if (!out || !n_blocks) return slm_fail(SLM_E_INVALID_ARG, "out is NULL or n_blocks is 0");
+    hipLaunchKernelGGL(k_calib_valu, dim3(n_blocks), dim3(256), 0, (hipStream_t)stream, out, n_iter,
+                       0.999f, 0.001f);
+    return slm_finish(stream, flags);
+}
+
 // ------------------------------------------------------------------ seed tiling
 //
 // Work order for the fields tracer: seeds are bucketed into 8x8-pixel tiles,
@@ -924,8 +950,8 @@ __global__ void k_paths(SlmParams P, const float2* __restrict__ seeds, const uin
             dt = (ax < ay) ? dtx : ((ay < ax) ? dty : fminf(dtx, dty));
             x = mad<C>(su, dt, x);
             y = mad<C>(sv, dt, y);
-            x = approx(x, res);
-            y = approx(y, res);
+            x = approx_abs(x, res, P.approx_wrap);
+            y = approx_abs(y, res, P.approx_wrap);
             x = fminf(fmaxf(x, -0.5f), P.nxf_mp5);
             y = fminf(fmaxf(y, -0.5f), P.nyf_mp5);
             dl = fast_length<C>(x - px, y - py);

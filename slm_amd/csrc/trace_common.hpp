@@ -134,6 +134,16 @@ __device__ __forceinline__ float approx_r(float v, float res, float inv_res) {
     if constexpr (R2) return c * inv_res;
     else return fdiv(c, res);
 }
+// approximate of an absolute position (the Euler tails, rungekutta.cl:115,
+// :237), where v*RES is usually outside char range.  wrap: the low byte of the
+// truncation, as a literal reading of (char) gives; otherwise the int32
+// truncation itself (the lattice value), which is what the reference's
+// recorded runs computed.
+__device__ __forceinline__ float approx_abs(float v, float res, uint32_t wrap) {
+    if (wrap) return approx(v, res);
+    const float c = (float)cvt_i32(v * res);
+    return (res == 128.0f) ? c * 0.0078125f : fdiv(c, res);
+}
 // compress (essentials.cl:71-74)
 __device__ __forceinline__ int8_t compress1(float v, float res) { return (int8_t)(cvt_i32(v * res) & 0xff); }
 

@@ -21,7 +21,7 @@ __all__ = ["Fields"]
 
 class Fields:
     def __init__(self, which_cl_platform=0, which_cl_device=0, cl_src_path=None, info=None,
-                 data=None, verbose=False, gpu_verbose=False, fp_contract=True):
+                 data=None, verbose=False, gpu_verbose=False, fp_contract=False):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("Fields needs a HIP device (torch.cuda.is_available() is False)")

@@ -33,6 +33,7 @@ _PROTOS = {
     "slm_device_count": (ctypes.c_int, []),
     "slm_rng_stream": (ctypes.c_int, [_U32, _U32, _U32, _P, _P, _P, _U32]),
     "slm_check_rounding": (ctypes.c_int, [_P, _U32, _P, _U32]),
+    "slm_calibrate_valu": (ctypes.c_int, [_U32, _U32, _P, _P, _U32]),
     "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _U32]),
@@ -196,6 +197,14 @@ def check_rounding(n_numerators: int = 8, stream=None):
     counts = torch.zeros(8, dtype=torch.int64, device=torch.device("cuda", torch.cuda.current_device()))
     check(load().slm_check_rounding(_ptr(counts), n_numerators, _stream(stream), SLM_FLAG_SYNC))
     return [int(v) for v in counts.cpu()]
+
+
+def calibrate_valu(n_blocks: int, n_iter: int, out, stream=None, sync=False):
+    """slm_calibrate_valu: n_blocks blocks of 256 threads, each wave issuing
+    16*n_iter independent v_fma_f32 (out: float32 [n_blocks], never written
+    for these arguments)."""
+    _require_cuda(out)
+    check(load().slm_calibrate_valu(n_blocks, n_iter, _ptr(out), _stream(stream), SLM_FLAG_SYNC if sync else 0))
 
 
 def fields_workspace_size(p: SlmParams) -> int:

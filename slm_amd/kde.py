@@ -47,7 +47,7 @@ def render_kde_macros(distbn, kernel_fn: str) -> list:
     return out
 
 
-def kde_params(distbn, fp_contract=True) -> SlmKdeParams:
+def kde_params(distbn, fp_contract=False) -> SlmKdeParams:
     """SlmKdeParams parsed from the rendered macro literals."""
     m = {}
     for opt in render_kde_macros(distbn, "pdf_univariate")[1::2]:
@@ -86,7 +86,7 @@ def prepare_filter(distbn):
     distbn.n_kdf_part_points_y = 0
 
 
-def estimate_univariate_pdf(distbn, fp_contract=True):
+def estimate_univariate_pdf(distbn, fp_contract=False):
     """kde.py:116-194 ('opencl' path, no detrending): histogram + filter on the
     GPU.  Returns (pdf, histogram) normalised as the reference does."""
     import torch
@@ -155,7 +155,7 @@ class Univariate_distribution:
         if self.verbose:
             print(*args, **kwargs)
 
-    def compute_kde_opencl(self, kernel="epanechnikov", bandwidth=0.15, fp_contract=True):
+    def compute_kde_opencl(self, kernel="epanechnikov", bandwidth=0.15, fp_contract=False):
         self.kernel = kernel
         self.bandwidth = bandwidth
         if self.kernel == "gaussian":

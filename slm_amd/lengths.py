@@ -17,7 +17,7 @@ from .useful import pick_seeds, vprint
 __all__ = ["hsl"]
 
 
-def hsl(info, data, do_use_ridges=False, verbose=False, fp_contract=True):
+def hsl(info, data, do_use_ridges=False, verbose=False, fp_contract=False):
     """Fills data.subsegment_hsl_array (float32, one entry per seed pixel).
     Returns False for an empty seed list, as the reference does."""
     import torch

@@ -45,7 +45,7 @@ class ChannelMapper:
               confluence tie-break); may be None when the channels are given
     """
 
-    def __init__(self, info, mask, uv, mapping, slt=None, slc=None, fp_contract=True, stream=None):
+    def __init__(self, info, mask, uv, mapping, slt=None, slc=None, fp_contract=False, stream=None):
         import torch
         self.info = info
         self.device = mapping.device

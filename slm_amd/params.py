@@ -17,7 +17,7 @@ import math
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 _FIELD_NAMES = [
     ("abi_version", ctypes.c_uint32),
@@ -47,7 +47,8 @@ _FIELD_NAMES = [
     ("pixel_size", ctypes.c_float),
     ("interchannel_max_n_steps", ctypes.c_uint32),
     ("segmentation_threshold", ctypes.c_uint32),
-    ("reserved", ctypes.c_uint32 * 5),
+    ("approx_wrap", ctypes.c_uint32),
+    ("reserved", ctypes.c_uint32 * 4),
 ]
 
 
@@ -215,7 +216,14 @@ def parse_macros(options: list) -> dict:
     return vals
 
 
-def params_from_macros(options: list, fp_contract: bool = True) -> SlmParams:
+def params_from_macros(options: list, fp_contract: bool = False, approx_wrap: bool = False) -> SlmParams:
+    """SlmParams from the reference's option list.  fp_contract: fuse a*b+c
+    where clang's OpenCL default would; off (every operation rounded) is what
+    the reference's own recorded runs computed.  approx_wrap: the Euler tail's
+    approximate() of an absolute position keeps the low byte of the (char)
+    conversion; off truncates it onto the lattice (the int32 value), which is
+    what the reference's recorded runs computed (their trajectory statistics
+    are reproduced exactly only that way)."""
     m = parse_macros(options)
     p = SlmParams()
     p.abi_version = ABI_VERSION
@@ -227,13 +235,14 @@ def params_from_macros(options: list, fp_contract: bool = True) -> SlmParams:
     p.downup_sign = float(np.float32(m["DOWNUP_SIGN"]))
     p.trajectory_resolution = float(np.float32(m["TRAJECTORY_RESOLUTION"]))
     p.fp_contract = 1 if fp_contract else 0
+    p.approx_wrap = 1 if approx_wrap else 0
     return p
 
 
 def kernel_params(info, downup_sign=1, kernel_def: str = "INTEGRATE_FIELDS",
-                  fp_contract: bool = True) -> SlmParams:
+                  fp_contract: bool = False, approx_wrap: bool = False) -> SlmParams:
     """SlmParams for one kernel launch, via the reference's own macro strings."""
-    return params_from_macros(render_macros(info, kernel_def, downup_sign), fp_contract)
+    return params_from_macros(render_macros(info, kernel_def, downup_sign), fp_contract, approx_wrap)
 
 
 def pad_seed_count(n_seed_points: int, n_work_items: int) -> int:

@@ -42,7 +42,7 @@ class _Stages:
         return out
 
 
-def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=True, device=None, condition=False,
+def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=False, device=None, condition=False,
              midslopes=True, midslope_filter_sigma=5.0, midslope_threshold=0.03, ridge_filter_sigma=2.0,
              ridge_threshold=0.8, fmm=False, return_grids=False):
     """Run the pipeline on a workloads.Job; returns (results dict, stage timings).
@@ -83,23 +83,24 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     st.mark("trajectories")
     # the two directions on two streams: 1000 trajectories fill a few waves,
     # so the launches overlap instead of running one after the other
-    lengths, nsteps = [], []
     main = torch.cuda.current_stream(dev)
     side = hip.side_stream(dev)
+    info.n_seed_points = np.uint32(n_t)
+    ptraj = [kernel_params(info, sign, "INTEGRATE_TRAJECTORY", fp_contract) for sign in (1, -1)]
+    # every output of both directions is allocated before the first launch and
+    # stays referenced until the side stream has joined: a tensor freed while
+    # its kernel runs could otherwise be handed to the other direction's
+    # outputs by the caching allocator
+    outs = [(torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=dev),
+             torch.empty(n_t, dtype=torch.int16, device=dev),
+             torch.empty(n_t, dtype=torch.float32, device=dev)) for p in ptraj]
     side.wait_stream(main)
-    for sign, stream in ((1, main), (-1, side)):
-        info.n_seed_points = np.uint32(n_t)
-        p = kernel_params(info, sign, "INTEGRATE_TRAJECTORY", fp_contract)
-        traj = torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=dev)
-        ns = torch.empty(n_t, dtype=torch.int16, device=dev)
-        ln = torch.empty(n_t, dtype=torch.float32, device=dev)
-        if stream is side:
-            for t in (traj, ns, ln):
-                t.record_stream(side)
+    for p, (traj, ns, ln), stream in zip(ptraj, outs, (main, side)):
         hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln, stream=stream)
-        lengths.append(ln)
-        nsteps.append(ns)
     main.wait_stream(side)
+    lengths = [o[2] for o in outs]
+    nsteps = [o[1] for o in outs]
+    del outs
     tl = torch.stack(lengths, 1).cpu().numpy()
     tn = torch.stack(nsteps, 1).cpu().numpy().view(np.uint16)
     stats = compute_stats(tl, tn, info.pixel_size)

@@ -18,7 +18,7 @@ __all__ = ["Trace"]
 
 
 class Trace:
-    def __init__(self, state, trace_params, geodata, preprocess, fp_contract=True):
+    def __init__(self, state, trace_params, geodata, preprocess, fp_contract=False):
         self.state = state
         self.params = trace_params
         for k, v in vars(trace_params).items():

@@ -22,7 +22,7 @@ __all__ = ["Trajectories"]
 class Trajectories:
     def __init__(self, which_cl_platform=0, which_cl_device=0, cl_src_path=None, info=None,
                  data=None, do_trace_downstream=True, do_trace_upstream=True, verbose=False,
-                 gpu_verbose=False, fp_contract=True):
+                 gpu_verbose=False, fp_contract=False):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("Trajectories needs a HIP device (torch.cuda.is_available() is False)")

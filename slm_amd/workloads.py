@@ -37,7 +37,7 @@ class Job:
     def shape(self):
         return self.mask.shape
 
-    def params(self, sign=1, kernel="INTEGRATE_FIELDS", fp_contract=True, n_seed_points=None,
+    def params(self, sign=1, kernel="INTEGRATE_FIELDS", fp_contract=False, n_seed_points=None,
                jitter=None) -> SlmParams:
         info = self.info
         info.n_seed_points = np.uint32(self.seeds.shape[0] if n_seed_points is None else n_seed_points)

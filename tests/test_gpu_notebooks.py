@@ -1,0 +1,37 @@
+"""The reference's three notebook jobs through the product path on the GPU:
+Geodata, Preprocess (slm_condition_uv: loops and blockages), Trace
+(slm_trace_paths for the trajectory table, slm_trace_fields_batch for the
+fields), with the settings of the reference's JSON files.  The blockage
+count, the field seed count and the printed trajectory table of each
+notebook must come out as recorded (IndianCreek_Test1's upstream means,
+recorded at another commit, to 1e-4; see test_reference_notebooks.py)."""
+import pytest
+
+from notebook_jobs import JOBS, NOTEBOOK, geodata, stats_mismatches
+
+pytestmark = pytest.mark.gpu
+
+TEST1_SKIP = {("upstream", "l", "mean"), ("upstream", "ds", "mean")}
+
+
+@pytest.mark.parametrize("name", JOBS)
+def test_notebook_job_through_trace(gpu, name):
+    from slm_amd.parameters import section
+    from slm_amd.preprocess import Preprocess
+    from slm_amd.trace import Trace
+    nb = NOTEBOOK[name]
+    print("notebook job", name, flush=True)
+    params, state, geo = geodata(name)
+    pre = Preprocess(state, section(params, "preprocess"), geo)
+    pre.do()
+    assert len(pre.where_blockages_array) == nb["blockages"]
+    tr = Trace(state, section(params, "trace"), geo, pre)
+    tr.do()
+    assert int(tr.fields.info.n_seed_points) == nb["field_seeds"]
+    df = tr.traj_stats_df
+    skip = TEST1_SKIP if name == "IndianCreek_Test1" else ()
+    assert stats_mismatches(name, df, skip) == []
+    for direction, col, stat in skip:
+        want = float(nb["stats"][direction][col][("min", "mean", "max").index(stat)])
+        assert abs(float(df[col][direction, stat]) - want) <= 1e-4 * want
+    assert tr.slc_array.shape == (geo.roi_nx + 4, geo.roi_ny + 4, 2)

@@ -133,18 +133,18 @@ def test_library_exports_header_symbols():
     out = subprocess.run(["nm", "-D", "--defined-only", hip.LIB_PATH], capture_output=True, text=True).stdout
     for name in declared:
         assert re.search(r"\bT %s$" % name, out, re.M), name
-    assert lib.slm_abi_version() == 1
+    assert lib.slm_abi_version() == 2
 
 
 def test_library_rejects_bad_params_without_gpu():
     """Argument checks run on the host before any device call."""
     from slm_amd import hip
-    from slm_amd.params import SlmParams
+    from slm_amd.params import ABI_VERSION, SlmParams
     lib = hip.load()
     p = SlmParams()
     p.abi_version = 99
     assert lib.slm_trace_fields(ctypes.byref(p), None, None, None, None, None, None, None, 0, None, 0) == -4
-    p.abi_version = 1
+    p.abi_version = ABI_VERSION
     p.nx_padded, p.ny_padded, p.nxy_padded = 10, 10, 99
     assert lib.slm_trace_paths(ctypes.byref(p), None, None, None, None, None, None, None, 0) == -1
     assert b"nxy_padded" in lib.slm_last_error()
