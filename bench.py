@@ -38,6 +38,7 @@ BYTES_PER_STEP = 73
 HBM_PEAK_GBS = 8000.0
 CALIB_ITERS = 8192          # slm_calibrate_valu rounds: 16 v_fma_f32 each per wave
 CALIB_BLOCKS_PER_CU = 8     # 32 waves per CU, 8 per SIMD
+VALU_PEAK_KINDS = (0, 2, 8)  # v_fma_f32, v_add_u32, v_mov_b32
 
 
 def _profile_values(path):
@@ -56,10 +57,12 @@ def _profile_values(path):
     return vals
 
 
-def valu_peak(stream=None, reps=5):
+def valu_peak(stream=None, reps=5, kind=0):
     """Sustained wave64 VALU issue rate of the whole chip (wave-instructions
     per second): slm_calibrate_valu, whose waves issue a known number of
-    independent v_fma_f32, timed with HIP events on its stream."""
+    independent instructions of one kind (0 v_fma_f32, 1 v_add_f32,
+    2 v_add_u32, 3 v_xor_b32, 4 v_mul_f32, 5 the two alternating), timed with HIP events on its
+    stream."""
     import torch
     from slm_amd import hip
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -67,16 +70,17 @@ def valu_peak(stream=None, reps=5):
     n_blocks = cus * CALIB_BLOCKS_PER_CU
     out = torch.zeros(n_blocks, dtype=torch.float32, device=dev)
     stream = stream or torch.cuda.current_stream(dev)
-    hip.calibrate_valu(n_blocks, CALIB_ITERS, out, stream=stream, sync=True)
+    hip.calibrate_valu(n_blocks, CALIB_ITERS, out, kind=kind, stream=stream, sync=True)
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     for a, b in ev:
         a.record(stream)
-        hip.calibrate_valu(n_blocks, CALIB_ITERS, out, stream=stream)
+        hip.calibrate_valu(n_blocks, CALIB_ITERS, out, kind=kind, stream=stream)
         b.record(stream)
     torch.cuda.synchronize(dev)
     ms = float(np.median([a.elapsed_time(b) for a, b in ev]))
     instr = n_blocks * 4 * 16 * CALIB_ITERS
-    return instr / (ms / 1e3), {"blocks": n_blocks, "iters": CALIB_ITERS, "wave_instructions": instr, "ms": ms}
+    return instr / (ms / 1e3), {"kind": kind, "blocks": n_blocks, "iters": CALIB_ITERS, "wave_instructions": instr,
+                                "ms": ms}
 
 
 def _cpu_info():
@@ -167,8 +171,9 @@ def tracer_roofline(args, kname, steps_per_launch, mean_launch_s, alg_gbs, strea
       - VALU issue: the VALU wave-instructions one launch issues (the
         counter SQ_INSTS_VALU per launch from the committed PMC summary,
         divided by what that counter reads per issued instruction on the
-        calibration kernel) per second, against the chip's sustained issue
-        rate measured now by the calibration kernel;
+        calibration kernel) per second, against the chip's fastest sustained
+        VALU issue rate, measured now by the calibration kernel over a few
+        instruction kinds;
       - HBM: measured DRAM-side bytes per launch (FETCH_SIZE + WRITE_SIZE,
         copy-calibrated, from the same summary) per second, against 8 TB/s.
     The bound is the resource with the larger fraction.  The algorithmic
@@ -192,10 +197,15 @@ def tracer_roofline(args, kname, steps_per_launch, mean_launch_s, alg_gbs, strea
     valu = prof.get("SQ_INSTS_VALU_per_launch")
     per_instr = cal.get("SQ_INSTS_VALU_per_issued_instruction")
     if valu and per_instr:
-        peak, meta = valu_peak(stream)
+        # the fastest sustained issue rate of any calibrated VALU instruction
+        # kind (v_mov_b32 issues faster than the 4-cycle arithmetic, and the
+        # tracer's mix holds both), and the v_fma_f32 rate beside it
+        peak, meta = max((valu_peak(stream, kind=k) for k in VALU_PEAK_KINDS), key=lambda r: r[0])
+        fma, _ = valu_peak(stream, kind=0)
         rate = valu / per_instr / mean_launch_s
         out["valu"] = {"achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instructions/s",
-                       "frac": rate / peak, "instructions_per_launch": valu / per_instr,
+                       "frac": rate / peak, "fma_rate": fma / 1e9, "frac_of_fma_rate": rate / fma,
+                       "instructions_per_launch": valu / per_instr,
                        "instructions_per_step": valu / per_instr / steps_per_launch, "calibration": meta}
     if "valu" in out and (hbm["frac"] is None or out["valu"]["frac"] >= hbm["frac"]):
         v = out["valu"]
@@ -478,8 +488,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     if args.valu_calibration:
-        rate, meta = valu_peak(reps=3)
-        print(json.dumps({"valu_calibration": meta, "wave_instructions_per_s": rate}), flush=True)
+        for kind in range(9):
+            rate, meta = valu_peak(reps=3, kind=kind)
+            print(json.dumps({"valu_calibration": meta, "wave_instructions_per_s": rate}), flush=True)
         return
     if args.copy_calibration:
         n = args.copy_calibration << 20

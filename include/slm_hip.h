@@ -126,11 +126,14 @@ int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void* stream, ui
 /*
  * Measurement hook with no counterpart in the reference: n_blocks blocks of
  * 256 threads in which every wave issues exactly 16*n_iter independent
- * v_fma_f32 (16 chains per lane), for timing the sustained VALU issue rate
- * and for calibrating the VALU instruction counter against a known count.
- * out: float [n_blocks], written only if a chain sums to an impossible value.
+ * instructions of one kind (16 chains per lane; kind 0 v_fma_f32, 1 v_add_f32,
+ * 2 v_add_u32, 3 v_xor_b32, 4 v_mul_f32, 5 v_fma_f32 and v_add_u32
+ * alternating, 6 v_cndmask_b32, 7 v_cmp_lt_f32, 8 v_mov_b32), for timing the sustained VALU issue
+ * rate and for calibrating the VALU instruction counter against a known count.
+ * out: float [n_blocks], written only if the chains end on an impossible value.
  */
-int slm_calibrate_valu(uint32_t n_blocks, uint32_t n_iter, float* out, void* stream, uint32_t flags);
+int slm_calibrate_valu(uint32_t kind, uint32_t n_blocks, uint32_t n_iter, float* out, void* stream,
+                       uint32_t flags);
 
 /*
  * Jittered streamline fields (slc / slt), one direction.

@@ -33,7 +33,7 @@ _PROTOS = {
     "slm_device_count": (ctypes.c_int, []),
     "slm_rng_stream": (ctypes.c_int, [_U32, _U32, _U32, _P, _P, _P, _U32]),
     "slm_check_rounding": (ctypes.c_int, [_P, _U32, _P, _U32]),
-    "slm_calibrate_valu": (ctypes.c_int, [_U32, _U32, _P, _P, _U32]),
+    "slm_calibrate_valu": (ctypes.c_int, [_U32, _U32, _U32, _P, _P, _U32]),
     "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _U32]),
@@ -199,12 +199,14 @@ def check_rounding(n_numerators: int = 8, stream=None):
     return [int(v) for v in counts.cpu()]
 
 
-def calibrate_valu(n_blocks: int, n_iter: int, out, stream=None, sync=False):
+def calibrate_valu(n_blocks: int, n_iter: int, out, kind: int = 0, stream=None, sync=False):
     """slm_calibrate_valu: n_blocks blocks of 256 threads, each wave issuing
-    16*n_iter independent v_fma_f32 (out: float32 [n_blocks], never written
-    for these arguments)."""
+    16*n_iter independent instructions of one kind (0 v_fma_f32, 1 v_add_f32,
+    2 v_add_u32, 3 v_xor_b32, 4 v_mul_f32, 5 fma/add_u32 alternating); out: float32
+    [n_blocks]."""
     _require_cuda(out)
-    check(load().slm_calibrate_valu(n_blocks, n_iter, _ptr(out), _stream(stream), SLM_FLAG_SYNC if sync else 0))
+    check(load().slm_calibrate_valu(kind, n_blocks, n_iter, _ptr(out), _stream(stream),
+                                    SLM_FLAG_SYNC if sync else 0))
 
 
 def fields_workspace_size(p: SlmParams) -> int:

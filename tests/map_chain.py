@@ -15,7 +15,7 @@ STEPS = ("connect", "thin", "heads", "count", "flag", "confluences", "segment", 
 
 
 class OracleMapper:
-    def __init__(self, info, mask, uv, mapping, slt=None, fp_contract=True):
+    def __init__(self, info, mask, uv, mapping, slt=None, fp_contract=False):
         self.info = info
         self.mask = np.ascontiguousarray(mask, dtype=bool)
         self.mask_u8 = self.mask.astype(np.uint8).reshape(-1)

@@ -98,7 +98,7 @@ def test_trajectories_and_fields_dropin(gpu):
     raw_c, raw_t = [], []
     for sign in (1, -1):
         info_f.downup_sign = float(sign)
-        p = kernel_params(info_f, sign, "INTEGRATE_FIELDS", True)
+        p = kernel_params(info_f, sign, "INTEGRATE_FIELDS", False)
         c, t, _ = oracle.trace_fields(p, seeds, job.mask.reshape(-1).astype(np.uint8), job.uv.reshape(-1))
         raw_c.append(c.reshape(nxp, nyp))
         raw_t.append(t.reshape(nxp, nyp))

@@ -50,7 +50,7 @@ def test_real_dtm_end_to_end(gpu, name, trace):
     nxp, nyp = d.mask_array.shape
     for k, sign in enumerate((1, -1)):
         f.info.downup_sign = float(sign)
-        p = kernel_params(f.info, sign, "INTEGRATE_FIELDS", True)
+        p = kernel_params(f.info, sign, "INTEGRATE_FIELDS", False)
         c, t, _ = oracle.trace_fields(p, d.seed_point_array, d.mask_array.reshape(-1).astype(np.uint8),
                                       d.uv_array.reshape(-1))
         assert np.array_equal(f.raw_slc[..., k], c.reshape(nxp, nyp)), (name, sign)
