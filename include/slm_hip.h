@@ -156,6 +156,27 @@ int slm_trace_fields(const SlmParams* p, const float* seeds_xy,
                      void* stream, uint32_t flags);
 
 /*
+ * slm_trace_fields without a workspace argument and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation).  The workspace lives in a fixed device array of
+ * slm_trace_fields_noalloc_max_bytes() bytes (64 MiB) that every device the
+ * library runs on holds until the process ends; a call whose grid and seeds
+ * need more (slm_trace_fields_workspace_size(p) > the maximum, about a 1024^2
+ * grid with one seed per pixel) fails with SLM_E_WORKSPACE and enqueues
+ * nothing.  Results are bit-identical to slm_trace_fields.  All calls on one
+ * device share the array: calls that overlap in time on one device (other
+ * streams, other threads) race; calls ordered on one stream do not.
+ * SLM_FLAG_KEEP_ORDER reuses the seed order and quads the previous call on
+ * the device left in the array.
+ */
+size_t slm_trace_fields_noalloc_max_bytes(void);
+int slm_trace_fields_noalloc(const SlmParams* p, const float* seeds_xy,
+                             const uint8_t* mask, const float* uv,
+                             uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                             void* stream, uint32_t flags);
+
+/*
  * Same result as slm_trace_fields (bit for bit), computed by LDS-owned tiles:
  * the grid is cut into tile x tile pixel tiles; each block steps the
  * streamlines currently in one tile with that

@@ -799,9 +799,9 @@ static uint32_t grid_for(size_t n) {
     return (uint32_t)(b > 4096 ? 4096 : (b ? b : 1));
 }
 
-extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
-                                const float* uv, uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
-                                void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
+static int trace_fields_ws(const SlmParams* p, const float* seeds_xy, const uint8_t* mask, const float* uv,
+                           uint32_t* slc, uint32_t* slt, uint64_t* steps_out, void* workspace,
+                           size_t workspace_bytes, void* stream, uint32_t flags) {
     int rc = check_params(p);
     if (rc) return rc;
     if (!mask || !uv || !slc || !slt) return slm_fail(SLM_E_INVALID_ARG, "NULL mask/uv/slc/slt");
@@ -884,6 +884,64 @@ extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const
         hipLaunchKernelGGL((k_fields<false, false, false>), dim3(grid), dim3(256), 0, s, *p, seeds, order, n_work, mask,
                            quad, acc, slc, slt, nullptr, ctrl, force, fixed);
     return slm_finish(stream, flags);
+}
+
+extern "C" int slm_trace_fields(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                const float* uv, uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                                void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
+    return trace_fields_ws(p, seeds_xy, mask, uv, slc, slt, steps_out, workspace, workspace_bytes, stream, flags);
+}
+
+// ------------------------------------------------------------------ fields without allocation
+//
+// slm_trace_fields_noalloc runs slm_trace_fields with its workspace in a
+// fixed device array instead of a caller buffer, for callers that cannot
+// allocate (graph capture, serving loops).  Each device holds its own copy of
+// the array for the life of the process.  In host code the array's name is
+// only a stand-in for the symbol: its device address on the current device is
+// looked up once per device with hipGetSymbolAddress and used everywhere.
+// Every kernel of the call clears what it needs before use (queue heads,
+// counting-sort counts, the packed accumulator), so nothing left by an earlier
+// call changes a result.  Calls that overlap on one device share the array and
+// race.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64];
+
+static int scratch_ptr(void** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return slm_fail(SLM_E_HIP, "device id %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return slm_fail(SLM_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return SLM_OK;
+}
+
+extern "C" size_t slm_trace_fields_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int slm_trace_fields_noalloc(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                        const float* uv, uint32_t* slc, uint32_t* slt, uint64_t* steps_out,
+                                        void* stream, uint32_t flags) {
+    int rc = check_params(p);
+    if (rc) return rc;
+    const size_t need = fields_ws(p).total;
+    if (need > kNoAllocScratchBytes)
+        return slm_fail(SLM_E_WORKSPACE, "grid and seeds need %zu bytes of scratch, more than the %zu-byte maximum "
+                        "of slm_trace_fields_noalloc (use slm_trace_fields with a workspace)",
+                        need, kNoAllocScratchBytes);
+    void* ws = nullptr;
+    rc = scratch_ptr(&ws);
+    if (rc) return rc;
+    return trace_fields_ws(p, seeds_xy, mask, uv, slc, slt, steps_out, ws, kNoAllocScratchBytes, stream, flags);
 }
 
 // ------------------------------------------------------------------ fields record (test hook)

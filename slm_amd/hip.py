@@ -37,6 +37,8 @@ _PROTOS = {
     "slm_trace_fields_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams)]),
     "slm_trace_fields": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P,
                                         ctypes.c_size_t, _P, _U32]),
+    "slm_trace_fields_noalloc_max_bytes": (ctypes.c_size_t, []),
+    "slm_trace_fields_noalloc": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P, _U32]),
     "slm_trace_fields_tiled_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(SlmParams),
                                                                 ctypes.POINTER(SlmTileOpts)]),
     "slm_trace_fields_tiled": (ctypes.c_int, [ctypes.POINTER(SlmParams), ctypes.POINTER(SlmTileOpts), _P, _P, _P,
@@ -231,6 +233,21 @@ def trace_fields(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, workspace=
                                   _ptr(slt), _ptr(steps), _ptr(workspace), workspace.numel(),
                                   _stream(stream), flags))
     return workspace
+
+
+def trace_fields_noalloc(p: SlmParams, seeds, mask, uv, slc, slt, steps=None, stream=None, accumulate=False,
+                         sync=False, keep_order=False):
+    """slm_trace_fields_noalloc: trace_fields with the library's fixed device
+    scratch array as workspace (no allocation by the library or here)."""
+    _require_cuda(seeds, mask, uv, slc, slt, steps)
+    flags = (SLM_FLAG_NO_ZERO if accumulate else 0) | (SLM_FLAG_SYNC if sync else 0) | \
+        (SLM_FLAG_KEEP_ORDER if keep_order else 0)
+    check(load().slm_trace_fields_noalloc(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(slc), _ptr(slt),
+                                          _ptr(steps), _stream(stream), flags))
+
+
+def noalloc_max_bytes() -> int:
+    return int(load().slm_trace_fields_noalloc_max_bytes())
 
 
 def tile_opts(tile=0, block=0, unit=0, cap_steps=0, halo=0) -> SlmTileOpts:
