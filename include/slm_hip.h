@@ -664,25 +664,32 @@ int slm_map_midslopes_ridges(const float* sla, const uint8_t* mask, uint32_t nx,
 int slm_fmm_pairs(const int32_t* label, uint32_t nx, uint32_t ny, const uint8_t* colour, uint32_t n_labels,
                   uint64_t* pairs, uint32_t cap, uint32_t* count, void* stream, uint32_t flags);
 size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny);
-/* Workspace for k colours relaxed together (14 B per pixel per colour plus
+/* Workspace for k colours solved together (46 B per pixel per colour plus
  * tile flags); slm_fmm_hsl takes as many colours per group as its workspace
- * holds, so a larger workspace means fewer, fuller relaxation launches.
+ * holds, so a larger workspace means fewer, fuller launches.
  * slm_fmm_workspace_size(nx, ny) == slm_fmm_workspace_size_k(nx, ny, 1). */
 size_t slm_fmm_workspace_size_k(uint32_t nx, uint32_t ny, uint32_t k);
 /* For every subsegment a (colour[a] < n_colours): phi = -1 on its
  * thin-channel pixels ((mapping & thin_flag) == thin_flag), +1 elsewhere on
  * its domain (the pixels with |label| == a and their 4-neighbours); the
- * first-order eikonal distance from the zero contour (scikit-fmm's
- * interface initialisation, grid step dx, float64); per subsegment:
- * sums[a] = sum of round(signed distance * 2^24) (int64 bit pattern) over the
- * points a channel reaches, cnt[a] their number, nch[a] the channel pixels,
- * nphi[a] the pixels with |label| == a.  Arrays of n_labels entries.
- * Synchronises the stream after every relaxation launch (each launch runs
- * up to 128 in-tile sweeps on every active tile of every colour of a group). */
+ * eikonal distance from the zero contour (scikit-fmm's interface
+ * initialisation, grid step dx, float64) of the given order:
+ *   order 2 (scikit-fmm's default): scikit-fmm's second-order update, points
+ *     accepted in (value, index) order, one lane per subsegment; where an
+ *     update meets a negative discriminant (scikit-fmm raises) the march of
+ *     that subsegment stops and failed[a] = 1 (failed: uint8 [n_labels]);
+ *   order 1: the first-order update relaxed to the fast march's fixed point
+ *     (failed may be NULL; synchronises the stream after every relaxation
+ *     launch, each running up to 128 in-tile sweeps on every active tile of
+ *     every colour of a group).
+ * Per subsegment: sums[a] = sum of round(signed distance * 2^24) (int64 bit
+ * pattern) over the points a channel reaches, cnt[a] their number, nch[a]
+ * the channel pixels, nphi[a] the pixels with |label| == a.  Arrays of
+ * n_labels entries (n_labels <= nx*ny + 1). */
 int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_flag, const uint8_t* colour,
-                uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx, uint64_t* sums,
-                uint32_t* cnt, uint32_t* nch, uint32_t* nphi, void* workspace, size_t workspace_bytes, void* stream,
-                uint32_t flags);
+                uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx, uint32_t order,
+                uint64_t* sums, uint32_t* cnt, uint32_t* nch, uint32_t* nphi, uint8_t* failed, void* workspace,
+                size_t workspace_bytes, void* stream, uint32_t flags);
 /* remap_hsl: grid[i] = hsl[|label[i]|] where the signed label is selected
  * (selected[|l|] bit 0 for l > 0, bit 1 for l < 0), else 0 (float32). */
 int slm_hsl_remap(const int32_t* label, uint32_t n, const int32_t* hsl, const uint8_t* selected, uint32_t n_labels,

@@ -8,10 +8,13 @@ slm_hsl_remap (slm_amd/hsl.py).  It restates python/streamlines/mapping.py:
       elsewhere; distance = skfmm.distance(phi, dx); hsl = 2 * mean(distance),
       kept (as int32, truncated) when n_phi / n_channel > 4
   parse_hsl + remap_hsl(threshold 0)     :923-985  each label's value on the grid
-with two stated differences:
-  * skfmm.distance (scikit-fmm, absent here; second order by default) is the
-    first-order fast marching of oracle/slm_oracle_fmm.c: parity unpinned
-    against the reference, pinned by closed forms in tests/test_fmm.py;
+with these stated points:
+  * skfmm.distance (scikit-fmm, absent here) is restated from its published
+    algorithm in oracle/slm_oracle_fmm.c: order 2 (scikit-fmm's default,
+    fmm_lengths(order=2)) or order 1; parity with scikit-fmm itself is
+    unpinned, the restatement is pinned by closed forms in tests/test_fmm.py;
+    a negative discriminant, on which scikit-fmm raises, leaves the
+    subsegment unmeasured as the reference's except clause does;
   * the mean distance of a subsegment is accumulated in fixed point
     (round(d * 2^24) summed in int64), so it does not depend on summation
     order; the reference's float64 mean differs by far less than the int32
@@ -37,6 +40,9 @@ def _lib():
                                       ctypes.c_double, ctypes.c_void_p]
     lib.slmo_fmm_update.restype = ctypes.c_double
     lib.slmo_fmm_update.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.slmo_fmm_distance2.restype = ctypes.c_int
+    lib.slmo_fmm_distance2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
     return lib
 
 
@@ -56,6 +62,29 @@ def fmm_distance(phi, dom, dx):
     return out
 
 
+class NegativeDiscriminant(ArithmeticError):
+    """The second-order update met a negative discriminant (scikit-fmm raises;
+    the reference then leaves the subsegment's length unset)."""
+
+
+def fmm_distance2(phi, dom, dx, order=2):
+    """Signed FMM distance of order 1 or 2 (scikit-fmm's second-order update,
+    points accepted in (value, index) order).  Raises NegativeDiscriminant
+    where scikit-fmm raises."""
+    phi = np.ascontiguousarray(np.where(np.asarray(phi) < 0, -1, 1).astype(np.int8))
+    dom = np.ascontiguousarray(np.asarray(dom, bool).astype(np.uint8))
+    nx, ny = phi.shape
+    out = np.empty((nx, ny), np.float64)
+    if order == 1:
+        return fmm_distance(phi, dom, dx)
+    rc = _lib().slmo_fmm_distance2(nx, ny, phi.ctypes.data, dom.ctypes.data, float(dx), 2, out.ctypes.data)
+    if rc == -1:
+        raise MemoryError("fmm oracle")
+    if rc == -2:
+        raise NegativeDiscriminant("negative discriminant in the second-order update")
+    return out
+
+
 def subsegment_problem(label, mapping, is_thinchannel, a):
     """(phi int8, domain bool, n_phi, n_channel) of subsegment |label| == a."""
     phi_bool = np.abs(label) == a
@@ -67,9 +96,12 @@ def subsegment_problem(label, mapping, is_thinchannel, a):
     return phi, dom, n_phi, n_ch
 
 
-def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size):
+def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, order=2):
     """Returns (selected int32[], hsl int32[], stats): stats maps |label| to
-    (fixed-point distance sum, domain size, n_phi, n_channel)."""
+    (fixed-point distance sum, domain size, n_phi, n_channel).  order 2 uses
+    fmm_distance2; a subsegment whose second-order
+    march meets a negative discriminant keeps hsl 0, as the reference's
+    except clause leaves it (stats entry (None, ...))."""
     label = np.asarray(label).astype(np.int64)
     mapping = np.asarray(mapping).astype(np.uint32)
     sel = np.unique(label[~mask].astype(np.int32))
@@ -85,12 +117,18 @@ def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size):
             else:
                 # points no channel reaches (a domain part without channel
                 # pixels) stay infinite and are left out of the mean
-                d = fmm_distance(phi, dom, pixel_size)[dom]
-                d = d[np.isfinite(d)]
-                q = int(np.rint(d * FIX).astype(np.int64).sum())
-                stats[a] = (q, int(d.size), n_phi, n_ch)
+                try:
+                    d = fmm_distance2(phi, dom, pixel_size, order)[dom]
+                except NegativeDiscriminant:
+                    d = None
+                if d is None:
+                    stats[a] = (None, int(dom.sum()), n_phi, n_ch)
+                else:
+                    d = d[np.isfinite(d)]
+                    q = int(np.rint(d * FIX).astype(np.int64).sum())
+                    stats[a] = (q, int(d.size), n_phi, n_ch)
         q, cnt, n_phi, n_ch = stats[a]
-        if n_ch == 0:
+        if n_ch == 0 or q is None:
             continue
         h = 2.0 * (q / FIX) / cnt
         if n_phi / n_ch > 4.0:

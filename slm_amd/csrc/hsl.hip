@@ -43,6 +43,8 @@ constexpr double kFix = 16777216.0;   // 2^24
 struct FmmLayout {
     uint8_t* base;
     size_t stride, off_neg, off_u, off_fixed, off_act0, off_act1;
+    // second-order march: per-label point lists, heap positions and heaps
+    size_t off_lcnt, off_lcur, off_pts, off_pos, off_heap;
 };
 __device__ __forceinline__ uint8_t* slot_base(const FmmLayout& L) { return L.base + blockIdx.y * L.stride; }
 
@@ -284,6 +286,224 @@ __global__ __launch_bounds__(256) void k_fmm_relax_tiles(uint32_t nx, uint32_t n
         act_out[blockIdx.x] = 1;
         atomicAdd(changed, 1u);
     }
+}
+
+// ---------------------------------------------------------------- second order
+//
+// scikit-fmm's default (distanceMarcher::updatePointOrderTwo, as published):
+// per axis the smaller frozen neighbour v1 and, when the point beyond it in
+// the same direction is frozen with a value not above v1, that value v2
+// (negated when it lies across the zero contour from the neighbour or from
+// the point) give (9/4)(u - t)^2/dx^2 with t = (4 v1 - v2)/3, otherwise
+// (u - v1)^2/dx^2; u is the larger root of their sum = 1.  v2 is not reset
+// when the other direction of the axis gives a smaller v1 without a usable
+// second point (as published).  Which points are frozen when a point is
+// updated depends on the order of acceptance, so the march runs in that
+// order: one lane per subsegment (the subsegments of a colour are disjoint),
+// a binary heap of (value, index) pairs in the workspace, a point's value
+// replaced at each update.  Pops come in (value, index) order, which is the
+// oracle's order, so the results are its bits.  A negative discriminant (on
+// which scikit-fmm raises, and the reference then leaves the subsegment
+// unmeasured) stops the lane and marks the subsegment failed.
+
+struct HeapEnt {
+    double v;
+    int32_t i, pad;
+};
+
+__device__ __forceinline__ bool hless(const HeapEnt& a, const HeapEnt& b) {
+    return a.v < b.v || (a.v == b.v && a.i < b.i);
+}
+
+__device__ void heap_up(HeapEnt* h, int32_t* pos, int k) {
+    const HeapEnt e = h[k];
+    while (k > 0) {
+        const int p = (k - 1) >> 1;
+        const HeapEnt hp = h[p];
+        if (!hless(e, hp)) break;
+        h[k] = hp;
+        pos[hp.i] = k;
+        k = p;
+    }
+    h[k] = e;
+    pos[e.i] = k;
+}
+
+__device__ void heap_down(HeapEnt* h, int32_t* pos, int n, int k) {
+    const HeapEnt e = h[k];
+    for (;;) {
+        int c = 2 * k + 1;
+        if (c >= n) break;
+        HeapEnt hc = h[c];
+        if (c + 1 < n) {
+            const HeapEnt h2 = h[c + 1];
+            if (hless(h2, hc)) {
+                hc = h2;
+                c++;
+            }
+        }
+        if (!hless(hc, e)) break;
+        h[k] = hc;
+        pos[hc.i] = k;
+        k = c;
+    }
+    h[k] = e;
+    pos[e.i] = k;
+}
+
+__global__ void k_fmm_lcount(uint32_t nx, uint32_t ny, FmmLayout L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    const uint32_t key = ((const uint32_t*)slot_base(L))[i];
+    if (key) atomicAdd((uint32_t*)(slot_base(L) + L.off_lcnt) + key, 1u);
+}
+
+// Exclusive scan of the per-label counts [0, n] in place (one 1024-thread
+// block per colour slot); the cursor copy feeds the scatter.
+__global__ __launch_bounds__(1024) void k_fmm_lscan(uint32_t n, FmmLayout L) {
+    uint32_t* __restrict__ cnt = (uint32_t*)(slot_base(L) + L.off_lcnt);
+    uint32_t* __restrict__ cur = (uint32_t*)(slot_base(L) + L.off_lcur);
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = min(n, t * per), e = min(n, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = (t == 0) ? 0u : part[t - 1];
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t c = cnt[i];
+        cnt[i] = run;
+        cur[i] = run;
+        run += c;
+    }
+}
+
+__global__ void k_fmm_lscatter(uint32_t nx, uint32_t ny, FmmLayout L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    const uint32_t key = ((const uint32_t*)slot_base(L))[i];
+    if (!key) return;
+    const uint32_t at = atomicAdd((uint32_t*)(slot_base(L) + L.off_lcur) + key, 1u);
+    ((uint32_t*)(slot_base(L) + L.off_pts))[at] = i;
+}
+
+// The second-order update of point (x, y) of problem key (state >= 2 frozen).
+__device__ double fmm_update2(const uint32_t* __restrict__ dom, const uint8_t* __restrict__ neg,
+                              const uint8_t* __restrict__ state, const double* __restrict__ u, uint32_t key, int x,
+                              int y, int nx, int ny, double dx, bool* bad) {
+    const double idx2 = 1.0 / (dx * dx), aa = 9.0 / 4.0, third = 1.0 / 3.0;
+    const size_t i = (size_t)x * ny + y;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int axis = 0; axis < 2; axis++) {
+        double v1 = INFINITY, v2 = INFINITY;
+        for (int s = -1; s <= 1; s += 2) {
+            const int xx = axis == 0 ? x + s : x, yy = axis == 1 ? y + s : y;
+            if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) continue;
+            const size_t n = (size_t)xx * ny + yy;
+            if (dom[n] != key || state[n] < 2 || !(u[n] < v1)) continue;
+            v1 = u[n];
+            const int x2 = axis == 0 ? x + 2 * s : x, y2 = axis == 1 ? y + 2 * s : y;
+            if (x2 < 0 || y2 < 0 || x2 >= nx || y2 >= ny) continue;
+            const size_t n2 = (size_t)x2 * ny + y2;
+            if (dom[n2] != key || state[n2] < 2 || !(u[n2] <= v1)) continue;
+            v2 = u[n2];
+            if (neg[n2] != neg[n] || neg[n2] != neg[i]) v2 = -v2;
+        }
+        if (v2 < INFINITY) {
+            const double tp = third * (4.0 * v1 - v2);
+            a += idx2 * aa;
+            b -= idx2 * 2.0 * aa * tp;
+            c += idx2 * aa * tp * tp;
+        } else if (v1 < INFINITY) {
+            a += idx2;
+            b -= idx2 * 2.0 * v1;
+            c += idx2 * v1 * v1;
+        }
+    }
+    c -= 1.0;
+    const double det = b * b - 4.0 * a * c;
+    if (det < 0.0) {
+        *bad = true;
+        return INFINITY;
+    }
+    return (-b + sqrt(det)) / 2.0 / a;
+}
+
+// One lane per label of the slot's colour: the fast march of that subsegment.
+__global__ __launch_bounds__(64) void k_fmm_march(uint32_t nx, uint32_t ny, double dx, FmmLayout L,
+                                                  const uint8_t* __restrict__ colour, uint32_t n_labels, uint32_t c0,
+                                                  uint8_t* __restrict__ failed) {
+    const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key == 0 || key >= n_labels || colour[key] != c0 + blockIdx.y) return;
+    uint8_t* sb = slot_base(L);
+    const uint32_t* __restrict__ dom = (const uint32_t*)sb;
+    const uint8_t* __restrict__ neg = sb + L.off_neg;
+    double* __restrict__ u = (double*)(sb + L.off_u);
+    uint8_t* __restrict__ state = sb + L.off_fixed;    // k_fmm_init: 1 next to the contour
+    const uint32_t* __restrict__ lst = (const uint32_t*)(sb + L.off_lcnt);
+    const uint32_t beg = lst[key], end = lst[key + 1];
+    if (beg == end) return;
+    const uint32_t* __restrict__ pts = (const uint32_t*)(sb + L.off_pts) + beg;
+    int32_t* __restrict__ pos = (int32_t*)(sb + L.off_pos);
+    HeapEnt* __restrict__ h = (HeapEnt*)(sb + L.off_heap) + beg;
+    const uint32_t m = end - beg;
+    int hn = 0;
+    // states: 0 far, 1 trial, 2 next to the contour (frozen), 3 accepted
+    for (uint32_t k = 0; k < m; k++) {
+        const uint32_t p = pts[k];
+        pos[p] = -1;
+        if (state[p]) {
+            state[p] = 2;
+            h[hn].v = u[p];
+            h[hn].i = (int32_t)p;
+            heap_up(h, pos, hn);
+            hn++;
+        }
+    }
+    bool bad = false;
+    while (hn > 0 && !bad) {
+        const HeapEnt top = h[0];
+        hn--;
+        if (hn > 0) {
+            h[0] = h[hn];
+            heap_down(h, pos, hn, 0);
+        }
+        const int32_t i = top.i;
+        pos[i] = -1;
+        state[i] = 3;
+        const int x = i / (int)ny, y = i % (int)ny;
+        const int nb[4][2] = {{x - 1, y}, {x + 1, y}, {x, y - 1}, {x, y + 1}};
+        for (int k = 0; k < 4 && !bad; k++) {
+            const int xx = nb[k][0], yy = nb[k][1];
+            if (xx < 0 || yy < 0 || xx >= (int)nx || yy >= (int)ny) continue;
+            const int32_t j = xx * (int)ny + yy;
+            if (dom[j] != key || state[j] >= 2) continue;
+            const double v = fmm_update2(dom, neg, state, u, key, xx, yy, (int)nx, (int)ny, dx, &bad);
+            if (bad || v == 0.0) continue;     // scikit-fmm skips a zero update
+            const double old = u[j];
+            u[j] = v;
+            state[j] = 1;
+            if (pos[j] >= 0) {
+                h[pos[j]].v = v;
+                if (v < old || (v == old)) heap_up(h, pos, pos[j]);
+                else heap_down(h, pos, hn, pos[j]);
+            } else {
+                h[hn].v = v;
+                h[hn].i = j;
+                heap_up(h, pos, hn);
+                hn++;
+            }
+        }
+    }
+    if (bad) failed[key] = 1;
 }
 
 // Per problem: fixed-point sum of the signed distances of the points a
@@ -625,7 +845,11 @@ namespace {
 size_t fmm_slot_bytes(uint32_t nx, uint32_t ny) {
     const size_t n = (size_t)nx * ny;
     const size_t n_tiles = (size_t)((nx + kT - 1) / kT) * ((ny + kT - 1) / kT);
-    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 2 * al256(n_tiles);
+    // dom, neg, u, state/fixed, two tile-activity maps; then, for the
+    // second-order march, label counts and cursors (labels < n + 1), point
+    // lists, heap positions and heaps
+    return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 2 * al256(n_tiles) + 2 * al256((n + 2) * 4) +
+           al256(n * 4) + al256(n * 4) + al256(n * sizeof(HeapEnt));
 }
 }  // namespace
 
@@ -637,11 +861,12 @@ extern "C" size_t slm_fmm_workspace_size_k(uint32_t nx, uint32_t ny, uint32_t k)
 extern "C" size_t slm_fmm_workspace_size(uint32_t nx, uint32_t ny) { return slm_fmm_workspace_size_k(nx, ny, 1); }
 
 extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32_t thin_flag, const uint8_t* colour,
-                           uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx,
-                           uint64_t* sums, uint32_t* cnt, uint32_t* nch, uint32_t* nphi, void* workspace,
-                           size_t workspace_bytes, void* stream, uint32_t flags) {
+                           uint32_t n_labels, uint32_t n_colours, uint32_t nx, uint32_t ny, double dx, uint32_t order,
+                           uint64_t* sums, uint32_t* cnt, uint32_t* nch, uint32_t* nphi, uint8_t* failed,
+                           void* workspace, size_t workspace_bytes, void* stream, uint32_t flags) {
     if (!label || !mapping || !colour || !sums || !cnt || !nch || !nphi || !workspace || !nx || !ny ||
-        (uint64_t)nx * ny >= 0x80000000ull || !(dx > 0.0) || n_colours > 255)
+        (uint64_t)nx * ny >= 0x80000000ull || !(dx > 0.0) || n_colours > 255 || (order != 1 && order != 2) ||
+        (order == 2 && !failed) || n_labels > (uint64_t)nx * ny + 1)
         return slm_fail(SLM_E_INVALID_ARG, "bad fmm_hsl arguments");
     if (workspace_bytes < slm_fmm_workspace_size(nx, ny))
         return slm_fail(SLM_E_WORKSPACE, "workspace %zu bytes, need %zu", workspace_bytes,
@@ -661,11 +886,17 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
     L.off_fixed = L.off_u + al256(n * 8);
     L.off_act0 = L.off_fixed + al256(n);
     L.off_act1 = L.off_act0 + al256(n_tiles);
+    L.off_lcnt = L.off_act1 + al256(n_tiles);
+    L.off_lcur = L.off_lcnt + al256((n + 2) * 4);
+    L.off_pts = L.off_lcur + al256((n + 2) * 4);
+    L.off_pos = L.off_pts + al256(n * 4);
+    L.off_heap = L.off_pos + al256(n * 4);
     hipStream_t s = (hipStream_t)stream;
     zero(sums, (size_t)n_labels * 8, s);
     zero(cnt, (size_t)n_labels * 4, s);
     zero(nch, (size_t)n_labels * 4, s);
     zero(nphi, (size_t)n_labels * 4, s);
+    if (failed) zero(failed, (size_t)n_labels, s);
     // a bound on the launches keeps a bad input from looping
     const uint32_t max_launches = 2u * (nx + ny) + 64u;
     for (uint32_t c0 = 0; c0 < n_colours; c0 += group) {
@@ -674,6 +905,16 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
         hipLaunchKernelGGL(k_fmm_setup, g, dim3(kBlock), 0, s, label, mapping, thin_flag, colour, n_labels, c0, nx,
                            ny, L);
         hipLaunchKernelGGL(k_fmm_init, g, dim3(kBlock), 0, s, nx, ny, dx, L);
+        if (order == 2) {
+            for (uint32_t j = 0; j < k; j++) zero(L.base + j * slot + L.off_lcnt, (size_t)(n_labels + 1) * 4, s);
+            hipLaunchKernelGGL(k_fmm_lcount, g, dim3(kBlock), 0, s, nx, ny, L);
+            hipLaunchKernelGGL(k_fmm_lscan, dim3(1, k), dim3(1024), 0, s, n_labels + 1, L);
+            hipLaunchKernelGGL(k_fmm_lscatter, g, dim3(kBlock), 0, s, nx, ny, L);
+            hipLaunchKernelGGL(k_fmm_march, dim3((n_labels + 63) / 64, k), dim3(64), 0, s, nx, ny, dx, L, colour,
+                               n_labels, c0, failed);
+            hipLaunchKernelGGL(k_fmm_reduce, g, dim3(kBlock), 0, s, L, label, nx, ny, (u64*)sums, cnt, nch, nphi);
+            continue;
+        }
         for (uint32_t j = 0; j < k; j++)
             hipLaunchKernelGGL(k_fill_bytes, grid_for(n_tiles), dim3(kBlock), 0, s, L.base + j * slot + L.off_act0,
                                (size_t)n_tiles, (uint8_t)1);

@@ -82,8 +82,8 @@ _PROTOS = {
     "slm_fmm_pairs": (ctypes.c_int, [_P, _U32, _U32, _P, _U32, _P, _U32, _P, _P, _U32]),
     "slm_fmm_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
     "slm_fmm_workspace_size_k": (ctypes.c_size_t, [_U32, _U32, _U32]),
-    "slm_fmm_hsl": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, ctypes.c_double, _P, _P, _P, _P, _P,
-                                   ctypes.c_size_t, _P, _U32]),
+    "slm_fmm_hsl": (ctypes.c_int, [_P, _P, _U32, _P, _U32, _U32, _U32, _U32, ctypes.c_double, _U32, _P, _P, _P, _P,
+                                   _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_hsl_remap": (ctypes.c_int, [_P, _U32, _P, _P, _U32, _P, _P, _U32]),
     "slm_map_hsl_workspace_size": (ctypes.c_size_t, [_U32, _U32]),
     "slm_map_hsl": (ctypes.c_int, [_P, _P, _U32, _U32, ctypes.c_int, ctypes.c_int, _P, _P, ctypes.c_size_t, _P, _U32]),

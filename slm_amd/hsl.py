@@ -7,10 +7,15 @@ Mirrors python/streamlines/mapping.py:
   remap_hsl(n_hsl_averaging_threshold=0) :953-985
 with the per-subsegment skfmm.distance loop replaced by one colouring of the
 subsegments and one solve per colour on the GPU (slm_fmm_pairs, slm_fmm_hsl,
-slm_hsl_remap in hsl.hip).  The distance is the first-order fast-marching
-solution (scikit-fmm, absent here, defaults to second order: parity with the
-reference unpinned); each subsegment's mean distance is summed in fixed
-point (2^-24), so the result does not depend on summation order.
+slm_hsl_remap in hsl.hip).  The distance is scikit-fmm's second-order fast
+march by default (order=2: scikit-fmm's published update, run in acceptance
+order, one GPU lane per subsegment; a subsegment whose march meets a negative
+discriminant, where scikit-fmm raises, stays unmeasured as the reference's
+except clause leaves it), or the first-order solution (order=1: a parallel
+relaxation to the fast march's fixed point).  scikit-fmm is absent here, so
+parity with it is unpinned beyond the published algorithm.  Each
+subsegment's mean distance is summed in fixed point (2^-24), so the result
+does not depend on summation order.
 
 Typical use, after the segmentation chain has labelled the grid:
     res = fmm_lengths(label, mapping, mask, info.is_thinchannel, pixel_size)
@@ -39,6 +44,7 @@ class FmmResult:
     cnt: np.ndarray               # uint32 per |label|: points a channel reaches
     nch: np.ndarray               # uint32 per |label|: channel pixels
     nphi: np.ndarray              # uint32 per |label|: own pixels
+    failed: np.ndarray | None = None   # uint8 per |label|: 1 where the second-order march failed
 
 
 def select_subsegments(label, mask):
@@ -67,11 +73,12 @@ def colour_labels(n_labels, problems, pairs):
     return colour, n_col
 
 
-def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None):
+def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2):
     """fmm_lengths (mapping.py:841-900) on device tensors: label int32
     [nx, ny] (signed flank labels), mapping int32 [nx, ny] (uint32 bits),
-    mask bool [nx, ny].  max_group caps the colours relaxed together (default:
-    all that fit in 8 GiB).  Returns an FmmResult (host arrays)."""
+    mask bool [nx, ny].  order: 2 (scikit-fmm's default) or 1.  max_group
+    caps the colours solved together (default: all that fit in 8 GiB).
+    Returns an FmmResult (host arrays)."""
     import torch
     hip._require_cuda(label, mapping, mask)
     dev = label.device
@@ -102,30 +109,32 @@ def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, m
     cnt = torch.empty(n_labels, dtype=torch.int32, device=dev)
     nch = torch.empty(n_labels, dtype=torch.int32, device=dev)
     nphi = torch.empty(n_labels, dtype=torch.int32, device=dev)
-    # all colours relaxed together when they fit in 8 GiB (14 B per pixel per colour)
+    failed = torch.empty(n_labels, dtype=torch.uint8, device=dev)
+    # all colours solved together when they fit in 8 GiB
     one = lib.slm_fmm_workspace_size_k(nx, ny, 1) - 512
     k = max(1, min(int(n_col), (8 << 30) // max(one, 1), int(max_group) if max_group else 1 << 30))
     ws_bytes = lib.slm_fmm_workspace_size_k(nx, ny, k)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     hip.check(lib.slm_fmm_hsl(hip._ptr(label), hip._ptr(mapping), int(is_thinchannel), hip._ptr(colour_d), n_labels,
-                              n_col, nx, ny, ctypes.c_double(float(pixel_size)), hip._ptr(sums), hip._ptr(cnt),
-                              hip._ptr(nch), hip._ptr(nphi), hip._ptr(ws), ws_bytes, hip._stream(stream),
-                              hip.SLM_FLAG_SYNC))
+                              n_col, nx, ny, ctypes.c_double(float(pixel_size)), int(order), hip._ptr(sums),
+                              hip._ptr(cnt), hip._ptr(nch), hip._ptr(nphi), hip._ptr(failed), hip._ptr(ws), ws_bytes,
+                              hip._stream(stream), hip.SLM_FLAG_SYNC))
     s_np, c_np = sums.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
+    f_np = failed.cpu().numpy() if order == 2 else np.zeros(n_labels, np.uint8)
     ch_np, ph_np = nch.cpu().numpy().view(np.uint32), nphi.cpu().numpy().view(np.uint32)
     # per selected label, in the reference's order of operations
     # (2 * mean distance; kept when n_phi / n_channel > 4; int32 truncation)
     a = np.abs(sel_np.astype(np.int64))
     q, c = s_np[a].astype(np.float64), c_np[a].astype(np.int64)
     ch, ph = ch_np[a].astype(np.int64), ph_np[a].astype(np.int64)
-    ok = (ch > 0) & (c > 0)
+    ok = (ch > 0) & (c > 0) & (f_np[a] == 0)
     with np.errstate(divide="ignore", invalid="ignore"):
         h = 2.0 * (q / FIX) / c
         keep = ok & (ph / np.where(ch > 0, ch, 1) > 4.0)
     hsl = np.where(keep, h, 0.0).astype(np.int32)
     pos = hsl[hsl > 0]
     mean = float(np.mean(pos)) if pos.size else None
-    return FmmResult(sel_np, hsl, mean, n_col, s_np, c_np, ch_np, ph_np)
+    return FmmResult(sel_np, hsl, mean, n_col, s_np, c_np, ch_np, ph_np, f_np)
 
 
 def remap_hsl(label, res: FmmResult, stream=None):

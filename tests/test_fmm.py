@@ -1,11 +1,14 @@
 """Pins of the eikonal (FMM) oracle behind the FMM hillslope lengths
 (oracle/slm_oracle_fmm.c, oracle/fmm_ref.py), on the CPU.
 
-scikit-fmm is absent, so the oracle is a first-order restatement of the
-fast marching method; these tests pin it by closed forms and invariants:
-the update formula's cases, a straight channel (exact distances k - 1/2),
-a single channel pixel (the hand-computed diagonal value), masked barriers,
-1-Lipschitz continuity and the symmetry of the grid."""
+scikit-fmm is absent, so the oracle restates the published fast marching
+method, first order and scikit-fmm's second-order update; these tests pin
+it by closed forms and invariants: the update formula's cases, a straight
+channel (exact distances k - 1/2, both orders), a diagonal channel (a plane
+front at 45 degrees: exact in both orders), a single channel pixel (the
+hand-computed values, including the second-order update across the zero
+contour), masked barriers, 1-Lipschitz continuity and the symmetry of the
+grid."""
 import numpy as np
 import pytest
 
@@ -164,3 +167,76 @@ def test_hsl_aspect_stats_disparity():
     assert skew["hsl_ns_disparity_confidence"] > even["hsl_ns_disparity_confidence"]
     assert 60.0 < skew["hsl_mean_azimuth"] < 120.0                  # the mean vector points north
     assert skew["hsl_ns_ks"][1] < 1e-6 and skew["hsl_ns_pp_diff"] > 50
+
+
+@pytest.mark.parametrize("dx", [1.0, 2.0])
+def test_second_order_straight_channel_exact(dx):
+    n, m, c = 21, 17, 9
+    phi = np.ones((n, m), np.int8)
+    phi[c, :] = -1
+    d = fmm_ref.fmm_distance2(phi, np.ones((n, m), bool), dx, order=2)
+    x = np.arange(n)[:, None].repeat(m, 1)
+    want = (np.abs(x - c) - 0.5) * dx
+    want[c, :] = -0.5 * dx
+    assert np.abs(d - want).max() < 1e-12 * dx
+
+
+def test_diagonal_front_exact_both_orders():
+    """Channel pixels on the diagonal: the zero contour is the line y = x + 1/2
+    (and its mirror), so the distance of (x, y) is (|y - x| - 1/2) / sqrt(2)."""
+    n = 24
+    phi = np.ones((n, n), np.int8)
+    phi[np.arange(n), np.arange(n)] = -1
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    k = np.abs(Y - X)
+    want = (k - 0.5) / np.sqrt(2.0)
+    inner = (k >= 1) & (X >= 3) & (X < n - 3) & (Y >= 3) & (Y < n - 3)
+    for order in (1, 2):
+        d = fmm_ref.fmm_distance2(phi, np.ones((n, n), bool), 1.0, order=order)
+        assert np.abs(d[inner] - want[inner]).max() < 1e-12, order
+
+
+def test_second_order_single_pixel_by_hand():
+    """One channel pixel at (4, 4): the interface points carry 1/2 and the
+    pixel -1/sqrt(8).  (2, 4) is updated when (3, 4) is accepted: along x, v1
+    = 1/2 and the point beyond, (4, 4), lies across the contour, so v2 =
+    -1/sqrt(8) and t = (4 v1 - v2)/3; along y nothing is frozen: u = t + 2/3.
+    (3, 3) takes first-order terms on both axes (the points beyond are not
+    frozen yet): (1 + sqrt(2))/2."""
+    phi = np.ones((9, 9), np.int8)
+    phi[4, 4] = -1
+    d = fmm_ref.fmm_distance2(phi, np.ones((9, 9), bool), 1.0, order=2)
+    assert d[4, 4] == -1.0 / np.sqrt(8.0)
+    assert d[3, 4] == 0.5
+    t = (1.0 / 3.0) * (4.0 * 0.5 + 1.0 / np.sqrt(8.0))
+    assert abs(d[2, 4] - (t + 2.0 / 3.0)) < 1e-15
+    assert abs(d[3, 3] - (1.0 + np.sqrt(2.0)) * 0.5) < 1e-15
+    assert np.array_equal(d, d.T) and np.array_equal(d, d[::-1, :]) and np.array_equal(d, d[:, ::-1])
+
+
+def _scattered(seed, n=48):
+    rs = np.random.default_rng(seed)
+    phi = np.ones((n, n), np.int8)
+    phi[rs.integers(4, n - 4, 10), rs.integers(4, n - 4, 10)] = -1
+    return phi, np.ones((n, n), bool)
+
+
+def test_second_order_close_to_first_and_lipschitz():
+    phi, dom = _scattered(2)
+    d1 = fmm_ref.fmm_distance2(phi, dom, 1.0, order=1)
+    d2 = fmm_ref.fmm_distance2(phi, dom, 1.0, order=2)
+    assert np.array_equal(np.sign(d1), np.sign(d2))
+    assert np.abs(d2 - d1).max() < 1.0
+    for ax in (0, 1):
+        assert np.abs(np.diff(np.abs(d2), axis=ax)).max() <= 1.0 + 1e-9
+
+
+def test_second_order_negative_discriminant_raises():
+    """Scattered single channel pixels: where two fronts meet, the
+    second-order terms of the two axes can leave the quadratic without a
+    real root; scikit-fmm raises there (the reference then leaves the
+    subsegment unmeasured), and so does the oracle."""
+    phi, dom = _scattered(0)
+    with pytest.raises(fmm_ref.NegativeDiscriminant):
+        fmm_ref.fmm_distance2(phi, dom, 1.0, order=2)
+    fmm_ref.fmm_distance2(phi, dom, 1.0, order=1)      # first order always has a root

@@ -1,11 +1,13 @@
 """GPU parity of the FMM hillslope lengths (slm_amd.hsl: slm_fmm_pairs,
 slm_fmm_hsl, slm_hsl_remap) against the CPU oracle (oracle/fmm_ref.py).
 
-Per subsegment the counts must match exactly, the fixed-point distance sums
-to within one unit per point (the GPU relaxation and the oracle's fast march
-reach the same discrete solution, but a value can come from a different
-update by an ulp), and the int32 HSL exactly unless twice the mean distance
-lies within 1e-6 of an integer."""
+Second order (the default, scikit-fmm's): the GPU marches every subsegment
+in the oracle's acceptance order, so counts, fixed-point sums, failure flags
+and HSL are the oracle's exactly.  First order: counts exactly, the
+fixed-point distance sums to within one unit per point (the GPU relaxation
+and the oracle's fast march reach the same discrete solution, but a value can
+come from a different update by an ulp), and the int32 HSL exactly unless
+twice the mean distance lies within 1e-6 of an integer."""
 import numpy as np
 import pytest
 
@@ -32,19 +34,28 @@ def _voronoi_labels(n, m, k, seed):
     return label, mapping, mask
 
 
-def _compare(gpu, label, mapping, mask, dx):
+def _compare(gpu, label, mapping, mask, dx, order=2):
     import torch
     from slm_amd import hsl
     lt = torch.from_numpy(label).to(gpu)
     mt = torch.from_numpy(mapping.view(np.int32)).to(gpu)
     kt = torch.from_numpy(mask).to(gpu)
-    res = hsl.fmm_lengths(lt, mt, kt, 2, dx)
-    sel, h_o, stats = fmm_ref.fmm_lengths(label, mapping, mask, 2, dx)
-    print("subsegments", sel.size, "colours", res.n_colours, flush=True)
+    res = hsl.fmm_lengths(lt, mt, kt, 2, dx, order=order)
+    sel, h_o, stats = fmm_ref.fmm_lengths(label, mapping, mask, 2, dx, order=order)
+    print("order", order, "subsegments", sel.size, "colours", res.n_colours, flush=True)
     assert np.array_equal(res.selected, sel)
     for s, hg, ho in zip(sel, res.hsl, h_o):
         a = abs(int(s))
         q, cnt, n_phi, n_ch = stats[a]
+        if order == 2:
+            assert bool(res.failed[a]) == (q is None), a
+            if q is None:
+                assert hg == 0 and ho == 0
+                continue
+            assert (int(res.cnt[a]), int(res.nphi[a]), int(res.nch[a])) == (cnt, n_phi, n_ch), a
+            assert int(res.sums[a]) == q, a
+            assert hg == ho, a
+            continue
         assert (int(res.cnt[a]), int(res.nphi[a]), int(res.nch[a])) == (cnt, n_phi, n_ch), a
         assert abs(int(res.sums[a]) - q) <= cnt, a
         if n_ch and n_phi / n_ch > 4.0:
@@ -60,9 +71,10 @@ def _compare(gpu, label, mapping, mask, dx):
 
 
 @pytest.mark.parametrize("shape,k,seed,dx", [((64, 50), 9, 0, 1.0), ((130, 97), 40, 1, 2.0), ((200, 200), 120, 2, 1.0)])
-def test_fmm_voronoi_subsegments(gpu, shape, k, seed, dx):
+@pytest.mark.parametrize("order", [2, 1])
+def test_fmm_voronoi_subsegments(gpu, shape, k, seed, dx, order):
     label, mapping, mask = _voronoi_labels(shape[0], shape[1], k, seed)
-    res = _compare(gpu, label, mapping, mask, dx)
+    res = _compare(gpu, label, mapping, mask, dx, order)
     assert (res.hsl > 0).sum() > 0
 
 
@@ -76,15 +88,19 @@ def test_fmm_colour_groups(gpu):
     lt = torch.from_numpy(label).to(gpu)
     mt = torch.from_numpy(mapping.view(np.int32)).to(gpu)
     kt = torch.from_numpy(mask).to(gpu)
-    one = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=1)
-    two = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=2)
-    allc = hsl.fmm_lengths(lt, mt, kt, 2, 2.0)
+    one = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=1, order=1)
+    two = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=2, order=1)
+    allc = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, order=1)
     print("colours", allc.n_colours, flush=True)
     assert allc.n_colours >= 3
     for r in (two, allc):
         assert np.array_equal(r.cnt, one.cnt) and np.array_equal(r.nch, one.nch) and np.array_equal(r.nphi, one.nphi)
         assert np.all(np.abs(r.sums.astype(np.int64) - one.sums.astype(np.int64)) <= one.cnt.astype(np.int64))
         assert np.array_equal(r.selected, one.selected)
+    # second order: the march is exact, so any grouping gives the same bits
+    one2 = hsl.fmm_lengths(lt, mt, kt, 2, 2.0, max_group=1)
+    all2 = hsl.fmm_lengths(lt, mt, kt, 2, 2.0)
+    assert np.array_equal(one2.sums, all2.sums) and np.array_equal(one2.failed, all2.failed)
 
 
 def test_fmm_traced_segmentation(gpu):
@@ -96,5 +112,25 @@ def test_fmm_traced_segmentation(gpu):
     g = res["grids"]
     label = g["label"].cpu().numpy().reshape(job.mask.shape).astype(np.int32)
     mapping = g["mapping"].cpu().numpy().reshape(job.mask.shape).view(np.uint32)
-    r = _compare(gpu, label, mapping, job.mask, job.pixel_size)
-    assert r.selected.size > 10 and (r.hsl > 0).sum() > 0
+    for order in (2, 1):
+        r = _compare(gpu, label, mapping, job.mask, job.pixel_size, order)
+        assert r.selected.size > 10 and (r.hsl > 0).sum() > 0
+
+
+def test_fmm_second_order_failures_match(gpu):
+    """Subsegments whose channel is a few scattered pixels: some second-order
+    marches meet a negative discriminant (scikit-fmm raises; the reference
+    leaves them unmeasured).  The GPU flags exactly the oracle's, and the
+    others match it bit for bit."""
+    import torch
+    from slm_amd import hsl
+    label, mapping, mask = _voronoi_labels(120, 110, 30, 7)
+    rs = np.random.default_rng(7)
+    mapping[:] = 0
+    pts = rs.integers(3, 117, size=(150, 2))
+    mapping[pts[:, 0], np.clip(pts[:, 1], 3, 106)] = 2
+    res = _compare(gpu, label, mapping, mask, 1.0, order=2)
+    sel, _, stats = fmm_ref.fmm_lengths(label, mapping, mask, 2, 1.0, order=2)
+    n_failed = sum(1 for v in stats.values() if v[0] is None)
+    print("failed second-order marches", n_failed, "of", len(stats), flush=True)
+    assert n_failed > 0 and int(res.failed.sum()) == n_failed
