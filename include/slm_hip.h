@@ -443,13 +443,20 @@ size_t slm_map_workspace_size(const SlmParams* p);
 int slm_connect_channels(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
                          const float* uv, uint32_t* mapping, void* stream, uint32_t flags);
 
-/* Stand-in for Mapping.thin_channels (mapping.py:707-719), which calls
- * scikit-image skeletonize(medial_axis(.)): Zhang-Suen thinning of the
- * IS_CHANNEL | IS_INTERCHANNEL pixels, result ORed as IS_THINCHANNEL.
- * Synchronises the stream once per iteration; *n_iter (may be NULL) receives
- * the iteration count. */
-int slm_thin_channels(const SlmParams* p, uint32_t* mapping, void* workspace,
-                      size_t workspace_bytes, uint32_t* n_iter, void* stream, uint32_t flags);
+/* Mapping.thin_channels (mapping.py:707-719): skeletonize(medial_axis(B))
+ * of B = the IS_CHANNEL | IS_INTERCHANNEL pixels, ORed into mapping as
+ * IS_THINCHANNEL.  medial_axis is scikit-image's published algorithm (keep
+ * table over 3x3 windows, pixels visited in order of squared distance to the
+ * unset pixels, corner score, then tie-breaker): tiebreak holds the
+ * tie-breaker of the n_set pixels of B in row-major order (scikit-image
+ * draws numpy's RandomState(0).permutation(n_set)); n_set must equal the
+ * number of pixels of B.  With tiebreak NULL the medial axis is skipped.
+ * skeletonize is the Zhang-Suen thinning (scikit-image is absent, so its
+ * thinning is restated from the paper).  Synchronises the stream once per
+ * eight medial-axis rounds and once per eight thinning iterations; *n_iter
+ * (may be NULL) receives the thinning's iteration count. */
+int slm_thin_channels(const SlmParams* p, uint32_t* mapping, const uint32_t* tiebreak, uint32_t n_set,
+                      void* workspace, size_t workspace_bytes, uint32_t* n_iter, void* stream, uint32_t flags);
 
 /* map_channel_heads (opencl/channelheads.cl:32-93), launched by
  * channelheads.map_channel_heads (channelheads.py:33-79): seeds = all unmasked

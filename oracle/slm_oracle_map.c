@@ -453,6 +453,23 @@ void slmo_label_confluences(const SlmParams* p, const float* seeds, const uint8_
 
 /* ------------------------------------------------------- segment_downchannels */
 
+/* Cycle skip of the full ownership walks (with the fast-forward on): Brent's
+ * detection on the pixel fires once the walk has gone round the cycle from a
+ * point on it; every offer after that repeats one already made.  (The owned
+ * walks cannot circle: the pixel where a walk entered a cycle is owned by one
+ * of its two inflows only.) */
+typedef struct { uint32_t sidx, shop, power; } LinkCycle;
+static LinkCycle link_cycle(uint32_t idx) { LinkCycle c = {idx, 0u, 1u}; return c; }
+static int link_cycle_step(LinkCycle* c, uint32_t idx, uint32_t hop) {
+    if (idx == c->sidx) return 1;
+    if (hop - c->shop == c->power) {
+        c->sidx = idx;
+        c->shop = hop;
+        c->power <<= 1;
+    }
+    return 0;
+}
+
 /* Ownership key of a hop prev -> link[prev]: larger count, then a major
  * inflow, then larger pixel index. */
 static inline uint64_t own_key(const uint32_t* mapping, const uint32_t* count, uint32_t prev) {
@@ -495,8 +512,10 @@ void slmo_segment_downchannels(const SlmParams* p, const float* seeds, const uin
     }
     for (uint32_t k = p->seeds_chunk_offset; k < p->n_seed_points; k++) {
         uint32_t prev = seed_pixel(p, seeds, k), idx = lk(p, link, prev);
+        LinkCycle cyc = link_cycle(prev);
         for (size_t hop = 0; hop < nxy && !mask[idx] && prev != idx && !(mapping[idx] & F_SSH); hop++) {
             kmax(&own[idx], own_key(mapping, count, prev));
+            if (g_fast_forward && link_cycle_step(&cyc, idx, (uint32_t)hop + 1u)) break;
             prev = idx;
             idx = lk(p, link, idx);
         }

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "common.hpp"
 #include "trace_common.hpp"
@@ -471,6 +472,28 @@ __global__ void k_seg_heads(SlmParams P, const float2* __restrict__ seeds, uint3
     mark[h] = 1;
 }
 
+// A full walk that returns to a pixel is on a link cycle and would circle to
+// its hop cap (a cycle of the medial-axis skeleton's links, for one).
+// Brent's detection on the pixel fires once the walk has gone round the whole
+// cycle from a point on it; every ownership offer after that repeats one
+// already made, so the walk stops there: the same bits as walking to the cap.
+// (The owned walks of the later phases cannot circle: the pixel where a walk
+// entered a cycle is owned by one of its two inflows only.)
+struct LinkCycle {
+    uint32_t sidx, shop, power;
+    __device__ LinkCycle(uint32_t idx) : sidx(idx), shop(0), power(1) {}
+    // after the hop that reached idx (hop count, from 1): true on detection
+    __device__ bool step(uint32_t idx, uint32_t hop) {
+        if (idx == sidx) return true;
+        if (hop - shop == power) {
+            sidx = idx;
+            shop = hop;
+            power <<= 1;
+        }
+        return false;
+    }
+};
+
 // Full walks: own[q] = max own_key(prev) over hops prev -> q.
 __global__ void k_seg_own(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
                           const uint32_t* __restrict__ mapping, const uint32_t* __restrict__ count,
@@ -478,8 +501,10 @@ __global__ void k_seg_own(SlmParams P, const float2* __restrict__ seeds, const u
     const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.n_seed_points) return;
     uint32_t prev = seed_pixel(P, seeds, k), idx = lk(P, link, prev);
+    LinkCycle cyc(prev);
     for (uint32_t hop = 0; hop < P.nxy_padded && !mask[idx] && prev != idx && !(mapping[idx] & F_SSH); hop++) {
         atomicMax(&own[idx], own_key(mapping, count, prev));
+        if (cyc.step(idx, hop + 1u)) break;
         prev = idx;
         idx = lk(P, link, idx);
     }
@@ -938,7 +963,203 @@ int check_map(const SlmParams* p, const void* seeds) {
     return SLM_OK;
 }
 
-size_t map_ws_bytes(const SlmParams* p) { return (size_t)16 * p->nxy_padded + 256; }
+// ------------------------------------------------------------------ medial axis
+//
+// scikit-image's medial_axis (the first step of the reference's channel
+// thinning, skeletonize(medial_axis(.)), mapping.py:717), as published: every
+// foreground pixel is visited once, in lexicographic order of (squared
+// distance to the background, corner score = 9 - set pixels of its 3x3
+// window, a tie-breaker permutation from the host), and takes the keep-table
+// value of its current 3x3 window (pixels outside the grid unset, pixels not
+// yet visited as they started).  A visit reads only the pixels visited before
+// it, so a pixel can be decided as soon as every neighbour that precedes it
+// has been: rounds decide all ready pixels at once, in any order, and give
+// the sequential visit's result.  Each pixel's order key packs the three
+// fields into one 64-bit integer.
+constexpr uint32_t kMaInf = 0x3fffffffu;
+
+// The keep table as 512 bits, passed by value.
+struct MaTable {
+    uint32_t w[16];
+};
+
+// Distance along x to the nearest unset pixel of the same column (y), both
+// ways; kMaInf when the column has none.  One thread per column.
+__global__ void k_ma_cols(const uint8_t* __restrict__ a, uint32_t nx, uint32_t ny, uint32_t* __restrict__ f) {
+    const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+    if (y >= ny) return;
+    uint32_t last = kMaInf;
+    for (uint32_t x = 0; x < nx; x++) {
+        const size_t i = (size_t)x * ny + y;
+        if (!a[i]) last = x;
+        f[i] = (last == kMaInf) ? kMaInf : x - last;
+    }
+    last = kMaInf;
+    for (uint32_t x = nx; x-- > 0;) {
+        const size_t i = (size_t)x * ny + y;
+        if (!a[i]) last = x;
+        if (last != kMaInf && last - x < f[i]) f[i] = last - x;
+    }
+}
+
+// Squared Euclidean distance of each set pixel to the nearest unset one:
+// min over columns y' of (y - y')^2 + f[x][y']^2, searched outward from y
+// until (y - y')^2 alone is no better.
+__global__ void k_ma_d2(const uint8_t* __restrict__ a, const uint32_t* __restrict__ f, uint32_t nx, uint32_t ny,
+                        u64* __restrict__ key) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx * ny) return;
+    if (!a[i]) {
+        key[i] = 0;
+        return;
+    }
+    const uint32_t x = i / ny, y = i % ny;
+    const size_t row = (size_t)x * ny;
+    u64 best = (f[i] == kMaInf) ? ~0ull : (u64)f[i] * f[i];
+    for (uint32_t dy = 1; dy < ny; dy++) {
+        const u64 dd = (u64)dy * dy;
+        if (dd >= best) break;
+        if (y >= dy) {
+            const uint32_t g = f[row + y - dy];
+            if (g != kMaInf && dd + (u64)g * g < best) best = dd + (u64)g * g;
+        }
+        if (y + dy < ny) {
+            const uint32_t g = f[row + y + dy];
+            if (g != kMaInf && dd + (u64)g * g < best) best = dd + (u64)g * g;
+        }
+    }
+    if (best > 0x0fffffffull) best = 0x0fffffffull;    // no unset pixel anywhere: last
+    key[i] = best << 36;
+}
+
+// Set pixels per 256-pixel block, then their exclusive prefix (one block).
+__global__ void k_ma_bcount(const uint8_t* __restrict__ a, uint32_t n, uint32_t* __restrict__ bsum) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool set = i < n && a[i];
+    const uint32_t c = __syncthreads_count(set);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(1024) void k_ma_bscan(uint32_t* __restrict__ bsum, uint32_t nb, uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nb + 1023u) / 1024u;
+    const uint32_t b = min(nb, t * per), e = min(nb, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; i++) s += bsum[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = (t == 0) ? 0u : part[t - 1];
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t c = bsum[i];
+        bsum[i] = run;
+        run += c;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
+// Row-major rank of each set pixel: the order key's tie-breaker is
+// tie[rank] (tie: the host's permutation of [0, n_set)); corner score; the
+// list of set pixels in rank order; state 0 (to visit) or 3 (unset).
+__global__ void k_ma_keys(const uint8_t* __restrict__ a, uint32_t nx, uint32_t ny,
+                          const uint32_t* __restrict__ bsum, const uint32_t* __restrict__ tie, uint32_t n_set,
+                          u64* __restrict__ key, uint32_t* __restrict__ list, uint8_t* __restrict__ state) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t n = nx * ny;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool set = i < n && a[i];
+    const u64 m = __ballot(set);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = bsum[blockIdx.x];
+    for (uint32_t k = 0; k < w; k++) off += wsum[k];
+    if (i >= n) return;
+    if (!set) {
+        state[i] = 3;
+        return;
+    }
+    const uint32_t rank = off + below;
+    if (rank >= n_set) {      // the host's count disagrees: leave the pixel unvisited (the call fails)
+        state[i] = 3;
+        return;
+    }
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    uint32_t cnt = 0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const int xx = x + dx, yy = y + dy;
+            if (xx >= 0 && yy >= 0 && xx < (int)nx && yy < (int)ny && a[(size_t)xx * ny + yy]) cnt++;
+        }
+    key[i] |= ((u64)(9u - cnt) << 32) | tie[rank];
+    list[rank] = i;
+    state[i] = 0;
+}
+
+// One round: every set pixel still to visit whose preceding neighbours are
+// all decided takes table[window]; state 1 kept, 2 removed.  pending counts
+// the pixels left undecided.
+__global__ void k_ma_round(uint32_t nx, uint32_t ny, const uint32_t* __restrict__ list, uint32_t n_set,
+                           const u64* __restrict__ key, uint8_t* __restrict__ state, MaTable table,
+                           uint32_t* __restrict__ pending) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_set) return;
+    const uint32_t i = list[r];
+    if (state[i] != 0) return;
+    const u64 k = key[i];
+    const int x = (int)(i / ny), y = (int)(i % ny);
+    uint32_t acc = 16u;
+    int bit = 0;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++, bit++) {
+            if (!dx && !dy) continue;
+            const int xx = x + dx, yy = y + dy;
+            if (xx < 0 || yy < 0 || xx >= (int)nx || yy >= (int)ny) continue;
+            const size_t q = (size_t)xx * ny + yy;
+            const uint8_t st = __hip_atomic_load(&state[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (st == 0 && key[q] < k) {       // a preceding neighbour not visited yet
+                atomicAdd(pending, 1u);
+                return;
+            }
+            if (st == 0 || st == 1) acc |= 1u << bit;
+        }
+    const uint8_t v = (table.w[acc >> 5] >> (acc & 31u)) & 1u ? 1 : 2;
+    __hip_atomic_store(&state[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The medial axis back into the byte image: set where kept.
+__global__ void k_ma_final(uint32_t n, const uint8_t* __restrict__ state, uint8_t* __restrict__ a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = state[i] == 1 ? 1 : 0;
+}
+
+// Workspace: two 8-byte-per-pixel areas (the byte images of the thinning,
+// the keys of the count and segment phases), the counters at 16 B per pixel,
+// then the medial axis' order keys, column distances, states, visit list and
+// block sums.
+inline size_t ma_al(size_t v) { return (v + 255) & ~(size_t)255; }
+struct MaWs {
+    size_t key, f, state, list, bsum, total, end;
+};
+MaWs ma_ws(size_t n) {
+    MaWs w;
+    w.key = ma_al(16 * n + 256);
+    w.f = w.key + ma_al(n * 8);
+    w.state = w.f + ma_al(n * 4);
+    w.list = w.state + ma_al(n);
+    w.bsum = w.list + ma_al(n * 4);
+    w.total = w.bsum + ma_al((n / 256 + 2) * 4);
+    w.end = w.total + 256;
+    return w;
+}
+size_t map_ws_bytes(const SlmParams* p) { return ma_ws(p->nxy_padded).end; }
 
 int check_ws(const SlmParams* p, void* ws, size_t bytes) {
     if (!ws) return slm_fail(SLM_E_WORKSPACE, "workspace is NULL");
@@ -1021,18 +1242,105 @@ extern "C" int slm_connect_channels(const SlmParams* p, const float* seeds_xy, c
     return slm_finish(stream, flags);
 }
 
-extern "C" int slm_thin_channels(const SlmParams* p, uint32_t* mapping, void* workspace, size_t workspace_bytes,
-                                 uint32_t* n_iter, void* stream, uint32_t flags) {
+namespace {
+// scikit-image's medial_axis keep table: bit (index) set when the centre of
+// the 3x3 pattern `index` (bit k = row k / 3, column k % 3; centre bit 4) is
+// set and either removing it changes the number of 8-connected components of
+// the pattern, or fewer than 3 of its pixels are set.
+int pattern_components(uint32_t idx) {
+    int lab[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, n = 0;
+    for (int k = 0; k < 9; k++) {
+        if (!(idx >> k & 1u) || lab[k]) continue;
+        n++;
+        int stack[9], top = 0;
+        stack[top++] = k;
+        lab[k] = n;
+        while (top) {
+            const int c = stack[--top], r = c / 3, q = c % 3;
+            for (int j = 0; j < 9; j++)
+                if ((idx >> j & 1u) && !lab[j] && std::abs(j / 3 - r) <= 1 && std::abs(j % 3 - q) <= 1) {
+                    lab[j] = n;
+                    stack[top++] = j;
+                }
+        }
+    }
+    return n;
+}
+
+MaTable ma_keep_table() {
+    MaTable t;
+    for (int w = 0; w < 16; w++) t.w[w] = 0;
+    for (uint32_t idx = 0; idx < 512; idx++) {
+        if (!(idx & 16u)) continue;
+        const bool keep = pattern_components(idx) != pattern_components(idx & ~16u) || __builtin_popcount(idx) < 3;
+        if (keep) t.w[idx >> 5] |= 1u << (idx & 31u);
+    }
+    return t;
+}
+
+// The medial axis of the byte image a, in place (see k_ma_round).  tie: the
+// host's tie-breaker permutation of the n_set set pixels.
+int medial_axis(uint8_t* a, uint32_t nx, uint32_t ny, const uint32_t* tie, uint32_t n_set, uint8_t* ws,
+                uint32_t* pending, hipStream_t s) {
+    const size_t n = (size_t)nx * ny;
+    const MaWs w = ma_ws(n);
+    u64* key = (u64*)(ws + w.key);
+    uint32_t* f = (uint32_t*)(ws + w.f);
+    uint8_t* state = ws + w.state;
+    uint32_t* list = (uint32_t*)(ws + w.list);
+    uint32_t* bsum = (uint32_t*)(ws + w.bsum);
+    uint32_t* total = (uint32_t*)(ws + w.total);
+    const uint32_t nb = (uint32_t)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_ma_cols, dim3((ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const uint8_t*)a, nx, ny, f);
+    hipLaunchKernelGGL(k_ma_d2, dim3(nb), dim3(kBlock), 0, s, (const uint8_t*)a, (const uint32_t*)f, nx, ny, key);
+    hipLaunchKernelGGL(k_ma_bcount, dim3(nb), dim3(kBlock), 0, s, (const uint8_t*)a, (uint32_t)n, bsum);
+    hipLaunchKernelGGL(k_ma_bscan, dim3(1), dim3(1024), 0, s, bsum, nb, total);
+    uint32_t count = 0;
+    HIP_TRY(hipMemcpyAsync(&count, total, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (count != n_set)
+        return slm_fail(SLM_E_INVALID_ARG, "the channel image has %u set pixels, the tie-breaker %u", count, n_set);
+    if (!count) return SLM_OK;
+    hipLaunchKernelGGL(k_ma_keys, dim3(nb), dim3(kBlock), 0, s, (const uint8_t*)a, nx, ny, (const uint32_t*)bsum,
+                       tie, n_set, key, list, state);
+    static const MaTable table = ma_keep_table();
+    // rounds, 8 per host check; each round decides at least the pixel first
+    // in the order, so n_set rounds always suffice
+    const uint32_t rb = (n_set + kBlock - 1) / kBlock;
+    for (uint32_t done = 0;; done += 8) {
+        for (int r = 0; r < 8; r++) {
+            zero_u64((u64*)pending, 1, s);
+            hipLaunchKernelGGL(k_ma_round, dim3(rb), dim3(kBlock), 0, s, nx, ny, (const uint32_t*)list, n_set,
+                               (const u64*)key, state, table, pending);
+        }
+        uint32_t left = 0;
+        HIP_TRY(hipMemcpyAsync(&left, pending, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!left) break;
+        if (done > n_set) return slm_fail(SLM_E_HIP, "medial axis rounds did not finish");
+    }
+    hipLaunchKernelGGL(k_ma_final, dim3(nb), dim3(kBlock), 0, s, (uint32_t)n, (const uint8_t*)state, a);
+    return SLM_OK;
+}
+}  // namespace
+
+extern "C" int slm_thin_channels(const SlmParams* p, uint32_t* mapping, const uint32_t* tiebreak, uint32_t n_set,
+                                 void* workspace, size_t workspace_bytes, uint32_t* n_iter, void* stream,
+                                 uint32_t flags) {
     int rc = check_grid(p);
     if (rc) return rc;
     if ((rc = check_ws(p, workspace, workspace_bytes))) return rc;
-    if (!mapping) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (!mapping || (n_set && !tiebreak)) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
     const uint32_t nxy = p->nxy_padded;
     uint8_t* a = (uint8_t*)ws_a(workspace);
     uint8_t* b = a + nxy;
     uint32_t* ctr = ws_ctr(workspace, p);
     hipLaunchKernelGGL(k_zs_init, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint32_t*)mapping, a);
+    if (tiebreak) {
+        if ((rc = medial_axis(a, p->nx_padded, p->ny_padded, tiebreak, n_set, (uint8_t*)workspace, ctr, s)))
+            return rc;
+    }
     uint32_t iters = 0;
     if ((rc = zs_thin(a, b, p->nx_padded, p->ny_padded, ctr, s, &iters))) return rc;
     hipLaunchKernelGGL(k_zs_final, pixel_grid(p), dim3(kBlock), 0, s, nxy, (const uint8_t*)a, mapping);

@@ -96,7 +96,7 @@ _PROTOS = {
     "slm_fields_finish_raw": (ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P, _U32]),
     "slm_fields_finish_scaled": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_float, ctypes.c_float, _P, _P, _P,
                                                 _U32]),
-    "slm_thin_channels": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, ctypes.c_size_t,
+    "slm_thin_channels": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _U32, _P, ctypes.c_size_t,
                                          ctypes.POINTER(_U32), _P, _U32]),
 }
 
@@ -381,16 +381,30 @@ def map_call(name: str, p: SlmParams, workspace=None, stream=None, sync=False, *
     return workspace
 
 
-def thin_channels(p: SlmParams, mapping, workspace=None, stream=None) -> int:
-    """Zhang-Suen stand-in for skeletonize(medial_axis(.)); returns iterations."""
+CHANNEL_BITS = 0x1 | 0x4   # IS_CHANNEL | IS_INTERCHANNEL, the pixels thin_channels thins
+
+
+def medial_axis_tiebreak(n_set: int):
+    """scikit-image medial_axis's tie-breaker for n_set pixels: numpy's legacy
+    RandomState(0).permutation (uint32)."""
+    return np.random.RandomState(0).permutation(np.arange(n_set)).astype(np.uint32)
+
+
+def thin_channels(p: SlmParams, mapping, workspace=None, stream=None, medial_axis=True) -> int:
+    """skeletonize(medial_axis(.)) of the channel pixels (medial_axis=False:
+    the thinning alone); returns the thinning's iterations."""
     import torch
     _require_cuda(mapping, workspace)
     need = map_workspace_size(p)
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=mapping.device)
+    tie, n_set = None, 0
+    if medial_axis:
+        n_set = int(((mapping & CHANNEL_BITS) != 0).sum().item())
+        tie = torch.from_numpy(medial_axis_tiebreak(n_set).view(np.int32)).to(mapping.device)
     it = _U32(0)
-    check(load().slm_thin_channels(ctypes.byref(p), _ptr(mapping), _ptr(workspace), workspace.numel(),
-                                   ctypes.byref(it), _stream(stream), 0))
+    check(load().slm_thin_channels(ctypes.byref(p), _ptr(mapping), _ptr(tie), n_set, _ptr(workspace),
+                                   workspace.numel(), ctypes.byref(it), _stream(stream), 0))
     return int(it.value)
 
 

@@ -6,7 +6,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from oracle import oracle
+from oracle import medial_axis_ref, oracle
 from slm_amd.params import kernel_params
 from slm_amd.useful import pick_seeds
 
@@ -48,9 +48,19 @@ class OracleMapper:
         self.run("connect_channels", s, uv=self.uv, mapping=self.mapping)
         return True
 
-    def thin_channels(self):
-        self.thin_iters = oracle.map_call("thin_channels", self.params("thin_channels", 0),
-                                          mapping=self.mapping.reshape(-1))
+    def thin_channels(self, medial_axis=True):
+        """skeletonize(medial_axis(channel pixels)) (mapping.py:717): the
+        medial-axis restatement, then the oracle's thinning of its result."""
+        info = self.info
+        bits = np.uint32(int(info.is_channel) | int(info.is_interchannel))
+        src = self.mapping
+        if medial_axis:
+            ma = medial_axis_ref.medial_axis((self.mapping & bits) != 0)
+            src = np.where(ma, np.uint32(info.is_channel), np.uint32(0)).astype(np.uint32)
+        tmp = np.ascontiguousarray(src, dtype=np.uint32).copy()
+        self.thin_iters = oracle.map_call("thin_channels", self.params("thin_channels", 0), mapping=tmp.reshape(-1))
+        thin = np.uint32(info.is_thinchannel)
+        self.mapping[(tmp & thin) == thin] |= thin
         return self.thin_iters
 
     def map_channel_heads(self):

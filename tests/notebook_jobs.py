@@ -51,3 +51,43 @@ def stats_mismatches(name, df, skip=()):
                 if got != want:
                     bad.append((direction, col, stat, got, want))
     return bad
+
+
+def pass1_counts(name, device):
+    """Mapping pass 1 of the named job (mapping.py:240-295) on the GPU after
+    Preprocess and Trace: mid-slope and ridge maps, then the coarse channel
+    mapping and segmentation at the JSON's coarse thresholds.  Returns (number
+    of segments, number of subsegments) as segment.py prints them."""
+    import torch
+    from slm_amd import hip
+    from slm_amd.mapping import ChannelMapper
+    from slm_amd.parameters import section
+    from slm_amd.preprocess import Preprocess
+    from slm_amd.trace import Trace
+    from slm_amd.useful import Data, Info, get_bbox
+    params, state, geo = geodata(name)
+    pre = Preprocess(state, section(params, "preprocess"), geo)
+    pre.do()
+    tr = Trace(state, section(params, "trace"), geo, pre)
+    tr.do()
+    mp = section(params, "mapping")
+    mask_array = state.merge_active_masks()
+    bbox, nxb, nyb = get_bbox(~mask_array)
+    info = Info(state, section(params, "trace"), geo.roi_pixel_size, mapping=mp)
+    info.set_xy(nxb, nyb, geo.pad_width)
+    info.set_thresholds(channel_threshold=np.float32(mp.coarse_channel_threshold),
+                        segmentation_threshold=np.uint32(mp.coarse_segmentation_threshold))
+    data = Data(info=info, bbox=bbox, pad=geo.pad_width, mapping_array=np.zeros(mask_array.shape, np.uint32),
+                mask_array=mask_array, uv_array=pre.uv_array, slc_array=tr.slc_array, slt_array=tr.slt_array,
+                sla_array=tr.sla_array)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    mask = t(data.mask_array)
+    mapping = torch.zeros(data.mask_array.shape, dtype=torch.int32, device=device)
+    hip.map_midslopes_ridges(t(data.sla_array), mask, mapping, int(info.is_midslope), int(info.is_ridge),
+                             mp.midslope_filter_sigma, mp.midslope_threshold, mp.ridge_filter_sigma,
+                             mp.ridge_threshold, sync=True)
+    cm = ChannelMapper(info, mask, t(data.uv_array), mapping, slt=t(data.slt_array[:, :, 0]),
+                       slc=t(data.slc_array[:, :, 0].astype(np.float64)))
+    assert cm.map_channels_segments()
+    lab = cm.label.cpu().numpy()
+    return int(cm.n_segments), int(np.unique(lab[lab != 0]).shape[0])

@@ -141,10 +141,39 @@ def test_thinning_bit_exact(gpu, shape, density, grow):
     m = np.where(img, np.uint32(1), np.uint32(0)).astype(np.uint32).reshape(-1)
     it_o = oracle.map_call("thin_channels", p, mapping=m)
     mg = torch.from_numpy(np.where(img, 1, 0).astype(np.int32).reshape(-1)).to(gpu)
-    it_g = hip.thin_channels(p, mg)
+    it_g = hip.thin_channels(p, mg, medial_axis=False)
     torch.cuda.synchronize()
     assert it_g == it_o
     assert np.array_equal(hip.as_u32(mg), m)
+
+
+@pytest.mark.parametrize("shape,density,grow,seed", [((300, 420), 0.985, 4, 7), ((257, 130), 0.996, 14, 8),
+                                                     ((130, 97), 0.97, 2, 9), ((70, 65), 1.0, 0, 1)])
+def test_medial_axis_thinning_bit_exact(gpu, shape, density, grow, seed):
+    """skeletonize(medial_axis(.)) as Mapping.thin_channels calls it: the
+    GPU's rounds against the oracle's one-by-one visit of the medial-axis
+    restatement, then the same thinning; wide blobs, many objects, and an
+    empty image."""
+    import torch
+    from scipy import ndimage
+    from oracle import medial_axis_ref, oracle
+    from slm_amd import hip
+    rs = np.random.default_rng(seed)
+    img = rs.random(shape) > density
+    if grow:
+        img = ndimage.binary_dilation(img, iterations=grow)
+    print("medial axis", shape, "set pixels", int(img.sum()), flush=True)
+    p = workloads.planar(n=8).params()
+    p.nx_padded, p.ny_padded, p.nxy_padded = img.shape[0], img.shape[1], img.size
+    ma = medial_axis_ref.medial_axis(img)
+    m = np.where(ma, np.uint32(1), np.uint32(0)).astype(np.uint32).reshape(-1)
+    it_o = oracle.map_call("thin_channels", p, mapping=m)
+    want = np.where(img, np.uint32(1), np.uint32(0)).reshape(-1) | (m & np.uint32(2))
+    mg = torch.from_numpy(np.where(img, 1, 0).astype(np.int32).reshape(-1)).to(gpu)
+    it_g = hip.thin_channels(p, mg)
+    torch.cuda.synchronize()
+    assert it_g == it_o
+    assert np.array_equal(hip.as_u32(mg), want)
 
 
 def test_mapping_entry_points_reject_bad_input(gpu):
@@ -164,3 +193,23 @@ def test_mapping_entry_points_reject_bad_input(gpu):
     assert rc == -5                                       # SLM_E_WORKSPACE
     p.n_seed_points = 0                                  # empty seed list: nothing to do, no error
     assert lib.slm_connect_channels(ctypes.byref(p), None, None, None, None, hip._stream(None), 1) == 0
+
+
+def test_segment_downchannels_on_link_cycles(gpu):
+    """segment_downchannels over a link graph with cycles (heads whose paths
+    run into them): the GPU's cycle-skipping walks give the oracle's grids."""
+    import torch
+    from oracle import oracle
+    from slm_amd import hip
+    from test_oracle_map import link_cycle_case
+    p, seeds, mask, count, link = link_cycle_case()
+    mapping = np.zeros(p.nxy_padded, np.uint32)
+    label = np.zeros(p.nxy_padded, np.uint32)
+    oracle.map_call("segment_downchannels", p, seeds=seeds, mask=mask, mapping=mapping, count=count, link=link,
+                    label=label)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a).view(np.int32) if a.dtype == np.uint32 else a).to(gpu)
+    mg, lg = t(np.zeros(p.nxy_padded, np.uint32)), t(np.zeros(p.nxy_padded, np.uint32))
+    hip.map_call("segment_downchannels", p, seeds=t(seeds), mask=t(mask), mapping=mg, count=t(count), link=t(link),
+                 label=lg, sync=True)
+    assert np.array_equal(hip.as_u32(mg), mapping)
+    assert np.array_equal(hip.as_u32(lg), label)

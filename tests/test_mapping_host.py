@@ -5,7 +5,7 @@ memory, and the whole chain must equal the numpy OracleMapper's."""
 import numpy as np
 import pytest
 
-from oracle import oracle
+from oracle import medial_axis_ref, oracle
 from slm_amd import hip, mapping, workloads
 
 from map_chain import OracleMapper, STEPS
@@ -21,8 +21,16 @@ def cpu_kernels(monkeypatch):
         oracle.map_call(name, p, **np_arrays)
         return workspace
 
-    def fake_thin(p, m, workspace=None, stream=None):
-        return int(oracle.map_call("thin_channels", p, mapping=m.numpy().view(np.uint32)))
+    def fake_thin(p, m, workspace=None, stream=None, medial_axis=True):
+        mm = m.numpy().view(np.uint32)
+        src = mm
+        if medial_axis:
+            img = ((mm & np.uint32(hip.CHANNEL_BITS)) != 0).reshape(p.nx_padded, p.ny_padded)
+            src = np.where(medial_axis_ref.medial_axis(img), np.uint32(1), np.uint32(0)).astype(np.uint32).reshape(-1)
+        tmp = np.ascontiguousarray(src, dtype=np.uint32).copy()
+        it = int(oracle.map_call("thin_channels", p, mapping=tmp))
+        mm |= tmp & np.uint32(2)
+        return it
 
     monkeypatch.setattr(hip, "map_call", fake_map_call)
     monkeypatch.setattr(hip, "thin_channels", fake_thin)

@@ -288,3 +288,64 @@ def test_cycle_fast_forward_is_exact():
     print("chain seconds with / without the skip:", times)
     for k in states[1]:
         assert np.array_equal(states[0][k], states[1][k]), k
+
+
+def link_cycle_case():
+    """A link graph with cycles for segment_downchannels: three heads whose
+    paths enter one 9-pixel cycle at different pixels, a second 5-pixel cycle
+    reached by one head, counts that rise along the paths and vary around the
+    cycles so that segment starts are marked on them (threshold 4)."""
+    from slm_amd import workloads
+    n = 24
+    p = workloads.planar(n=8).params()
+    p.nx_padded = p.ny_padded = n
+    p.nxy_padded = n * n
+    p.segmentation_threshold = 4
+    pad = int(p.pad_width)
+    link = np.arange(n * n, dtype=np.uint32)            # self links: walks stop
+    count = np.zeros(n * n, np.uint32)
+    rs = np.random.default_rng(5)
+
+    def pix(x, y):
+        return x * n + y
+
+    cyc = [pix(10, 3 + k) for k in range(9)]
+    for a, b in zip(cyc, cyc[1:] + cyc[:1]):
+        link[a] = b
+    cyc2 = [pix(18, 4 + k) for k in range(5)]
+    for a, b in zip(cyc2, cyc2[1:] + cyc2[:1]):
+        link[a] = b
+    heads = []
+    for hx, entry, length in ((3, cyc[0], 6), (5, cyc[4], 3), (14, cyc2[2], 8)):
+        path = [pix(hx, 3 + k) for k in range(length)]
+        for a, b in zip(path, path[1:]):
+            link[a] = b
+        link[path[-1]] = entry
+        count[path] = np.arange(1, length + 1) * 3
+        heads.append(path[0])
+    count[cyc] = rs.integers(5, 40, len(cyc))
+    count[cyc2] = rs.integers(5, 40, len(cyc2))
+    seeds = np.array([[h // n - pad, h % n - pad] for h in heads], np.float32)
+    p.n_seed_points = len(heads)
+    mask = np.zeros(n * n, np.uint8)
+    return p, seeds, mask, count, link
+
+
+def test_segment_walks_on_link_cycles_fast_forward_exact():
+    """The full ownership walks of segment_downchannels circle on link cycles
+    (the walk never meets a segment head there); skipping them on detection
+    gives the same grids as walking to the hop cap."""
+    p, seeds, mask, count, link = link_cycle_case()
+    out = []
+    try:
+        for ff in (True, False):
+            oracle.map_set_fast_forward(ff)
+            mapping = np.zeros(p.nxy_padded, np.uint32)
+            label = np.zeros(p.nxy_padded, np.uint32)
+            oracle.map_call("segment_downchannels", p, seeds=seeds, mask=mask, mapping=mapping, count=count,
+                            link=link, label=label)
+            out.append((mapping.copy(), label.copy()))
+    finally:
+        oracle.map_set_fast_forward(True)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert (out[0][0] & np.uint32(0x4000)).sum() > 3 * 0x4000       # segment starts beyond the heads
