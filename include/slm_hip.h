@@ -230,7 +230,9 @@ typedef struct SlmBatchOpts {
     uint32_t chunk;          /* work items per queue grab (0: 64)                   */
     uint32_t blocks_per_cu;  /* resident 256-thread blocks per CU (0: occupancy max) */
     uint32_t queue_slices;   /* work-list slices / queue heads, 1..8 (0: 8, one per XCD) */
-    uint32_t reserved[5];
+    uint32_t lds_slots;      /* per-wave LDS accumulation table: 256, 512 or 1024 slots
+                                (a 16x16, 16x32 or 32x32 pixel window; 0: 512)         */
+    uint32_t reserved[4];
 } SlmBatchOpts;
 
 size_t slm_trace_fields_batch_workspace_size(const SlmFieldsJob* jobs, uint32_t n_jobs);

@@ -452,12 +452,23 @@ __device__ __forceinline__ uint32_t slot_of(uint32_t ix, uint32_t iy) {
     return ((ix & 15u) << 5) | (iy & 31u);
 }
 
-struct RunTable {
+// Table sizes of the batched tracer (a tuning option): 256 slots map a 16x16
+// window of pixels, 512 a 16x32 window, 1024 a 32x32 window, collision-free.
+template <uint32_t SLOTS>
+__device__ __forceinline__ uint32_t slot_of_t(uint32_t ix, uint32_t iy) {
+    static_assert(SLOTS == 256 || SLOTS == 512 || SLOTS == 1024, "table size");
+    if constexpr (SLOTS == 256) return ((ix & 15u) << 4) | (iy & 15u);
+    else if constexpr (SLOTS == 512) return ((ix & 15u) << 5) | (iy & 31u);
+    else return ((ix & 31u) << 5) | (iy & 31u);
+}
+
+template <uint32_t SLOTS>
+struct RunTableT {
     uint32_t* tag;
     unsigned long long* val;
 
     __device__ __forceinline__ void init(uint32_t lane) {
-        for (uint32_t i = lane; i < kRunSlots; i += 64u) {
+        for (uint32_t i = lane; i < SLOTS; i += 64u) {
             __hip_atomic_store(&tag[i], kRunEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             __hip_atomic_store(&val[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
@@ -492,13 +503,14 @@ struct RunTable {
 
     __device__ __forceinline__ void drain(uint32_t lane, unsigned long long* __restrict__ acc) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        for (uint32_t i = lane; i < kRunSlots; i += 64u) {
+        for (uint32_t i = lane; i < SLOTS; i += 64u) {
             const uint32_t t = __hip_atomic_load(&tag[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             const unsigned long long v = __hip_atomic_load(&val[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
             if (t != kRunEmpty && v) atomicAdd(&acc[t], v);
         }
     }
 };
+using RunTable = RunTableT<kRunSlots>;
 
 // One persistent wave pulls streamlines from the queue; a lane that finishes
 // takes the next item at once, so divergent lengths do not idle the wave.
@@ -1924,7 +1936,7 @@ __global__ void k_put_desc(DescPack pack, uint32_t first, uint32_t count, BatchD
     if (i < count) out[first + i] = pack.d[i];
 }
 
-template <bool C, bool PACKED, bool R2>
+template <bool C, bool PACKED, bool R2, uint32_t SLOTS = kRunSlots>
 __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restrict__ desc, uint32_t n_jobs,
                                                       uint32_t n_work, unsigned long long* __restrict__ steps,
                                                       FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
@@ -1938,9 +1950,9 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
 
     // per-wave accumulation table; its tags are pixel indices into the
     // concatenated accumulator of all jobs (fewer than 2^32 - 1 pixels)
-    __shared__ uint32_t s_tag[PACKED ? 4u * kRunSlots : 1u];
-    __shared__ unsigned long long s_val[PACKED ? 4u * kRunSlots : 1u];
-    RunTable table{s_tag + (threadIdx.x >> 6) * kRunSlots, s_val + (threadIdx.x >> 6) * kRunSlots};
+    __shared__ uint32_t s_tag[PACKED ? 4u * SLOTS : 1u];
+    __shared__ unsigned long long s_val[PACKED ? 4u * SLOTS : 1u];
+    RunTableT<SLOTS> table{s_tag + (threadIdx.x >> 6) * SLOTS, s_val + (threadIdx.x >> 6) * SLOTS};
     if constexpr (PACKED) table.init(lane);
 
     // per-lane job context
@@ -2041,7 +2053,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                         cx = ix;
                         cy = iy;
                         run_idx = idx;
-                        run_slot = slot_of(ix, iy);
+                        run_slot = slot_of_t<SLOTS>(ix, iy);
                         run_n = 1u;
                         run_t = slt_inc(0.f);
                         active = (0.f < P.max_length) && (0u < P.max_n_steps);
@@ -2098,7 +2110,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     f_n = run_n;
                     f_t = run_t;
                     run_idx = nidx;
-                    run_slot = slot_of(nix, niy);
+                    run_slot = slot_of_t<SLOTS>(nix, niy);
                     run_n = 0;
                     run_t = 0;
                 }
@@ -2226,14 +2238,17 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
         if (J.params.n_seed_points > J.params.seeds_chunk_offset && !J.seeds_xy)
             return slm_fail(SLM_E_INVALID_ARG, "job %u: NULL seeds", j);
     }
-    uint32_t chunk = 64, blocks_per_cu = 0, slices = kQueues;
+    uint32_t chunk = 64, blocks_per_cu = 0, slices = kQueues, slots = kRunSlots;
     if (opts) {
         if (opts->chunk) chunk = opts->chunk;
         if (opts->blocks_per_cu) blocks_per_cu = opts->blocks_per_cu;
         if (opts->queue_slices) slices = opts->queue_slices;
+        if (opts->lds_slots) slots = opts->lds_slots;
     }
     if (chunk < 1 || chunk > 65536) return slm_fail(SLM_E_INVALID_ARG, "chunk %u outside [1, 65536]", chunk);
     if (slices < 1 || slices > kQueues) return slm_fail(SLM_E_INVALID_ARG, "queue_slices %u outside [1, 8]", slices);
+    if (slots != 256 && slots != 512 && slots != 1024)
+        return slm_fail(SLM_E_INVALID_ARG, "lds_slots %u is not 256, 512 or 1024", slots);
     DevInfo di;
     rc = device_info(di);
     if (rc) return rc;
@@ -2312,10 +2327,17 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     int per_cu = 0;
     bool r2 = true;
     for (uint32_t j = 0; j < n_jobs; j++) r2 = r2 && slm_res_is_pow2(&jobs[j].params);
-    if (C0)
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<true, true, true>, 256, 0));
-    else
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<false, true, true>, 256, 0));
+#define SLM_OCC(CC, SS) HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fields_batch<CC, true, true, SS>, 256, 0))
+    if (C0) {
+        if (slots == 256) SLM_OCC(true, 256);
+        else if (slots == 1024) SLM_OCC(true, 1024);
+        else SLM_OCC(true, 512);
+    } else {
+        if (slots == 256) SLM_OCC(false, 256);
+        else if (slots == 1024) SLM_OCC(false, 1024);
+        else SLM_OCC(false, 512);
+    }
+#undef SLM_OCC
     if (per_cu < 1) per_cu = 1;
     if (blocks_per_cu && (int)blocks_per_cu < per_cu) per_cu = (int)blocks_per_cu;
     uint32_t grid = (uint32_t)di.cus * (uint32_t)per_cu;
@@ -2323,16 +2345,23 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (grid > max_useful) grid = max_useful ? max_useful : 1;
     unsigned long long* st = (unsigned long long*)steps_out;
     if (n_work) {
-#define SLM_LAUNCH_BATCH(CC, RR)                                                                     \
-    hipLaunchKernelGGL((k_fields_batch<CC, true, RR>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, \
+#define SLM_LAUNCH_BATCH(CC, RR, SS)                                                                     \
+    hipLaunchKernelGGL((k_fields_batch<CC, true, RR, SS>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, \
                        n_jobs, n_work, st, ctrl, force, chunk, slices, acc_all)
+#define SLM_LAUNCH_SLOTS(CC, RR)                          \
+    do {                                                  \
+        if (slots == 256) SLM_LAUNCH_BATCH(CC, RR, 256);   \
+        else if (slots == 1024) SLM_LAUNCH_BATCH(CC, RR, 1024); \
+        else SLM_LAUNCH_BATCH(CC, RR, 512);                \
+    } while (0)
         if (C0) {
-            if (r2) SLM_LAUNCH_BATCH(true, true);
-            else SLM_LAUNCH_BATCH(true, false);
+            if (r2) SLM_LAUNCH_SLOTS(true, true);
+            else SLM_LAUNCH_SLOTS(true, false);
         } else {
-            if (r2) SLM_LAUNCH_BATCH(false, true);
-            else SLM_LAUNCH_BATCH(false, false);
+            if (r2) SLM_LAUNCH_SLOTS(false, true);
+            else SLM_LAUNCH_SLOTS(false, false);
         }
+#undef SLM_LAUNCH_SLOTS
 #undef SLM_LAUNCH_BATCH
     }
     hipLaunchKernelGGL(k_fields_sumlow, dim3(grid_for(w.n_pix)), dim3(256), 0, s, acc_all, (size_t)w.n_pix, ctrl);

@@ -310,9 +310,9 @@ def hillslope_lengths(p: SlmParams, seeds, mask, uv, mapping, thin_flag, from_fl
                                        SLM_FLAG_SYNC if sync else 0))
 
 
-def batch_opts(chunk=0, blocks_per_cu=0, queue_slices=0) -> SlmBatchOpts:
+def batch_opts(chunk=0, blocks_per_cu=0, queue_slices=0, lds_slots=0) -> SlmBatchOpts:
     o = SlmBatchOpts()
-    o.chunk, o.blocks_per_cu, o.queue_slices = chunk, blocks_per_cu, queue_slices
+    o.chunk, o.blocks_per_cu, o.queue_slices, o.lds_slots = chunk, blocks_per_cu, queue_slices, lds_slots
     return o
 
 

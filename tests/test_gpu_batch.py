@@ -111,3 +111,21 @@ def test_autotuned_opts_same_bits(gpu):
     hip.trace_fields_batch(arr, opts=opts, sync=True)
     for (_, _, _, _, slc, slt), (rc, rt) in zip(jobs, ref):
         assert np.array_equal(hip.as_u32(slc), rc) and np.array_equal(hip.as_u32(slt), rt)
+    assert opts.lds_slots in autotune.LDS_SLOTS
+
+
+@pytest.mark.parametrize("slots", [256, 512, 1024])
+def test_lds_table_sizes_same_bits(gpu, slots):
+    """Every LDS accumulation-table size the tuner may pick gives the bits of
+    separate single-tile calls (and the packed-sum fallback still agrees)."""
+    import torch
+    from slm_amd import hip
+    jobs, arr = _batch(gpu, n_tiles=5, seed=11, d=2)
+    ref, total = _single(gpu, jobs)
+    for force in (False, True):
+        steps = torch.zeros(1, dtype=torch.int64, device=gpu)
+        hip.trace_fields_batch(arr, steps=steps, opts=hip.batch_opts(lds_slots=slots), force_fallback=force, sync=True)
+        if not force:
+            assert int(steps.item()) == total
+        for (_, _, _, _, slc, slt), (rc, rt) in zip(jobs, ref):
+            assert np.array_equal(hip.as_u32(slc), rc) and np.array_equal(hip.as_u32(slt), rt)
