@@ -256,6 +256,23 @@ int slm_trace_paths(const SlmParams* p, const float* seeds_xy,
                     void* stream, uint32_t flags);
 
 /*
+ * Packing of recorded trajectories end to end on the device, replacing the
+ * per-seed host slices traj[k, :nsteps[k]] that build streamline_arrays_list
+ * in Trajectories.gpu_compute_trajectories (python/streamlines/
+ * trajectories.py:320-325); the reference has no kernel for it.
+ * slm_paths_offsets: offsets (uint64 [n + 1]) = exclusive prefix sum of
+ *   min(nsteps[k], max_n_steps); offsets[n] is the total step count.
+ * slm_paths_pack: packed (int8x2 [offsets[n]]) receives row k's first
+ *   min(nsteps[k], max_n_steps) steps of traj (int8x2 [n][max_n_steps]) at
+ *   step offsets[k], given the offsets slm_paths_offsets wrote.
+ */
+int slm_paths_offsets(const uint16_t* nsteps, uint32_t n, uint32_t max_n_steps,
+                      uint64_t* offsets, void* stream, uint32_t flags);
+int slm_paths_pack(const int8_t* traj, const uint16_t* nsteps, const uint64_t* offsets,
+                   uint32_t n, uint32_t max_n_steps, int8_t* packed,
+                   void* stream, uint32_t flags);
+
+/*
  * Test hook with no reference counterpart: records the positions of selected
  * jittered fields streamlines exactly as slm_trace_fields integrates them.
  * For j < n_sel: streamline (seed_sel[j], sub_sel[j]) writes its positions

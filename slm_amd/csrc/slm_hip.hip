@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <stdarg.h>
+#include <algorithm>
 #include <mutex>
 #include "../../include/slm_hip.h"
 #include "trace_common.hpp"
@@ -1101,6 +1102,133 @@ extern "C" int slm_trace_paths(const SlmParams* p, const float* seeds_xy, const 
     else
         hipLaunchKernelGGL(k_paths<false>, dim3((n + 63) / 64), dim3(64), 0, s, *p, (const float2*)seeds_xy, mask,
                            (const float2*)uv, (char2*)traj, nsteps, length, n);
+    return slm_finish(stream, flags);
+}
+
+// ------------------------------------------------------------------ trajectory packing
+//
+// The recorded trajectories of slm_trace_paths sit in rows of max_n_steps
+// steps, mostly unused.  Packing them end to end takes an exclusive prefix
+// sum of the step counts (offsets, n + 1 entries) and one copy of the used
+// steps, so that only those leave HBM.  The sum runs in place over offsets,
+// in blocks of kPackBlock counts: pass 1 leaves block b's total in its first
+// slot offsets[b * kPackBlock]; pass 2 (one block) turns those slots into
+// exclusive block bases and writes the grand total to offsets[n]; pass 3
+// scans each block from its base.  Counts above max_n_steps are clamped in
+// every pass, so the packed rows never read past their row.
+
+constexpr uint32_t kPackThreads = 256;
+constexpr uint32_t kPackPer = 4;
+constexpr uint32_t kPackBlock = kPackThreads * kPackPer;
+
+// exclusive scan of one value per thread of a kPackThreads block; returns the
+// thread's prefix and leaves the block total in *total (sh: kPackThreads slots)
+__device__ inline uint64_t pack_block_scan(uint64_t v, uint64_t* sh, uint64_t* total) {
+    const uint32_t t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (uint32_t d = 1; d < kPackThreads; d <<= 1) {
+        const uint64_t add = t >= d ? sh[t - d] : 0ull;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const uint64_t incl = sh[t];
+    *total = sh[kPackThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ inline uint32_t pack_count(const uint16_t* nsteps, uint32_t k, uint32_t n, uint32_t max_n_steps) {
+    return k < n ? min((uint32_t)nsteps[k], max_n_steps) : 0u;
+}
+
+__global__ __launch_bounds__(kPackThreads) void k_pack_sums(const uint16_t* __restrict__ nsteps, uint32_t n,
+                                                            uint32_t max_n_steps, uint64_t* __restrict__ offsets) {
+    __shared__ uint64_t sh[kPackThreads];
+    const size_t k0 = (size_t)blockIdx.x * kPackBlock + threadIdx.x * kPackPer;
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < kPackPer; ++j)
+        if (k0 + j < n) s += pack_count(nsteps, (uint32_t)(k0 + j), n, max_n_steps);
+    uint64_t total;
+    pack_block_scan(s, sh, &total);
+    if (threadIdx.x == 0) offsets[(size_t)blockIdx.x * kPackBlock] = total;
+}
+
+__global__ __launch_bounds__(kPackThreads) void k_pack_bases(uint32_t n_blocks, uint32_t n,
+                                                             uint64_t* __restrict__ offsets) {
+    __shared__ uint64_t sh[kPackThreads];
+    uint64_t carry = 0;
+    for (uint32_t b0 = 0; b0 < n_blocks; b0 += kPackThreads) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t v = b < n_blocks ? offsets[(size_t)b * kPackBlock] : 0ull;
+        uint64_t total;
+        const uint64_t pre = pack_block_scan(v, sh, &total);
+        if (b < n_blocks) offsets[(size_t)b * kPackBlock] = carry + pre;
+        carry += total;
+    }
+    if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+__global__ __launch_bounds__(kPackThreads) void k_pack_scan(const uint16_t* __restrict__ nsteps, uint32_t n,
+                                                            uint32_t max_n_steps, uint64_t* __restrict__ offsets) {
+    __shared__ uint64_t sh[kPackThreads];
+    const size_t first = (size_t)blockIdx.x * kPackBlock;
+    const uint64_t base = offsets[first];      // read by every thread before any write below
+    const size_t k0 = first + threadIdx.x * kPackPer;
+    uint32_t c[kPackPer];
+    uint64_t s = 0;
+    for (uint32_t j = 0; j < kPackPer; ++j) {
+        c[j] = k0 + j < n ? pack_count(nsteps, (uint32_t)(k0 + j), n, max_n_steps) : 0u;
+        s += c[j];
+    }
+    uint64_t total;
+    uint64_t o = base + pack_block_scan(s, sh, &total);    // its barriers order the reads of base first
+    for (uint32_t j = 0; j < kPackPer; ++j) {
+        if (k0 + j < n) offsets[k0 + j] = o;
+        o += c[j];
+    }
+}
+
+// one wave per record: its used steps (2 bytes each) copied to their offset
+__global__ __launch_bounds__(kPackThreads) void k_pack_copy(const uint16_t* __restrict__ traj,
+                                                            const uint16_t* __restrict__ nsteps,
+                                                            const uint64_t* __restrict__ offsets, uint32_t n,
+                                                            uint32_t max_n_steps, uint16_t* __restrict__ packed) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const size_t waves = (size_t)gridDim.x * (kPackThreads / 64);
+    for (size_t k = (size_t)blockIdx.x * (kPackThreads / 64) + (threadIdx.x >> 6); k < n; k += waves) {
+        const uint32_t cnt = min((uint32_t)nsteps[k], max_n_steps);
+        const uint16_t* src = traj + k * max_n_steps;
+        uint16_t* dst = packed + offsets[k];
+        for (uint32_t i = lane; i < cnt; i += 64) dst[i] = src[i];
+    }
+}
+
+extern "C" int slm_paths_offsets(const uint16_t* nsteps, uint32_t n, uint32_t max_n_steps, uint64_t* offsets,
+                                 void* stream, uint32_t flags) {
+    if (!offsets || (n && !nsteps)) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, (unsigned long long*)offsets, (size_t)1);
+        return slm_finish(stream, flags);
+    }
+    const uint32_t nb = (uint32_t)(((size_t)n + kPackBlock - 1) / kPackBlock);
+    hipLaunchKernelGGL(k_pack_sums, dim3(nb), dim3(kPackThreads), 0, s, nsteps, n, max_n_steps, offsets);
+    hipLaunchKernelGGL(k_pack_bases, dim3(1), dim3(kPackThreads), 0, s, nb, n, offsets);
+    hipLaunchKernelGGL(k_pack_scan, dim3(nb), dim3(kPackThreads), 0, s, nsteps, n, max_n_steps, offsets);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_paths_pack(const int8_t* traj, const uint16_t* nsteps, const uint64_t* offsets, uint32_t n,
+                              uint32_t max_n_steps, int8_t* packed, void* stream, uint32_t flags) {
+    if (n == 0) return slm_finish(stream, flags);
+    if (!traj || !nsteps || !offsets || !packed) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
+    if (max_n_steps == 0) return slm_fail(SLM_E_INVALID_ARG, "max_n_steps is 0");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t blocks = std::min<size_t>(((size_t)n + 3) / 4, 65536);
+    hipLaunchKernelGGL(k_pack_copy, dim3((uint32_t)blocks), dim3(kPackThreads), 0, s, (const uint16_t*)traj, nsteps,
+                       offsets, n, max_n_steps, (uint16_t*)packed);
     return slm_finish(stream, flags);
 }
 

@@ -44,6 +44,8 @@ _PROTOS = {
     "slm_trace_fields_tiled": (ctypes.c_int, [ctypes.POINTER(SlmParams), ctypes.POINTER(SlmTileOpts), _P, _P, _P,
                                               _P, _P, _P, _P, ctypes.c_size_t, _P, _U32]),
     "slm_trace_paths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _P, _P, _U32]),
+    "slm_paths_offsets": (ctypes.c_int, [_P, _U32, _U32, _P, _P, _U32]),
+    "slm_paths_pack": (ctypes.c_int, [_P, _P, _P, _U32, _U32, _P, _P, _U32]),
     "slm_trace_fields_record": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _P, _U32,
                                                _U32, _P, _P, _P, _U32]),
     "slm_hillslope_lengths": (ctypes.c_int, [ctypes.POINTER(SlmParams), _P, _P, _P, _P, _U32, _U32, _P, _P,
@@ -289,6 +291,27 @@ def trace_paths(p: SlmParams, seeds, mask, uv, traj, nsteps, length, stream=None
     check(load().slm_trace_paths(ctypes.byref(p), _ptr(seeds), _ptr(mask), _ptr(uv), _ptr(traj),
                                  _ptr(nsteps), _ptr(length), _stream(stream),
                                  SLM_FLAG_SYNC if sync else 0))
+
+
+def paths_pack(traj, nsteps, max_n_steps, stream=None, sync=False):
+    """Recorded trajectories packed end to end on the device
+    (slm_paths_offsets, then slm_paths_pack): traj int8 [n, max_n_steps, 2],
+    nsteps int16 [n] (uint16 bits).  Returns (offsets int64 [n + 1],
+    packed int8 [offsets[n], 2]); reading the total waits for the stream."""
+    import torch
+    _require_cuda(traj, nsteps)
+    n = nsteps.numel()
+    if traj.numel() != n * max_n_steps * 2:
+        raise ValueError("traj holds %d bytes, not n * max_n_steps * 2 = %d" % (traj.numel(), n * max_n_steps * 2))
+    offsets = torch.empty(n + 1, dtype=torch.int64, device=nsteps.device)
+    check(load().slm_paths_offsets(_ptr(nsteps), n, int(max_n_steps), _ptr(offsets), _stream(stream), 0))
+    if stream is not None:
+        stream.synchronize()
+    total = int(offsets[n].item())
+    packed = torch.empty((total, 2), dtype=torch.int8, device=nsteps.device)
+    check(load().slm_paths_pack(_ptr(traj), _ptr(nsteps), _ptr(offsets), n, int(max_n_steps), _ptr(packed),
+                                _stream(stream), SLM_FLAG_SYNC if sync else 0))
+    return offsets, packed
 
 
 def trace_fields_record(p: SlmParams, seeds, mask, uv, seed_sel, sub_sel, max_rec, xy, nrec,

@@ -91,11 +91,14 @@ class Trajectories:
             outs[idx] = (traj_d, nsteps_d, length_d)
         torch.cuda.synchronize(dev)
         for idx, (traj_d, nsteps_d, length_d) in outs.items():
-            nsteps = nsteps_d.cpu().numpy().view(np.uint16).copy()
-            traj = traj_d.cpu().numpy()
-            traj_nsteps_array[:, idx] = nsteps
+            # only the used steps leave the device: packed end to end there
+            # (trajectories.py:320-325 slices each seed's row on the host)
+            offsets_d, packed_d = hip.paths_pack(traj_d, nsteps_d, nl)
+            offsets = offsets_d.cpu().numpy()
+            packed = packed_d.cpu().numpy()
+            traj_nsteps_array[:, idx] = nsteps_d.cpu().numpy().view(np.uint16)
             traj_length_array[:, idx] = length_d.cpu().numpy()
-            streamline_arrays_list[idx] = [traj[k, :nsteps[k]].copy() for k in range(ns)]
+            streamline_arrays_list[idx] = np.split(packed, offsets[1:-1])
         self.data.streamline_arrays_list = streamline_arrays_list
         self.data.traj_nsteps_array = traj_nsteps_array
         self.data.traj_length_array = traj_length_array
