@@ -20,6 +20,7 @@ Fields compared (planar float32 [3][nx*ny] over the tracer's grid):
   predict         inference on overlapping tiles, stitched into the grid
   gated_map       predict, re-trace a random share of the tiles, replace
                   failures, fall back to plain_map when too many fail
+  check_tiles     the gate's checks applied to any whole-map fields
   plain_map       the same fields by tracing every pixel
   compare         median relative log-density error and channel-pixel F1
 """
@@ -434,6 +435,57 @@ def _tile_check(pred, truth, valid, max_err, min_f1, min_channel_px):
     return ok, med, f1, tp + fn
 
 
+def _check_windows(grid, origins, chosen, size, halo):
+    """Trace the core of each chosen tile as a grid of its own with a margin
+    of max_length + 2 px (every streamline that can reach the core starts
+    inside that window), all windows in one batched launch.  Returns (jobs,
+    cores, steps): the job tuples, (i0, i1, j0, j1, r0, r1, c0, c1) per tile
+    (core and window in grid pixels), and the device step count."""
+    import torch
+    margin = int(math.ceil(grid.max_length_px)) + 2
+    c = size - 2 * halo
+    jobs, cores = [], []
+    for t in chosen:
+        i0, j0 = int(origins[t][0]) + halo, int(origins[t][1]) + halo
+        i1, j1 = min(grid.nx, i0 + c), min(grid.ny, j0 + c)
+        r0, r1 = max(0, i0 - margin), min(grid.nx, i1 + margin)
+        c0, c1 = max(0, j0 - margin), min(grid.ny, j1 + margin)
+        jobs.append(grid.region(r0, r1, c0, c1))
+        cores.append((i0, i1, j0, j1, r0, r1, c0, c1))
+    steps = torch.zeros(1, dtype=torch.int64, device=grid.dev)
+    hip.trace_fields_batch(hip.fields_jobs(jobs), steps=steps)
+    return jobs, cores, steps
+
+
+def _checked(grid, jobs, cores, theta, fields, max_err, min_f1, min_channel_px):
+    """Per checked tile, in order: (check dict, predicted core view of
+    fields, traced core)."""
+    view = fields.view(3, grid.nx, grid.ny)
+    gm = grid.mask.view(grid.nx, grid.ny)
+    for (p, seeds, mask, uv, slc, slt), (i0, i1, j0, j1, r0, r1, c0, c1) in zip(jobs, cores):
+        f = fields_log(slc, slt, mask, grid.density, theta).view(3, r1 - r0, c1 - c0)
+        tr = f[:, i0 - r0:i1 - r0, j0 - c0:j1 - c0]
+        pr = view[:, i0:i1, j0:j1]
+        ok, med, f1, nch = _tile_check(pr, tr, gm[i0:i1, j0:j1] == 0, max_err, min_f1, min_channel_px)
+        yield {"core": [i0, i1, j0, j1], "median_rel_err": med, "f1": f1, "channel_px": nch, "ok": ok}, pr, tr
+
+
+def _chosen(n_tiles, check_frac, rng_seed):
+    k = max(1, int(round(check_frac * n_tiles)))
+    return np.sort(np.random.default_rng(rng_seed).choice(n_tiles, k, replace=False))
+
+
+def check_tiles(fields, grid, theta, size=256, halo=16, check_frac=0.02, rng_seed=2, max_err=0.05, min_f1=0.95,
+                min_channel_px=200):
+    """The gate's checks (the same tiles, windows and criteria as gated_map)
+    applied to given fields [3, nxy] over the grid, e.g. a whole-map
+    prediction or another trace: every chosen tile is checked and fields is
+    not changed.  Returns the list of check dicts."""
+    origins = tile_origins(grid.nx, grid.ny, size, halo)
+    jobs, cores, _ = _check_windows(grid, origins, _chosen(len(origins), check_frac, rng_seed), size, halo)
+    return [c for c, _, _ in _checked(grid, jobs, cores, theta, fields, max_err, min_f1, min_channel_px)]
+
+
 def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rng_seed=2, max_err=0.05,
               min_f1=0.95, min_channel_px=200, max_fail_frac=0.25, workspace=None):
     """Surrogate map with tracer checks.  A random check_frac of the tiles
@@ -451,33 +503,15 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
     import torch
     origins = tile_origins(grid.nx, grid.ny, size, halo)
     n_tiles = len(origins)
-    k = max(1, int(round(check_frac * n_tiles)))
-    chosen = np.sort(np.random.default_rng(rng_seed).choice(n_tiles, k, replace=False))
+    chosen = _chosen(n_tiles, check_frac, rng_seed)
+    k = len(chosen)
     out = torch.empty((3, grid.nxy), dtype=torch.float32, device=grid.dev)
     predict_tiles(net, grid, origins[chosen], size, halo, batch, out)
-    margin = int(math.ceil(grid.max_length_px)) + 2
-    c = size - 2 * halo
-    jobs, cores = [], []
-    for t in chosen:
-        i0, j0 = int(origins[t][0]) + halo, int(origins[t][1]) + halo
-        i1, j1 = min(grid.nx, i0 + c), min(grid.ny, j0 + c)
-        r0, r1 = max(0, i0 - margin), min(grid.nx, i1 + margin)
-        c0, c1 = max(0, j0 - margin), min(grid.ny, j1 + margin)
-        jobs.append(grid.region(r0, r1, c0, c1))
-        cores.append((i0, i1, j0, j1, r0, r1, c0, c1))
-    arr = hip.fields_jobs(jobs)
-    steps = torch.zeros(1, dtype=torch.int64, device=grid.dev)
-    hip.trace_fields_batch(arr, steps=steps)
-    view = out.view(3, grid.nx, grid.ny)
-    gm = grid.mask.view(grid.nx, grid.ny)
+    jobs, cores, steps = _check_windows(grid, origins, chosen, size, halo)
     checks, n_fail = [], 0
-    for (p, seeds, mask, uv, slc, slt), (i0, i1, j0, j1, r0, r1, c0, c1) in zip(jobs, cores):
-        f = fields_log(slc, slt, mask, grid.density, theta).view(3, r1 - r0, c1 - c0)
-        tr = f[:, i0 - r0:i1 - r0, j0 - c0:j1 - c0]
-        pr = view[:, i0:i1, j0:j1]
-        ok, med, f1, nch = _tile_check(pr, tr, gm[i0:i1, j0:j1] == 0, max_err, min_f1, min_channel_px)
-        checks.append({"core": [i0, i1, j0, j1], "median_rel_err": med, "f1": f1, "channel_px": nch, "ok": ok})
-        if not ok:
+    for check, pr, tr in _checked(grid, jobs, cores, theta, out, max_err, min_f1, min_channel_px):
+        checks.append(check)
+        if not check["ok"]:
             n_fail += 1
             if n_fail > max_fail_frac * k:
                 break   # the map falls back whatever the remaining checks say

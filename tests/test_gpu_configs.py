@@ -10,6 +10,9 @@
 - Batched uneven tiles: 8 tiles with sides 256-512 px, 1 m and 2 m, two of
   them fans with unfilled pits, MAX_N_STEPS 5000, both directions: 16 jobs in
   one persistent launch, every job bit-exact against the CPU oracle.
+- The surrogate's training set (8192 tiles of 128^2) and gated 8192^2 map:
+  one epoch beats the constant predictor; the gate's fallback is the plain
+  map bit for bit.
 """
 import numpy as np
 import pytest
@@ -133,3 +136,34 @@ def test_batched_uneven_tiles_bit_exact_vs_oracle(gpu):
             assert np.array_equal(hip.as_u32(c), oc), (t.name, s)
             assert np.array_equal(hip.as_u32(tt), ot), (t.name, s)
     assert int(steps.item()) == total
+
+
+def test_surrogate_training_set_and_gated_map_at_size(gpu):
+    """The surrogate's configurations at their own sizes.  Training: 32
+    traced family DTMs of 2048^2 cut into 8192 tiles of 128^2, one epoch of
+    Adam (batch 32, 256 steps); the epoch's mean loss is below the best
+    constant predictor's.  Gated map: the 8192^2 family DTM (seed 2) in 1369
+    tiles of 256^2 with 27 checked; a network one epoch old misses the
+    tolerance, so the map falls back and equals the plain map bit for bit."""
+    import torch
+    from slm_amd import gated
+    from slm_amd import surrogate as S
+    print("training set 32 x 2048^2", flush=True)
+    data = gated.training_set(n_dtms=32, n=2048, size=128, seed0=1000, device=gpu)
+    assert data.n_tiles == 8192 and int(data.n_valid.sum()) > 0.99 * 8192 * 128 * 128
+    const = gated.constant_loss(data)
+    net, hist = gated.train(data, epochs=1, batch=32, lr=1e-3, device=gpu)
+    print("constant %.4f, first epoch %.4f" % (const, hist["loss"][0]), flush=True)
+    assert hist["steps"] == 256 and hist["loss"][0] < const
+    inf = S.UNet(net.params_numpy(), device=gpu, inference=True)
+    theta = data.theta
+    del data, net
+    torch.cuda.empty_cache()
+    print("gated map 8192^2", flush=True)
+    grid = gated.DeviceGrid(gated.family_dtm(8192, 2), device=gpu)
+    out, rep = gated.gated_map(inf, grid, theta, size=256, halo=16, batch=64)
+    print({k: rep[k] for k in ("tiles", "checked", "evaluated", "failed", "fallback")}, flush=True)
+    assert rep["tiles"] == 1369 and rep["checked"] == 27
+    assert rep["fallback"]
+    plain, _ = gated.plain_map(grid, theta)
+    assert torch.equal(out, plain)

@@ -41,8 +41,10 @@ def test_notebook_pass1_segment_counts(gpu):
     """Mapping pass 1 at the JSON's coarse thresholds: GuadalupeMtns_Test1
     prints 3 segments and 5 subsegments, and so does this path.
     IndianCreek_Test2's 4 / 9 is not reproduced (8 / 14 here): its network
-    depends on scikit-image's skeletonize, restated here as Zhang-Suen, and on
-    which legal schedule the reference's racy kernels took (both unpinned)."""
+    depends on scikit-image's medial_axis and skeletonize, restated here from
+    their published algorithms (medial axis, then Zhang-Suen) but not checked
+    against scikit-image's own output, and on which legal schedule the
+    reference's racy kernels took (both unpinned)."""
     from notebook_jobs import pass1_counts
     nb = NOTEBOOK["GuadalupeMtns_Test1"]
     assert pass1_counts("GuadalupeMtns_Test1", gpu) == (nb["n_segments"], nb["n_subsegments"])
