@@ -594,9 +594,11 @@ int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int lddy, float* d
                       int N, int H, int W, int Cin, int Cout, int taps, void* stream, uint32_t flags);
 /* Fused: dy[p][c] = 0 where y[p][c] <= 0 (skipped when y is NULL), then
  * db[c] += sum_p dy[p][c] (skipped when db is NULL).  C a multiple of 32, at
- * most 256; buffers 16-byte aligned, strides multiples of 8. */
-int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C, void* stream,
-                          uint32_t flags);
+ * most 256; buffers 16-byte aligned, strides multiples of 8.  partials (fp32,
+ * n_partials >= C floats, e.g. 512 * 256) holds per-block column sums that a
+ * second kernel adds into db; NULL adds them by atomics from fewer blocks. */
+int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C,
+                          float* partials, int n_partials, void* stream, uint32_t flags);
 /* dy[p][c] = 0 where y[p][c] <= 0 (ReLU backward on the stored output). */
 int slm_nn_relu_bwd(const void* y, int ldy, void* dy, int lddy, int M, int C, void* stream, uint32_t flags);
 /* 2x2 max pool, stride 2; H, W are the input's (even) sizes. */

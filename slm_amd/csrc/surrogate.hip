@@ -11,8 +11,11 @@
 //                             bias and ReLU fused into the epilogue
 //   conv backward-data        the same kernel on dY with weights flipped and
 //                             transposed (k_wflip)
-//   conv backward-weight      dW = dY^T X: pixels are the reduction; tiles of
-//                             dY and im2col(X) staged transposed in LDS
+//   conv backward-weight      dW = dY^T X: pixels are the reduction; on
+//                             16..128-column images dY and X (with its halo)
+//                             are staged once per chunk of rows (k_wgrad_halo),
+//                             other shapes stage tiles of dY and im2col(X)
+//                             (k_wgrad_t)
 //   maxpool 2x2, nearest upsample 2x, ReLU mask, add, loss, Adam
 //
 // MFMA 32x32x16 operand layout (gfx950, checked on the device): lane l holds
@@ -333,6 +336,133 @@ __global__ __launch_bounds__(256) void k_wgrad_t(const bf16* __restrict__ x, int
     }
 }
 
+// Weight gradient with one LDS copy of X per pixel chunk (3x3 or 1x1).  A chunk is
+// R = 256 / W whole rows of one image (W = 16 .. 128 columns).  Staged
+// channel-major, each 8-pixel x 8-channel block read with eight 16-byte loads
+// and written transposed as eight 16-byte rows:
+//   dyT [CT][256 px]                               the A operands
+//   xh  [CIT][R + 2 rows][8 zeros | W | 8 zeros]   X with its row and column halo
+// so X leaves L2 once per chunk (plus its two halo rows) instead of once per
+// tap.  Wave (cw, dh) owns output channels [32 cw, +32) and the three taps of
+// row offset dh - 1: per 16-pixel step it reads its A operand, the aligned
+// 8 columns of its B operand and the words either side, and forms the column
+// offsets -1 and +1 by shifting bf16 pairs across neighbouring words
+// (v_alignbyte), so three MFMAs share one set of aligned LDS reads.  The
+// plane strides are 8 (mod 128) elements, so the 32 rows of an operand read
+// start on different 16-byte bank groups.  Chunks [blockIdx.z * cpb, +cpb)
+// of the N * ceil(H / R); partial sums leave by fp32 atomics.  For 1x1
+// (T = 1) there is no halo: one wave per 32 output channels, one MFMA per step.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int T, int W, int CT, int CIT>
+__global__ __launch_bounds__((T == 9 ? 192 : 64) * (CT / 32)) void k_wgrad_halo(
+    const bf16* __restrict__ x, int ldx, const bf16* __restrict__ dy, int lddy, float* __restrict__ dw, int N, int H,
+    int Cin, int cpb) {
+    constexpr int R = 256 / W;
+    constexpr int HALO = T == 9 ? 1 : 0;                // rows of halo above and below
+    constexpr int WP = W + 16;
+    constexpr int XROW = (R + 2 * HALO) * WP;
+    constexpr int XS = XROW + ((8 - XROW % 128) + 128) % 128;
+    constexpr int DS = 256 + 8;
+    constexpr int NT = (T == 9 ? 192 : 64) * (CT / 32);
+    constexpr int XG = W / 8 + 2;                       // staged 8-column groups per row
+    static_assert(T == 1 || T == 9, "taps");
+    static_assert(W % 16 == 0 && W >= 16 && W <= 256, "W");
+    static_assert(XS % 128 == 8 && DS % 128 == 8, "strides");
+    __shared__ __attribute__((aligned(16))) bf16 dyT[CT * DS];
+    __shared__ __attribute__((aligned(16))) bf16 xh[CIT * XS];
+    const int K = T * Cin;
+    const int c0 = blockIdx.x * CT, ci0 = blockIdx.y * CIT;
+    const int t = threadIdx.x, wid = t >> 6, lane = t & 63;
+    const int cw = T == 9 ? wid / 3 : wid, dh = T == 9 ? wid % 3 : 0;
+    const int r = lane & 31, half = lane >> 5;
+    const int per_image = (H + R - 1) / R;
+    const int ch_end = min(N * per_image, (int)(blockIdx.z + 1) * cpb);
+    const bool b_on = CIT == 32 || r < CIT;
+    f32x16 acc0 = {}, acc1 = {}, acc2 = {};
+    const bf16x8 zero = {};
+    for (int ch = blockIdx.z * cpb; ch < ch_end; ++ch) {
+        const int n = ch / per_image, r0 = (ch % per_image) * R;
+        for (int blk = t; blk < 32 * (CT / 8); blk += NT) {
+            const int cg = blk % (CT / 8), q = (blk / (CT / 8)) * 8;
+            const int row = q / W, col = q % W;
+            const bool in = r0 + row < H;
+            const bf16* src = dy + ((size_t)(n * H + r0 + row) * W + col) * lddy + c0 + cg * 8;
+            bf16x8 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = in ? *(const bf16x8*)(src + (size_t)i * lddy) : zero;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                bf16x8 c;
+#pragma unroll
+                for (int i = 0; i < 8; i++) c[i] = v[i][j];
+                *(bf16x8*)&dyT[(cg * 8 + j) * DS + q] = c;
+            }
+        }
+        for (int blk = t; blk < (R + 2 * HALO) * XG * (CIT / 8); blk += NT) {
+            const int cg = blk % (CIT / 8), g = (blk / (CIT / 8)) % XG, rr = blk / ((CIT / 8) * XG);
+            const int row = r0 - HALO + rr, col = (g - 1) * 8;
+            const bool in = row >= 0 && row < H && g >= 1 && g <= W / 8;
+            bf16x8 v[8];
+            if (in) {
+                const bf16* src = x + ((size_t)(n * H + row) * W + col) * ldx + ci0 + cg * 8;
+#pragma unroll
+                for (int i = 0; i < 8; i++) v[i] = *(const bf16x8*)(src + (size_t)i * ldx);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 8; i++) v[i] = zero;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                bf16x8 c;
+#pragma unroll
+                for (int i = 0; i < 8; i++) c[i] = v[i][j];
+                *(bf16x8*)&xh[(cg * 8 + j) * XS + rr * WP + g * 8] = c;
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ks = 0; ks < 16; ++ks) {
+            const int q = ks * 16 + half * 8, row = q / W, col = q % W;
+            const bf16x8 a = *(const bf16x8*)&dyT[(cw * 32 + r) * DS + q];
+            u32x4 g = {0u, 0u, 0u, 0u};
+            uint32_t prev = 0u, next = 0u;
+            if (b_on) {
+                const bf16* bp = &xh[r * XS + (row + dh) * WP + col + 8];
+                g = *(const u32x4*)bp;
+                if (T == 9) {
+                    prev = *(const uint32_t*)(bp - 2);
+                    next = *(const uint32_t*)(bp + 8);
+                }
+            }
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, g), acc1, 0, 0, 0);
+            if (T == 9) {
+                const u32x4 gm = {__builtin_amdgcn_alignbyte(g[0], prev, 2), __builtin_amdgcn_alignbyte(g[1], g[0], 2),
+                                  __builtin_amdgcn_alignbyte(g[2], g[1], 2), __builtin_amdgcn_alignbyte(g[3], g[2], 2)};
+                const u32x4 gp = {__builtin_amdgcn_alignbyte(g[1], g[0], 2), __builtin_amdgcn_alignbyte(g[2], g[1], 2),
+                                  __builtin_amdgcn_alignbyte(g[3], g[2], 2), __builtin_amdgcn_alignbyte(next, g[3], 2)};
+                acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, gm), acc0, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, gp), acc2, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (!b_on) return;
+    const int ci = ci0 + r;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int co = c0 + cw * 32 + (v >> 2) * 8 + half * 4 + (v & 3);
+        float* d = dw + (size_t)co * K + (size_t)(dh * 3) * Cin + ci;
+        if (T == 9) {
+            atomicAdd(d, acc0[v]);
+            atomicAdd(d + Cin, acc1[v]);
+            atomicAdd(d + 2 * Cin, acc2[v]);
+        } else {
+            atomicAdd(d, acc1[v]);
+        }
+    }
+}
+
 // db[co] += sum over pixels of dY[p][co].  A thread sums 8 channels (one
 // 16-byte load per pixel) over a strided set of pixels; the block's 256/(C/8)
 // pixel lanes are reduced in LDS and each channel gets one atomic per block
@@ -365,16 +495,19 @@ __global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, int 
 // ------------------------------------------------------------------ elementwise
 
 // Fused ReLU backward and bias gradient: dY *= (Y > 0) when y is given (the
-// stored post-ReLU output), then db[c] += sum over pixels of dY[p][c].  A
-// thread handles 8 channels (16-byte accesses) over strided pixels; lanes are
-// reduced in LDS, one atomic per channel per block (few blocks).
+// stored post-ReLU output), then the column sums of dY.  A thread handles 8
+// channels (16-byte accesses) over strided pixels; lanes are reduced in LDS.
+// Each block's C sums go to row blockIdx.x of partials when given (summed by
+// k_colsum), else straight into db by atomics (few blocks then: the atomics
+// of every block share C addresses).
 __global__ __launch_bounds__(256) void k_relu_bias(const bf16* __restrict__ y, int ldy, bf16* __restrict__ dy,
-                                                   int lddy, float* __restrict__ db, int M, int C) {
+                                                   int lddy, float* __restrict__ db, float* __restrict__ partials,
+                                                   int M, int C) {
     __shared__ float red[256 * 8];
     const int C8 = C / 8;                       // 4 .. 32: divides 256
     const int t = threadIdx.x, tc = t % C8, tp = t / C8, P = 256 / C8;
     float acc[8] = {};
-#pragma unroll 2
+#pragma unroll 4
     for (size_t p = (size_t)blockIdx.x * P + tp; p < (size_t)M; p += (size_t)gridDim.x * P) {
         bf16x8 g = *(const bf16x8*)(dy + p * lddy + tc * 8);
         if (y) {
@@ -396,8 +529,35 @@ __global__ __launch_bounds__(256) void k_relu_bias(const bf16* __restrict__ y, i
         for (int q = 0; q < P; q++)
 #pragma unroll
             for (int j = 0; j < 8; j++) sum[j] += red[(q * C8 + tc) * 8 + j];
+        if (partials) {
 #pragma unroll
-        for (int j = 0; j < 8; j++) atomicAdd(&db[tc * 8 + j], sum[j]);
+            for (int j = 0; j < 8; j++) partials[(size_t)blockIdx.x * C + tc * 8 + j] = sum[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) atomicAdd(&db[tc * 8 + j], sum[j]);
+        }
+    }
+}
+
+// db[c] += sum over rows r < n_rows of partials[r][c]: block b takes columns
+// [32 b, +32), its 32 row lanes stride the rows with four independent sums
+// each, reduced in LDS.
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ partials, int n_rows, int C,
+                                                 float* __restrict__ db) {
+    __shared__ float red[1024];
+    const int c = blockIdx.x * 32 + (threadIdx.x & 31), lane_r = threadIdx.x >> 5;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+    int r = lane_r;
+    for (; r + 96 < n_rows; r += 128)
+#pragma unroll
+        for (int u = 0; u < 4; u++) s[u] += partials[(size_t)(r + 32 * u) * C + c];
+    for (; r < n_rows; r += 32) s[0] += partials[(size_t)r * C + c];
+    red[threadIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float sum = 0.f;
+        for (int q = 0; q < 32; q++) sum += red[q * 32 + threadIdx.x];
+        db[c] += sum;
     }
 }
 
@@ -825,6 +985,30 @@ extern "C" int slm_nn_conv_wgrad(const void* x, int ldx, const void* dy, int ldd
         else
             hipLaunchKernelGGL(k_wgrad<1>, grid, dim3(256), 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, M, H,
                                W, Cin, Cout, ppb);
+    } else if ((W == 16 || W == 32 || W == 64 || W == 128) && (Cin % 32 == 0 || Cin == 16) &&
+               (size_t)N * H * W == (size_t)M) {
+        // LDS-staged kernel: tiles of 32 or 64 output x 16 or 32 input channels x the taps;
+        // at least 4 chunks per block (every block ends in a full tile of atomics),
+        // about 512 blocks when there is work for them
+        const int ct = Cout % 64 == 0 ? 64 : 32, cit = Cin % 32 == 0 ? 32 : 16;
+        const int R = 256 / W;
+        const int chunks = N * ((H + R - 1) / R);
+        const int tiles = (Cout / ct) * (Cin / cit);
+        int splits = std::max(1, std::min((512 + tiles - 1) / tiles, chunks / 4));
+        const int cpb = (chunks + splits - 1) / splits;
+        splits = (chunks + cpb - 1) / cpb;
+        const dim3 grid(Cout / ct, Cin / cit, splits), block((taps == 9 ? 192 : 64) * (ct / 32));
+#define WH_L(TT, WW, C, CI) hipLaunchKernelGGL((k_wgrad_halo<TT, WW, C, CI>), grid, block, 0, s, (const bf16*)x, ldx, (const bf16*)dy, lddy, dw, N, H, Cin, cpb)
+#define WH_C(TT, WW) \
+        if (ct == 64) { if (cit == 32) WH_L(TT, WW, 64, 32); else WH_L(TT, WW, 64, 16); } \
+        else { if (cit == 32) WH_L(TT, WW, 32, 32); else WH_L(TT, WW, 32, 16); }
+#define WH_W(TT) \
+        if (W == 16) { WH_C(TT, 16) } else if (W == 32) { WH_C(TT, 32) } else if (W == 64) { WH_C(TT, 64) } \
+        else { WH_C(TT, 128) }
+        if (taps == 9) { WH_W(9) } else { WH_W(1) }
+#undef WH_W
+#undef WH_C
+#undef WH_L
     } else {
         // 64 x 64 tiles, or 32 x 128 when Cout is 32; pixel splits of whole 128-pixel
         // chunks, enough to fill the chip (~1024 blocks)
@@ -963,15 +1147,24 @@ extern "C" int slm_nn_fields_log(const uint32_t* slc, const uint32_t* slt, const
     return slm_finish(stream, flags);
 }
 
-extern "C" int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C, void* stream,
-                                     uint32_t flags) {
+extern "C" int slm_nn_relu_bias_grad(const void* y, int ldy, void* dy, int lddy, float* db, int M, int C,
+                                     float* partials, int n_partials, void* stream, uint32_t flags) {
     if (!dy || M < 0 || C <= 0 || C % 32 || C > 256 || lddy % 8 || ((uintptr_t)dy) % 16 ||
-        (y && (ldy % 8 || ((uintptr_t)y) % 16)))
+        (y && (ldy % 8 || ((uintptr_t)y) % 16)) || (partials && n_partials < C))
         return slm_fail(SLM_E_INVALID_ARG, "bad relu_bias_grad arguments");
     if (!M || (!y && !db)) return slm_finish(stream, flags);
+    hipStream_t s = (hipStream_t)stream;
     const int P = 256 / (C / 8);
-    const int blocks = std::min(128, (M + P - 1) / P);
-    hipLaunchKernelGGL(k_relu_bias, dim3(blocks), dim3(256), 0, (hipStream_t)stream, (const bf16*)y, ldy, (bf16*)dy,
-                       lddy, db, M, C);
+    if (db && partials) {
+        // two blocks per CU keep enough loads in flight; their sums are reduced by k_colsum
+        const int blocks = std::min(std::min(512, n_partials / C), (M + P - 1) / P);
+        hipLaunchKernelGGL(k_relu_bias, dim3(blocks), dim3(256), 0, s, (const bf16*)y, ldy, (bf16*)dy, lddy, db,
+                           partials, M, C);
+        hipLaunchKernelGGL(k_colsum, dim3(C / 32), dim3(1024), 0, s, (const float*)partials, blocks, C, db);
+    } else {
+        const int blocks = std::min(db ? 128 : 512, (M + P - 1) / P);
+        hipLaunchKernelGGL(k_relu_bias, dim3(blocks), dim3(256), 0, s, (const bf16*)y, ldy, (bf16*)dy, lddy, db,
+                           (float*)nullptr, M, C);
+    }
     return slm_finish(stream, flags);
 }

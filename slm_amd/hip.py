@@ -57,7 +57,7 @@ _PROTOS = {
     "slm_nn_conv_up": (ctypes.c_int, [_P, _I, _P, _I, _I, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_wflip": (ctypes.c_int, [_P, _P, _I, _I, _I, _P, _U32]),
     "slm_nn_conv_wgrad": (ctypes.c_int, [_P, _I, _P, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _U32]),
-    "slm_nn_relu_bias_grad": (ctypes.c_int, [_P, _I, _P, _I, _P, _I, _I, _P, _U32]),
+    "slm_nn_relu_bias_grad": (ctypes.c_int, [_P, _I, _P, _I, _P, _I, _I, _P, _I, _P, _U32]),
     "slm_nn_relu_bwd": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _P, _U32]),
     "slm_nn_pool": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),
     "slm_nn_pool_bwd": (ctypes.c_int, [_P, _I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _P, _U32]),

@@ -68,10 +68,13 @@ def relu_bwd(y, ldy, dy, lddy, M, C):
     hip.check(hip.load().slm_nn_relu_bwd(_p(y), ldy, _p(dy), lddy, M, C, _s(), 0))
 
 
-def relu_bias_grad(y, ldy, dy, lddy, db, M, C):
-    """dy *= (y > 0) when y is given, then db += column sums of dy."""
+def relu_bias_grad(y, ldy, dy, lddy, db, M, C, partials=None):
+    """dy *= (y > 0) when y is given, then db += column sums of dy (per-block
+    sums staged in partials, fp32 with at least C elements, when given)."""
     hip.check(hip.load().slm_nn_relu_bias_grad(_p(y) if y is not None else None, ldy, _p(dy), lddy,
-                                               _p(db) if db is not None else None, M, C, _s(), 0))
+                                               _p(db) if db is not None else None, M, C,
+                                               _p(partials) if partials is not None else None,
+                                               partials.numel() if partials is not None else 0, _s(), 0))
 
 
 def pool(x, ldx, y, ldy, N, H, W, C):
@@ -157,6 +160,8 @@ class UNet:
             self.g = torch.zeros_like(self.p)
             self.m = torch.zeros_like(self.p)
             self.v = torch.zeros_like(self.p)
+            # per-block bias-gradient sums: 512 blocks x up to 256 channels
+            self.partials = torch.empty(512 * 256, dtype=torch.float32, device=self.dev)
         self.pb = self.p.to(torch.bfloat16)          # bf16 copy (biases unused in bf16)
         self.wt = {name: torch.empty(cout * taps * cin, dtype=torch.bfloat16, device=self.dev)
                    for name, cin, cout, taps, _ in LAYERS}
@@ -279,7 +284,7 @@ class UNet:
         mask applied here) -> weight/bias grads (+=) and dx (written)."""
         cin, cout, taps, relu = self._io(name)
         M = n * h * w
-        relu_bias_grad(yout if relu else None, ldy, dy, lddy, self.gb(name), M, cout)
+        relu_bias_grad(yout if relu else None, ldy, dy, lddy, self.gb(name), M, cout, self.partials)
         wgrad(xin, ldx, dy, lddy, self.gw(name), None, n, h, w, cin, cout, taps)
         if need_dx:
             wflip(self.w(name), self.wt[name], taps, cin, cout)

@@ -69,16 +69,21 @@ def test_conv_forward(gpu, taps, relu, cin, cout, ld_extra, H, W):
     _close_bf16(y2.float().cpu().numpy(), y.float().cpu().numpy(), frac=0.98)
 
 
-@pytest.mark.parametrize("taps,cin,cout", [(9, 16, 32), (9, 64, 128), (1, 32, 32), (9, 96, 32), (9, 384, 128),
-                                           (9, 128, 256), (1, 32, 64)])
-def test_conv_backward(gpu, taps, cin, cout):
+# (N, H, W): 16-, 32-, 64- and 128-column images run the LDS-staged
+# weight-gradient kernel (chunks of 256 / W rows, the last one partial when
+# H is not a multiple of them); 48-column images the im2col-tile kernel
+@pytest.mark.parametrize("taps,cin,cout,N,H,W", [
+    (9, 16, 32, 2, 16, 16), (9, 64, 128, 2, 16, 16), (1, 32, 32, 2, 16, 16), (9, 96, 32, 2, 16, 16),
+    (9, 384, 128, 2, 16, 16), (9, 128, 256, 2, 16, 16), (1, 32, 64, 2, 16, 16), (9, 16, 32, 2, 5, 128),
+    (9, 32, 32, 1, 7, 128), (9, 64, 64, 2, 7, 64), (9, 96, 32, 3, 9, 32), (9, 192, 64, 1, 6, 64),
+    (9, 16, 64, 2, 3, 64), (9, 32, 32, 2, 4, 48), (1, 32, 32, 1, 5, 128), (1, 64, 32, 2, 4, 48)])
+def test_conv_backward(gpu, taps, cin, cout, N, H, W):
     """Backward-weight (fp32) and backward-data (flipped weights through the
     forward kernel) against torch autograd on the CPU."""
     import torch
     import torch.nn.functional as F
     from slm_amd import surrogate as S
-    rs = np.random.default_rng(7 + cin)
-    N, H, W = 2, 16, 16
+    rs = np.random.default_rng(7 + cin + 1000 * W + 10 * H)
     x = _bf(rs.standard_normal((N, H, W, cin)))
     w = _bf(rs.standard_normal((cout, taps, cin)) / np.sqrt(cin * taps))
     dy = _bf(rs.standard_normal((N, H, W, cout)))
@@ -114,30 +119,35 @@ def test_conv_backward(gpu, taps, cin, cout):
         _close_bf16(dx.float().cpu().numpy(), dx_ref, frac=0.97)
 
 
-@pytest.mark.parametrize("C,ld", [(32, 32), (64, 96), (256, 256)])
-def test_relu_bias_grad(gpu, C, ld):
+@pytest.mark.parametrize("C,ld,M,staged", [(32, 32, 3 * 1024 + 32, False), (64, 96, 3 * 1024 + 32, False),
+                                            (256, 256, 3 * 1024 + 32, False), (32, 32, 3 * 1024 + 32, True),
+                                            (64, 96, 40000, True), (256, 256, 9000, True), (32, 96, 300000, True)])
+def test_relu_bias_grad(gpu, C, ld, M, staged):
+    """ReLU mask and column sums; staged: per-block sums in a partials buffer
+    reduced by a second kernel (up to 512 blocks), else atomics."""
     import torch
     from slm_amd import surrogate as S
-    rs = np.random.default_rng(C)
-    M = 3 * 1024 + 32
+    rs = np.random.default_rng(C + M)
     y = _bf(rs.standard_normal((M, ld))).to(gpu)
     dy = _bf(rs.standard_normal((M, ld))).to(gpu)
     dy0 = dy.float().cpu().numpy()
-    db = torch.zeros(C, dtype=torch.float32, device=gpu)
-    S.relu_bias_grad(y, ld, dy, ld, db, M, C)
+    db = torch.full((C,), 0.5, dtype=torch.float32, device=gpu)         # sums are added to what db holds
+    partials = torch.full((512 * 256,), float("nan"), dtype=torch.float32, device=gpu) if staged else None
+    S.relu_bias_grad(y, ld, dy, ld, db, M, C, partials)
     torch.cuda.synchronize()
     yn = y.float().cpu().numpy()
     want = np.where(yn[:, :C] > 0, dy0[:, :C], 0.0)
     got = dy.float().cpu().numpy()
     assert np.array_equal(got[:, :C], want)
     assert np.array_equal(got[:, C:], dy0[:, C:])          # channels beyond C untouched
-    assert np.allclose(db.cpu().numpy(), want.sum(0), rtol=1e-5, atol=1e-3)
+    tol = 1e-5 * np.abs(want).sum(0) + 1e-3
+    assert np.all(np.abs(db.cpu().numpy() - (0.5 + want.astype(np.float64).sum(0))) <= tol)
     # bias only (no ReLU): dy unchanged
     db.zero_()
-    S.relu_bias_grad(None, 0, dy, ld, db, M, C)
+    S.relu_bias_grad(None, 0, dy, ld, db, M, C, partials)
     torch.cuda.synchronize()
     assert np.array_equal(dy.float().cpu().numpy(), got)
-    assert np.allclose(db.cpu().numpy(), got[:, :C].sum(0), rtol=1e-5, atol=1e-3)
+    assert np.all(np.abs(db.cpu().numpy() - got[:, :C].astype(np.float64).sum(0)) <= tol)
 
 
 def test_pool_up_add_loss_adam(gpu):
