@@ -51,6 +51,8 @@ extern "C" {
                                       (test hook; both compute the same products) */
 #define SLM_FLAG_FORCE_FALLBACK 8u /* test hook: reject the packed 64-bit accumulation so the
                                       32-bit fallback path runs (results are identical) */
+#define SLM_FLAG_FMM_LANES 32u    /* test hook: slm_fmm_hsl (second order) marches every
+                                      subsegment one lane each, with no LDS heaps (same bits) */
 
 /*
  * Every kernel constant of the reference's -D macro set (pocl.py:225-297, values

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
+#include <mutex>
 #include "../../include/slm_hip.h"
 #include "common.hpp"
 
@@ -43,8 +44,10 @@ constexpr double kFix = 16777216.0;   // 2^24
 struct FmmLayout {
     uint8_t* base;
     size_t stride, off_neg, off_u, off_fixed, off_act0, off_act1;
-    // second-order march: per-label point lists, heap positions and heaps
-    size_t off_lcnt, off_lcur, off_pts, off_pos, off_heap;
+    // second-order march: per-label point lists, heap positions and heaps;
+    // the lists of subsegments marched with LDS heaps (counters, then two
+    // lists of big_cap keys)
+    size_t off_lcnt, off_lcur, off_pts, off_pos, off_heap, off_big, big_cap;
 };
 __device__ __forceinline__ uint8_t* slot_base(const FmmLayout& L) { return L.base + blockIdx.y * L.stride; }
 
@@ -396,26 +399,47 @@ __global__ void k_fmm_lscatter(uint32_t nx, uint32_t ny, FmmLayout L) {
 }
 
 // The second-order update of point (x, y) of problem key (state >= 2 frozen).
+// Every value it may read (the first and second neighbours along each axis,
+// both sides) is loaded first, in one round of independent loads; the
+// arithmetic then follows the march's rules: per axis the smaller first
+// neighbour, the second neighbour only behind it and not larger, its sign
+// flipped across the contour, and v2 kept from the first side when the
+// second side improves v1 alone.
 __device__ double fmm_update2(const uint32_t* __restrict__ dom, const uint8_t* __restrict__ neg,
                               const uint8_t* __restrict__ state, const double* __restrict__ u, uint32_t key, int x,
                               int y, int nx, int ny, double dx, bool* bad) {
     const double idx2 = 1.0 / (dx * dx), aa = 9.0 / 4.0, third = 1.0 / 3.0;
     const size_t i = (size_t)x * ny + y;
+    bool ok1[4], ok2[4], flip[4];
+    double u1[4], u2[4];
+    const uint8_t ni = neg[i];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int axis = q >> 1, sd = (q & 1) ? 1 : -1;
+        const int x1 = axis == 0 ? x + sd : x, y1 = axis == 1 ? y + sd : y;
+        const int x2 = axis == 0 ? x + 2 * sd : x, y2 = axis == 1 ? y + 2 * sd : y;
+        const bool in1 = x1 >= 0 && y1 >= 0 && x1 < nx && y1 < ny;
+        const bool in2 = x2 >= 0 && y2 >= 0 && x2 < nx && y2 < ny;
+        const size_t n1 = in1 ? (size_t)x1 * ny + y1 : i, n2 = in2 ? (size_t)x2 * ny + y2 : i;
+        const uint32_t d1 = dom[n1], d2 = dom[n2];
+        const uint8_t s1 = state[n1], s2 = state[n2], e1 = neg[n1], e2 = neg[n2];
+        u1[q] = u[n1];
+        u2[q] = u[n2];
+        ok1[q] = in1 && d1 == key && s1 >= 2;
+        ok2[q] = in2 && d2 == key && s2 >= 2;
+        flip[q] = e2 != e1 || e2 != ni;
+    }
     double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
     for (int axis = 0; axis < 2; axis++) {
         double v1 = INFINITY, v2 = INFINITY;
-        for (int s = -1; s <= 1; s += 2) {
-            const int xx = axis == 0 ? x + s : x, yy = axis == 1 ? y + s : y;
-            if (xx < 0 || yy < 0 || xx >= nx || yy >= ny) continue;
-            const size_t n = (size_t)xx * ny + yy;
-            if (dom[n] != key || state[n] < 2 || !(u[n] < v1)) continue;
-            v1 = u[n];
-            const int x2 = axis == 0 ? x + 2 * s : x, y2 = axis == 1 ? y + 2 * s : y;
-            if (x2 < 0 || y2 < 0 || x2 >= nx || y2 >= ny) continue;
-            const size_t n2 = (size_t)x2 * ny + y2;
-            if (dom[n2] != key || state[n2] < 2 || !(u[n2] <= v1)) continue;
-            v2 = u[n2];
-            if (neg[n2] != neg[n] || neg[n2] != neg[i]) v2 = -v2;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int q = axis * 2 + side;
+            if (!ok1[q] || !(u1[q] < v1)) continue;
+            v1 = u1[q];
+            if (!ok2[q] || !(u2[q] <= v1)) continue;
+            v2 = flip[q] ? -u2[q] : u2[q];
         }
         if (v2 < INFINITY) {
             const double tp = third * (4.0 * v1 - v2);
@@ -438,9 +462,10 @@ __device__ double fmm_update2(const uint32_t* __restrict__ dom, const uint8_t* _
 }
 
 // One lane per label of the slot's colour: the fast march of that subsegment.
+// Subsegments with lds_lo < m <= lds_hi points are left to k_fmm_march_lds.
 __global__ __launch_bounds__(64) void k_fmm_march(uint32_t nx, uint32_t ny, double dx, FmmLayout L,
                                                   const uint8_t* __restrict__ colour, uint32_t n_labels, uint32_t c0,
-                                                  uint8_t* __restrict__ failed) {
+                                                  uint8_t* __restrict__ failed, uint32_t lds_lo, uint32_t lds_hi) {
     const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key == 0 || key >= n_labels || colour[key] != c0 + blockIdx.y) return;
     uint8_t* sb = slot_base(L);
@@ -450,7 +475,7 @@ __global__ __launch_bounds__(64) void k_fmm_march(uint32_t nx, uint32_t ny, doub
     uint8_t* __restrict__ state = sb + L.off_fixed;    // k_fmm_init: 1 next to the contour
     const uint32_t* __restrict__ lst = (const uint32_t*)(sb + L.off_lcnt);
     const uint32_t beg = lst[key], end = lst[key + 1];
-    if (beg == end) return;
+    if (beg == end || (end - beg > lds_lo && end - beg <= lds_hi)) return;
     const uint32_t* __restrict__ pts = (const uint32_t*)(sb + L.off_pts) + beg;
     int32_t* __restrict__ pos = (int32_t*)(sb + L.off_pos);
     HeapEnt* __restrict__ h = (HeapEnt*)(sb + L.off_heap) + beg;
@@ -504,6 +529,398 @@ __global__ __launch_bounds__(64) void k_fmm_march(uint32_t nx, uint32_t ny, doub
         }
     }
     if (bad) failed[key] = 1;
+}
+
+// The subsegments of each slot's colour with lds_lo < m <= cap_a points go to
+// the slot's list 0, those with cap_a < m <= cap_b to its list 1 (counters
+// big[0] and big[2]).
+__global__ void k_fmm_classify(FmmLayout L, const uint8_t* __restrict__ colour, uint32_t n_labels, uint32_t c0,
+                               uint32_t lds_lo, uint32_t cap_a, uint32_t cap_b) {
+    const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key == 0 || key >= n_labels || colour[key] != c0 + blockIdx.y) return;
+    uint8_t* sb = slot_base(L);
+    const uint32_t* __restrict__ lst = (const uint32_t*)(sb + L.off_lcnt);
+    const uint32_t m = lst[key + 1] - lst[key];
+    if (m <= lds_lo || m > cap_b) return;
+    uint32_t* big = (uint32_t*)(sb + L.off_big);
+    const uint32_t cls = m > cap_a ? 1u : 0u;
+    const uint32_t at = atomicAdd(&big[2 * cls], 1u);
+    big[4 + cls * L.big_cap + at] = key;
+}
+
+// The march of a subsegment held whole in LDS by one 64-lane wave.  Its
+// points are sorted by pixel index, so local index order is pixel order and
+// the lane kernel's heap order (value, pixel index) is (value, local index).
+// Per point: u (double), flags (bits 0-1 the lane kernel's state, bit 2 the
+// channel side, bits 3-4 whether the points before and after in the row,
+// local index -1 and +1, belong to the subsegment) and the local indices of
+// the pixels one row before and after (0xFFFF: not in the subsegment).  The
+// trial front is split by local index mod 64: lane L lists its own points
+// that are in the front (at most ceil(cap / 64)) and keeps their minimum, so
+// the next point to accept is a reduction over the 64 lane minima, and only
+// the lane that owned it, or whose minimum grew, scans its list again
+// (dropping accepted points).
+constexpr uint32_t kFmmNone = 0xFFFFu;
+constexpr uint8_t kFmmNeg = 4, kFmmYm = 8, kFmmYp = 16;
+
+struct FmmLds {
+    double* u;
+    uint16_t* xn;    // [2 * cap]: row before, row after
+    uint16_t* list;  // [64][per_lane]
+    uint8_t* f;
+    uint32_t per_lane;
+};
+
+__host__ __device__ constexpr uint32_t fmm_lds_per_lane(uint32_t cap) { return (cap + 63) / 64; }
+__host__ __device__ constexpr size_t fmm_lds_bytes(uint32_t cap) {
+    return ((size_t)cap * 13 + (size_t)64 * fmm_lds_per_lane(cap) * 2 + 15) & ~(size_t)15;
+}
+
+__device__ __forceinline__ bool fless(double va, uint32_t la, double vb, uint32_t lb) {
+    return va < vb || (va == vb && la < lb);
+}
+
+// fmm_update2 on the LDS copy: the same values, by local index, read in three
+// rounds of independent LDS loads (the point; its first neighbours; their
+// neighbours beyond), then the same arithmetic.
+__device__ double fmm_update2_lds(const FmmLds& S, uint32_t l, double dx, bool* bad) {
+    const double idx2 = 1.0 / (dx * dx), aa = 9.0 / 4.0, third = 1.0 / 3.0;
+    const uint8_t fl = S.f[l];
+    const uint16_t xa = S.xn[2 * l], xb = S.xn[2 * l + 1];
+    const bool nl = (fl & kFmmNeg) != 0;
+    uint32_t n1[4], n2[4];
+    n1[0] = xa;
+    n1[1] = xb;
+    n1[2] = (fl & kFmmYm) ? l - 1 : kFmmNone;
+    n1[3] = (fl & kFmmYp) ? l + 1 : kFmmNone;
+    uint8_t f1[4], f2[4];
+    double u1[4], u2[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t a1 = n1[q] != kFmmNone ? n1[q] : l;
+        f1[q] = S.f[a1];
+        u1[q] = S.u[a1];
+        if (q < 2) n2[q] = S.xn[2 * a1 + q];
+    }
+    if (n1[0] == kFmmNone) n2[0] = kFmmNone;
+    if (n1[1] == kFmmNone) n2[1] = kFmmNone;
+    n2[2] = (n1[2] != kFmmNone && (f1[2] & kFmmYm)) ? l - 2 : kFmmNone;
+    n2[3] = (n1[3] != kFmmNone && (f1[3] & kFmmYp)) ? l + 2 : kFmmNone;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t a2 = n2[q] != kFmmNone ? n2[q] : l;
+        f2[q] = S.f[a2];
+        u2[q] = S.u[a2];
+    }
+    double a = 0.0, b = 0.0, c = 0.0;
+#pragma unroll
+    for (int axis = 0; axis < 2; axis++) {
+        double v1 = INFINITY, v2 = INFINITY;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int q = axis * 2 + side;
+            if (n1[q] == kFmmNone || (f1[q] & 3) < 2 || !(u1[q] < v1)) continue;
+            v1 = u1[q];
+            if (n2[q] == kFmmNone || (f2[q] & 3) < 2 || !(u2[q] <= v1)) continue;
+            const bool e1 = (f1[q] & kFmmNeg) != 0, e2 = (f2[q] & kFmmNeg) != 0;
+            v2 = (e2 != e1 || e2 != nl) ? -u2[q] : u2[q];
+        }
+        if (v2 < INFINITY) {
+            const double tp = third * (4.0 * v1 - v2);
+            a += idx2 * aa;
+            b -= idx2 * 2.0 * aa * tp;
+            c += idx2 * aa * tp * tp;
+        } else if (v1 < INFINITY) {
+            a += idx2;
+            b -= idx2 * 2.0 * v1;
+            c += idx2 * v1 * v1;
+        }
+    }
+    c -= 1.0;
+    const double det = b * b - 4.0 * a * c;
+    if (det < 0.0) {
+        *bad = true;
+        return INFINITY;
+    }
+    return (-b + sqrt(det)) / 2.0 / a;
+}
+
+// The block is one wave: its LDS accesses complete in order, so a fence at
+// wavefront scope (which keeps the compiler from moving them) orders one
+// lane's writes before other lanes' later reads.
+__device__ __forceinline__ void fmm_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One step of a min-reduction of (value, local index): each lane takes the
+// smaller of its own and the one CTRL (a DPP pattern) brings it.
+template <int CTRL>
+__device__ __forceinline__ void fmm_dpp_min(double& v, uint32_t& l) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)bits, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(bits >> 32), CTRL, 0xF, 0xF, false);
+    const uint32_t ol = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)l, CTRL, 0xF, 0xF, false);
+    const double ov = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    if (fless(ov, ol, v, l)) {
+        v = ov;
+        l = ol;
+    }
+}
+
+__device__ __forceinline__ double fmm_readlane(double v, int lane) {
+    const uint64_t bits = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bits, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(bits >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// The minimum (value, local index) over the wave, in every lane (all 64 lanes
+// active); empty lanes hold (inf, ~0).  Within each row of 16 lanes by DPP
+// (quad swaps, then rotations by 4 and 8), then across the four rows.
+__device__ __forceinline__ void fmm_wave_min(double& bv, uint32_t& bl) {
+    fmm_dpp_min<0xB1>(bv, bl);      // quad_perm [1, 0, 3, 2]
+    fmm_dpp_min<0x4E>(bv, bl);      // quad_perm [2, 3, 0, 1]
+    fmm_dpp_min<0x124>(bv, bl);     // row_ror:4
+    fmm_dpp_min<0x128>(bv, bl);     // row_ror:8
+    double v = fmm_readlane(bv, 0);
+    uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)bl, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const double ov = fmm_readlane(bv, r);
+        const uint32_t ol = (uint32_t)__builtin_amdgcn_readlane((int)bl, r);
+        if (fless(ov, ol, v, l)) {
+            v = ov;
+            l = ol;
+        }
+    }
+    bv = v;
+    bl = l;
+}
+
+// The whole wave scans the front list of lane o, 64 entries at a time:
+// accepted points are dropped (the list compacted in place) and lane o
+// receives the minimum of the rest and the new count.
+__device__ void fmm_wave_scan(const FmmLds& S, uint32_t lane, uint32_t o, uint32_t& cnt, double& mv, uint32_t& ml) {
+    const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)o);
+    uint16_t* lst = S.list + o * S.per_lane;
+    double bv = INFINITY;
+    uint32_t bl = 0xFFFFFFFFu, w = 0;
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        bool live = false;
+        uint16_t e = 0;
+        if (j < n) {
+            e = lst[j];
+            live = (S.f[e] & 3) != 3;
+            if (live) {
+                const double v = S.u[e];
+                if (fless(v, e, bv, bl)) {
+                    bv = v;
+                    bl = e;
+                }
+            }
+        }
+        const uint64_t bal = __ballot(live);
+        if (live) lst[w + __popcll(bal & ((1ull << lane) - 1ull))] = e;
+        w += __popcll(bal);
+    }
+    fmm_wave_min(bv, bl);
+    if (lane == o) {
+        mv = bv;
+        ml = bl;
+        cnt = w;
+    }
+}
+
+// Lane's scan of its front list: drops accepted points, returns the minimum.
+__device__ void fmm_lane_scan(const FmmLds& S, uint32_t lane, uint32_t& cnt, double& mv, uint32_t& ml) {
+    uint16_t* lst = S.list + lane * S.per_lane;
+    mv = INFINITY;
+    ml = 0xFFFFFFFFu;
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < cnt; j++) {
+        const uint16_t e = lst[j];
+        if ((S.f[e] & 3) == 3) continue;
+        lst[w++] = e;
+        const double v = S.u[e];
+        if (fless(v, e, mv, ml)) {
+            mv = v;
+            ml = e;
+        }
+    }
+    cnt = w;
+}
+
+// The march of k_fmm_march for the subsegments of list cls of the k slots,
+// one 64-lane block each (blocks take the next entry of the slots' lists,
+// counted by next[cls], until they are done), with the subsegment held in LDS
+// (fmm_lds_bytes(cap) bytes, at most cap points).  Set-up: the point list is
+// sorted (bitonic, in LDS) and written back sorted, pos maps each pixel to
+// its local index, and the neighbour indices, flags and u are gathered.  Per
+// accepted point, lanes 0-3 compute the four neighbour updates at once (an
+// update reads only frozen values, and the updates of one point freeze
+// nothing); the results are applied in the lane kernel's neighbour order up
+// to a negative discriminant, as it stops there, each by its point's owner
+// lane.  Global memory is only read until the march
+// ends; then u is written back.
+__global__ __launch_bounds__(64) void k_fmm_march_lds(uint32_t nx, uint32_t ny, double dx, FmmLayout L, uint32_t k_slots,
+                                                      uint32_t* __restrict__ next, uint8_t* __restrict__ failed,
+                                                      uint32_t cls, uint32_t cap) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_fmm[];
+    __shared__ uint32_t cur;
+    FmmLds S;
+    S.u = (double*)lds_fmm;
+    S.xn = (uint16_t*)(S.u + cap);
+    S.per_lane = fmm_lds_per_lane(cap);
+    S.list = S.xn + 2 * cap;
+    S.f = (uint8_t*)(S.list + 64 * S.per_lane);
+    uint32_t* scratch = (uint32_t*)lds_fmm;          // 2 * cap entries, before u is filled
+    const uint32_t lane = threadIdx.x;
+    for (;;) {
+        if (lane == 0) cur = atomicAdd(&next[cls], 1u);
+        __syncthreads();
+        uint32_t e = cur;
+        __syncthreads();
+        // the entry's slot: the slots' lists follow one another
+        uint32_t g = 0;
+        for (; g < k_slots; g++) {
+            const uint32_t c = ((const uint32_t*)(L.base + g * L.stride + L.off_big))[2 * cls];
+            if (e < c) break;
+            e -= c;
+        }
+        if (g == k_slots) break;
+        uint8_t* sb = L.base + g * L.stride;
+        const uint32_t* __restrict__ dom = (const uint32_t*)sb;
+        const uint8_t* __restrict__ neg = sb + L.off_neg;
+        double* __restrict__ u = (double*)(sb + L.off_u);
+        const uint8_t* __restrict__ state = sb + L.off_fixed;
+        const uint32_t* __restrict__ lst = (const uint32_t*)(sb + L.off_lcnt);
+        int32_t* __restrict__ pos = (int32_t*)(sb + L.off_pos);
+        const uint32_t key = ((const uint32_t*)(sb + L.off_big))[4 + cls * L.big_cap + e];
+        uint32_t* __restrict__ pts = (uint32_t*)(sb + L.off_pts) + lst[key];
+        const uint32_t m = lst[key + 1] - lst[key];
+        // sort the points by pixel index
+        uint32_t p2 = 1;
+        while (p2 < m) p2 <<= 1;
+        for (uint32_t k = lane; k < p2; k += 64) scratch[k] = k < m ? pts[k] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t kk = 2; kk <= p2; kk <<= 1) {
+            for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+                for (uint32_t i = lane; i < p2; i += 64) {
+                    const uint32_t ixj = i ^ jj;
+                    if (ixj > i) {
+                        const uint32_t x0 = scratch[i], x1 = scratch[ixj];
+                        if (((i & kk) == 0) == (x0 > x1)) {
+                            scratch[i] = x1;
+                            scratch[ixj] = x0;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t k = lane; k < m; k += 64) {
+            const uint32_t p = scratch[k];
+            pts[k] = p;
+            pos[p] = (int32_t)k;
+        }
+        __syncthreads();
+        // neighbours, flags, u; each lane lists its own frozen points
+        uint32_t cnt = 0;
+        for (uint32_t k = lane; k < m; k += 64) {
+            const uint32_t p = pts[k];
+            const uint32_t x = p / ny, y = p % ny;
+            uint16_t up = kFmmNone, dn = kFmmNone;
+            if (x > 0 && dom[p - ny] == key) up = (uint16_t)pos[p - ny];
+            if (x + 1 < nx && dom[p + ny] == key) dn = (uint16_t)pos[p + ny];
+            uint8_t fl = neg[p] ? kFmmNeg : 0;
+            if (y > 0 && dom[p - 1] == key) fl |= kFmmYm;
+            if (y + 1 < ny && dom[p + 1] == key) fl |= kFmmYp;
+            if (state[p]) {
+                fl |= 2;
+                S.list[lane * S.per_lane + cnt++] = (uint16_t)k;
+            }
+            S.u[k] = u[p];
+            S.xn[2 * k] = up;
+            S.xn[2 * k + 1] = dn;
+            S.f[k] = fl;
+        }
+        __syncthreads();
+        double mv;
+        uint32_t ml;
+        fmm_lane_scan(S, lane, cnt, mv, ml);
+        bool bad = false, rescan = false;
+        for (;;) {
+            // the next point: the smallest (value, local index) of the lane minima
+            double bv = mv;
+            uint32_t bl = ml;
+            fmm_wave_min(bv, bl);
+            if (bad || bl == 0xFFFFFFFFu) break;
+            const uint32_t i = bl;
+            if (lane == 0) S.f[i] = (uint8_t)(S.f[i] | 3u);
+            rescan = (i & 63u) == lane;              // its owner scans again (dropping it)
+            fmm_wave_sync();
+            int32_t flag = 0;
+            double v = 0.0;
+            uint32_t l = kFmmNone;
+            if (lane < 4) {
+                const uint8_t fi = S.f[i];
+                if (lane < 2) l = S.xn[2 * i + lane];
+                else if (lane == 2 && (fi & kFmmYm)) l = i - 1;
+                else if (lane == 3 && (fi & kFmmYp)) l = i + 1;
+                if (l != kFmmNone && (S.f[l] & 3) < 2) {
+                    bool bd = false;
+                    v = fmm_update2_lds(S, l, dx, &bd);
+                    flag = bd ? 2 : (v == 0.0 ? 0 : 1);     // scikit-fmm skips a zero update
+                }
+            }
+            // the results of lanes 0-3 in neighbour order, up to the first negative
+            // discriminant; the four points are distinct, so each owner lane applies
+            // its own (reading its point's value and state before it writes them)
+            int32_t fk[4];
+            uint32_t lk[4];
+            double vk[4];
+            int kstop = 4;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                fk[k] = __builtin_amdgcn_readlane(flag, k);
+                lk[k] = (uint32_t)__builtin_amdgcn_readlane((int)l, k);
+                vk[k] = fmm_readlane(v, k);
+                if (fk[k] == 2 && kstop == 4) kstop = k;
+            }
+            if (kstop < 4) bad = true;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k >= kstop || fk[k] != 1 || (lk[k] & 63u) != lane) continue;
+                const uint32_t q = lk[k];
+                const double old = S.u[q];
+                const uint8_t fq = S.f[q];
+                const bool in_front = (fq & 3) == 1;
+                S.u[q] = vk[k];
+                S.f[q] = (uint8_t)((fq & ~3u) | 1u);
+                if (!in_front) S.list[lane * S.per_lane + cnt++] = (uint16_t)q;
+                if (!rescan) {
+                    if (fless(vk[k], q, mv, ml)) {
+                        mv = vk[k];
+                        ml = q;
+                    } else if (ml == q && vk[k] > old) {
+                        rescan = true;               // its minimum grew
+                    }
+                }
+            }
+            fmm_wave_sync();
+            for (uint64_t want = __ballot(rescan); want; want &= want - 1) {
+                fmm_wave_scan(S, lane, (uint32_t)__ffsll((unsigned long long)want) - 1u, cnt, mv, ml);
+                fmm_wave_sync();
+            }
+            rescan = false;
+        }
+        for (uint32_t k = lane; k < m; k += 64) u[pts[k]] = S.u[k];
+        if (lane == 0 && bad) failed[key] = 1;
+        __syncthreads();
+    }
 }
 
 // Per problem: fixed-point sum of the signed distances of the points a
@@ -841,6 +1258,15 @@ extern "C" int slm_fmm_pairs(const int32_t* label, uint32_t nx, uint32_t ny, con
 }
 
 namespace {
+// subsegments marched with LDS heaps have more than kLdsLo points; those of
+// at most kCapA points are marched from 32 KiB of LDS, larger ones from the
+// device's LDS per block
+constexpr uint32_t kLdsLo = 512;
+constexpr uint32_t kCapA = 2048;
+
+// capacity of each LDS-march list of a slot: its subsegments are disjoint
+size_t fmm_big_cap(uint32_t nx, uint32_t ny) { return (size_t)nx * ny / kLdsLo + 2; }
+
 // bytes of one colour slot, and the slot layout
 size_t fmm_slot_bytes(uint32_t nx, uint32_t ny) {
     const size_t n = (size_t)nx * ny;
@@ -849,7 +1275,48 @@ size_t fmm_slot_bytes(uint32_t nx, uint32_t ny) {
     // second-order march, label counts and cursors (labels < n + 1), point
     // lists, heap positions and heaps
     return al256(n * 4) + al256(n) + al256(n * 8) + al256(n) + 2 * al256(n_tiles) + 2 * al256((n + 2) * 4) +
-           al256(n * 4) + al256(n * 4) + al256(n * sizeof(HeapEnt));
+           al256(n * 4) + al256(n * 4) + al256(n * sizeof(HeapEnt)) + al256(16 + 8 * fmm_big_cap(nx, ny));
+}
+
+// A second stream per device (default priority) with fork and join events.
+struct SideStream {
+    hipStream_t s;
+    hipEvent_t fork, join;
+};
+
+int fmm_side_stream(SideStream** out) {
+    static std::mutex mu;
+    static SideStream side[64];
+    static bool made[64];
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return slm_fail(SLM_E_INVALID_ARG, "device %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!made[dev]) {
+        HIP_TRY(hipStreamCreateWithFlags(&side[dev].s, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&side[dev].fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&side[dev].join, hipEventDisableTiming));
+        made[dev] = true;
+    }
+    *out = &side[dev];
+    return SLM_OK;
+}
+
+// Largest subsegment marched in LDS: fmm_lds_bytes(cap) within the device's
+// LDS per block (less 512 bytes for the kernel's static LDS), local indices
+// below 0xFFFF.
+uint32_t fmm_lds_cap() {
+    static uint32_t cap[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (!cap[dev]) {
+        int bytes = 0;
+        if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) bytes = 65536;
+        uint32_t c = (uint32_t)std::min<long>(65534, std::max<long>(0, ((long)bytes - 512) / 13));
+        while (c && fmm_lds_bytes(c) + 512 > (size_t)bytes) c--;
+        cap[dev] = c;
+    }
+    return cap[dev];
 }
 }  // namespace
 
@@ -891,6 +1358,8 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
     L.off_pts = L.off_lcur + al256((n + 2) * 4);
     L.off_pos = L.off_pts + al256(n * 4);
     L.off_heap = L.off_pos + al256(n * 4);
+    L.off_big = L.off_heap + al256(n * sizeof(HeapEnt));
+    L.big_cap = fmm_big_cap(nx, ny);
     hipStream_t s = (hipStream_t)stream;
     zero(sums, (size_t)n_labels * 8, s);
     zero(cnt, (size_t)n_labels * 4, s);
@@ -910,8 +1379,46 @@ extern "C" int slm_fmm_hsl(const int32_t* label, const uint32_t* mapping, uint32
             hipLaunchKernelGGL(k_fmm_lcount, g, dim3(kBlock), 0, s, nx, ny, L);
             hipLaunchKernelGGL(k_fmm_lscan, dim3(1, k), dim3(1024), 0, s, n_labels + 1, L);
             hipLaunchKernelGGL(k_fmm_lscatter, g, dim3(kBlock), 0, s, nx, ny, L);
-            hipLaunchKernelGGL(k_fmm_march, dim3((n_labels + 63) / 64, k), dim3(64), 0, s, nx, ny, dx, L, colour,
-                               n_labels, c0, failed);
+            // subsegments of more than kLdsLo points march in LDS (test hook: all one lane each)
+            const uint32_t cap_b = (flags & SLM_FLAG_FMM_LANES) ? 0u : fmm_lds_cap();
+            const uint32_t lo = cap_b > kLdsLo ? kLdsLo : 0u, hi = cap_b > kLdsLo ? cap_b : 0u;
+            if (!hi)
+                hipLaunchKernelGGL(k_fmm_march, dim3((n_labels + 63) / 64, k), dim3(64), 0, s, nx, ny, dx, L, colour,
+                                   n_labels, c0, failed, lo, hi);
+            if (hi) {
+                uint32_t* next = ctr + 1;              // two queue cursors beside the relaxation counter
+                zero(next, 2 * sizeof(uint32_t), s);
+                for (uint32_t j = 0; j < k; j++) zero(L.base + j * slot + L.off_big, 16, s);
+                const uint32_t cap_a = std::min(kCapA, hi);
+                hipLaunchKernelGGL(k_fmm_classify, dim3((n_labels + 255) / 256, k), dim3(256), 0, s, L, colour,
+                                   n_labels, c0, lo, cap_a, hi);
+                const size_t lds_a = fmm_lds_bytes(cap_a), lds_b = fmm_lds_bytes(hi);
+                static bool attr_set = false;
+                if (!attr_set) {
+                    HIP_TRY(hipFuncSetAttribute((const void*)k_fmm_march_lds,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b));
+                    attr_set = true;
+                }
+                // The largest subsegments set the pace (one wave each, one per CU for
+                // their LDS): they start first on the caller's stream, while the small
+                // LDS marches and the lane kernel run beside them on a second stream.
+                SideStream* side = nullptr;
+                int rc = fmm_side_stream(&side);
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(side->fork, s));
+                HIP_TRY(hipStreamWaitEvent(side->s, side->fork, 0));
+                if (hi > cap_a)
+                    hipLaunchKernelGGL(k_fmm_march_lds, dim3(256), dim3(64), lds_b, s, nx, ny, dx, L, k, next,
+                                       failed, 1u, hi);
+                // persistent blocks: several per CU for the small subsegments
+                hipLaunchKernelGGL(k_fmm_march_lds, dim3(1280), dim3(64), lds_a, side->s, nx, ny, dx, L, k, next,
+                                   failed, 0u, cap_a);
+                hipLaunchKernelGGL(k_fmm_march, dim3((n_labels + 63) / 64, k), dim3(64), 0, side->s, nx, ny, dx, L,
+                                   colour, n_labels, c0, failed, lo, hi);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(side->join, side->s));
+                HIP_TRY(hipStreamWaitEvent(s, side->join, 0));
+            }
             hipLaunchKernelGGL(k_fmm_reduce, g, dim3(kBlock), 0, s, L, label, nx, ny, (u64*)sums, cnt, nch, nphi);
             continue;
         }

@@ -19,6 +19,7 @@ SLM_FLAG_SYNC = 1
 SLM_FLAG_NO_ZERO = 2
 SLM_FLAG_KEEP_ORDER = 4
 SLM_FLAG_FORCE_FALLBACK = 8
+SLM_FLAG_FMM_LANES = 32
 
 _ERR_NAMES = {-1: "SLM_E_INVALID_ARG", -2: "SLM_E_HIP", -3: "SLM_E_OOM", -4: "SLM_E_ABI",
               -5: "SLM_E_WORKSPACE"}

@@ -73,11 +73,14 @@ def colour_labels(n_labels, problems, pairs):
     return colour, n_col
 
 
-def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2):
+def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2,
+                lanes_only=False):
     """fmm_lengths (mapping.py:841-900) on device tensors: label int32
     [nx, ny] (signed flank labels), mapping int32 [nx, ny] (uint32 bits),
     mask bool [nx, ny].  order: 2 (scikit-fmm's default) or 1.  max_group
     caps the colours solved together (default: all that fit in 8 GiB).
+    lanes_only (test hook): second-order marches one lane per subsegment,
+    without the LDS heaps of the larger ones (the same bits).
     Returns an FmmResult (host arrays)."""
     import torch
     hip._require_cuda(label, mapping, mask)
@@ -118,7 +121,7 @@ def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, m
     hip.check(lib.slm_fmm_hsl(hip._ptr(label), hip._ptr(mapping), int(is_thinchannel), hip._ptr(colour_d), n_labels,
                               n_col, nx, ny, ctypes.c_double(float(pixel_size)), int(order), hip._ptr(sums),
                               hip._ptr(cnt), hip._ptr(nch), hip._ptr(nphi), hip._ptr(failed), hip._ptr(ws), ws_bytes,
-                              hip._stream(stream), hip.SLM_FLAG_SYNC))
+                              hip._stream(stream), hip.SLM_FLAG_SYNC | (hip.SLM_FLAG_FMM_LANES if lanes_only else 0)))
     s_np, c_np = sums.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
     f_np = failed.cpu().numpy() if order == 2 else np.zeros(n_labels, np.uint8)
     ch_np, ph_np = nch.cpu().numpy().view(np.uint32), nphi.cpu().numpy().view(np.uint32)

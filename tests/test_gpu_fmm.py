@@ -70,12 +70,36 @@ def _compare(gpu, label, mapping, mask, dx, order=2):
     return res
 
 
-@pytest.mark.parametrize("shape,k,seed,dx", [((64, 50), 9, 0, 1.0), ((130, 97), 40, 1, 2.0), ((200, 200), 120, 2, 1.0)])
+# cells of ~330 px march one lane each; ~1600 px (the 32 KiB LDS heaps) and
+# ~5600 px (the large LDS heaps) one 64-lane block each
+@pytest.mark.parametrize("shape,k,seed,dx", [((64, 50), 9, 0, 1.0), ((130, 97), 40, 1, 2.0), ((200, 200), 120, 2, 1.0),
+                                             ((256, 256), 40, 4, 2.0), ((300, 262), 14, 3, 1.0)])
 @pytest.mark.parametrize("order", [2, 1])
 def test_fmm_voronoi_subsegments(gpu, shape, k, seed, dx, order):
     label, mapping, mask = _voronoi_labels(shape[0], shape[1], k, seed)
     res = _compare(gpu, label, mapping, mask, dx, order)
     assert (res.hsl > 0).sum() > 0
+
+
+def test_fmm_lds_heaps_match_lanes(gpu):
+    """Second order on the traced 2048^2 tile's segmentation (subsegments up
+    to ~10^4 px): the LDS-heap marches against one lane per subsegment, bit
+    for bit (sums, counts, failures)."""
+    import torch
+    from slm_amd import hsl, pipeline, workloads
+    job = workloads.tile_2048()
+    res, _ = pipeline.map_tile(job, return_grids=True)
+    g = res["grids"]
+    lab = g["label"].cpu().numpy()
+    sizes = np.bincount(np.abs(lab.astype(np.int64)).ravel())[1:]
+    print("subsegments", int((sizes > 0).sum()), "largest", int(sizes.max()), "over 512 px", int((sizes > 512).sum()),
+          flush=True)
+    assert sizes.max() > 2048
+    a = hsl.fmm_lengths(g["label"], g["mapping"], g["mask"], int(job.info.is_thinchannel), job.pixel_size)
+    b = hsl.fmm_lengths(g["label"], g["mapping"], g["mask"], int(job.info.is_thinchannel), job.pixel_size,
+                        lanes_only=True)
+    for f in ("sums", "cnt", "nch", "nphi", "failed", "hsl", "selected"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
 def test_fmm_colour_groups(gpu):
