@@ -330,7 +330,10 @@ def hsl_aspect_stats(haa, df, n_hsl_split=4):
         out["hsl_ns_mannwhitney"] = (float(mw.statistic) / (nn * ns), float(mw.pvalue))
         out["hsl_ns_ranksum"] = (float(rsum.statistic), float(rsum.pvalue))
         out["hsl_ns_welch"] = (float(welch.statistic), float(welch.pvalue))
-        n_min, s_min, n_max, s_max = min(hsl_n), min(hsl_s), max(hsl_n), max(hsl_s)
+        # the reference's builtin min/max over the Series (Python floats, so the
+        # averages below are float64): the same values, vectorised
+        n_min, s_min = float(hsl_n.min()), float(hsl_s.min())
+        n_max, s_max = float(hsl_n.max()), float(hsl_s.max())
         ns_min, ns_max = (n_min + s_min) / 2, (n_max + s_max) / 2
         pc = np.linspace(0, 100, num=50)
         pn, ps = np.percentile(hsl_n, pc), np.percentile(hsl_s, pc)
