@@ -705,9 +705,13 @@ size_t slm_fmm_workspace_size_k(uint32_t nx, uint32_t ny, uint32_t k);
  * eikonal distance from the zero contour (scikit-fmm's interface
  * initialisation, grid step dx, float64) of the given order:
  *   order 2 (scikit-fmm's default): scikit-fmm's second-order update, points
- *     accepted in (value, index) order, one lane per subsegment; where an
- *     update meets a negative discriminant (scikit-fmm raises) the march of
- *     that subsegment stops and failed[a] = 1 (failed: uint8 [n_labels]);
+ *     accepted in (value, index) order; subsegments of more than 512 points
+ *     march in LDS, one wave each (those too large for the device's LDS, and
+ *     every one under SLM_FLAG_FMM_LANES, one lane each, with the same bits),
+ *     partly on a second stream of the library's that joins the caller's
+ *     before the call's last kernel; where an update meets a negative
+ *     discriminant (scikit-fmm raises) the march of that subsegment stops and
+ *     failed[a] = 1 (failed: uint8 [n_labels]);
  *   order 1: the first-order update relaxed to the fast march's fixed point
  *     (failed may be NULL; synchronises the stream after every relaxation
  *     launch, each running up to 128 in-tile sweeps on every active tile of
