@@ -53,7 +53,9 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     mapping defaults) are set before the channels are mapped.  fmm: also the
     FMM hillslope lengths of every subsegment (slm_amd.hsl, the reference's
     live HSL path) and their grid.  return_grids: results["grids"] holds the
-    device grids (uv, mask, sla, slt, slc, mapping, label, fmm_hsl)."""
+    device grids (uv, mask, sla, slt, slc, mapping, label, fmm_hsl) and the
+    trajectories' host lengths and step counts [n, 2] (traj_length,
+    traj_nsteps)."""
     import torch
     dev = device or torch.device("cuda", torch.cuda.current_device())
     info = job.info
@@ -207,5 +209,6 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     results["ridge_pixels"] = int(((cm.mapping & int(info.is_ridge)) != 0).sum().item())
     if return_grids:
         results["grids"] = {"uv": uv, "mask": mask_b, "sla": sla_f, "slt": slt_f, "slc": slc_f,
-                            "mapping": cm.mapping, "label": cm.label, "fmm_hsl": fmm_grid}
+                            "mapping": cm.mapping, "label": cm.label, "fmm_hsl": fmm_grid,
+                            "traj_length": tl, "traj_nsteps": tn}
     return results, timings

@@ -70,13 +70,13 @@ def job_from_dtm(name, roi, pixel_size, trace_overrides=None, state_overrides=No
     # note: a DTM pixel (i, j) sits at cropped padded index (i + pad - origin[0], j + pad - origin[1])
 
 
-def small_256(jitter=1.0, seed=0, n=256, d=1, max_error=None, resolution=None):
+def small_256(jitter=1.0, seed=0, n=256, d=1, max_error=None, resolution=None, max_length=1000.0):
     """max_error overrides max_integration_step_error (small values make the
     adaptive RK2 step actually adapt); resolution overrides
     trajectory_resolution (128 by default)."""
     roi = synth.make_dtm(n, pixel_size=1.0, relief=20.0, tilt=0.05, seed=seed, nan_border=2)
     tr = {"subpixel_seed_point_density": d, "jitter_magnitude": float(jitter),
-          "max_length": 1000.0, "integrator_step_factor": 0.5}
+          "max_length": float(max_length), "integrator_step_factor": 0.5}
     if max_error is not None:
         tr["max_integration_step_error"] = float(max_error)
     if resolution is not None:

@@ -45,3 +45,39 @@ def test_map_tile_with_conditioning(gpu):
     res, tim = pipeline.map_tile(job, condition=True)
     assert res["chain_ok"] and res["n_segments"] > 0 and res["hsl_seeds"] > 0
     assert "uv" in tim
+
+
+def test_two_stream_trajectories_match_serial_calls(gpu):
+    """The pipeline traces the two trajectory directions on two streams. With
+    a short max_length every output (traj [1000, 200, 2] int8 = 400 KB, steps,
+    lengths) sits in the caching allocator's small pool, where a tensor freed
+    while its kernel still ran could be handed to the other direction; two
+    passes in a row must give what serial slm_trace_paths calls give."""
+    import torch
+    from slm_amd import hip
+    from slm_amd.params import kernel_params
+    from slm_amd.pipeline import map_tile
+    from slm_amd.useful import create_seeds
+    job = workloads.small_256(jitter=1.0, d=1, max_length=100.0)
+    info = job.info
+    outs = [map_tile(job, midslopes=False, return_grids=True)[0]["grids"] for _ in range(2)]
+    tseeds, n_t, _ = create_seeds(job.mask, int(info.pad_width), int(info.n_work_items),
+                                  n_seed_points=info.n_trajectory_seed_points, do_shuffle=info.do_shuffle,
+                                  rng_seed=info.shuffle_rng_seed)
+    info.n_seed_points = np.uint32(n_t)
+    ts = torch.from_numpy(tseeds).to(gpu)
+    mask = torch.from_numpy(job.mask.astype(np.uint8).reshape(-1)).to(gpu)
+    uv = torch.from_numpy(np.ascontiguousarray(job.uv).reshape(-1)).to(gpu)
+    want_l, want_n = [], []
+    for sign in (1, -1):
+        p = kernel_params(info, sign, "INTEGRATE_TRAJECTORY", False)
+        assert n_t * int(p.max_n_steps) * 2 <= 1 << 20
+        traj = torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=gpu)
+        ns = torch.empty(n_t, dtype=torch.int16, device=gpu)
+        ln = torch.empty(n_t, dtype=torch.float32, device=gpu)
+        hip.trace_paths(p, ts, mask, uv, traj, ns, ln, sync=True)
+        want_l.append(ln.cpu().numpy())
+        want_n.append(ns.cpu().numpy().view(np.uint16))
+    for g in outs:
+        assert np.array_equal(g["traj_length"], np.stack(want_l, 1))
+        assert np.array_equal(g["traj_nsteps"], np.stack(want_n, 1))
