@@ -93,6 +93,8 @@ __global__ __launch_bounds__(256) void k_conv(const bf16* __restrict__ x, int ld
 // 32-wide row, or two 16-wide rows) and all CT channels: RW x CT/32
 // accumulator tiles of 32x32.
 // Input channels [0, c_up) come from x2 at half resolution, as in k_conv.
+// The next input chunk's halo and weights are loaded into registers while
+// the current chunk computes from LDS.
 template <int BW, int RW, int CT, int KC, bool RELU>
 __global__ __launch_bounds__(256, 2) void k_conv_lds(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ x2,
                                                      int ldx2, int c_up, const bf16* __restrict__ w,
@@ -124,29 +126,67 @@ __global__ __launch_bounds__(256, 2) void k_conv_lds(const bf16* __restrict__ x,
         pcol[i] = r % BW;
     }
     const bf16x8 zero = {};
-    for (int c0 = 0; c0 < Cin; c0 += KC) {
-        __syncthreads();
-        constexpr int XV = (BH + 2) * (BW + 2) * (KC / 8);
-        for (int v = t; v < XV; v += 256) {
-            const int px = v / (KC / 8), part = v % (KC / 8);
-            const int hr = px / (BW + 2), hc = px % (BW + 2);
-            const int gh = h0 + hr - 1, gw = w0 + hc - 1;
+    // staging: the halo ((BH+2) x (BW+2) pixels x KC channels) and the weights
+    // (CT x 9 x KC) of one input chunk, in 16-byte vectors, NXV + NWV per
+    // thread; the next chunk's vectors are loaded into registers while the
+    // current chunk computes from LDS
+    constexpr int XV = (BH + 2) * (BW + 2) * (KC / 8);
+    constexpr int WV = CT * 9 * (KC / 8);
+    constexpr int NXV = (XV + 255) / 256, NWV = (WV + 255) / 256;
+    bf16x8 xr[NXV], wr[NWV];
+    auto load_chunk = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < NXV; i++) {
+            const int v = t + i * 256;
             bf16x8 val = zero;
-            if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
-                const int c = c0 + part * 8;
-                val = c < c_up ? *(const bf16x8*)(x2 + ((size_t)(n * (H / 2) + gh / 2) * (W / 2) + gw / 2) * ldx2 + c)
-                               : *(const bf16x8*)(x + ((size_t)(n * H + gh) * W + gw) * ldx + c);
+            if (v < XV) {
+                const int px = v / (KC / 8), part = v % (KC / 8);
+                const int hr = px / (BW + 2), hc = px % (BW + 2);
+                const int gh = h0 + hr - 1, gw = w0 + hc - 1;
+                if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
+                    const int c = c0 + part * 8;
+                    val = c < c_up
+                              ? *(const bf16x8*)(x2 + ((size_t)(n * (H / 2) + gh / 2) * (W / 2) + gw / 2) * ldx2 + c)
+                              : *(const bf16x8*)(x + ((size_t)(n * H + gh) * W + gw) * ldx + c);
+                }
             }
-            *(bf16x8*)&xs[px * XP + part * 8] = val;
+            xr[i] = val;
         }
-        constexpr int WV = CT * 9 * (KC / 8);
-        for (int v = t; v < WV; v += 256) {
-            const int co = v / (9 * KC / 8), rem = v % (9 * KC / 8);
-            const int tap = rem / (KC / 8), part = rem % (KC / 8);
-            *(bf16x8*)&ws[co * WP + tap * KC + part * 8] =
-                *(const bf16x8*)(w + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 + part * 8);
+#pragma unroll
+        for (int i = 0; i < NWV; i++) {
+            const int v = t + i * 256;
+            if (v < WV) {
+                const int co = v / (9 * KC / 8), rem = v % (9 * KC / 8);
+                const int tap = rem / (KC / 8), part = rem % (KC / 8);
+                wr[i] = *(const bf16x8*)(w + ((size_t)(co0 + co) * 9 + tap) * Cin + c0 + part * 8);
+            }
         }
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int i = 0; i < NXV; i++) {
+            const int v = t + i * 256;
+            if (v < XV) {
+                const int px = v / (KC / 8), part = v % (KC / 8);
+                *(bf16x8*)&xs[px * XP + part * 8] = xr[i];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NWV; i++) {
+            const int v = t + i * 256;
+            if (v < WV) {
+                const int co = v / (9 * KC / 8), rem = v % (9 * KC / 8);
+                const int tap = rem / (KC / 8), part = rem % (KC / 8);
+                *(bf16x8*)&ws[co * WP + tap * KC + part * 8] = wr[i];
+            }
+        }
+    };
+    load_chunk(0);
+    for (int c0 = 0; c0 < Cin; c0 += KC) {
+        __syncthreads();                      // the previous chunk's reads of LDS are done
+        store_chunk();
         __syncthreads();
+        if (c0 + KC < Cin) load_chunk(c0 + KC);
 #pragma unroll 1
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
