@@ -390,7 +390,8 @@ __global__ __launch_bounds__(256) void k_wgrad_t(const bf16* __restrict__ x, int
 // (v_alignbyte), so three MFMAs share one set of aligned LDS reads.  The
 // plane strides are 8 (mod 128) elements, so the 32 rows of an operand read
 // start on different 16-byte bank groups.  Chunks [blockIdx.z * cpb, +cpb)
-// of the N * ceil(H / R); partial sums leave by fp32 atomics.  For 1x1
+// of the N * ceil(H / R), the next one loaded into registers while the
+// current one computes; partial sums leave by fp32 atomics.  For 1x1
 // (T = 1) there is no halo: one wave per 32 output channels, one MFMA per step.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -421,46 +422,74 @@ __global__ __launch_bounds__((T == 9 ? 192 : 64) * (CT / 32)) void k_wgrad_halo(
     const bool b_on = CIT == 32 || r < CIT;
     f32x16 acc0 = {}, acc1 = {}, acc2 = {};
     const bf16x8 zero = {};
-    for (int ch = blockIdx.z * cpb; ch < ch_end; ++ch) {
+    // staging blocks (8 pixels x 8 channels) per thread; the next chunk's are
+    // loaded into registers while the current chunk computes from LDS
+    constexpr int NBD = 32 * (CT / 8), NBX = (R + 2 * HALO) * XG * (CIT / 8);
+    constexpr int ND = (NBD + NT - 1) / NT, NX = (NBX + NT - 1) / NT;
+    bf16x8 dv[ND][8], xv[NX][8];
+    auto load_chunk = [&](int ch) {
         const int n = ch / per_image, r0 = (ch % per_image) * R;
-        for (int blk = t; blk < 32 * (CT / 8); blk += NT) {
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            const int blk = t + k * NT;
             const int cg = blk % (CT / 8), q = (blk / (CT / 8)) * 8;
             const int row = q / W, col = q % W;
-            const bool in = r0 + row < H;
+            const bool in = blk < NBD && r0 + row < H;
             const bf16* src = dy + ((size_t)(n * H + r0 + row) * W + col) * lddy + c0 + cg * 8;
-            bf16x8 v[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) v[i] = in ? *(const bf16x8*)(src + (size_t)i * lddy) : zero;
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                bf16x8 c;
-#pragma unroll
-                for (int i = 0; i < 8; i++) c[i] = v[i][j];
-                *(bf16x8*)&dyT[(cg * 8 + j) * DS + q] = c;
-            }
+            for (int i = 0; i < 8; i++) dv[k][i] = in ? *(const bf16x8*)(src + (size_t)i * lddy) : zero;
         }
-        for (int blk = t; blk < (R + 2 * HALO) * XG * (CIT / 8); blk += NT) {
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int blk = t + k * NT;
             const int cg = blk % (CIT / 8), g = (blk / (CIT / 8)) % XG, rr = blk / ((CIT / 8) * XG);
             const int row = r0 - HALO + rr, col = (g - 1) * 8;
-            const bool in = row >= 0 && row < H && g >= 1 && g <= W / 8;
-            bf16x8 v[8];
+            const bool in = blk < NBX && row >= 0 && row < H && g >= 1 && g <= W / 8;
             if (in) {
                 const bf16* src = x + ((size_t)(n * H + row) * W + col) * ldx + ci0 + cg * 8;
 #pragma unroll
-                for (int i = 0; i < 8; i++) v[i] = *(const bf16x8*)(src + (size_t)i * ldx);
+                for (int i = 0; i < 8; i++) xv[k][i] = *(const bf16x8*)(src + (size_t)i * ldx);
             } else {
 #pragma unroll
-                for (int i = 0; i < 8; i++) v[i] = zero;
+                for (int i = 0; i < 8; i++) xv[k][i] = zero;
             }
+        }
+    };
+    auto store_chunk = [&]() {
+#pragma unroll
+        for (int k = 0; k < ND; k++) {
+            const int blk = t + k * NT;
+            if (blk >= NBD) continue;
+            const int cg = blk % (CT / 8), q = (blk / (CT / 8)) * 8;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 bf16x8 c;
 #pragma unroll
-                for (int i = 0; i < 8; i++) c[i] = v[i][j];
+                for (int i = 0; i < 8; i++) c[i] = dv[k][i][j];
+                *(bf16x8*)&dyT[(cg * 8 + j) * DS + q] = c;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            const int blk = t + k * NT;
+            if (blk >= NBX) continue;
+            const int cg = blk % (CIT / 8), g = (blk / (CIT / 8)) % XG, rr = blk / ((CIT / 8) * XG);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                bf16x8 c;
+#pragma unroll
+                for (int i = 0; i < 8; i++) c[i] = xv[k][i][j];
                 *(bf16x8*)&xh[(cg * 8 + j) * XS + rr * WP + g * 8] = c;
             }
         }
+    };
+    const int ch_begin = blockIdx.z * cpb;
+    if (ch_begin < ch_end) load_chunk(ch_begin);
+    for (int ch = ch_begin; ch < ch_end; ++ch) {
+        __syncthreads();                      // the previous chunk's reads of LDS are done
+        store_chunk();
         __syncthreads();
+        if (ch + 1 < ch_end) load_chunk(ch + 1);
 #pragma unroll 4
         for (int ks = 0; ks < 16; ++ks) {
             const int q = ks * 16 + half * 8, row = q / W, col = q % W;
@@ -485,7 +514,6 @@ __global__ __launch_bounds__((T == 9 ? 192 : 64) * (CT / 32)) void k_wgrad_halo(
                 acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, gp), acc2, 0, 0, 0);
             }
         }
-        __syncthreads();
     }
     if (!b_on) return;
     const int ci = ci0 + r;
