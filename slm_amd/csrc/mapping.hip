@@ -68,10 +68,10 @@ template <bool C>
 __device__ __forceinline__ float map_move(const SlmParams& P, const StepOut& o, float& x, float& y,
                                           uint32_t& idx) {
     float2 nx = o.next;
-    float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+    float dl = fast_length_cr<C>(o.dxy2.x, o.dxy2.y);
     if (dl < P.integration_halt_threshold) {
         nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
-        dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+        dl = fast_length_cr<C>(o.dxy1.x, o.dxy1.y);
     }
     x = nx.x;
     y = nx.y;
@@ -95,6 +95,7 @@ __global__ void k_connect(SlmParams P, const float2* __restrict__ seeds, const u
     const uint32_t cap = P.interchannel_max_n_steps;
     const float res = P.trajectory_resolution;
     const float2 s0 = seeds[k];
+    const bool fixed = dt_is_fixed(P);
     uint32_t n_total = 0;
     for (int pass = 0; pass < 2; pass++) {
         if (pass == 1 && !(n_total > 2 && n_total < cap)) return;
@@ -111,13 +112,12 @@ __global__ void k_connect(SlmParams P, const float2* __restrict__ seeds, const u
                     if (!((mapping[idx] & F_CHANNEL) == 0 && !mask[idx] && n < cap)) break;
                 }
                 StepOut o;
-                compute_step<C, false>(P, uv, dt, x, y, rng, o);
+                compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
                 idx = o.idx;
                 if (mask[idx]) continue;
-                const float err = step_error<C>(P, o);
                 const float dl = map_move<C>(P, o, x, y, idx);
                 const bool stuck = dl < P.integration_halt_threshold;
-                if (!stuck) dt = next_dt(P, dt, err);
+                if (!stuck && !fixed) dt = next_dt(P, dt, step_error<C>(P, o));
                 if (pass == 1) {
                     // replay record n: the flag loop's step n
                     if (mask[ridx] || n >= n_total - 1u) return;
@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void k_heads(SlmParams P, const float2* __rest
                                                uint32_t* __restrict__ mapping) {
     float x = 0.f, y = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0, idx = 0, prev_idx = 0;
+    const bool fixed = dt_is_fixed(P);
     CycleSkip cyc;
     auto finish = [&]() {
         if (n >= P.max_n_steps) return;
@@ -260,12 +261,11 @@ __global__ __launch_bounds__(256) void k_heads(SlmParams P, const float2* __rest
             return false;
         }
         StepOut o;
-        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
         idx = o.idx;
-        const float err = step_error<C>(P, o);
         map_move<C>(P, o, x, y, idx);
         n++;
-        dt = next_dt(P, dt, err);
+        if (!fixed) dt = next_dt(P, dt, step_error<C>(P, o));
         // a repeated state inside the seed pixel never leaves it: the trace
         // runs to MAX_N_STEPS and changes nothing
         if (prev_idx == idx && cyc.check(x, y, dt, 0u, n, P.max_n_steps)) return false;
@@ -277,17 +277,17 @@ __global__ __launch_bounds__(256) void k_heads(SlmParams P, const float2* __rest
 // --------------------------------------------------------- count_downchannels
 
 // countlink_runge_kutta_step (rungekutta.cl:382-406): true when stuck.
+// fixed: dt_is_fixed(P), so dt stays DT_MAX and the step error is not needed.
 template <bool C>
 __device__ __forceinline__ bool countlink_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
-                                               float& x, float& y, uint32_t& idx) {
+                                               float& x, float& y, uint32_t& idx, bool fixed) {
     StepOut o;
     uint32_t rng = 0;
-    compute_step<C, false>(P, uv, dt, x, y, rng, o);
+    compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
     idx = o.idx;
-    const float err = step_error<C>(P, o);
     const float dl = map_move<C>(P, o, x, y, idx);
     if (dl < P.integration_halt_threshold) return true;
-    dt = next_dt(P, dt, err);
+    if (!fixed) dt = next_dt(P, dt, step_error<C>(P, o));
     return false;
 }
 
@@ -302,6 +302,7 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
     const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.n_seed_points) return;
     const uint32_t cap = 1000u * P.max_n_steps;
+    const bool fixed = dt_is_fixed(P);
     const float2 s = seeds[k];
     float x = s.x, y = s.y, dt = P.dt_max;
     uint32_t idx = array_idx(P, x, y), prev = idx, n = 0, counter = 2, cprev = 1;
@@ -314,7 +315,7 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
     uint32_t sprev = prev, sn = n, scounter = counter, power = 1;
     bool ff = true;
     while (!mask[idx] && n < cap) {
-        if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
+        if (countlink_step<C>(P, uv, dt, x, y, idx, fixed)) break;
         n++;
         if (prev != idx) {
             ++counter;
@@ -556,6 +557,7 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
     const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.n_seed_points) return;
     const uint32_t cap = 100u * P.max_n_steps;
+    const bool fixed = dt_is_fixed(P);
     const float2 s = seeds[k];
     float x = s.x, y = s.y, dt = P.dt_max;
     uint32_t idx = array_idx(P, x, y), prev = idx, n = 0;
@@ -563,7 +565,7 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
     uint32_t sn = 0, power = 1;
     while (prev == idx && !mask[idx] && n < cap) {
         prev = idx;
-        if (countlink_step<C>(P, uv, dt, x, y, idx)) break;
+        if (countlink_step<C>(P, uv, dt, x, y, idx, fixed)) break;
         n++;
         if (prev == idx) {
             // a repeated state inside the seed pixel never leaves it: no link
@@ -583,20 +585,20 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
 // ------------------------------------------------------ hillslope segmentation
 
 // segment_runge_kutta_step (rungekutta.cl:435-459); probe = the pixel of
-// compute_step_vec's next point, before the dxy1 retry.
+// compute_step_vec's next point, before the dxy1 retry.  fixed:
+// dt_is_fixed(P), so dt stays DT_MAX and the step error is not needed.
 template <bool C, bool R2>
 __device__ __forceinline__ bool segment_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
                                              float& x, float& y, uint32_t& idx, uint32_t& n, uint32_t& probe,
-                                             float inv_res) {
+                                             float inv_res, bool fixed) {
     StepOut o;
     uint32_t rng = 0;
-    compute_step<C, false, R2>(P, uv, dt, x, y, rng, o, inv_res);
+    compute_step<C, false, R2, true>(P, uv, dt, x, y, rng, o, inv_res);
     probe = o.idx;
-    const float err = step_error<C>(P, o);
     const float dl = map_move<C>(P, o, x, y, idx);
     n += 1u;
     if (dl < P.integration_halt_threshold) return true;
-    dt = next_dt(P, dt, err);
+    if (!fixed) dt = next_dt(P, dt, step_error<C>(P, o));
     return false;
 }
 
@@ -607,6 +609,7 @@ __global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __res
                                                 const uint32_t* __restrict__ mapping,
                                                 uint32_t* __restrict__ label) {
     const float inv_res = 1.0f / P.trajectory_resolution;
+    const bool fixed = dt_is_fixed(P);
     float x = 0.f, y = 0.f, dt = 0.f;
     uint32_t n = 0, probe = 0, idx = 0, hs = 0;
     CycleSkip cyc;
@@ -625,7 +628,7 @@ __global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __res
     };
     auto step = [&]() -> bool {
         if (!(!mask[idx] && ((~mapping[idx]) & F_THIN) && n < P.max_n_steps) ||
-            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res)) {
+            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res, fixed)) {
             finish();
             return false;
         }
@@ -697,6 +700,7 @@ __global__ __launch_bounds__(256) void k_sub_flanks(SlmParams P, const float2* _
                                                     const uint32_t* __restrict__ mapping,
                                                     uint32_t* __restrict__ label) {
     const float inv_res = 1.0f / P.trajectory_resolution;
+    const bool fixed = dt_is_fixed(P);
     float x = 0.f, y = 0.f, dt = 0.f;
     uint32_t n = 0, probe = 0, idx = 0, hs = 0;
     CycleSkip cyc;
@@ -712,7 +716,7 @@ __global__ __launch_bounds__(256) void k_sub_flanks(SlmParams P, const float2* _
     };
     auto step = [&]() -> bool {
         if (!(!mask[idx] && !(mapping[idx] & F_LEFT) && !(mapping[idx] & F_THIN) && n < P.max_n_steps) ||
-            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res)) {
+            segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res, fixed)) {
             if (mapping[idx] & F_LEFT) label[hs] |= LFA;
             return false;
         }
@@ -730,6 +734,7 @@ __global__ __launch_bounds__(256) void k_fix(SlmParams P, const float2* __restri
                                              const uint32_t* __restrict__ mapping,
                                              const uint32_t* __restrict__ snap, uint32_t* __restrict__ label) {
     const float inv_res = 1.0f / P.trajectory_resolution;
+    const bool fixed = dt_is_fixed(P);
     float x = 0.f, y = 0.f, dt = 0.f;
     uint32_t n = 0, idx = 0, hs = 0, own = 0;
     auto lab = [&](uint32_t q) -> uint32_t { return q == hs ? own : snap[q]; };
@@ -761,7 +766,7 @@ __global__ __launch_bounds__(256) void k_fix(SlmParams P, const float2* __restri
             return false;
         }
         uint32_t probe;
-        const bool stuck = segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res);
+        const bool stuck = segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res, fixed);
         if ((own & ~LFA) != (lab(probe) & ~LFA)) own = lab(probe);
         if (stuck) {
             finish();

@@ -520,8 +520,9 @@ using RunTable = RunTableT<kRunSlots>;
 // recorded.  !PACKED: the fallback, which runs only when the packed result was
 // rejected and adds runs to slc/slt with two 32-bit atomics.
 // dt_fixed: the host has proven that the adaptive step can never leave DT_MAX
-// (every dxy component lies in [-1, 1), so step_error <= sqrt(8)/GRID_SCALE,
-// and GRID_SCALE*ADJUSTED_MAX_ERROR >= sqrt(8)*(1+2^-20) makes
+// (slm::dt_is_fixed: every dxy component is c/RES with c in [-128, 127], so
+// step_error <= sqrt(8)*(128/RES)/GRID_SCALE, and a GRID_SCALE *
+// ADJUSTED_MAX_ERROR above that bound by a rounding margin makes
 // fmin(DT_MAX, ADJUSTED_MAX_ERROR*DT_MAX/step_error) == DT_MAX); step_error
 // is then not computed.
 template <bool C, bool PACKED, bool R2>
@@ -746,11 +747,8 @@ __global__ void k_build_quads(SlmParams P, const float2* __restrict__ uv, Quad* 
     }
 }
 
-// True when next_dt can only return DT_MAX (see k_fields).
-static uint32_t dt_is_fixed(const SlmParams* p) {
-    return ((double)p->grid_scale * (double)p->adjusted_max_error >= 2.8284271247461903 * (1.0 + 1.0 / 1048576.0))
-               ? 1u : 0u;
-}
+// True when next_dt can only return DT_MAX (slm::dt_is_fixed).
+static uint32_t dt_is_fixed(const SlmParams* p) { return slm::dt_is_fixed(*p) ? 1u : 0u; }
 
 __global__ void k_fields_init(FieldsCtrl* ctrl) {
     if (threadIdx.x < kQueues) ctrl->queue[threadIdx.x * kQueueStride] = 0u;
@@ -1036,23 +1034,23 @@ __global__ void k_paths(SlmParams P, const float2* __restrict__ seeds, const uin
     float l = 0.f, dl = 0.f, dt = P.dt_max;
     uint32_t nst = 0, rng = 0;
     uint32_t idx = array_idx(P, x, y);
+    const bool fixed = dt_is_fixed(P);
     while (!mask[idx] && (l < P.max_length && nst < P.max_n_steps)) {
         StepOut o;
-        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
         idx = o.idx;
         if (!mask[idx]) {
-            const float err = step_error<C>(P, o);
-            dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+            dl = fast_length_cr<C>(o.dxy2.x, o.dxy2.y);
             float2 nx = o.next;
             if (dl < P.integration_halt_threshold) {
                 nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
-                dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+                dl = fast_length_cr<C>(o.dxy1.x, o.dxy1.y);
             }
             x = nx.x;
             y = nx.y;
             idx = array_idx(P, x, y);
             const bool stuck = dl < P.integration_halt_threshold;
-            if (!stuck) dt = next_dt(P, dt, err);
+            if (!stuck && !fixed) dt = next_dt(P, dt, step_error<C>(P, o));
             l += dl;
             rec[nst] = make_char2(compress1(x - px, res), compress1(y - py, res));
             nst += 1u;
@@ -1244,6 +1242,7 @@ __global__ __launch_bounds__(256) void k_hsl(SlmParams P, const float2* __restri
     float x = 0.f, y = 0.f, l = 0.f, dt = 0.f;
     uint32_t n = 0, rng = 0, idx = 0, k = 0;
     bool moved_off = false;
+    const bool fixed = dt_is_fixed(P);
     CycleSkip cyc;
     auto start = [&](uint32_t kk) -> bool {
         const float2 s = seeds[kk];
@@ -1264,23 +1263,22 @@ __global__ __launch_bounds__(256) void k_hsl(SlmParams P, const float2* __restri
             return false;
         }
         StepOut o;
-        compute_step<C, false>(P, uv, dt, x, y, rng, o);
+        compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
         idx = o.idx;
         if (mask[idx]) return false;
         if (!moved_off && ((~mapping[idx]) & from_flag)) moved_off = true;
         else if (moved_off && (mapping[idx] & from_flag)) return false;
-        const float err = step_error<C>(P, o);
-        float dl = fast_length<C>(o.dxy2.x, o.dxy2.y);
+        float dl = fast_length_cr<C>(o.dxy2.x, o.dxy2.y);
         float2 nx = o.next;
         if (dl < P.integration_halt_threshold) {
             nx = make_float2(x + o.dxy1.x, y + o.dxy1.y);
-            dl = fast_length<C>(o.dxy1.x, o.dxy1.y);
+            dl = fast_length_cr<C>(o.dxy1.x, o.dxy1.y);
         }
         x = nx.x;
         y = nx.y;
         idx = array_idx(P, x, y);
         if (dl < P.integration_halt_threshold) return false;
-        dt = next_dt(P, dt, err);
+        if (!fixed) dt = next_dt(P, dt, step_error<C>(P, o));
         l += dl;
         n += 1u;
         // A repeated state (position, dt, moved_off) comes back around a cycle

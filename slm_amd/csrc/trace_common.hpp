@@ -207,6 +207,56 @@ __device__ __forceinline__ float2 speed_interp(const SlmParams& P, const float2*
     return make_float2(s0 * inv, s1 * inv);
 }
 
+// Two neighbouring float2 of one uv row in one 16-byte load (8-byte aligned;
+// gfx950 global loads take any dword alignment).
+typedef float SlmFloat4A8 __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ float4 load_uv_pair(const float2* __restrict__ p) {
+    const SlmFloat4A8 v = *reinterpret_cast<const SlmFloat4A8*>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// speed_interp with the same arithmetic (so the same bits): pixels (xl, yd)
+// and (xl, yu) are neighbours in memory, as are (xr, yd) and (xr, yu), so two
+// 16-byte loads fetch the four corners.  On the last row (yu == yd) each pair
+// is read one element lower and its upper half serves both corners.  The
+// normalisation uses the short sqrt / divide forms.
+template <bool C>
+__device__ __forceinline__ float2 speed_interp_cr(const SlmParams& P, const float2* __restrict__ uv,
+                                                  float x, float y) {
+    const uint32_t top = P.ny_padded - 1u;
+    const uint32_t xl = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
+    const uint32_t yd = pix(y, P.pad_width_pp5, top);
+    const uint32_t xr = min(P.nx_padded - 1u, xl + 1u);
+    const bool last = yd >= top;
+    const uint32_t y0 = last ? yd - 1u : yd;
+    const float4 L = load_uv_pair(uv + (P.ny_padded * xl + y0));
+    const float4 R = load_uv_pair(uv + (P.ny_padded * xr + y0));
+    const float ax = last ? L.z : L.x, ay = last ? L.w : L.y;
+    const float bx = last ? R.z : R.x, by = last ? R.w : R.y;
+    const float rx = x - (float)xl + (float)P.pad_width;
+    const float lx = 1.0f - rx;
+    const float uy = y - (float)yd + (float)P.pad_width;
+    const float dy = 1.0f - uy;
+    const float dwn0 = mad<C>(ax, lx, bx * rx), dwn1 = mad<C>(ay, lx, by * rx);
+    const float upp0 = mad<C>(L.z, lx, R.z * rx), upp1 = mad<C>(L.w, lx, R.w * rx);
+    const float s0 = mad<C>(dwn0, dy, upp0 * uy), s1 = mad<C>(dwn1, dy, upp1 * uy);
+    const float inv = inv_sqrt_cr(dot2<C>(s0, s1));
+    return make_float2(s0 * inv, s1 * inv);
+}
+
+// The adaptive step can never leave DT_MAX.  Every dxy component is c/RES with
+// c in [-128, 127] (approximate), so a component of dxy2 - dxy1 is below
+// 256/RES and step_error <= sqrt(8) * (128/RES) / GRID_SCALE.  When
+// GRID_SCALE * ADJUSTED_MAX_ERROR exceeds that bound by a margin that covers
+// the roundings of step_error and of next_dt's quotient, ADJ*DT_MAX/err >=
+// DT_MAX for every err, so next_dt returns DT_MAX at every step and a trace
+// started at DT_MAX keeps it.  Kernels then skip step_error and next_dt.
+__host__ __device__ inline bool dt_is_fixed(const SlmParams& p) {
+    if (!(p.trajectory_resolution > 0.0f)) return false;
+    const double bound = 2.8284271247461903 * (128.0 / (double)p.trajectory_resolution) * (1.0 + 1.0 / 1048576.0);
+    return (double)p.grid_scale * (double)p.adjusted_max_error >= bound;
+}
+
 struct StepOut {
     float2 uv1, dxy1, dxy2, next;
     uint32_t idx;
@@ -214,31 +264,42 @@ struct StepOut {
 
 // compute_step_vec (computestep.cl:42-57) / compute_step_vec_jittered (:86-106).
 // R2: TRAJECTORY_RESOLUTION is a power of two and inv_res its reciprocal.
-template <bool C, bool JITTER, bool R2 = false>
+// CR: the paired-load interpolation and the short sqrt / divide forms (the
+// same bits).
+template <bool C, bool JITTER, bool R2 = false, bool CR = false>
 __device__ __forceinline__ void compute_step(const SlmParams& P, const float2* __restrict__ uv,
                                              float dt, float x, float y, uint32_t& rng, StepOut& o,
                                              float inv_res = 0.0f) {
     const float res = P.trajectory_resolution;
     const float combo = P.combo_factor;
-    float2 u1 = speed_interp<C>(P, uv, x, y);
+    float2 u1 = CR ? speed_interp_cr<C>(P, uv, x, y) : speed_interp<C>(P, uv, x, y);
     if constexpr (JITTER) {
         const float2 r = rand_vec(rng);
         u1.x = mad<C>(r.x, P.jitter_magnitude, u1.x);
         u1.y = mad<C>(r.y, P.jitter_magnitude, u1.y);
-        const float len = fast_length<C>(u1.x, u1.y);
-        u1.x = fdiv(u1.x, len);
-        u1.y = fdiv(u1.y, len);
+        if constexpr (CR) {
+            u1 = div_by_length<C>(u1.x, u1.y);
+        } else {
+            const float len = fast_length<C>(u1.x, u1.y);
+            u1.x = fdiv(u1.x, len);
+            u1.y = fdiv(u1.y, len);
+        }
     }
     o.uv1 = u1;
     o.dxy1 = make_float2(approx_r<R2>(u1.x * combo * dt, res, inv_res), approx_r<R2>(u1.y * combo * dt, res, inv_res));
-    float2 u2 = speed_interp<C>(P, uv, x + o.dxy1.x, y + o.dxy1.y);
+    float2 u2 = CR ? speed_interp_cr<C>(P, uv, x + o.dxy1.x, y + o.dxy1.y)
+                   : speed_interp<C>(P, uv, x + o.dxy1.x, y + o.dxy1.y);
     if constexpr (JITTER) {
         const float2 r = rand_vec(rng);
         u2.x = mad<C>(r.x, P.jitter_magnitude, u2.x);
         u2.y = mad<C>(r.y, P.jitter_magnitude, u2.y);
-        const float len = fast_length<C>(u2.x, u2.y);
-        u2.x = fdiv(u2.x, len);
-        u2.y = fdiv(u2.y, len);
+        if constexpr (CR) {
+            u2 = div_by_length<C>(u2.x, u2.y);
+        } else {
+            const float len = fast_length<C>(u2.x, u2.y);
+            u2.x = fdiv(u2.x, len);
+            u2.y = fdiv(u2.y, len);
+        }
     }
     o.dxy2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, o.dxy1.x), res, inv_res),
                          approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, o.dxy1.y), res, inv_res));

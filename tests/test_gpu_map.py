@@ -31,13 +31,18 @@ def _case(kind):
         job, m = workloads.v_valley(n=96)
         m = channels_from_mapping(m, job.info)
         slt = np.zeros(m.shape, np.float32)
-    elif kind in ("y", "y100"):
+    elif kind in ("y", "y100", "y_adapt"):
         job, m = workloads.y_valley(n=128, channel_width=3)
         slt = np.random.default_rng(1).random(m.shape).astype(np.float32)
         if kind == "y100":
             # a TRAJECTORY_RESOLUTION that is not a power of two: the traces
             # divide in approximate instead of scaling
             job.info.trajectory_resolution = np.uint32(100)
+        if kind == "y_adapt":
+            # a step-error tolerance small enough that dt adapts (the kernels
+            # skip the step error only when it provably cannot)
+            job.info.max_integration_step_error = np.float32(1e-7)
+            job.info.adjusted_max_error = 0.85 * np.sqrt(np.float64(job.info.max_integration_step_error))
     elif kind == "pits":
         job, m, slt = _traced_case(_pitted_job())
     else:
@@ -89,7 +94,8 @@ def _load_oracle(om, gm):
         setattr(om, k, None if v is None else _u32(v).reshape(v.shape))
 
 
-@pytest.mark.parametrize("kind,contract", [("v", True), ("y", True), ("y", False), ("y100", True), ("traced", True),
+@pytest.mark.parametrize("kind,contract", [("v", True), ("y", True), ("y", False), ("y100", True), ("y_adapt", True),
+                                           ("traced", True),
                                            ("pits", True)])
 def test_chain_stepwise_bit_exact(gpu, kind, contract):
     import torch
