@@ -133,11 +133,13 @@ extern "C" int slm_rng_stream(uint32_t idx0, uint32_t n, uint32_t n_draws, uint3
 // The short sqrt / divide forms of trace_common.hpp against the operators,
 // bit for bit.  counts (uint64 [8], accumulated):
 //   [0]/[1] sqrt_short(d) != sqrtf(d), of every float d in [2^-96, 2^100]
-//   [2]/[3] div_short(1, b, recip_short(b)) != 1/b, of every b in [2^-48, 2^50]
-//   [4]/[5] div_short(a, b, recip_short(b)) != a/b, n_num numerators per b with
-//           |a| in [2^-70, 8b], either sign (hashed exponent and mantissa)
-//   [6]     sqrt_cr(d) / inv_sqrt_cr(d) differ from the operators, d = any bits,
-//           or sqrt_short(+0) differs from sqrtf(+0)
+//   [2]/[3] recip_short(b) != 1/b, of every b in [2^-48, 2^50]
+//   [4]/[5] div_short(a, b, recip_short(b)) != a/b: n_num numerators per b with
+//           |a| in [2^-70, 8b], either sign (hashed exponent and mantissa), and
+//           every significand of a in [1, 2) for 2048 significands of b in
+//           [1, 2) (the 1024 largest, where a*RN(1/b) is furthest from a/b,
+//           and 1024 hashed; scaling both by powers of two changes no bit)
+//   [6]     sqrt_cr(d) / inv_sqrt_cr(d) differ from the operators, d = any bits
 //   [7]     div_by_length(x, y) differs from x/len, y/len, x = any bits, y hashed
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -162,7 +164,7 @@ __global__ void k_check_rounding(unsigned long long* __restrict__ counts, uint32
         if (d >= 0x1p-48f && d <= 0x1p50f) {
             c[3]++;
             const float r = recip_short(d);
-            if (__float_as_uint(div_short(1.0f, d, r)) != __float_as_uint(fdiv(1.0f, d))) c[2]++;
+            if (__float_as_uint(r) != __float_as_uint(fdiv(1.0f, d))) c[2]++;
             const uint32_t eb = (bits >> 23) & 0xffu;
             const uint32_t lo = 127u - 70u, span = eb + 3u - lo + 1u;
             for (uint32_t j = 0; j < n_num; j++) {
@@ -176,7 +178,6 @@ __global__ void k_check_rounding(unsigned long long* __restrict__ counts, uint32
             }
         }
         if (__float_as_uint(sqrt_cr(d)) != __float_as_uint(fsqrt(d)) ||
-            (d == 0.0f && __float_as_uint(sqrt_short(fabsf(d))) != __float_as_uint(fsqrt(fabsf(d)))) ||
             __float_as_uint(inv_sqrt_cr(d)) != __float_as_uint(fdiv(1.0f, fsqrt(d))))
             c[6]++;
         const float y = __uint_as_float(mix32(bits ^ 0x5bd1e995u));
@@ -189,10 +190,28 @@ __global__ void k_check_rounding(unsigned long long* __restrict__ counts, uint32
         if (c[k]) atomicAdd(&counts[k], c[k]);
 }
 
+// Every dividend significand against one divisor significand per blockIdx.y.
+__global__ void k_check_quotients(unsigned long long* __restrict__ counts) {
+    const uint32_t j = blockIdx.y;
+    const uint32_t bm = j < 1024u ? 0x7fffffu - j : (mix32(j * 0x9e3779b9u + 11u) & 0x7fffffu);
+    const float b = __uint_as_float((127u << 23) | bm);
+    const float y = recip_short(b);
+    unsigned long long bad = 0, n = 0;
+    for (uint32_t am = blockIdx.x * blockDim.x + threadIdx.x; am < (1u << 23); am += gridDim.x * blockDim.x) {
+        const float a = __uint_as_float((127u << 23) | am);
+        n++;
+        if (__float_as_uint(div_short(a, b, y)) != __float_as_uint(fdiv(a, b))) bad++;
+    }
+    if (bad) atomicAdd(&counts[4], bad);
+    atomicAdd(&counts[5], n);
+}
+
 extern "C" int slm_check_rounding(uint64_t* counts, uint32_t n_numerators, void* stream, uint32_t flags) {
     if (!counts) return slm_fail(SLM_E_INVALID_ARG, "counts is NULL");
     hipLaunchKernelGGL(k_check_rounding, dim3(8192), dim3(256), 0, (hipStream_t)stream,
                        (unsigned long long*)counts, n_numerators);
+    hipLaunchKernelGGL(k_check_quotients, dim3(256, 2048), dim3(256), 0, (hipStream_t)stream,
+                       (unsigned long long*)counts);
     return slm_finish(stream, flags);
 }
 
@@ -2083,8 +2102,8 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
 
     // per-lane job context
     SlmParams P;
-    const uint8_t* mask = nullptr;
-    const Quad* quad = nullptr;
+    GBytePtr mask = nullptr;
+    GQuadPtr quad = nullptr;
     uint32_t* slc = nullptr;
     uint32_t* slt = nullptr;
     uint32_t dt_fixed = 0, pix_begin = 0;
@@ -2152,8 +2171,8 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     }
                     const BatchDesc* D = desc + lo;
                     P = D->P;
-                    mask = D->mask;
-                    quad = D->quad;
+                    mask = as_global(D->mask);
+                    quad = as_global(D->quad);
                     slc = D->slc;
                     slt = D->slt;
                     dt_fixed = D->dt_fixed;
@@ -2209,15 +2228,18 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
             const uint32_t nix = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
             const uint32_t niy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
             const uint32_t nidx = niy + P.ny_padded * nix;
+            // the end pixel's mask byte and quad, issued together: Q is taken
+            // whatever the mask says (a masked end pixel ends the streamline,
+            // and Q is not read again before a refill sets it), so the quad
+            // load is not deferred behind the mask test
             const uint8_t mk = mask[nidx];
-            const Quad Qn = load_quad(quad, nidx);
+            Q = load_quad(quad, nidx);
             if (mk) {
                 active = false;
             } else {
                 const float dl = fast_length_cr<C>(d2.x, d2.y);
                 x = nx;
                 y = ny;
-                Q = Qn;
                 cx = nix;
                 cy = niy;
                 const bool stuck = dl < P.integration_halt_threshold;

@@ -32,42 +32,44 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
 
 // Correctly rounded sqrt and divide, short forms.  gfx950 has no correctly
 // rounded f32 sqrt or divide instruction; the compiler expands '/' and
-// __builtin_sqrtf into IEEE sequences:
-//   sqrt(d):  s = v_sqrt_f32(d), then s-1ulp or s+1ulp by the signs of the fma
-//             residuals d - (s-1ulp)*s and d - (s+1ulp)*s, between an input
-//             scaling by 2^32 (d < 2^-96) and a pass-through of +-0 and +inf;
-//   a / b:    r = v_rcp_f32(b) refined by one fma Newton step, q = a*r
-//             corrected twice by fma residuals, between v_div_scale (operand
-//             scaling), v_div_fmas and v_div_fixup (special values).
-// For d in [2^-96, 2^100], and for b in [2^-48, 2^50] with |a| in [2^-70, 8b],
-// the scaling and fix-up steps leave every value unchanged, so the short forms
-// below (the same instructions without them) return the same bits; the
-// refined reciprocal of a divisor can then also be shared by two quotients.
-// slm_check_rounding compares them with the operators on the device over
-// every float in range (and 6.6e9 quotients).  Outside the ranges the
-// operators are used.
+// __builtin_sqrtf into IEEE sequences (operand scaling, a hardware estimate
+// refined by fma residuals, special-value fix-ups).  The short forms below are
+// shorter refinements that return the same bits in their ranges:
+//   sqrt_short(d), d in [2^-96, 2^100]: y = v_rsq_f32(d), s = d*y, then one
+//       correction s + (d - s*s)*(y/2);
+//   recip_short(b), b in [2^-48, 2^50]: one fma Newton step on v_rcp_f32,
+//       which is already the correctly rounded 1/b;
+//   div_short(a, b, y) with y = recip_short(b) = RN(1/b), |a| in [2^-70, 8b]:
+//       q = a*y corrected once, q + (a - b*q)*y (Markstein's theorem: one
+//       correction of a quotient from the correctly rounded reciprocal is
+//       the correctly rounded quotient).  One reciprocal serves two quotients.
+// slm_check_rounding compares them with the operators on the device: every
+// float in the sqrt and reciprocal ranges, every dividend significand for
+// 2048 divisor significands (half of them the largest, where the first
+// quotient estimate is worst), and 6.6e9 hashed quotients.  The guarded
+// wrappers below (sqrt_cr, inv_sqrt_cr, div_by_length) use the operators
+// outside the ranges; sqrt_short itself is only called inside them.
+// Each form trades the compares and selects of the IEEE sequences for
+// multiplies, which gfx950 can issue two at a time.
 __device__ __forceinline__ bool sqrt_short_ok(float d) { return d >= 0x1p-96f && d <= 0x1p100f; }
 __device__ __forceinline__ float sqrt_short(float d) {
-    const float s = __builtin_amdgcn_sqrtf(d);
-    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
-    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
-    const float rm = __builtin_fmaf(-sm, s, d);
-    const float rp = __builtin_fmaf(-sp, s, d);
-    const float t = (rm <= 0.0f) ? sm : s;
-    return (rp > 0.0f) ? sp : t;
+    const float y = __builtin_amdgcn_rsqf(d);
+    const float s = d * y;
+    const float h = 0.5f * y;
+    const float r = __builtin_fmaf(-s, s, d);
+    return __builtin_fmaf(r, h, s);
 }
-// refined reciprocal of the divide sequence (an input of div_short, not 1/b)
+// RN(1/b) for b in [2^-48, 2^50]
 __device__ __forceinline__ float recip_short(float b) {
     const float r = __builtin_amdgcn_rcpf(b);
     const float e = __builtin_fmaf(-b, r, 1.0f);
     return __builtin_fmaf(e, r, r);
 }
-__device__ __forceinline__ float div_short(float a, float b, float rb) {
-    float q = a * rb;
-    float e = __builtin_fmaf(-b, q, a);
-    q = __builtin_fmaf(e, rb, q);
-    e = __builtin_fmaf(-b, q, a);
-    return __builtin_fmaf(e, rb, q);
+// RN(a/b) from y = recip_short(b)
+__device__ __forceinline__ float div_short(float a, float b, float y) {
+    const float q = a * y;
+    const float e = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(e, y, q);
 }
 // sqrt(d) and 1/sqrt(d), each rounding as the operators do
 __device__ __forceinline__ float sqrt_cr(float d) {
@@ -76,8 +78,7 @@ __device__ __forceinline__ float sqrt_cr(float d) {
 }
 __device__ __forceinline__ float inv_sqrt_cr(float d) {
     if (__builtin_expect(sqrt_short_ok(d), 1)) {
-        const float s = sqrt_short(d);   // in [2^-48, 2^50]
-        return div_short(1.0f, s, recip_short(s));
+        return recip_short(sqrt_short(d));   // the root is in [2^-48, 2^50]
     }
     return fdiv(1.0f, fsqrt(d));
 }
@@ -101,11 +102,27 @@ __device__ __forceinline__ float2 div_by_length(float x, float y) {
     return make_float2(fdiv(x, len), fdiv(y, len));
 }
 
+// The hardware float -> integer conversions: v_cvt_u32_f32 / v_cvt_i32_f32
+// truncate toward zero, saturate out-of-range values (infinities included) to
+// the type's range and map NaN to 0.  C++ leaves those inputs undefined, so the
+// instructions are issued directly; one instruction replaces the compare and
+// select a defined C++ form needs.
+__device__ __forceinline__ uint32_t cvt_u32_sat(float f) {
+    uint32_t r;
+    __asm__("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
+__device__ __forceinline__ int32_t cvt_i32_sat(float f) {
+    int32_t r;
+    __asm__("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+    return r;
+}
+
 // Pixel coordinate of a position: min(n-1, (uint)max(0, v+PAD_WIDTH_PP5))
-// (essentials.cl:26-27, :57-58).  t is never NaN after fmaxf.
+// (essentials.cl:26-27, :57-58), the conversion saturating (negative and NaN
+// sums give 0, so the max is the conversion's own).
 __device__ __forceinline__ uint32_t pix(float v, float pp5, uint32_t nm1) {
-    const float t = fmaxf(0.0f, v + pp5);
-    return (t >= (float)nm1) ? nm1 : (uint32_t)t;
+    return min(cvt_u32_sat(v + pp5), nm1);
 }
 
 // get_array_idx (essentials.cl:56-59)
@@ -114,8 +131,8 @@ __device__ __forceinline__ uint32_t array_idx(const SlmParams& P, float x, float
            P.ny_padded * pix(x, P.pad_width_pp5, P.nx_padded - 1u);
 }
 
-// (int)f with NaN -> 0; callers keep |f| < 2^31.
-__device__ __forceinline__ int32_t cvt_i32(float f) { return (int32_t)((f == f) ? f : 0.0f); }
+// (int)f: saturating truncation, NaN -> 0
+__device__ __forceinline__ int32_t cvt_i32(float f) { return cvt_i32_sat(f); }
 
 // approximate (essentials.cl:104-109): (char)(v*RES)/RES.  For RES = 128 the
 // division is an exact scaling, done as a multiply.
@@ -316,11 +333,8 @@ __device__ __forceinline__ float step_error(const SlmParams& P, const StepOut& o
 __device__ __forceinline__ float next_dt(const SlmParams& P, float dt, float err) {
     return (err == 0.0f) ? P.dt_max : fminf(P.dt_max, fdiv(P.adjusted_max_error * dt, err));
 }
-// (uint)(l+0.5f) of writearray.cl:37, :78 (l >= 0, or NaN -> 0)
-__device__ __forceinline__ uint32_t slt_inc(float l) {
-    const float t = l + 0.5f;
-    return (t > 0.0f) ? ((t >= 4294967296.0f) ? 0xffffffffu : (uint32_t)t) : 0u;
-}
+// (uint)(l+0.5f) of writearray.cl:37, :78 (saturating, NaN -> 0)
+__device__ __forceinline__ uint32_t slt_inc(float l) { return cvt_u32_sat(l + 0.5f); }
 // sub-pixel start offset i*STEP-HALFSPAN (integratefields.cl:108-110)
 template <bool C>
 __device__ __forceinline__ float sub_offset(const SlmParams& P, uint32_t i) {
@@ -345,6 +359,26 @@ __device__ __forceinline__ Quad load_quad(const Quad* __restrict__ q, uint32_t i
     Quad r;
     r.lo = q[idx].lo;
     r.hi = q[idx].hi;
+    return r;
+}
+
+// Global-memory views of pointers a kernel loads from a descriptor.  The
+// compiler cannot tell where such a pointer points and issues flat loads, which
+// count against the LDS wait counter as well as the vector-memory one: every
+// wait for an LDS result then also waits for the flat loads in flight.  The
+// buffers are device allocations, so the global forms are exact.
+typedef const __attribute__((address_space(1))) Quad* GQuadPtr;
+typedef const __attribute__((address_space(1))) uint8_t* GBytePtr;
+__device__ __forceinline__ GQuadPtr as_global(const Quad* p) { return (GQuadPtr)p; }
+__device__ __forceinline__ GBytePtr as_global(const uint8_t* p) { return (GBytePtr)p; }
+typedef float SlmF4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) SlmF4* GF4Ptr;
+__device__ __forceinline__ Quad load_quad(GQuadPtr q, uint32_t idx) {
+    const GF4Ptr p = (GF4Ptr)(q + idx);
+    const SlmF4 a = p[0], b = p[1];
+    Quad r;
+    r.lo = make_float4(a.x, a.y, a.z, a.w);
+    r.hi = make_float4(b.x, b.y, b.z, b.w);
     return r;
 }
 
