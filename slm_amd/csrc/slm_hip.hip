@@ -2208,8 +2208,8 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 c_next += take;
             }
         }
-        bool f_has = false;
-        uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
+        bool f_has = false, stepped = false;
+        uint32_t nidx = 0, nix = 0, niy = 0;
         if (active) {
             // compute_step_vec_jittered (computestep.cl:86-106) on quads, with
             // the short sqrt / divide forms where their inputs allow
@@ -2225,23 +2225,25 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
             const float2 d2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, d1.x), res, inv_res),
                                           approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, d1.y), res, inv_res));
             const float nx = x + d2.x, ny = y + d2.y;
-            const uint32_t nix = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
-            const uint32_t niy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
-            const uint32_t nidx = niy + P.ny_padded * nix;
+            nix = pix(nx, P.pad_width_pp5, P.nx_padded - 1u);
+            niy = pix(ny, P.pad_width_pp5, P.ny_padded - 1u);
+            nidx = niy + P.ny_padded * nix;
             // the end pixel's mask byte and quad, issued together: Q is taken
             // whatever the mask says (a masked end pixel ends the streamline,
             // and Q is not read again before a refill sets it), so the quad
             // load is not deferred behind the mask test
             const uint8_t mk = mask[nidx];
             Q = load_quad(quad, nidx);
+            // the position and its pixel advance whatever the mask says, for
+            // the same reason
+            x = nx;
+            y = ny;
+            cx = nix;
+            cy = niy;
             if (mk) {
                 active = false;
             } else {
                 const float dl = fast_length_cr<C>(d2.x, d2.y);
-                x = nx;
-                y = ny;
-                cx = nix;
-                cy = niy;
                 const bool stuck = dl < P.integration_halt_threshold;
                 if (!stuck && !dt_fixed) {
                     StepOut o;
@@ -2251,23 +2253,24 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 }
                 l += dl;
                 n += 1u;
-                if (nidx != run_idx) {
-                    f_has = true;
-                    f_idx = run_idx;
-                    f_slot = run_slot;
-                    f_n = run_n;
-                    f_t = run_t;
-                    run_idx = nidx;
-                    run_slot = slot_of_t<SLOTS>(nix, niy);
-                    run_n = 0;
-                    run_t = 0;
-                }
-                run_n += 1u;
-                run_t += slt_inc(l);
+                stepped = true;
+                f_has = nidx != run_idx;
                 active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
             }
         }
-        emit(f_has, f_idx, f_slot, f_n, f_t);
+        // a step into another pixel closes the run of the one it left; the
+        // run state is passed as it stands, and the new run starts after
+        emit(f_has, run_idx, run_slot, run_n, run_t);
+        if (f_has) {
+            run_idx = nidx;
+            run_slot = slot_of_t<SLOTS>(nix, niy);
+            run_n = 0;
+            run_t = 0;
+        }
+        if (stepped) {
+            run_n += 1u;
+            run_t += slt_inc(l);
+        }
         const bool ended = !active && run_n;
         if (__any(ended)) {
             emit(ended, run_idx, run_slot, run_n, run_t);
