@@ -168,12 +168,14 @@ def cpu_baseline(job, target_s=12.0):
 def tracer_roofline(args, kname, steps_per_launch, mean_launch_s, alg_gbs, stream):
     """Roofline of the fields tracer.  Two resources are compared, each from
     the kernel's mean launch time measured here:
-      - VALU issue: the VALU wave-instructions one launch issues (the
-        counter SQ_INSTS_VALU per launch from the committed PMC summary,
-        divided by what that counter reads per issued instruction on the
-        calibration kernel) per second, against the chip's fastest sustained
-        VALU issue rate, measured now by the calibration kernel over a few
-        instruction kinds;
+      - VALU issue: a SIMD issues VALU work in quad-cycle slots, one
+        instruction or a dual-issued pair per slot.  The slots one launch
+        takes (SQ_INSTS_VALU minus SQ_ACTIVE_INST_VALU2 per launch, from the
+        committed PMC summary, scaled by what SQ_INSTS_VALU reads per issued
+        instruction on the calibration kernel) per second, against the slot
+        rate of the v_fma_f32 calibration kernel (never dual-issued, so one
+        instruction per slot), measured now; the instruction rate against
+        the fastest calibrated kind is kept beside it;
       - HBM: measured DRAM-side bytes per launch (FETCH_SIZE + WRITE_SIZE,
         copy-calibrated, from the same summary) per second, against 8 TB/s.
     The bound is the resource with the larger fraction.  The algorithmic
@@ -195,18 +197,23 @@ def tracer_roofline(args, kname, steps_per_launch, mean_launch_s, alg_gbs, strea
            "mean_launch_ms": mean_launch_s * 1e3, "traffic": traffic, "hbm": hbm,
            "l2_hit_rate": prof.get("TCC_hit_rate")}
     valu = prof.get("SQ_INSTS_VALU_per_launch")
+    dual = prof.get("SQ_ACTIVE_INST_VALU2_per_launch")
     per_instr = cal.get("SQ_INSTS_VALU_per_issued_instruction")
-    if valu and per_instr:
-        # the fastest sustained issue rate of any calibrated VALU instruction
-        # kind (v_mov_b32 issues faster than the 4-cycle arithmetic, and the
-        # tracer's mix holds both), and the v_fma_f32 rate beside it
-        peak, meta = max((valu_peak(stream, kind=k) for k in VALU_PEAK_KINDS), key=lambda r: r[0])
-        fma, _ = valu_peak(stream, kind=0)
-        rate = valu / per_instr / mean_launch_s
-        out["valu"] = {"achieved": rate / 1e9, "peak": peak / 1e9, "unit": "G wave-instructions/s",
-                       "frac": rate / peak, "fma_rate": fma / 1e9, "frac_of_fma_rate": rate / fma,
-                       "instructions_per_launch": valu / per_instr,
-                       "instructions_per_step": valu / per_instr / steps_per_launch, "calibration": meta}
+    if valu and dual is not None and per_instr:
+        instr = valu / per_instr
+        slots = instr - dual / per_instr
+        fma, meta = valu_peak(stream, kind=0)
+        # the fastest sustained instruction rate of any calibrated kind
+        # (v_mov_b32, which pairs), for the instruction view
+        ipeak, _ = max((valu_peak(stream, kind=k) for k in VALU_PEAK_KINDS), key=lambda r: r[0])
+        rate = slots / mean_launch_s
+        irate = instr / mean_launch_s
+        out["valu"] = {"achieved": rate / 1e9, "peak": fma / 1e9, "unit": "G VALU issue slots/s",
+                       "frac": rate / fma, "slots_per_launch": slots, "instructions_per_launch": instr,
+                       "dual_issued_pairs_per_launch": dual / per_instr,
+                       "instructions_per_step": instr / steps_per_launch,
+                       "instruction_rate": irate / 1e9, "fastest_kind_instruction_rate": ipeak / 1e9,
+                       "instruction_frac": irate / ipeak, "calibration": meta}
     if "valu" in out and (hbm["frac"] is None or out["valu"]["frac"] >= hbm["frac"]):
         v = out["valu"]
         out.update({"bound": "valu", "achieved": v["achieved"], "peak": v["peak"], "unit": v["unit"],
