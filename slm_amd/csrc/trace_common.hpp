@@ -416,8 +416,9 @@ __device__ __forceinline__ float2 jitter_unit(const SlmParams& P, float2 u, uint
 }
 
 // interp_quad_px and jitter_unit with the short sqrt / divide forms where
-// their inputs allow (same bits).  Used by k_fields_batch, where they measure
-// faster; in k_fields the operator forms above measure faster.
+// their inputs allow (same bits).  The fields tracers (k_fields,
+// k_fields_batch) use these; the operator forms above serve the kernels whose
+// steps are a small part of their time.
 template <bool C>
 __device__ __forceinline__ float2 interp_quad_px_cr(const SlmParams& P, const Quad& Q, float x, float y, uint32_t xl,
                                                     uint32_t yd) {
