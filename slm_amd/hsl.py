@@ -73,58 +73,82 @@ def colour_labels(n_labels, problems, pairs):
     return colour, n_col
 
 
-def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2,
-                lanes_only=False):
-    """fmm_lengths (mapping.py:841-900) on device tensors: label int32
-    [nx, ny] (signed flank labels), mapping int32 [nx, ny] (uint32 bits),
-    mask bool [nx, ny].  order: 2 (scikit-fmm's default) or 1.  max_group
-    caps the colours solved together (default: all that fit in 8 GiB).
-    lanes_only (test hook): second-order marches one lane per subsegment,
-    without the LDS heaps of the larger ones (the same bits).
-    Returns an FmmResult (host arrays)."""
+@dataclass
+class FmmPending:
+    """slm_fmm_hsl in flight (fmm_start): its device outputs, workspace and
+    stream, held until fmm_finish copies the results back."""
+    sel_np: np.ndarray
+    n_col: int
+    order: int
+    stream: object
+    sums: object
+    cnt: object
+    nch: object
+    nphi: object
+    failed: object
+    ws: object
+
+
+def fmm_start(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2,
+              lanes_only=False):
+    """The first half of fmm_lengths: select and colour the subsegments (host
+    round trips) and launch the solve on stream (default: the current one)
+    without waiting for it.  Work on other streams runs beside the march;
+    fmm_finish(pending) waits for it and returns the FmmResult."""
     import torch
     hip._require_cuda(label, mapping, mask)
     dev = label.device
+    stream = stream if stream is not None else torch.cuda.current_stream(dev)
     nx, ny = label.shape
     lib = hip.load()
-    sel = select_subsegments(label, mask)
-    sel_np = sel.cpu().numpy().astype(np.int32)
-    absl = label.to(torch.int64).abs()
-    n_labels = int(absl.max().item()) + 1 if label.numel() else 1
-    problems = np.unique(np.abs(sel_np.astype(np.int64)))
-    flag = np.full(n_labels, 0xFF, np.uint8)
-    flag[problems] = 0
-    flag_d = torch.from_numpy(flag).to(dev)
-    count = torch.zeros(1, dtype=torch.int32, device=dev)
-    cap = max(1 << 16, 2 * (nx + ny))
-    while True:
-        pairs = torch.empty(cap, dtype=torch.int64, device=dev)
-        hip.check(lib.slm_fmm_pairs(hip._ptr(label), nx, ny, hip._ptr(flag_d), n_labels, hip._ptr(pairs), cap,
-                                    hip._ptr(count), hip._stream(stream), hip.SLM_FLAG_SYNC))
-        found = int(count.item())
-        if found <= cap:
-            break
-        cap = found
-    uniq = torch.unique(pairs[:found]).cpu().numpy() if found else np.zeros(0, np.int64)
-    colour, n_col = colour_labels(n_labels, problems, uniq)
-    colour_d = torch.from_numpy(colour).to(dev)
-    sums = torch.empty(n_labels, dtype=torch.int64, device=dev)
-    cnt = torch.empty(n_labels, dtype=torch.int32, device=dev)
-    nch = torch.empty(n_labels, dtype=torch.int32, device=dev)
-    nphi = torch.empty(n_labels, dtype=torch.int32, device=dev)
-    failed = torch.empty(n_labels, dtype=torch.uint8, device=dev)
-    # all colours solved together when they fit in 8 GiB
-    one = lib.slm_fmm_workspace_size_k(nx, ny, 1) - 512
-    k = max(1, min(int(n_col), (8 << 30) // max(one, 1), int(max_group) if max_group else 1 << 30))
-    ws_bytes = lib.slm_fmm_workspace_size_k(nx, ny, k)
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-    hip.check(lib.slm_fmm_hsl(hip._ptr(label), hip._ptr(mapping), int(is_thinchannel), hip._ptr(colour_d), n_labels,
-                              n_col, nx, ny, ctypes.c_double(float(pixel_size)), int(order), hip._ptr(sums),
-                              hip._ptr(cnt), hip._ptr(nch), hip._ptr(nphi), hip._ptr(failed), hip._ptr(ws), ws_bytes,
-                              hip._stream(stream), hip.SLM_FLAG_SYNC | (hip.SLM_FLAG_FMM_LANES if lanes_only else 0)))
-    s_np, c_np = sums.cpu().numpy(), cnt.cpu().numpy().view(np.uint32)
-    f_np = failed.cpu().numpy() if order == 2 else np.zeros(n_labels, np.uint8)
-    ch_np, ph_np = nch.cpu().numpy().view(np.uint32), nphi.cpu().numpy().view(np.uint32)
+    with torch.cuda.stream(stream):
+        sel = select_subsegments(label, mask)
+        sel_np = sel.cpu().numpy().astype(np.int32)
+        absl = label.to(torch.int64).abs()
+        n_labels = int(absl.max().item()) + 1 if label.numel() else 1
+        problems = np.unique(np.abs(sel_np.astype(np.int64)))
+        flag = np.full(n_labels, 0xFF, np.uint8)
+        flag[problems] = 0
+        flag_d = torch.from_numpy(flag).to(dev)
+        count = torch.zeros(1, dtype=torch.int32, device=dev)
+        cap = max(1 << 16, 2 * (nx + ny))
+        while True:
+            pairs = torch.empty(cap, dtype=torch.int64, device=dev)
+            hip.check(lib.slm_fmm_pairs(hip._ptr(label), nx, ny, hip._ptr(flag_d), n_labels, hip._ptr(pairs), cap,
+                                        hip._ptr(count), hip._stream(stream), hip.SLM_FLAG_SYNC))
+            found = int(count.item())
+            if found <= cap:
+                break
+            cap = found
+        uniq = torch.unique(pairs[:found]).cpu().numpy() if found else np.zeros(0, np.int64)
+        colour, n_col = colour_labels(n_labels, problems, uniq)
+        colour_d = torch.from_numpy(colour).to(dev)
+        sums = torch.empty(n_labels, dtype=torch.int64, device=dev)
+        cnt = torch.empty(n_labels, dtype=torch.int32, device=dev)
+        nch = torch.empty(n_labels, dtype=torch.int32, device=dev)
+        nphi = torch.empty(n_labels, dtype=torch.int32, device=dev)
+        failed = torch.empty(n_labels, dtype=torch.uint8, device=dev)
+        # all colours solved together when they fit in 8 GiB
+        one = lib.slm_fmm_workspace_size_k(nx, ny, 1) - 512
+        k = max(1, min(int(n_col), (8 << 30) // max(one, 1), int(max_group) if max_group else 1 << 30))
+        ws_bytes = lib.slm_fmm_workspace_size_k(nx, ny, k)
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+        hip.check(lib.slm_fmm_hsl(hip._ptr(label), hip._ptr(mapping), int(is_thinchannel), hip._ptr(colour_d),
+                                  n_labels, n_col, nx, ny, ctypes.c_double(float(pixel_size)), int(order),
+                                  hip._ptr(sums), hip._ptr(cnt), hip._ptr(nch), hip._ptr(nphi), hip._ptr(failed),
+                                  hip._ptr(ws), ws_bytes, hip._stream(stream),
+                                  hip.SLM_FLAG_FMM_LANES if lanes_only else 0))
+    return FmmPending(sel_np, int(n_col), int(order), stream, sums, cnt, nch, nphi, failed, ws)
+
+
+def fmm_finish(pend: FmmPending):
+    """Wait for fmm_start's solve and return its FmmResult."""
+    import torch
+    with torch.cuda.stream(pend.stream):
+        s_np, c_np = pend.sums.cpu().numpy(), pend.cnt.cpu().numpy().view(np.uint32)
+        f_np = pend.failed.cpu().numpy() if pend.order == 2 else np.zeros(s_np.shape[0], np.uint8)
+        ch_np, ph_np = pend.nch.cpu().numpy().view(np.uint32), pend.nphi.cpu().numpy().view(np.uint32)
+    sel_np = pend.sel_np
     # per selected label, in the reference's order of operations
     # (2 * mean distance; kept when n_phi / n_channel > 4; int32 truncation)
     a = np.abs(sel_np.astype(np.int64))
@@ -137,7 +161,20 @@ def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, m
     hsl = np.where(keep, h, 0.0).astype(np.int32)
     pos = hsl[hsl > 0]
     mean = float(np.mean(pos)) if pos.size else None
-    return FmmResult(sel_np, hsl, mean, n_col, s_np, c_np, ch_np, ph_np, f_np)
+    return FmmResult(sel_np, hsl, mean, pend.n_col, s_np, c_np, ch_np, ph_np, f_np)
+
+
+def fmm_lengths(label, mapping, mask, is_thinchannel, pixel_size, stream=None, max_group=None, order=2,
+                lanes_only=False):
+    """fmm_lengths (mapping.py:841-900) on device tensors: label int32
+    [nx, ny] (signed flank labels), mapping int32 [nx, ny] (uint32 bits),
+    mask bool [nx, ny].  order: 2 (scikit-fmm's default) or 1.  max_group
+    caps the colours solved together (default: all that fit in 8 GiB).
+    lanes_only (test hook): second-order marches one lane per subsegment,
+    without the LDS heaps of the larger ones (the same bits).
+    Returns an FmmResult (host arrays)."""
+    return fmm_finish(fmm_start(label, mapping, mask, is_thinchannel, pixel_size, stream, max_group, order,
+                                lanes_only))
 
 
 def remap_hsl(label, res: FmmResult, stream=None):
@@ -178,10 +215,16 @@ def parse_hsl(res: FmmResult, coarse_label=0):
 
 def trace_mean_hsl(grid, hsl_averaging_threshold=5.0):
     """remap_hsl's trace_mean_hsl (mapping.py:977-985): numpy mean of the grid
-    values >= the threshold, or None."""
-    g = grid.cpu().numpy() if hasattr(grid, "cpu") else np.asarray(grid)
-    v = g[~np.isnan(g)]
-    v = v[v >= hsl_averaging_threshold]
+    values >= the threshold, or None.  A device grid is filtered on the device
+    (NaN compares false, so the NaN pixels drop out as in the reference's
+    isnan filter) and only the kept values, in the grid's order, come back:
+    numpy averages the same float32 array."""
+    if hasattr(grid, "cpu"):
+        v = grid[grid >= hsl_averaging_threshold].cpu().numpy()
+    else:
+        g = np.asarray(grid)
+        v = g[~np.isnan(g)]
+        v = v[v >= hsl_averaging_threshold]
     return np.mean(v) if v.shape[0] else None
 
 

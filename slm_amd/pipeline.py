@@ -82,41 +82,38 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     mask_b = mask_full[ox:ox + nxp, oy:oy + nyp].contiguous()
     mask = mask_b.to(torch.uint8).reshape(-1)
 
-    st.mark("trajectories")
-    # the two directions on two streams: 1000 trajectories fill a few waves,
-    # so the launches overlap instead of running one after the other
+    st.mark("fields")
+    # the fields pass on the main stream, and beside it the two trajectory
+    # directions (1000 seeds each) on the side stream: the fields pass needs
+    # their statistics only at the finishing stage.  Every output is
+    # allocated before the first launch and stays referenced until the side
+    # stream has joined (a tensor freed while its kernel runs could otherwise
+    # be handed to another output by the caching allocator).
     main = torch.cuda.current_stream(dev)
     side = hip.side_stream(dev)
     info.n_seed_points = np.uint32(n_t)
     ptraj = [kernel_params(info, sign, "INTEGRATE_TRAJECTORY", fp_contract) for sign in (1, -1)]
-    # every output of both directions is allocated before the first launch and
-    # stays referenced until the side stream has joined: a tensor freed while
-    # its kernel runs could otherwise be handed to the other direction's
-    # outputs by the caching allocator
     outs = [(torch.empty((n_t, int(p.max_n_steps), 2), dtype=torch.int8, device=dev),
              torch.empty(n_t, dtype=torch.int16, device=dev),
              torch.empty(n_t, dtype=torch.float32, device=dev)) for p in ptraj]
-    side.wait_stream(main)
-    for p, (traj, ns, ln), stream in zip(ptraj, outs, (main, side)):
-        hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln, stream=stream)
-    main.wait_stream(side)
-    lengths = [o[2] for o in outs]
-    nsteps = [o[1] for o in outs]
-    del outs
-    tl = torch.stack(lengths, 1).cpu().numpy()
-    tn = torch.stack(nsteps, 1).cpu().numpy().view(np.uint16)
-    stats = compute_stats(tl, tn, info.pixel_size)
-
-    st.mark("fields")
     nxy = nxp * nyp
     slc = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     slt = torch.empty((2, nxy), dtype=torch.int32, device=dev)
     steps = torch.zeros(1, dtype=torch.int64, device=dev)
+    side.wait_stream(main)
+    for p, (traj, ns, ln) in zip(ptraj, outs):
+        hip.trace_paths(p, ts, mask, uv.reshape(-1), traj, ns, ln, stream=side)
     info.n_seed_points = np.uint32(job.seeds.shape[0])
     # both directions in one persistent launch (no end-of-launch tail between them)
     jobs = hip.fields_jobs([(kernel_params(info, sign, "INTEGRATE_FIELDS", fp_contract), seeds, mask,
                              uv.reshape(-1), slc[k], slt[k]) for k, sign in enumerate((1, -1))])
     hip.trace_fields_batch(jobs, steps)
+    with torch.cuda.stream(side):
+        tl = torch.stack([o[2] for o in outs], 1).cpu().numpy()
+        tn = torch.stack([o[1] for o in outs], 1).cpu().numpy().view(np.uint16)
+    main.wait_stream(side)
+    del outs
+    stats = compute_stats(tl, tn, info.pixel_size)
 
     st.mark("finish")
     slt_f, sla_f = hip.fields_finish_raw(slc, slt, int(info.subpixel_seed_point_density))
@@ -140,43 +137,66 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
     cm = ChannelMapper(info, mask_b, uv, mapping0, slt=slt_dn, slc=slc_dn, fp_contract=fp_contract)
     ok_chain = cm.map_channels_segments()
 
-    st.mark("hsl")
-    if midslopes:
-        # lengths.hsl (lengths.py): trace down from the mid-slope pixels
-        hs_seeds = pick_seeds(mask=mask_b, map=cm.mapping, flag=info.is_midslope, pad=pad)
-    else:
-        # without a mid-slope map: from every pixel that is not a thin channel
-        hs_seeds = pick_seeds(mask=mask_b, map=~cm.mapping, flag=info.is_thinchannel, pad=pad)
-    info.n_seed_points = np.uint32(hs_seeds.shape[0])
-    p = kernel_params(info, 1, "HILLSLOPE_LENGTHS", fp_contract)
-    hsl = torch.zeros(hs_seeds.shape[0], dtype=torch.float32, device=dev)
-    if hs_seeds.shape[0]:
-        hip.hillslope_lengths(p, hs_seeds, mask, uv.reshape(-1), cm.mapping.reshape(-1),
-                              int(info.is_thinchannel), int(info.is_midslope), hsl)
+    def traced_lengths():
+        # lengths.hsl (lengths.py): trace down from the mid-slope pixels, or
+        # without a mid-slope map from every pixel that is not a thin channel;
+        # then the Gaussian KDE of their logarithms
+        if midslopes:
+            hs = pick_seeds(mask=mask_b, map=cm.mapping, flag=info.is_midslope, pad=pad)
+        else:
+            hs = pick_seeds(mask=mask_b, map=~cm.mapping, flag=info.is_thinchannel, pad=pad)
+        info.n_seed_points = np.uint32(hs.shape[0])
+        p = kernel_params(info, 1, "HILLSLOPE_LENGTHS", fp_contract)
+        length = torch.zeros(hs.shape[0], dtype=torch.float32, device=dev)
+        if hs.shape[0]:
+            hip.hillslope_lengths(p, hs, mask, uv.reshape(-1), cm.mapping.reshape(-1), int(info.is_thinchannel),
+                                  int(info.is_midslope), length)
+        return hs, length
 
-    st.mark("kde")
-    pos = hsl[hsl > 0]
-    dist = None
-    if pos.numel() > 1:
-        logx = torch.log(pos).cpu().numpy().astype(np.float32)
-        dist = Univariate_distribution(logx_array=logx, logy_array=logx, n_hist_bins=512, n_pdf_points=256)
-        dist.compute_kde_opencl(kernel="gaussian", bandwidth=0.15, fp_contract=fp_contract)
+    def kde(length):
+        reached = length[length > 0]
+        d = None
+        if reached.numel() > 1:
+            logx = torch.log(reached).cpu().numpy().astype(np.float32)
+            d = Univariate_distribution(logx_array=logx, logy_array=logx, n_hist_bins=512, n_pdf_points=256)
+            d.compute_kde_opencl(kernel="gaussian", bandwidth=0.15, fp_contract=fp_contract)
+        return reached, d
+
     fmm_res = fmm_grid = fmm_trace_mean = ha_stats = None
-    if fmm:
+    if not fmm:
+        st.mark("hsl")
+        hs_seeds, hsl = traced_lengths()
+        st.mark("kde")
+        pos, dist = kde(hsl)
+    else:
         # the reference's live HSL path (mapping.py:490-494): FMM lengths per
-        # subsegment, remapped onto the grid
+        # subsegment, remapped onto the grid.  The march of the largest
+        # subsegments holds a few CUs for most of the stage, so the traced
+        # lengths, their KDE and the slope and aspect maps of the aspect
+        # stage (none of which reads the FMM result) run beside it on the
+        # side stream.
         st.mark("fmm")
-        fmm_res = fmm_hsl.fmm_lengths(cm.label, cm.mapping, mask_b, int(info.is_thinchannel), job.pixel_size)
+        # the side stream joins the mapping outputs before the solve is queued
+        # on the main stream, so it does not wait for the solve
+        side.wait_stream(main)
+        pend = fmm_hsl.fmm_start(cm.label, cm.mapping, mask_b, int(info.is_thinchannel), job.pixel_size,
+                                 stream=main)
+        pad_w = int(info.pad_width)
+        with torch.cuda.stream(side):
+            hs_seeds, hsl = traced_lengths()
+            pos, dist = kde(hsl)
+            slope_full = fmm_hsl.slope_deg(roi, pad_w, job.pixel_size)
+            slope_c = slope_full[ox:ox + nxp, oy:oy + nyp].contiguous()
+            aspect, aspect_mask = fmm_hsl.map_aspect(slope_c, uv, cm.mapping, int(info.is_channel), 4.0, 50.0,
+                                                     job.pixel_size)
+        fmm_res = fmm_hsl.fmm_finish(pend)
+        del pend
         fmm_grid = fmm_hsl.remap_hsl(cm.label, fmm_res)
         fmm_trace_mean = fmm_hsl.trace_mean_hsl(fmm_grid)
         # map_hsl, map_aspect, combine_hsl_aspect, hsl_aspect_stats (mapping.py:987-1251)
-        st.mark("aspect")
-        pad_w = int(info.pad_width)
-        slope_full = fmm_hsl.slope_deg(roi, pad_w, job.pixel_size)
-        slope_c = slope_full[ox:ox + nxp, oy:oy + nyp].contiguous()
         hsl_smoothed = fmm_hsl.map_hsl(fmm_grid, mask_b, 51, 30, job.pixel_size)
-        aspect, aspect_mask = fmm_hsl.map_aspect(slope_c, uv, cm.mapping, int(info.is_channel), 4.0, 50.0,
-                                                 job.pixel_size)
+        main.wait_stream(side)
+        st.mark("aspect")
         df_ha, _, haa = fmm_hsl.combine_hsl_aspect(aspect.cpu().numpy(), aspect_mask.cpu().numpy(),
                                                    hsl_smoothed.cpu().numpy(), mask_b.cpu().numpy(), pad_w)
         ha_stats = fmm_hsl.hsl_aspect_stats(haa, df_ha)

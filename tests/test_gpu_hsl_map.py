@@ -83,3 +83,24 @@ def test_map_tile_fmm_products(gpu):
     ha = res["hsl_aspect"]
     assert ha["hsl_mean"] > 0 and np.isfinite(ha["hsl_mean_azimuth"])
     assert "fmm" in tim and "aspect" in tim
+
+
+def test_map_tile_fmm_overlap_matches_serial(gpu):
+    """map_tile(fmm=True) runs the traced hillslope lengths, their KDE and the
+    slope and aspect maps on the side stream while the FMM march runs on the
+    main one.  Their results equal those of the serial stages of
+    map_tile(fmm=False), and the FMM results equal a serial fmm_lengths on the
+    same grids."""
+    from slm_amd import hsl as fmm_hsl
+    from slm_amd import pipeline, workloads
+    job = workloads.small_256()
+    ov, _ = pipeline.map_tile(job, fmm=True, return_grids=True)
+    se, _ = pipeline.map_tile(job, fmm=False)
+    for k in ("hsl_seeds", "hsl_reached", "hsl_median_m", "kde_mode_x", "n_segments", "channel_pixels"):
+        assert ov[k] == se[k], k
+    g = ov["grids"]
+    info = job.info
+    ref = fmm_hsl.fmm_lengths(g["label"], g["mapping"], g["mask"], int(info.is_thinchannel), job.pixel_size)
+    assert ov["fmm_subsegments"] == ref.selected.size and ov["fmm_kept"] == int((ref.hsl > 0).sum())
+    assert ov["fmm_mean_hsl_m"] == ref.fmm_mean_hsl
+    assert ov["fmm_trace_mean_hsl_m"] == float(fmm_hsl.trace_mean_hsl(fmm_hsl.remap_hsl(g["label"], ref).cpu().numpy()))

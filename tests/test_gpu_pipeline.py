@@ -17,7 +17,8 @@ def test_map_tile_small_256(gpu):
     job = workloads.small_256(jitter=1.0, d=2)
     res, tim = map_tile(job)
     print(res, {k: round(v["gpu_ms"], 2) for k, v in tim.items()}, flush=True)
-    assert set(tim) == {"uv", "trajectories", "fields", "finish", "mapping", "hsl", "kde"}
+    # the trajectories run beside the fields pass, inside its stage
+    assert set(tim) == {"uv", "fields", "finish", "mapping", "hsl", "kde"}
     assert res["chain_ok"] and res["n_segments"] > 0 and res["channel_pixels"] > 0
     # HSL traces start at the mid-slope pixels (lengths.py) and stop, unmeasured,
     # when they meet another mid-slope line: a share of them reaches a channel
@@ -48,11 +49,12 @@ def test_map_tile_with_conditioning(gpu):
 
 
 def test_two_stream_trajectories_match_serial_calls(gpu):
-    """The pipeline traces the two trajectory directions on two streams. With
-    a short max_length every output (traj [1000, 200, 2] int8 = 400 KB, steps,
-    lengths) sits in the caching allocator's small pool, where a tensor freed
-    while its kernel still ran could be handed to the other direction; two
-    passes in a row must give what serial slm_trace_paths calls give."""
+    """The pipeline traces the two trajectory directions on the side stream
+    while the fields pass runs on the main one. With a short max_length every
+    output (traj [1000, 200, 2] int8 = 400 KB, steps, lengths) sits in the
+    caching allocator's small pool, where a tensor freed while its kernel still
+    ran could be handed to another output; two passes in a row must give what
+    serial slm_trace_paths calls give."""
     import torch
     from slm_amd import hip
     from slm_amd.params import kernel_params
