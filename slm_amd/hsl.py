@@ -298,20 +298,37 @@ def combine_hsl_aspect(aspect, aspect_mask, hsl_smoothed, mask, pad, n_bins=60, 
     (aspect in degrees, HSL >= threshold) pairs sorted by aspect, the mean HSL
     in n_bins + 1 aspect bins centred on -180, ..., 180 degrees (the two end
     bins averaged, since they are one direction), and the [hsl, bin centre in
-    radians] array.  mask: the merged active masks (True = masked).
-    Returns (DataFrame of pairs, per-bin means, averages array)."""
+    radians] array.  mask: the merged active masks (True = masked).  The
+    grids are numpy arrays, or torch tensors (the selection then runs on
+    their device).  Returns (DataFrame of pairs, per-bin means, averages
+    array)."""
     import pandas as pd
-    a = np.asarray(aspect, np.float32)
-    h = np.asarray(hsl_smoothed, np.float32)
     inner = np.index_exp[pad:-pad, pad:-pad]
-    m = np.asarray(mask, bool)[inner] | np.asarray(aspect_mask, bool)[inner]
-    asp, hv = a[inner][~m], h[inner][~m]
-    if hv.shape == (0,):
-        # nothing unmasked: the reference falls back to every pixel
-        asp, hv = a[inner].ravel(), h[inner].ravel()
-    asp = np.rad2deg(asp).astype(np.float32)
-    keep = hv >= hsl_averaging_threshold
-    pairs = np.stack((hv[keep], asp[keep]), axis=1)
+    if hasattr(aspect, "cpu"):
+        # torch tensors (any device): the pixel selection runs where the grids
+        # are, and only the kept pairs, in the same row-major order, come back
+        import torch
+        a, h = aspect.float()[inner], hsl_smoothed.float()[inner]
+        m = mask.bool()[inner] | aspect_mask.bool()[inner]
+        sel = ~m
+        if not bool(sel.any()):
+            # nothing unmasked: the reference falls back to every pixel
+            sel = torch.ones_like(m)
+        sel = sel & (h >= hsl_averaging_threshold)
+        hv = h[sel].cpu().numpy()
+        asp = np.rad2deg(a[sel].cpu().numpy()).astype(np.float32)
+        pairs = np.stack((hv, asp), axis=1)
+    else:
+        a = np.asarray(aspect, np.float32)
+        h = np.asarray(hsl_smoothed, np.float32)
+        m = np.asarray(mask, bool)[inner] | np.asarray(aspect_mask, bool)[inner]
+        asp, hv = a[inner][~m], h[inner][~m]
+        if hv.shape == (0,):
+            # nothing unmasked: the reference falls back to every pixel
+            asp, hv = a[inner].ravel(), h[inner].ravel()
+        asp = np.rad2deg(asp).astype(np.float32)
+        keep = hv >= hsl_averaging_threshold
+        pairs = np.stack((hv[keep], asp[keep]), axis=1)
     df = pd.DataFrame(columns=["hsl", "aspect"], data=pairs[pairs[:, 1].argsort()])
     half = 180 / n_bins
     width = 2 * half

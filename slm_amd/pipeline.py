@@ -197,8 +197,7 @@ def map_tile(job, segmentation_threshold=20, channel_quantile=0.97, fp_contract=
         hsl_smoothed = fmm_hsl.map_hsl(fmm_grid, mask_b, 51, 30, job.pixel_size)
         main.wait_stream(side)
         st.mark("aspect")
-        df_ha, _, haa = fmm_hsl.combine_hsl_aspect(aspect.cpu().numpy(), aspect_mask.cpu().numpy(),
-                                                   hsl_smoothed.cpu().numpy(), mask_b.cpu().numpy(), pad_w)
+        df_ha, _, haa = fmm_hsl.combine_hsl_aspect(aspect, aspect_mask, hsl_smoothed, mask_b, pad_w)
         ha_stats = fmm_hsl.hsl_aspect_stats(haa, df_ha)
     st.mark("end")
     timings = st.report()

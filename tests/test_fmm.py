@@ -154,6 +154,26 @@ def test_combine_hsl_aspect_bins():
     assert np.all(np.diff(df.aspect.values) >= 0)
 
 
+def test_combine_hsl_aspect_tensor_inputs():
+    """Grids given as torch tensors are selected where they live; the pairs,
+    bin means and statistics equal those of the numpy path, with masked
+    pixels, NaN lengths and the all-masked fallback."""
+    import torch
+    from slm_amd import hsl as H
+    aspect, amask, h, mask = _aspect_case(north_extra=5.0, seed=3)
+    rs = np.random.default_rng(4)
+    amask = rs.random(amask.shape) < 0.2
+    mask = rs.random(mask.shape) < 0.1
+    h = np.where(rs.random(h.shape) < 0.05, np.nan, h).astype(np.float32)
+    for m in (mask, np.ones_like(mask)):
+        df0, me0, haa0 = H.combine_hsl_aspect(aspect, amask, h, m, pad=2)
+        t = [torch.from_numpy(np.ascontiguousarray(x)) for x in (aspect, amask, h, m)]
+        df1, me1, haa1 = H.combine_hsl_aspect(*t, pad=2)
+        assert df0.equals(df1) and me0.equals(me1) and np.array_equal(haa0, haa1, equal_nan=True)
+    s0, s1 = H.hsl_aspect_stats(haa0, df0), H.hsl_aspect_stats(haa1, df1)
+    assert all(np.array_equal(np.asarray(s0[k]), np.asarray(s1[k])) for k in s0)
+
+
 def test_hsl_aspect_stats_disparity():
     from slm_amd import hsl as H
 
