@@ -51,7 +51,13 @@ __device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
 // outside the ranges; sqrt_short itself is only called inside them.
 // Each form trades the compares and selects of the IEEE sequences for
 // multiplies, which gfx950 can issue two at a time.
-__device__ __forceinline__ bool sqrt_short_ok(float d) { return d >= 0x1p-96f && d <= 0x1p100f; }
+// d in [2^-96, 2^100] as one unsigned compare of the bits: the bit patterns
+// of non-negative floats are ordered as the values, and negative values, NaN
+// and infinity fall outside (the same result as the two float compares for
+// every bit pattern)
+__device__ __forceinline__ bool sqrt_short_ok(float d) {
+    return __float_as_uint(d) - 0x0f800000u <= 0x71800000u - 0x0f800000u;
+}
 __device__ __forceinline__ float sqrt_short(float d) {
     const float y = __builtin_amdgcn_rsqf(d);
     const float s = d * y;
@@ -166,18 +172,17 @@ __device__ __forceinline__ int8_t compress1(float v, float res) { return (int8_t
 
 // lehmer_rand_uint (rng.cl:37-45): ((u64)(uint)(s+1) * 279470273) mod M with
 // M = 2^32-5, bit-identical to the 64-bit %.  With p = a*279470273 = hi*2^32 +
-// lo, 2^32 == 5 (mod M) gives p == 5*hi + lo; 5*hi < 1.4e9 fits 32 bits, and
-// the carry c of lo + 5*hi folds once more as 5c: u = (lo + 5*hi mod 2^32) +
-// 5c stays below 2^32 (c = 1 leaves the sum below 1.4e9) and == p (mod M).
-// One conditional subtraction of M finishes: u + 5 wraps to u - M exactly
-// when u >= M, so min(u, u + 5) is the residue.  One 32x32->64 multiply; the
-// rest is full-rate 32-bit arithmetic.
+// lo, 2^32 == 5 (mod M) gives p == r = 5*hi + lo, formed by one 64-bit
+// multiply-add; r < 2^32 + 5*2^29, so r = rhi*2^32 + rlo with rhi in {0, 1}
+// and r == u = 5*rhi + rlo (mod M), u < 2^32 (rhi = 1 leaves rlo below
+// 5*2^29).  One conditional subtraction of M finishes: u + 5 wraps to u - M
+// exactly when u >= M, so min(u, u + 5) is the residue.  Checked against the
+// 64-bit % for all 2^32 states.
 __device__ __forceinline__ uint32_t lehmer(uint32_t& s) {
     const uint32_t a = s + 1u;                 // wraps in 32 bits first
     const uint64_t p = (uint64_t)a * 279470273u;
-    const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
-    const uint32_t t = lo + ((hi << 2) + hi);
-    const uint32_t u = t + ((t < lo) ? 5u : 0u);
+    const uint64_t r = (uint64_t)(uint32_t)(p >> 32) * 5u + (uint32_t)p;
+    const uint32_t u = (uint32_t)r + 5u * (uint32_t)(r >> 32);
     s = min(u, u + 5u);
     return s;
 }
