@@ -667,9 +667,9 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 c_next += take;
             }
         }
-        // f_has: the streamline entered another pixel, which ends its run
-        bool f_has = false, stepped = false;
-        uint32_t nidx = 0, nix = 0, niy = 0;
+        // run that ends this step because the streamline entered another pixel
+        bool f_has = false;
+        uint32_t f_idx = 0, f_slot = 0, f_n = 0, f_t = 0;
         if (active) {
             // compute_step_vec_jittered (computestep.cl:86-106) on quads, with
             // the short sqrt / divide forms where their inputs allow
@@ -682,23 +682,20 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
             const float2 d2 = make_float2(approx_r<R2>(0.5f * mad<C>(u2.x * combo, dt, d1.x), res, inv_res),
                                           approx_r<R2>(0.5f * mad<C>(u2.y * combo, dt, d1.y), res, inv_res));
             const float nx = x + d2.x, ny = y + d2.y;
-            nix = pix(nx, P.pad_width_pp5, nxm1);
-            niy = pix(ny, P.pad_width_pp5, nym1);
-            nidx = niy + P.ny_padded * nix;
+            const uint32_t nix = pix(nx, P.pad_width_pp5, nxm1), niy = pix(ny, P.pad_width_pp5, nym1);
+            const uint32_t nidx = niy + P.ny_padded * nix;
             const uint8_t mk = mask[nidx];
-            // the end pixel's quad, position and pixel are taken whatever the
-            // mask says: a masked end pixel ends the streamline, which reads
-            // none of them again before a refill sets them
-            Q = load_quad(quad, nidx);
-            x = nx;
-            y = ny;
-            cx = nix;
-            cy = niy;
+            const Quad Qn = load_quad(quad, nidx);
             if (mk) {
                 active = false;  // end pixel masked: step dropped, loop ends
             } else {
                 // runge_kutta_step_write_sl_data (rungekutta.cl:163-186)
                 const float dl = fast_length_cr<C>(d2.x, d2.y);
+                x = nx;
+                y = ny;
+                Q = Qn;
+                cx = nix;
+                cy = niy;
                 const bool stuck = dl < P.integration_halt_threshold;
                 if (!stuck && !dt_fixed) {
                     StepOut o;
@@ -708,25 +705,24 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
                 }
                 l += dl;
                 n += 1u;
-                stepped = true;
-                f_has = nidx != run_idx;
+                // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced
+                if (nidx != run_idx) {
+                    f_has = true;
+                    f_idx = run_idx;
+                    f_slot = run_slot;
+                    f_n = run_n;
+                    f_t = run_t;
+                    run_idx = nidx;
+                    run_slot = slot_of(nix, niy);
+                    run_n = 0;
+                    run_t = 0;
+                }
+                run_n += 1u;
+                run_t += slt_inc(l);
                 active = !stuck && (l < P.max_length) && (n < P.max_n_steps);
             }
         }
-        // check_atomic_write_sl_data (writearray.cl:70-82), run-coalesced: a
-        // step into another pixel closes the run of the one it left, passed as
-        // it stands; the new run starts after
-        emit(f_has, run_idx, run_slot, run_n, run_t);
-        if (f_has) {
-            run_idx = nidx;
-            run_slot = slot_of(nix, niy);
-            run_n = 0;
-            run_t = 0;
-        }
-        if (stepped) {
-            run_n += 1u;
-            run_t += slt_inc(l);
-        }
+        emit(f_has, f_idx, f_slot, f_n, f_t);
         // runs of streamlines that ended (this step, or at their seed)
         const bool ended = !active && run_n;
         if (__any(ended)) {
