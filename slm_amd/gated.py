@@ -420,16 +420,18 @@ def plain_map(grid, theta, steps=None, workspace=None):
 
 
 def _tile_check(pred, truth, valid, max_err, min_f1, min_channel_px):
-    """Per-tile check of predicted against traced fields (views [3, a, b])."""
+    """Per-tile check of predicted against traced fields (views [3, a, b]).
+    The median error and the confusion counts come back in one copy."""
     import torch
     v = valid
     t1 = truth[1][v]
     err = ((pred[1][v] - t1).abs() / t1.clamp_min(1e-6))
-    med = float(err.median().item()) if err.numel() else 0.0
     pc, tc = (pred[2] > 0) & v, (truth[2] > 0) & v
-    tp = int((pc & tc).sum().item())
-    fp = int((pc & ~tc).sum().item())
-    fn = int((~pc & tc).sum().item())
+    med_t = err.median().double() if err.numel() else torch.zeros((), dtype=torch.float64, device=err.device)
+    got = torch.stack([med_t, (pc & tc).sum().double(), (pc & ~tc).sum().double(),
+                       (~pc & tc).sum().double()]).cpu().numpy()
+    med = float(np.float32(got[0]))
+    tp, fp, fn = int(got[1]), int(got[2]), int(got[3])
     f1 = 2 * tp / max(1, 2 * tp + fp + fn) if (tp + fp + fn) else 1.0
     ok = med <= max_err and (f1 >= min_f1 or (tp + fn) < min_channel_px)
     return ok, med, f1, tp + fn
@@ -497,27 +499,35 @@ def gated_map(net, grid, theta, size=256, halo=16, batch=64, check_frac=0.02, rn
     channel pixels), fails.  When more than max_fail_frac of the checked
     tiles fail, the map is plain_map and no other tile is predicted;
     otherwise the other tiles are predicted and each failed checked tile
-    keeps its trace.  The checks stop at the failure that decides the
-    fallback (report["evaluated"] of report["checked"]).  Returns (fields
-    [3, nxy], report)."""
+    keeps its trace.  The checks run in their order and stop at the failure
+    that decides the fallback (report["evaluated"] of report["checked"]);
+    the first floor(max_fail_frac * k) + 1 of them, the fewest that can
+    decide it, are predicted, set up and traced first, and the others only
+    when the map has not fallen back by then.  Returns (fields [3, nxy],
+    report)."""
     import torch
     origins = tile_origins(grid.nx, grid.ny, size, halo)
     n_tiles = len(origins)
     chosen = _chosen(n_tiles, check_frac, rng_seed)
     k = len(chosen)
     out = torch.empty((3, grid.nxy), dtype=torch.float32, device=grid.dev)
-    predict_tiles(net, grid, origins[chosen], size, halo, batch, out)
-    jobs, cores, steps = _check_windows(grid, origins, chosen, size, halo)
-    checks, n_fail = [], 0
-    for check, pr, tr in _checked(grid, jobs, cores, theta, out, max_err, min_f1, min_channel_px):
-        checks.append(check)
-        if not check["ok"]:
-            n_fail += 1
-            if n_fail > max_fail_frac * k:
-                break   # the map falls back whatever the remaining checks say
-            pr.copy_(tr)
+    first = min(k, int(math.floor(max_fail_frac * k)) + 1)
+    checks, n_fail, check_steps = [], 0, 0
+    for part in (chosen[:first], chosen[first:]):
+        if not len(part) or n_fail > max_fail_frac * k:
+            continue
+        predict_tiles(net, grid, origins[part], size, halo, batch, out)
+        jobs, cores, steps = _check_windows(grid, origins, part, size, halo)
+        check_steps += int(steps.item())
+        for check, pr, tr in _checked(grid, jobs, cores, theta, out, max_err, min_f1, min_channel_px):
+            checks.append(check)
+            if not check["ok"]:
+                n_fail += 1
+                if n_fail > max_fail_frac * k:
+                    break   # the map falls back whatever the remaining checks say
+                pr.copy_(tr)
     report = {"tiles": n_tiles, "checked": k, "evaluated": len(checks), "failed": n_fail,
-              "check_steps": int(steps.item()), "fallback": False, "checks": checks}
+              "check_steps": check_steps, "fallback": False, "checks": checks}
     if n_fail > max_fail_frac * k:
         out, _ = plain_map(grid, theta, workspace=workspace)
         report["fallback"] = True
