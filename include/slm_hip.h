@@ -606,7 +606,9 @@ int slm_nn_relu_bwd(const void* y, int ldy, void* dy, int lddy, int M, int C, vo
 /* 2x2 max pool, stride 2; H, W are the input's (even) sizes. */
 int slm_nn_pool(const void* x, int ldx, void* y, int ldy, int N, int H, int W, int C, void* stream,
                 uint32_t flags);
-/* Max-pool backward: each window's gradient goes to its first maximum (row-major). */
+/* Max-pool backward: each window's gradient goes to its first maximum (row-major),
+ * the window's other pixels get 0; with SLM_FLAG_NO_ZERO it is added to dx
+ * instead (dx + gradient, dx + 0 elsewhere). */
 int slm_nn_pool_bwd(const void* x, int ldx, const void* y, int ldy, const void* dy, int lddy, void* dx,
                     int lddx, int N, int H, int W, int C, void* stream, uint32_t flags);
 /* Nearest 2x upsample; H, W are the output's sizes. */
@@ -629,6 +631,22 @@ int slm_nn_loss(const void* head, int ldh, const float* target, const uint8_t* v
  * writes the bf16 copy pb when not NULL. */
 int slm_nn_adam(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1,
                 float b2, float eps, int step, void* stream, uint32_t flags);
+
+/* One layer's weights within the flat parameter vector of slm_nn_adam_flip:
+ * p[offset .. offset + cout*taps*cin) as [cout][taps][cin]; wt receives them
+ * as slm_nn_wflip writes them (bf16 [cin][taps][cout], taps reversed). */
+typedef struct SlmNnFlip {
+    uint64_t offset;
+    uint32_t taps, cin, cout, reserved;
+    void* wt;
+} SlmNnFlip;
+
+/* slm_nn_adam that also writes the flipped and transposed bf16 weights of up
+ * to 32 layers (the backward-data weights), from the same bf16 values as pb:
+ * one launch instead of one slm_nn_wflip per layer and step. */
+int slm_nn_adam_flip(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1,
+                     float b2, float eps, int step, const SlmNnFlip* layers, int n_layers, void* stream,
+                     uint32_t flags);
 
 /* Tiles: S x S windows of an nx x ny grid (x-major, pixel (i, j) at i*ny + j)
  * with top-left corners tiles[2t], tiles[2t+1] (may lie outside the grid;

@@ -653,7 +653,10 @@ __global__ void k_pool(const bf16* __restrict__ x, int ldx, bf16* __restrict__ y
     y[q * ldy + c] = f2b(m);
 }
 
-// gradient to the first maximum of each 2x2 window (row-major), others 0
+// gradient to the first maximum of each 2x2 window (row-major), others 0;
+// ACC: added to dx instead (dx + g, or dx + 0: what adding a written
+// gradient to dx gives, in one pass)
+template <bool ACC>
 __global__ void k_pool_bwd(const bf16* __restrict__ x, int ldx, const bf16* __restrict__ y, int ldy,
                            const bf16* __restrict__ dy, int lddy, bf16* __restrict__ dx, int lddx, int N, int H,
                            int W, int C) {
@@ -670,7 +673,9 @@ __global__ void k_pool_bwd(const bf16* __restrict__ x, int ldx, const bf16* __re
         for (int b = 0; b < 2; b++) {
             const size_t o = (size_t)(n * H + 2 * ho + a) * W + 2 * wo + b;
             const bool hit = !done && b2f(x[o * ldx + c]) == m;
-            dx[o * lddx + c] = hit ? g : f2b(0.0f);
+            const bf16 r = hit ? g : f2b(0.0f);
+            if constexpr (ACC) dx[o * lddx + c] = f2b(b2f(dx[o * lddx + c]) + b2f(r));
+            else dx[o * lddx + c] = r;
             done = done || hit;
         }
 }
@@ -778,6 +783,39 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
     if (pb) pb[i] = f2b(pi);
+}
+
+// k_adam, also writing each listed layer's weights flipped and transposed for
+// the backward-data convolution (k_wflip's wt from the new bf16 value)
+constexpr int kFlipMax = 32;
+struct FlipTable {
+    SlmNnFlip l[kFlipMax];
+    int n;
+};
+__global__ void k_adam_flip(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, bf16* __restrict__ pb, int n, float lr, float b1, float b2,
+                            float eps, float c1, float c2, FlipTable T) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    const bf16 w = f2b(pi);
+    pb[i] = w;
+    for (int k = 0; k < T.n; k++) {
+        const SlmNnFlip& L = T.l[k];
+        const int nw = (int)(L.cout * L.taps * L.cin);
+        const int j = i - (int)L.offset;
+        if (j < 0 || j >= nw) continue;
+        const int cin = (int)L.cin, taps = (int)L.taps, cout = (int)L.cout;
+        const int ci = j % cin, tap = (j / cin) % taps, co = j / (cin * taps);
+        ((bf16*)L.wt)[((size_t)ci * taps + (taps - 1 - tap)) * cout + co] = w;
+        break;
+    }
 }
 
 // ------------------------------------------------------------------ tiles
@@ -1120,8 +1158,14 @@ extern "C" int slm_nn_pool(const void* x, int ldx, void* y, int ldy, int N, int 
 extern "C" int slm_nn_pool_bwd(const void* x, int ldx, const void* y, int ldy, const void* dy, int lddy, void* dx,
                                int lddx, int N, int H, int W, int C, void* stream, uint32_t flags) {
     if (!x || !y || !dy || !dx || H % 2 || W % 2) return slm_fail(SLM_E_INVALID_ARG, "bad pool_bwd arguments");
-    hipLaunchKernelGGL(k_pool_bwd, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0, (hipStream_t)stream,
-                       (const bf16*)x, ldx, (const bf16*)y, ldy, (const bf16*)dy, lddy, (bf16*)dx, lddx, N, H, W, C);
+    if (flags & SLM_FLAG_NO_ZERO)
+        hipLaunchKernelGGL(k_pool_bwd<true>, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0,
+                           (hipStream_t)stream, (const bf16*)x, ldx, (const bf16*)y, ldy, (const bf16*)dy, lddy,
+                           (bf16*)dx, lddx, N, H, W, C);
+    else
+        hipLaunchKernelGGL(k_pool_bwd<false>, g1((size_t)N * (H / 2) * (W / 2) * C), dim3(256), 0,
+                           (hipStream_t)stream, (const bf16*)x, ldx, (const bf16*)y, ldy, (const bf16*)dy, lddy,
+                           (bf16*)dx, lddx, N, H, W, C);
     return slm_finish(stream, flags);
 }
 
@@ -1164,6 +1208,27 @@ extern "C" int slm_nn_adam(float* p, const float* g, float* m, float* v, void* p
     const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
     hipLaunchKernelGGL(k_adam, g1((size_t)n), dim3(256), 0, (hipStream_t)stream, p, g, m, v, (bf16*)pb, n, lr, b1, b2,
                        eps, c1, c2);
+    return slm_finish(stream, flags);
+}
+
+extern "C" int slm_nn_adam_flip(float* p, const float* g, float* m, float* v, void* pb, int n, float lr, float b1,
+                                float b2, float eps, int step, const SlmNnFlip* layers, int n_layers, void* stream,
+                                uint32_t flags) {
+    if (!p || !g || !m || !v || !pb || n <= 0 || step <= 0 || n_layers < 0 || n_layers > kFlipMax ||
+        (n_layers && !layers))
+        return slm_fail(SLM_E_INVALID_ARG, "bad adam_flip arguments");
+    FlipTable T;
+    T.n = n_layers;
+    for (int k = 0; k < n_layers; k++) {
+        const SlmNnFlip& L = layers[k];
+        if (!L.wt || (L.taps != 1 && L.taps != 9) || !L.cin || !L.cout ||
+            L.offset + (uint64_t)L.cout * L.taps * L.cin > (uint64_t)n)
+            return slm_fail(SLM_E_INVALID_ARG, "adam_flip layer %d: bad shape, offset or buffer", k);
+        T.l[k] = L;
+    }
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam_flip, g1((size_t)n), dim3(256), 0, (hipStream_t)stream, p, g, m, v, (bf16*)pb, n, lr, b1,
+                       b2, eps, c1, c2, T);
     return slm_finish(stream, flags);
 }
 

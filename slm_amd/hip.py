@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .params import SlmFieldsJob, SlmBatchOpts, SlmKdeParams, SlmParams, SlmTileOpts, ABI_VERSION
+from .params import SlmFieldsJob, SlmBatchOpts, SlmKdeParams, SlmNnFlip, SlmParams, SlmTileOpts, ABI_VERSION
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libslm_hip.so")
@@ -69,6 +69,8 @@ _PROTOS = {
                                    _U32]),
     "slm_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                    ctypes.c_float, _I, _P, _U32]),
+    "slm_nn_adam_flip": (ctypes.c_int, [_P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                        ctypes.c_float, _I, ctypes.POINTER(SlmNnFlip), _I, _P, _U32]),
     "slm_nn_tile_inputs": (ctypes.c_int, [_P, _P, _P, _I, _I, _P, _I, _I, ctypes.c_float, _P, _P, _P, _U32]),
     "slm_nn_tile_targets": (ctypes.c_int, [_P, _P, _P, _I, _I, _P, _I, _I, _I, ctypes.c_float, _P, _P, _P, _U32]),
     "slm_nn_tile_stitch": (ctypes.c_int, [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P, _U32]),

@@ -82,6 +82,15 @@ class SlmKdeParams(ctypes.Structure):
 assert ctypes.sizeof(SlmKdeParams) == 64
 
 
+class SlmNnFlip(ctypes.Structure):
+    """Mirror of ``struct SlmNnFlip`` in include/slm_hip.h (32 bytes)."""
+    _fields_ = [("offset", ctypes.c_uint64), ("taps", ctypes.c_uint32), ("cin", ctypes.c_uint32),
+                ("cout", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("wt", ctypes.c_void_p)]
+
+
+assert ctypes.sizeof(SlmNnFlip) == 32
+
+
 class SlmTileOpts(ctypes.Structure):
     """Mirror of ``struct SlmTileOpts`` in include/slm_hip.h (32 bytes)."""
     _fields_ = [("tile", ctypes.c_uint32), ("block", ctypes.c_uint32), ("unit", ctypes.c_uint32),

@@ -81,8 +81,10 @@ def pool(x, ldx, y, ldy, N, H, W, C):
     hip.check(hip.load().slm_nn_pool(_p(x), ldx, _p(y), ldy, N, H, W, C, _s(), 0))
 
 
-def pool_bwd(x, ldx, y, ldy, dy, lddy, dx, lddx, N, H, W, C):
-    hip.check(hip.load().slm_nn_pool_bwd(_p(x), ldx, _p(y), ldy, _p(dy), lddy, _p(dx), lddx, N, H, W, C, _s(), 0))
+def pool_bwd(x, ldx, y, ldy, dy, lddy, dx, lddx, N, H, W, C, accumulate=False):
+    """accumulate: add the routed gradient to dx instead of writing it."""
+    hip.check(hip.load().slm_nn_pool_bwd(_p(x), ldx, _p(y), ldy, _p(dy), lddy, _p(dx), lddx, N, H, W, C, _s(),
+                                         hip.SLM_FLAG_NO_ZERO if accumulate else 0))
 
 
 def up(x, ldx, y, ldy, N, H, W, C):
@@ -165,6 +167,14 @@ class UNet:
         self.pb = self.p.to(torch.bfloat16)          # bf16 copy (biases unused in bf16)
         self.wt = {name: torch.empty(cout * taps * cin, dtype=torch.bfloat16, device=self.dev)
                    for name, cin, cout, taps, _ in LAYERS}
+        if not inference:
+            # the backward-data weights: flipped here once, then by every Adam step
+            self.flip = (hip.SlmNnFlip * len(LAYERS))()
+            for k, (name, cin, cout, taps, _) in enumerate(LAYERS):
+                wflip(self.w(name), self.wt[name], taps, cin, cout)
+                self.flip[k].offset, self.flip[k].taps = self.offsets[name][0], taps
+                self.flip[k].cin, self.flip[k].cout = cin, cout
+                self.flip[k].wt = self.wt[name].data_ptr()
         self.step_count = 0
         self.shape = None
 
@@ -225,9 +235,6 @@ class UNet:
             return
         # gradient buffers
         G = {k: t.empty_like(v) for k, v in A.items() if k not in ("x",)}
-        G["tmp0"] = e(N, H, W, 32)
-        G["tmp1"] = e(N, H2, W2, 64)
-        G["tmp2"] = e(N, H4, W4, 128)
         self.G = G
 
     def forward(self, x_nhwc, N, H, W):
@@ -287,7 +294,7 @@ class UNet:
         relu_bias_grad(yout if relu else None, ldy, dy, lddy, self.gb(name), M, cout, self.partials)
         wgrad(xin, ldx, dy, lddy, self.gw(name), None, n, h, w, cin, cout, taps)
         if need_dx:
-            wflip(self.w(name), self.wt[name], taps, cin, cout)
+            # self.wt[name]: the flipped weights, current since the last Adam step
             conv(dy, lddy, self.wt[name], None, dx, lddx, n, h, w, cout, cin, taps, False)
 
     def backward(self, dhead):
@@ -313,18 +320,15 @@ class UNet:
         cb("bna", A["p2"], 128, A["bna"], 256, G["bna"], 256, G["p2"], 128, N, H8, W8)
         # e2b's output gradient = skip slice of dcat2 + pool backward
         cat2_e2, gcat2_e2 = A["cat2"][:, 256:], G["cat2"][:, 256:]
-        pool_bwd(cat2_e2, 384, A["p2"], 128, G["p2"], 128, G["tmp2"], 128, N, H4, W4, 128)
-        add(gcat2_e2, 384, G["tmp2"], 128, N * H4 * W4, 128)
+        pool_bwd(cat2_e2, 384, A["p2"], 128, G["p2"], 128, gcat2_e2, 384, N, H4, W4, 128, accumulate=True)
         cb("e2b", A["e2a"], 128, cat2_e2, 384, gcat2_e2, 384, G["e2a"], 128, N, H4, W4)
         cb("e2a", A["p1"], 64, A["e2a"], 128, G["e2a"], 128, G["p1"], 64, N, H4, W4)
         cat1_e1, gcat1_e1 = A["cat1"][:, 128:], G["cat1"][:, 128:]
-        pool_bwd(cat1_e1, 192, A["p1"], 64, G["p1"], 64, G["tmp1"], 64, N, H2, W2, 64)
-        add(gcat1_e1, 192, G["tmp1"], 64, N * H2 * W2, 64)
+        pool_bwd(cat1_e1, 192, A["p1"], 64, G["p1"], 64, gcat1_e1, 192, N, H2, W2, 64, accumulate=True)
         cb("e1b", A["e1a"], 64, cat1_e1, 192, gcat1_e1, 192, G["e1a"], 64, N, H2, W2)
         cb("e1a", A["p0"], 32, A["e1a"], 64, G["e1a"], 64, G["p0"], 32, N, H2, W2)
         cat0_e0, gcat0_e0 = A["cat0"][:, 64:], G["cat0"][:, 64:]
-        pool_bwd(cat0_e0, 96, A["p0"], 32, G["p0"], 32, G["tmp0"], 32, N, H, W, 32)
-        add(gcat0_e0, 96, G["tmp0"], 32, N * H * W, 32)
+        pool_bwd(cat0_e0, 96, A["p0"], 32, G["p0"], 32, gcat0_e0, 96, N, H, W, 32, accumulate=True)
         cb("e0b", A["e0a"], 32, cat0_e0, 96, gcat0_e0, 96, G["e0a"], 32, N, H, W)
         cb("e0a", A["x"], IN_CH, A["e0a"], 32, G["e0a"], 32, None, 0, N, H, W, need_dx=False)
 
@@ -349,9 +353,10 @@ class UNet:
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
-        hip.check(hip.load().slm_nn_adam(_p(self.p), _p(self.g), _p(self.m), _p(self.v), _p(self.pb), self.n_params,
-                                         ctypes.c_float(lr), ctypes.c_float(b1), ctypes.c_float(b2),
-                                         ctypes.c_float(eps), self.step_count, _s(), 0))
+        hip.check(hip.load().slm_nn_adam_flip(_p(self.p), _p(self.g), _p(self.m), _p(self.v), _p(self.pb),
+                                              self.n_params, ctypes.c_float(lr), ctypes.c_float(b1),
+                                              ctypes.c_float(b2), ctypes.c_float(eps), self.step_count, self.flip,
+                                              len(LAYERS), _s(), 0))
 
     def flops_per_pixel(self):
         """Forward multiply-adds x 2 per output pixel of the input grid."""

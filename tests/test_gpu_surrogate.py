@@ -310,3 +310,63 @@ def test_training_beats_constant_predictor(gpu):
     print("loss first %.4f last %.4f constant %.4f" % (losses[0], losses[-1], const), flush=True)
     assert losses[-1] < 0.5 * losses[0]
     assert losses[-1] < 0.5 * const
+
+
+def test_pool_bwd_accumulate_and_adam_flip(gpu):
+    """The two fused training ops: the max-pool backward added into a
+    gradient buffer (the skip connection's) equals the written backward
+    added to it; Adam with a flip table updates the parameters as
+    slm_nn_adam does and leaves each layer's flipped weights equal to
+    slm_nn_wflip of the new bf16 copy."""
+    import torch
+    from slm_amd import hip
+    from slm_amd import surrogate as S
+    rs = np.random.default_rng(7)
+    N, H, W, C = 2, 8, 12, 32
+    xg = _bf(rs.standard_normal((N * H * W, C))).to(gpu)
+    y = torch.empty((N * H * W // 4, C), dtype=torch.bfloat16, device=gpu)
+    S.pool(xg, C, y, C, N, H, W, C)
+    dy = _bf(rs.standard_normal((N * H * W // 4, C))).to(gpu)
+    dx = torch.empty((N * H * W, C), dtype=torch.bfloat16, device=gpu)
+    S.pool_bwd(xg, C, y, C, dy, C, dx, C, N, H, W, C)
+    base = _bf(rs.standard_normal((N * H * W, 2 * C)))
+    base[:3] = 0.0
+    base[0, C:] = -0.0
+    acc = base.to(gpu)
+    S.pool_bwd(xg, C, y, C, dy, C, acc[:, C:], 2 * C, N, H, W, C, accumulate=True)
+    want = base.clone()
+    want[:, C:] = (base[:, C:].float() + dx.float().cpu()).to(torch.bfloat16)
+    assert torch.equal(acc.cpu().view(torch.int16), want.view(torch.int16))
+    # Adam with flips: two layers (3x3 and 1x1) and a bias gap between them
+    shapes = [(9, 16, 32), (1, 32, 8)]
+    offs, n = [], 0
+    for taps, cin, cout in shapes:
+        offs.append(n)
+        n += cout * taps * cin + cout
+    p0 = rs.standard_normal(n).astype(np.float32)
+    g0 = rs.standard_normal(n).astype(np.float32)
+    res = []
+    for flip in (False, True):
+        p = torch.from_numpy(p0.copy()).to(gpu)
+        g = torch.from_numpy(g0).to(gpu)
+        m, v = torch.zeros_like(p), torch.zeros_like(p)
+        pb = torch.empty(n, dtype=torch.bfloat16, device=gpu)
+        wts = [torch.empty(cout * taps * cin, dtype=torch.bfloat16, device=gpu) for taps, cin, cout in shapes]
+        args = (S._p(p), S._p(g), S._p(m), S._p(v), S._p(pb), n, S.ctypes.c_float(1e-2), S.ctypes.c_float(0.9),
+                S.ctypes.c_float(0.999), S.ctypes.c_float(1e-8), 3)
+        if flip:
+            tab = (hip.SlmNnFlip * 2)()
+            for k, ((taps, cin, cout), o) in enumerate(zip(shapes, offs)):
+                tab[k].offset, tab[k].taps, tab[k].cin, tab[k].cout = o, taps, cin, cout
+                tab[k].wt = wts[k].data_ptr()
+            hip.check(hip.load().slm_nn_adam_flip(*args, tab, 2, S._s(), 1))
+        else:
+            hip.check(hip.load().slm_nn_adam(*args, S._s(), 1))
+        res.append((p.cpu(), pb.cpu(), pb, wts))
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1].view(torch.int16), res[1][1].view(torch.int16))
+    pb = res[1][2]
+    for (taps, cin, cout), o, wt in zip(shapes, offs, res[1][3]):
+        ref = torch.empty_like(wt)
+        S.wflip(pb[o:o + cout * taps * cin], ref, taps, cin, cout)
+        torch.cuda.synchronize()
+        assert torch.equal(ref.cpu().view(torch.int16), wt.cpu().view(torch.int16))
