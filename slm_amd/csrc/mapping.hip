@@ -278,12 +278,13 @@ __global__ __launch_bounds__(256) void k_heads(SlmParams P, const float2* __rest
 
 // countlink_runge_kutta_step (rungekutta.cl:382-406): true when stuck.
 // fixed: dt_is_fixed(P), so dt stays DT_MAX and the step error is not needed.
-template <bool C>
+// R2: TRAJECTORY_RESOLUTION is a power of two and inv_res its reciprocal.
+template <bool C, bool R2>
 __device__ __forceinline__ bool countlink_step(const SlmParams& P, const float2* __restrict__ uv, float& dt,
-                                               float& x, float& y, uint32_t& idx, bool fixed) {
+                                               float& x, float& y, uint32_t& idx, bool fixed, float inv_res) {
     StepOut o;
     uint32_t rng = 0;
-    compute_step<C, false, false, true>(P, uv, dt, x, y, rng, o);
+    compute_step<C, false, R2, true>(P, uv, dt, x, y, rng, o, inv_res);
     idx = o.idx;
     const float dl = map_move<C>(P, o, x, y, idx);
     if (dl < P.integration_halt_threshold) return true;
@@ -295,27 +296,59 @@ __device__ __forceinline__ bool countlink_step(const SlmParams& P, const float2*
 // pixel by max (counter, seed) in keyA.  PASS 1: traces stop on entering a
 // pixel ranked higher by another trace, flag it, count = max, and offer
 // link keys (counter at prev, next pixel) into keyL.
-template <bool C, int PASS>
-__global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
-                        const float2* __restrict__ uv, uint32_t* __restrict__ mapping,
-                        uint32_t* __restrict__ count, u64* __restrict__ keyA, u64* __restrict__ keyL) {
-    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= P.n_seed_points) return;
+template <bool C, int PASS, bool R2>
+__global__ __launch_bounds__(kBlock) void k_count(SlmParams P, const float2* __restrict__ seeds,
+                                                  const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
+                                                  uint32_t* __restrict__ mapping, uint32_t* __restrict__ count,
+                                                  u64* __restrict__ keyA, u64* __restrict__ keyL) {
     const uint32_t cap = 1000u * P.max_n_steps;
     const bool fixed = dt_is_fixed(P);
-    const float2 s = seeds[k];
-    float x = s.x, y = s.y, dt = P.dt_max;
-    uint32_t idx = array_idx(P, x, y), prev = idx, n = 0, counter = 2, cprev = 1;
-    if (PASS == 0)
-        atomicMax(&keyA[idx], key2(1u, k));
-    else
-        atomicMax(&count[idx], 1u);
-    // Brent cycle detection on (x, y, dt, prev); see k_flag_walk for the skip rule
-    float sx = x, sy = y, sdt = dt;
-    uint32_t sprev = prev, sn = n, scounter = counter, power = 1;
+    const float inv_res = 1.0f / P.trajectory_resolution;
+    // the trace of seed k: position, dt, pixel, the pixel before, steps,
+    // counter, and Brent cycle detection on (x, y, dt, prev) (see k_flag_walk
+    // for the skip rule); lanes take the next seed when theirs ends
+    // (persist_seeds): every write is an atomic max, and, or or a read of
+    // pass 0's final keyA, so the order of the traces changes nothing
+    float x = 0.f, y = 0.f, dt = 0.f, sx = 0.f, sy = 0.f, sdt = 0.f;
+    uint32_t k = 0, idx = 0, prev = 0, n = 0, counter = 0, cprev = 0, sprev = 0, sn = 0, scounter = 0, power = 1;
     bool ff = true;
-    while (!mask[idx] && n < cap) {
-        if (countlink_step<C>(P, uv, dt, x, y, idx, fixed)) break;
+    auto finish = [&]() {
+        if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
+    };
+    auto start = [&](uint32_t seed) -> bool {
+        k = seed;
+        const float2 s = seeds[k];
+        x = s.x;
+        y = s.y;
+        dt = P.dt_max;
+        idx = array_idx(P, x, y);
+        prev = idx;
+        n = 0;
+        counter = 2;
+        cprev = 1;
+        if (PASS == 0)
+            atomicMax(&keyA[idx], key2(1u, k));
+        else
+            atomicMax(&count[idx], 1u);
+        sx = x;
+        sy = y;
+        sdt = dt;
+        sprev = prev;
+        sn = n;
+        scounter = counter;
+        power = 1;
+        ff = true;
+        if (mask[idx] || n >= cap) {
+            finish();
+            return false;
+        }
+        return true;
+    };
+    auto step = [&]() -> bool {
+        if (countlink_step<C, R2>(P, uv, dt, x, y, idx, fixed, inv_res)) {
+            finish();
+            return false;
+        }
         n++;
         if (prev != idx) {
             ++counter;
@@ -326,7 +359,10 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
                 atomicOr(&mapping[idx], F_THIN);
                 atomicMax(&keyL[prev], key2(cprev, idx));
                 const u64 a = keyA[idx];
-                if (counter < (uint32_t)(a >> 32) && (uint32_t)a != k) break;
+                if (counter < (uint32_t)(a >> 32) && (uint32_t)a != k) {
+                    finish();
+                    return false;
+                }
                 atomicMax(&count[idx], counter);
                 cprev = counter;
             }
@@ -351,8 +387,13 @@ __global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uin
                 power <<= 1;
             }
         }
-    }
-    if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
+        if (mask[idx] || n >= cap) {
+            finish();
+            return false;
+        }
+        return true;
+    };
+    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
 }
 
 __global__ void k_link_final(uint32_t nxy, const u64* __restrict__ keyL, uint32_t* __restrict__ link) {
@@ -551,13 +592,14 @@ __global__ void k_seg_final(uint32_t nxy, const uint8_t* __restrict__ mark, uint
 // ------------------------------------------------------------ link_hillslopes
 
 // hillslopes (hillslopes.cl:31-91).
-template <bool C>
+template <bool C, bool R2>
 __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
                           const float2* __restrict__ uv, uint32_t* __restrict__ link) {
     const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= P.n_seed_points) return;
     const uint32_t cap = 100u * P.max_n_steps;
     const bool fixed = dt_is_fixed(P);
+    const float inv_res = 1.0f / P.trajectory_resolution;
     const float2 s = seeds[k];
     float x = s.x, y = s.y, dt = P.dt_max;
     uint32_t idx = array_idx(P, x, y), prev = idx, n = 0;
@@ -565,7 +607,7 @@ __global__ void k_link_hs(SlmParams P, const float2* __restrict__ seeds, const u
     uint32_t sn = 0, power = 1;
     while (prev == idx && !mask[idx] && n < cap) {
         prev = idx;
-        if (countlink_step<C>(P, uv, dt, x, y, idx, fixed)) break;
+        if (countlink_step<C, R2>(P, uv, dt, x, y, idx, fixed, inv_res)) break;
         n++;
         if (prev == idx) {
             // a repeated state inside the seed pixel never leaves it: no link
@@ -1381,18 +1423,22 @@ extern "C" int slm_count_downchannels(const SlmParams* p, const float* seeds_xy,
     u64* keyA = ws_a(workspace);
     u64* keyL = ws_b(workspace, p);
     zero_u64(keyA, (size_t)2 * p->nxy_padded, s);
-    const dim3 g = seed_grid(p);
+#define SLM_LAUNCH_COUNT(CC, RR)                                                                          \
+    do {                                                                                                  \
+        hipLaunchKernelGGL((k_count<CC, 0, RR>), persist_grid(k_count<CC, 0, RR>, n_work(p)), dim3(kBlock), 0, s, \
+                           SEED_ARGS(p), mask, (const float2*)uv, mapping, count, keyA, keyL);            \
+        hipLaunchKernelGGL((k_count<CC, 1, RR>), persist_grid(k_count<CC, 1, RR>, n_work(p)), dim3(kBlock), 0, s, \
+                           SEED_ARGS(p), mask, (const float2*)uv, mapping, count, keyA, keyL);            \
+    } while (0)
+    const bool r2 = slm_res_is_pow2(p);
     if (p->fp_contract) {
-        hipLaunchKernelGGL((k_count<true, 0>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
-                           mapping, count, keyA, keyL);
-        hipLaunchKernelGGL((k_count<true, 1>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
-                           mapping, count, keyA, keyL);
+        if (r2) SLM_LAUNCH_COUNT(true, true);
+        else SLM_LAUNCH_COUNT(true, false);
     } else {
-        hipLaunchKernelGGL((k_count<false, 0>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
-                           mapping, count, keyA, keyL);
-        hipLaunchKernelGGL((k_count<false, 1>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv,
-                           mapping, count, keyA, keyL);
+        if (r2) SLM_LAUNCH_COUNT(false, true);
+        else SLM_LAUNCH_COUNT(false, false);
     }
+#undef SLM_LAUNCH_COUNT
     hipLaunchKernelGGL(k_link_final, pixel_grid(p), dim3(kBlock), 0, s, p->nxy_padded, (const u64*)keyL, link);
     return slm_finish(stream, flags);
 }
@@ -1458,7 +1504,18 @@ extern "C" int slm_link_hillslopes(const SlmParams* p, const float* seeds_xy, co
     if (!n_work(p)) return slm_finish(stream, flags);
     if (!mask || !uv || !link) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
-    LAUNCH_C(k_link_hs, seed_grid(p), SEED_ARGS(p), mask, (const float2*)uv, link);
+    const bool r2 = slm_res_is_pow2(p);
+    if (p->fp_contract) {
+        if (r2) hipLaunchKernelGGL((k_link_hs<true, true>), seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask,
+                                   (const float2*)uv, link);
+        else hipLaunchKernelGGL((k_link_hs<true, false>), seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask,
+                                (const float2*)uv, link);
+    } else {
+        if (r2) hipLaunchKernelGGL((k_link_hs<false, true>), seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask,
+                                   (const float2*)uv, link);
+        else hipLaunchKernelGGL((k_link_hs<false, false>), seed_grid(p), dim3(kBlock), 0, s, SEED_ARGS(p), mask,
+                                (const float2*)uv, link);
+    }
     return slm_finish(stream, flags);
 }
 
