@@ -297,58 +297,27 @@ __device__ __forceinline__ bool countlink_step(const SlmParams& P, const float2*
 // pixel ranked higher by another trace, flag it, count = max, and offer
 // link keys (counter at prev, next pixel) into keyL.
 template <bool C, int PASS, bool R2>
-__global__ __launch_bounds__(kBlock) void k_count(SlmParams P, const float2* __restrict__ seeds,
-                                                  const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
-                                                  uint32_t* __restrict__ mapping, uint32_t* __restrict__ count,
-                                                  u64* __restrict__ keyA, u64* __restrict__ keyL) {
+__global__ void k_count(SlmParams P, const float2* __restrict__ seeds, const uint8_t* __restrict__ mask,
+                        const float2* __restrict__ uv, uint32_t* __restrict__ mapping,
+                        uint32_t* __restrict__ count, u64* __restrict__ keyA, u64* __restrict__ keyL) {
+    const uint32_t k = P.seeds_chunk_offset + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= P.n_seed_points) return;
     const uint32_t cap = 1000u * P.max_n_steps;
     const bool fixed = dt_is_fixed(P);
     const float inv_res = 1.0f / P.trajectory_resolution;
-    // the trace of seed k: position, dt, pixel, the pixel before, steps,
-    // counter, and Brent cycle detection on (x, y, dt, prev) (see k_flag_walk
-    // for the skip rule); lanes take the next seed when theirs ends
-    // (persist_seeds): every write is an atomic max, and, or or a read of
-    // pass 0's final keyA, so the order of the traces changes nothing
-    float x = 0.f, y = 0.f, dt = 0.f, sx = 0.f, sy = 0.f, sdt = 0.f;
-    uint32_t k = 0, idx = 0, prev = 0, n = 0, counter = 0, cprev = 0, sprev = 0, sn = 0, scounter = 0, power = 1;
+    const float2 s = seeds[k];
+    float x = s.x, y = s.y, dt = P.dt_max;
+    uint32_t idx = array_idx(P, x, y), prev = idx, n = 0, counter = 2, cprev = 1;
+    if (PASS == 0)
+        atomicMax(&keyA[idx], key2(1u, k));
+    else
+        atomicMax(&count[idx], 1u);
+    // Brent cycle detection on (x, y, dt, prev); see k_flag_walk for the skip rule
+    float sx = x, sy = y, sdt = dt;
+    uint32_t sprev = prev, sn = n, scounter = counter, power = 1;
     bool ff = true;
-    auto finish = [&]() {
-        if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
-    };
-    auto start = [&](uint32_t seed) -> bool {
-        k = seed;
-        const float2 s = seeds[k];
-        x = s.x;
-        y = s.y;
-        dt = P.dt_max;
-        idx = array_idx(P, x, y);
-        prev = idx;
-        n = 0;
-        counter = 2;
-        cprev = 1;
-        if (PASS == 0)
-            atomicMax(&keyA[idx], key2(1u, k));
-        else
-            atomicMax(&count[idx], 1u);
-        sx = x;
-        sy = y;
-        sdt = dt;
-        sprev = prev;
-        sn = n;
-        scounter = counter;
-        power = 1;
-        ff = true;
-        if (mask[idx] || n >= cap) {
-            finish();
-            return false;
-        }
-        return true;
-    };
-    auto step = [&]() -> bool {
-        if (countlink_step<C, R2>(P, uv, dt, x, y, idx, fixed, inv_res)) {
-            finish();
-            return false;
-        }
+    while (!mask[idx] && n < cap) {
+        if (countlink_step<C, R2>(P, uv, dt, x, y, idx, fixed, inv_res)) break;
         n++;
         if (prev != idx) {
             ++counter;
@@ -359,10 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_count(SlmParams P, const float2* __r
                 atomicOr(&mapping[idx], F_THIN);
                 atomicMax(&keyL[prev], key2(cprev, idx));
                 const u64 a = keyA[idx];
-                if (counter < (uint32_t)(a >> 32) && (uint32_t)a != k) {
-                    finish();
-                    return false;
-                }
+                if (counter < (uint32_t)(a >> 32) && (uint32_t)a != k) break;
                 atomicMax(&count[idx], counter);
                 cprev = counter;
             }
@@ -387,13 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_count(SlmParams P, const float2* __r
                 power <<= 1;
             }
         }
-        if (mask[idx] || n >= cap) {
-            finish();
-            return false;
-        }
-        return true;
-    };
-    persist_seeds(P.seeds_chunk_offset, P.n_seed_points - P.seeds_chunk_offset, start, step);
+    }
+    if (PASS == 1) atomicMax(&keyL[prev], key2(cprev, idx));
 }
 
 __global__ void k_link_final(uint32_t nxy, const u64* __restrict__ keyL, uint32_t* __restrict__ link) {
@@ -1423,12 +1384,13 @@ extern "C" int slm_count_downchannels(const SlmParams* p, const float* seeds_xy,
     u64* keyA = ws_a(workspace);
     u64* keyL = ws_b(workspace, p);
     zero_u64(keyA, (size_t)2 * p->nxy_padded, s);
+    const dim3 g = seed_grid(p);
 #define SLM_LAUNCH_COUNT(CC, RR)                                                                          \
     do {                                                                                                  \
-        hipLaunchKernelGGL((k_count<CC, 0, RR>), persist_grid(k_count<CC, 0, RR>, n_work(p)), dim3(kBlock), 0, s, \
-                           SEED_ARGS(p), mask, (const float2*)uv, mapping, count, keyA, keyL);            \
-        hipLaunchKernelGGL((k_count<CC, 1, RR>), persist_grid(k_count<CC, 1, RR>, n_work(p)), dim3(kBlock), 0, s, \
-                           SEED_ARGS(p), mask, (const float2*)uv, mapping, count, keyA, keyL);            \
+        hipLaunchKernelGGL((k_count<CC, 0, RR>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv, \
+                           mapping, count, keyA, keyL);                                                   \
+        hipLaunchKernelGGL((k_count<CC, 1, RR>), g, dim3(kBlock), 0, s, SEED_ARGS(p), mask, (const float2*)uv, \
+                           mapping, count, keyA, keyL);                                                   \
     } while (0)
     const bool r2 = slm_res_is_pow2(p);
     if (p->fp_contract) {
