@@ -2146,7 +2146,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                         const uint32_t lo = q * slice, hi = min(n_work, lo + slice);
                         uint32_t b = 0;
                         if (lane == 0) b = atomicAdd(&ctrl->queue[q * kQueueStride], chunk);
-                        b = __shfl(b, 0);
+                        b = __builtin_amdgcn_readfirstlane(b);   // lane 0's, wave-uniform
                         if (b < slice && lo + b < hi) {
                             base = lo + b;
                             lim = hi;
@@ -2162,14 +2162,17 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                 const uint32_t take = min((uint32_t)__popcll(need), c_end - c_next);
                 const uint32_t rank = __popcll(need & lt_mask);
                 const bool mine = ((need >> lane) & 1ull) && rank < take;
+                // the job holding work item c_next, found once for the wave
+                // (c_next is uniform: scalar loads); items before the next
+                // job's begin belong to it too, and only a lane whose item
+                // lies beyond (a chunk across a job boundary) searches again
+                uint32_t j0;
+                BATCH_FIND(desc, n_jobs, work_begin, c_next, j0);
+                const uint32_t j0_end = j0 + 1u < n_jobs ? desc[j0 + 1u].work_begin : 0xffffffffu;
                 if (mine) {
                     const uint32_t w = c_next + rank;
-                    // the job holding work item w
-                    uint32_t lo = 0, hi = n_jobs;
-                    while (hi - lo > 1u) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (desc[mid].work_begin <= w) lo = mid; else hi = mid;
-                    }
+                    uint32_t lo = j0;
+                    if (w >= j0_end) BATCH_FIND(desc, n_jobs, work_begin, w, lo);
                     const BatchDesc* D = desc + lo;
                     P = D->P;
                     mask = as_global(D->mask);
@@ -2180,12 +2183,26 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     pix_begin = D->pix_begin;
                     inv_res = 1.0f / P.trajectory_resolution;
                     const uint32_t d = P.subpixel_seed_point_density ? P.subpixel_seed_point_density : 1u;
-                    const uint32_t dd = d * d;
                     const uint32_t wl = w - D->work_begin;
-                    const uint32_t sub = wl % dd;
-                    const SeedRec r = D->order[wl / dd];
-                    x = r.x + sub_offset<C>(P, sub % d);
-                    y = r.y + sub_offset<C>(P, sub / d);
+                    // work item wl = seed * d^2 + sub, sub = j * d + i (i along x);
+                    // d = 1 and 2 without the integer divisions
+                    uint32_t seed, si, sj;
+                    if (d == 1u) {
+                        seed = wl;
+                        si = sj = 0;
+                    } else if (d == 2u) {
+                        seed = wl >> 2;
+                        si = wl & 1u;
+                        sj = (wl >> 1) & 1u;
+                    } else {
+                        const uint32_t dd = d * d, sub = wl % dd;
+                        seed = wl / dd;
+                        si = sub % d;
+                        sj = sub / d;
+                    }
+                    const SeedRec r = D->order[seed];
+                    x = r.x + sub_offset<C>(P, si);
+                    y = r.y + sub_offset<C>(P, sj);
                     rng = rng_seed(r.seed_idx);
                     l = 0.f;
                     n = 0;
@@ -2193,9 +2210,13 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
                     const uint32_t ix = pix(x, P.pad_width_pp5, P.nx_padded - 1u);
                     const uint32_t iy = pix(y, P.pad_width_pp5, P.ny_padded - 1u);
                     const uint32_t idx = iy + P.ny_padded * ix;
-                    if (!mask[idx]) {
+                    // the seed pixel's quad is loaded with its mask byte (a
+                    // masked seed leaves the lane idle, and Q is set again at
+                    // its next refill)
+                    const uint8_t mk = mask[idx];
+                    Q = load_quad(quad, idx);
+                    if (!mk) {
                         // atomic_write_sl_data at the seed (jittertrajectory.cl:60-62)
-                        Q = load_quad(quad, idx);
                         cx = ix;
                         cy = iy;
                         run_idx = idx;
