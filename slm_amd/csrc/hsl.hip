@@ -558,8 +558,8 @@ __global__ void k_fmm_classify(FmmLayout L, const uint8_t* __restrict__ colour, 
 // trial front is split by local index mod 64: lane L lists its own points
 // that are in the front (at most ceil(cap / 64)) and keeps their minimum, so
 // the next point to accept is a reduction over the 64 lane minima, and only
-// the lane that owned it, or whose minimum grew, scans its list again
-// (dropping accepted points).
+// the lanes that owned it, or whose minimum grew, scan their lists again
+// (dropping accepted points), all at once.
 constexpr uint32_t kFmmNone = 0xFFFFu;
 constexpr uint8_t kFmmNeg = 4, kFmmYm = 8, kFmmYp = 16;
 
@@ -699,58 +699,40 @@ __device__ __forceinline__ void fmm_wave_min(double& bv, uint32_t& bl) {
     bl = l;
 }
 
-// The whole wave scans the front list of lane o, 64 entries at a time:
-// accepted points are dropped (the list compacted in place) and lane o
-// receives the minimum of the rest and the new count.
-__device__ void fmm_wave_scan(const FmmLds& S, uint32_t lane, uint32_t o, uint32_t& cnt, double& mv, uint32_t& ml) {
-    const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)o);
-    uint16_t* lst = S.list + o * S.per_lane;
-    double bv = INFINITY;
-    uint32_t bl = 0xFFFFFFFFu, w = 0;
-    for (uint32_t j0 = 0; j0 < n; j0 += 64) {
-        const uint32_t j = j0 + lane;
-        bool live = false;
-        uint16_t e = 0;
-        if (j < n) {
-            e = lst[j];
-            live = (S.f[e] & 3) != 3;
-            if (live) {
-                const double v = S.u[e];
-                if (fless(v, e, bv, bl)) {
-                    bv = v;
-                    bl = e;
-                }
-            }
-        }
-        const uint64_t bal = __ballot(live);
-        if (live) lst[w + __popcll(bal & ((1ull << lane) - 1ull))] = e;
-        w += __popcll(bal);
-    }
-    fmm_wave_min(bv, bl);
-    if (lane == o) {
-        mv = bv;
-        ml = bl;
-        cnt = w;
-    }
-}
-
-// Lane's scan of its front list: drops accepted points, returns the minimum.
-__device__ void fmm_lane_scan(const FmmLds& S, uint32_t lane, uint32_t& cnt, double& mv, uint32_t& ml) {
+// Lane's scan of its front list (count in LDS): drops accepted points and
+// returns the minimum (value, local index).  Eight entries at a time: the
+// group's list reads, then their states and values, are issued together,
+// and the survivors are written back in place (every write goes to a
+// position the group has already read).
+__device__ void fmm_lane_scan(const FmmLds& S, uint32_t lane, uint32_t* cnt_lds, double& mv, uint32_t& ml) {
     uint16_t* lst = S.list + lane * S.per_lane;
+    const uint32_t cnt = cnt_lds[lane];
     mv = INFINITY;
     ml = 0xFFFFFFFFu;
     uint32_t w = 0;
-    for (uint32_t j = 0; j < cnt; j++) {
-        const uint16_t e = lst[j];
-        if ((S.f[e] & 3) == 3) continue;
-        lst[w++] = e;
-        const double v = S.u[e];
-        if (fless(v, e, mv, ml)) {
-            mv = v;
-            ml = e;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 8) {
+        const uint32_t left = cnt - j0;
+        uint16_t e[8];
+        uint8_t f[8];
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) e[k] = lst[j0 + ((uint32_t)k < left ? (uint32_t)k : 0u)];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            f[k] = S.f[e[k]];
+            v[k] = S.u[e[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if ((uint32_t)k >= left || (f[k] & 3) == 3) continue;
+            lst[w++] = e[k];
+            if (fless(v[k], e[k], mv, ml)) {
+                mv = v[k];
+                ml = e[k];
+            }
         }
     }
-    cnt = w;
+    cnt_lds[lane] = w;
 }
 
 // The march of k_fmm_march for the subsegments of list cls of the k slots,
@@ -761,15 +743,17 @@ __device__ void fmm_lane_scan(const FmmLds& S, uint32_t lane, uint32_t& cnt, dou
 // its local index, and the neighbour indices, flags and u are gathered.  Per
 // accepted point, lanes 0-3 compute the four neighbour updates at once (an
 // update reads only frozen values, and the updates of one point freeze
-// nothing); the results are applied in the lane kernel's neighbour order up
-// to a negative discriminant, as it stops there, each by its point's owner
-// lane.  Global memory is only read until the march
-// ends; then u is written back.
+// nothing) and write those before the first negative discriminant, as the
+// lane kernel stops there; a point new to the front joins its owner's list
+// (lengths in LDS), and the owner lanes fold the values into their minima in
+// neighbour order.  Global memory is only read until the march ends; then u
+// is written back.
 __global__ __launch_bounds__(64) void k_fmm_march_lds(uint32_t nx, uint32_t ny, double dx, FmmLayout L, uint32_t k_slots,
                                                       uint32_t* __restrict__ next, uint8_t* __restrict__ failed,
                                                       uint32_t cls, uint32_t cap) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_fmm[];
     __shared__ uint32_t cur;
+    __shared__ uint32_t s_cnt[64];   // each lane's front list length
     FmmLds S;
     S.u = (double*)lds_fmm;
     S.xn = (uint16_t*)(S.u + cap);
@@ -847,11 +831,13 @@ __global__ __launch_bounds__(64) void k_fmm_march_lds(uint32_t nx, uint32_t ny, 
             S.xn[2 * k + 1] = dn;
             S.f[k] = fl;
         }
+        s_cnt[lane] = cnt;
         __syncthreads();
         double mv;
         uint32_t ml;
-        fmm_lane_scan(S, lane, cnt, mv, ml);
-        bool bad = false, rescan = false;
+        fmm_lane_scan(S, lane, s_cnt, mv, ml);
+        fmm_wave_sync();
+        bool bad = false;
         for (;;) {
             // the next point: the smallest (value, local index) of the lane minima
             double bv = mv;
@@ -860,62 +846,65 @@ __global__ __launch_bounds__(64) void k_fmm_march_lds(uint32_t nx, uint32_t ny, 
             if (bad || bl == 0xFFFFFFFFu) break;
             const uint32_t i = bl;
             if (lane == 0) S.f[i] = (uint8_t)(S.f[i] | 3u);
-            rescan = (i & 63u) == lane;              // its owner scans again (dropping it)
+            bool rescan = (i & 63u) == lane;         // its owner scans again (dropping it)
             fmm_wave_sync();
+            // lanes 0-3: the update of neighbour k (x before, x after, y before,
+            // y after) of a point not frozen
             int32_t flag = 0;
             double v = 0.0;
             uint32_t l = kFmmNone;
+            uint8_t fl = 0;
             if (lane < 4) {
                 const uint8_t fi = S.f[i];
                 if (lane < 2) l = S.xn[2 * i + lane];
                 else if (lane == 2 && (fi & kFmmYm)) l = i - 1;
                 else if (lane == 3 && (fi & kFmmYp)) l = i + 1;
-                if (l != kFmmNone && (S.f[l] & 3) < 2) {
-                    bool bd = false;
-                    v = fmm_update2_lds(S, l, dx, &bd);
-                    flag = bd ? 2 : (v == 0.0 ? 0 : 1);     // scikit-fmm skips a zero update
-                }
-            }
-            // the results of lanes 0-3 in neighbour order, up to the first negative
-            // discriminant; the four points are distinct, so each owner lane applies
-            // its own (reading its point's value and state before it writes them)
-            int32_t fk[4];
-            uint32_t lk[4];
-            double vk[4];
-            int kstop = 4;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                fk[k] = __builtin_amdgcn_readlane(flag, k);
-                lk[k] = (uint32_t)__builtin_amdgcn_readlane((int)l, k);
-                vk[k] = fmm_readlane(v, k);
-                if (fk[k] == 2 && kstop == 4) kstop = k;
-            }
-            if (kstop < 4) bad = true;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (k >= kstop || fk[k] != 1 || (lk[k] & 63u) != lane) continue;
-                const uint32_t q = lk[k];
-                const double old = S.u[q];
-                const uint8_t fq = S.f[q];
-                const bool in_front = (fq & 3) == 1;
-                S.u[q] = vk[k];
-                S.f[q] = (uint8_t)((fq & ~3u) | 1u);
-                if (!in_front) S.list[lane * S.per_lane + cnt++] = (uint16_t)q;
-                if (!rescan) {
-                    if (fless(vk[k], q, mv, ml)) {
-                        mv = vk[k];
-                        ml = q;
-                    } else if (ml == q && vk[k] > old) {
-                        rescan = true;               // its minimum grew
+                if (l != kFmmNone) {
+                    fl = S.f[l];
+                    if ((fl & 3) < 2) {
+                        bool bd = false;
+                        v = fmm_update2_lds(S, l, dx, &bd);
+                        flag = bd ? 2 : (v == 0.0 ? 0 : 1);     // scikit-fmm skips a zero update
                     }
                 }
             }
-            fmm_wave_sync();
-            for (uint64_t want = __ballot(rescan); want; want &= want - 1) {
-                fmm_wave_scan(S, lane, (uint32_t)__ffsll((unsigned long long)want) - 1u, cnt, mv, ml);
-                fmm_wave_sync();
+            // the lane kernel applies them in neighbour order and stops at the
+            // first negative discriminant; the four points are distinct and an
+            // update reads only frozen values, so lane k writes update k itself
+            // when no earlier one failed, and appends a point new to the front
+            // to its owner's list
+            const uint64_t negs = __ballot(flag == 2) & 0xFull;
+            const uint32_t kstop = negs ? (uint32_t)__ffsll((unsigned long long)negs) - 1u : 4u;
+            if (negs) bad = true;
+            const bool apply = flag == 1 && lane < kstop;
+            if (apply) {
+                S.u[l] = v;
+                S.f[l] = (uint8_t)((fl & ~3u) | 1u);
+                if ((fl & 3) != 1) {
+                    const uint32_t o = l & 63u;
+                    S.list[o * S.per_lane + atomicAdd(&s_cnt[o], 1u)] = (uint16_t)l;
+                }
             }
-            rescan = false;
+            // each owner lane folds its applied values into its minimum, in
+            // neighbour order; when a lane's minimum is q its value is q's
+            // value before the update, so a larger one means the minimum grew
+            const uint32_t la = apply ? l : kFmmNone;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t q = (uint32_t)__builtin_amdgcn_readlane((int)la, k);
+                const double vq = fmm_readlane(v, k);
+                if (q == kFmmNone || (q & 63u) != lane || rescan) continue;
+                if (fless(vq, q, mv, ml)) {
+                    mv = vq;
+                    ml = q;
+                } else if (ml == q && vq > mv) {
+                    rescan = true;
+                }
+            }
+            fmm_wave_sync();
+            // the lanes that rescan do so together, each its own list
+            if (rescan) fmm_lane_scan(S, lane, s_cnt, mv, ml);
+            fmm_wave_sync();
         }
         for (uint32_t k = lane; k < m; k += 64) u[pts[k]] = S.u[k];
         if (lane == 0 && bad) failed[key] = 1;
