@@ -354,11 +354,32 @@ __device__ void heap_down(HeapEnt* h, int32_t* pos, int n, int k) {
     pos[e.i] = k;
 }
 
+// The lanes of a wave that hold one key (neighbouring pixels mostly share
+// one) act through one atomic: per distinct non-zero key, the lowest such
+// lane adds the group's size and the group's lanes take consecutive
+// positions after the old value (in lane order).  Returns the lane's
+// position, or 0 for key 0.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t fmm_group_add(uint32_t key, uint32_t* __restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t todo = __ballot(key != 0u);
+    uint32_t at = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint64_t same = __ballot(key == lk) & todo;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if ((same >> lane) & 1ull) at = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return at;
+}
+
 __global__ void k_fmm_lcount(uint32_t nx, uint32_t ny, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx * ny) return;
-    const uint32_t key = ((const uint32_t*)slot_base(L))[i];
-    if (key) atomicAdd((uint32_t*)(slot_base(L) + L.off_lcnt) + key, 1u);
+    const uint32_t key = i < nx * ny ? ((const uint32_t*)slot_base(L))[i] : 0u;
+    fmm_group_add(key, (uint32_t*)(slot_base(L) + L.off_lcnt));
 }
 
 // Exclusive scan of the per-label counts [0, n] in place (one 1024-thread
@@ -389,13 +410,14 @@ __global__ __launch_bounds__(1024) void k_fmm_lscan(uint32_t n, FmmLayout L) {
     }
 }
 
+// Each label's points in any order: the march sorts them by pixel index, and
+// the lane kernel's heap pops in (value, pixel index) order whatever order
+// they were pushed in.
 __global__ void k_fmm_lscatter(uint32_t nx, uint32_t ny, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx * ny) return;
-    const uint32_t key = ((const uint32_t*)slot_base(L))[i];
-    if (!key) return;
-    const uint32_t at = atomicAdd((uint32_t*)(slot_base(L) + L.off_lcur) + key, 1u);
-    ((uint32_t*)(slot_base(L) + L.off_pts))[at] = i;
+    const uint32_t key = i < nx * ny ? ((const uint32_t*)slot_base(L))[i] : 0u;
+    const uint32_t at = fmm_group_add(key, (uint32_t*)(slot_base(L) + L.off_lcur));
+    if (key) ((uint32_t*)(slot_base(L) + L.off_pts))[at] = i;
 }
 
 // The second-order update of point (x, y) of problem key (state >= 2 frozen).
@@ -913,8 +935,8 @@ __global__ __launch_bounds__(64) void k_fmm_march_lds(uint32_t nx, uint32_t ny, 
 }
 
 // Per problem: fixed-point sum of the signed distances of the points a
-// channel reaches, their count, the channel pixels and the own pixels.  A
-// wave whose active lanes share one key adds once.
+// channel reaches, their count, the channel pixels and the own pixels.  The
+// lanes of a wave that share a key add once.
 __global__ void k_fmm_reduce(FmmLayout L, const int32_t* __restrict__ label, uint32_t nx, uint32_t ny,
                              u64* __restrict__ sums, uint32_t* __restrict__ cnt, uint32_t* __restrict__ nch,
                              uint32_t* __restrict__ nphi) {
@@ -937,27 +959,30 @@ __global__ void k_fmm_reduce(FmmLayout L, const int32_t* __restrict__ label, uin
             }
         }
     }
-    const uint32_t first = __shfl(key, 0);
-    const bool uniform = __all(key == first);
-    if (uniform) {
-        if (!first) return;
+    // per distinct non-zero key of the wave, the sums of its lanes (the
+    // others contribute zero) reach every lane, and the lowest lane of the
+    // key adds them: integer sums, so the grouping changes nothing
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t todo = __ballot(key != 0u);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const bool in = key == lk;
+        long long gq = in ? q : 0;
+        uint32_t gc = in ? c : 0u, gch = in ? ch : 0u, gown = in ? own : 0u;
         for (int o = 32; o > 0; o >>= 1) {
-            q += __shfl_down(q, o);
-            c += __shfl_down(c, o);
-            ch += __shfl_down(ch, o);
-            own += __shfl_down(own, o);
+            gq += __shfl_xor(gq, o);
+            gc += __shfl_xor(gc, o);
+            gch += __shfl_xor(gch, o);
+            gown += __shfl_xor(gown, o);
         }
-        if ((threadIdx.x & 63u) == 0u) {
-            if (q) atomicAdd(&sums[first], (u64)q);
-            if (c) atomicAdd(&cnt[first], c);
-            if (ch) atomicAdd(&nch[first], ch);
-            if (own) atomicAdd(&nphi[first], own);
+        if ((int)lane == leader) {
+            if (gq) atomicAdd(&sums[lk], (u64)gq);
+            if (gc) atomicAdd(&cnt[lk], gc);
+            if (gch) atomicAdd(&nch[lk], gch);
+            if (gown) atomicAdd(&nphi[lk], gown);
         }
-    } else if (key) {
-        if (q) atomicAdd(&sums[key], (u64)q);
-        if (c) atomicAdd(&cnt[key], c);
-        if (ch) atomicAdd(&nch[key], ch);
-        if (own) atomicAdd(&nphi[key], own);
+        todo &= ~__ballot(in);
     }
 }
 
