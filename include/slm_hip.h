@@ -553,6 +553,18 @@ int slm_subsegment_flanks(const SlmParams* p, const float* seeds_xy, const uint8
                           const float* uv, const uint32_t* mapping, uint32_t* label,
                           void* stream, uint32_t flags);
 
+/* segment_hillslopes (opencl/segment.cl:133-188) and subsegment_flanks
+ * (:360-414) in one walk per seed, for seeds = unmasked non-thin pixels,
+ * called after slm_subsegment_channel_edges (which segment.subsegment_flanks
+ * runs after segment_hillslopes; neither reads what the other writes).  No
+ * single reference kernel: the flank walk is the hillslope walk up to its
+ * first left-flank pixel, so one walk gives both, and the labels equal those
+ * of slm_segment_hillslopes, slm_subsegment_channel_edges and
+ * slm_subsegment_flanks in the reference's order. */
+int slm_segment_hillslopes_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                  const float* uv, const uint32_t* mapping, uint32_t* label,
+                                  void* stream, uint32_t flags);
+
 /* fix_right_flanks (opencl/segment.cl:441-504) and fix_left_flanks (:530-594),
  * third and fourth launches of segment.subsegment_flanks: relabel a pixel
  * from the pixels its trace crosses on its own side of the channel.  Reads of

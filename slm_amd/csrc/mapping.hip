@@ -605,8 +605,16 @@ __device__ __forceinline__ bool segment_step(const SlmParams& P, const float2* _
     return false;
 }
 
-// segment_hillslopes (segment.cl:133-188).
-template <bool C, bool R2>
+// segment_hillslopes (segment.cl:133-188).  FLANKS: subsegment_flanks
+// (:360-414) in the same walk, for seeds = the unmasked non-thin pixels once
+// subsegment_channel_edges has flagged the left flanks.  The flank walk is
+// this walk up to its first left-flank pixel (the same steps and the same
+// cycle skips, and it also stops wherever this one does), so
+// LEFT_FLANK_ADDITION is decided there and ORed into the label written at
+// the end, as the reference writes the channel's label and ORs the flag
+// after (a seed that is itself a left flank, flagged by the channel edges,
+// stops its flank walk at once).
+template <bool C, bool R2, bool FLANKS>
 __global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __restrict__ seeds,
                                                 const uint8_t* __restrict__ mask, const float2* __restrict__ uv,
                                                 const uint32_t* __restrict__ mapping,
@@ -615,9 +623,12 @@ __global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __res
     const bool fixed = dt_is_fixed(P);
     float x = 0.f, y = 0.f, dt = 0.f;
     uint32_t n = 0, probe = 0, idx = 0, hs = 0;
+    bool open = false, lfa = false;   // the flank walk goes on; its flag
     CycleSkip cyc;
     auto finish = [&]() {
-        if (mapping[idx] & F_THIN) label[hs] = label[idx];
+        if (FLANKS && open) lfa = (mapping[idx] & F_LEFT) != 0u;
+        if (mapping[idx] & F_THIN) label[hs] = label[idx] | (lfa ? LFA : 0u);
+        else if (lfa) label[hs] |= LFA;
     };
     auto start = [&](uint32_t k) -> bool {
         const float2 s = seeds[k];
@@ -627,9 +638,16 @@ __global__ __launch_bounds__(256) void k_seg_hs(SlmParams P, const float2* __res
         n = 0;
         cyc.reset();
         idx = hs = array_idx(P, x, y);
+        open = FLANKS;
+        lfa = false;
         return true;
     };
     auto step = [&]() -> bool {
+        if (FLANKS && open &&
+            (mask[idx] || (mapping[idx] & (F_LEFT | F_THIN)) || n >= P.max_n_steps)) {
+            open = false;
+            lfa = (mapping[idx] & F_LEFT) != 0u;
+        }
         if (!(!mask[idx] && ((~mapping[idx]) & F_THIN) && n < P.max_n_steps) ||
             segment_step<C, R2>(P, uv, dt, x, y, idx, n, probe, inv_res, fixed)) {
             finish();
@@ -1481,17 +1499,18 @@ extern "C" int slm_link_hillslopes(const SlmParams* p, const float* seeds_xy, co
     return slm_finish(stream, flags);
 }
 
-extern "C" int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
-                                      const float* uv, const uint32_t* mapping, uint32_t* label, void* stream,
-                                      uint32_t flags) {
+namespace {
+template <bool FLANKS>
+int seg_hs_launch(const SlmParams* p, const float* seeds_xy, const uint8_t* mask, const float* uv,
+                  const uint32_t* mapping, uint32_t* label, void* stream, uint32_t flags) {
     int rc = check_map(p, seeds_xy);
     if (rc) return rc;
     if (!n_work(p)) return slm_finish(stream, flags);
     if (!mask || !uv || !mapping || !label) return slm_fail(SLM_E_INVALID_ARG, "NULL buffer");
     hipStream_t s = (hipStream_t)stream;
 #define SLM_LAUNCH_SEG(CC, RR)                                                                              \
-    hipLaunchKernelGGL((k_seg_hs<CC, RR>), persist_grid(k_seg_hs<CC, RR>, n_work(p)), dim3(kBlock), 0, s, SEED_ARGS(p), mask, \
-                       (const float2*)uv, mapping, label)
+    hipLaunchKernelGGL((k_seg_hs<CC, RR, FLANKS>), persist_grid(k_seg_hs<CC, RR, FLANKS>, n_work(p)), dim3(kBlock), 0, \
+                       s, SEED_ARGS(p), mask, (const float2*)uv, mapping, label)
     const bool r2 = slm_res_is_pow2(p);
     if (p->fp_contract) {
         if (r2) SLM_LAUNCH_SEG(true, true);
@@ -1502,6 +1521,19 @@ extern "C" int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy,
     }
 #undef SLM_LAUNCH_SEG
     return slm_finish(stream, flags);
+}
+}  // namespace
+
+extern "C" int slm_segment_hillslopes(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                      const float* uv, const uint32_t* mapping, uint32_t* label, void* stream,
+                                      uint32_t flags) {
+    return seg_hs_launch<false>(p, seeds_xy, mask, uv, mapping, label, stream, flags);
+}
+
+extern "C" int slm_segment_hillslopes_flanks(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,
+                                             const float* uv, const uint32_t* mapping, uint32_t* label,
+                                             void* stream, uint32_t flags) {
+    return seg_hs_launch<true>(p, seeds_xy, mask, uv, mapping, label, stream, flags);
 }
 
 extern "C" int slm_subsegment_channel_edges(const SlmParams* p, const float* seeds_xy, const uint8_t* mask,

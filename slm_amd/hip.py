@@ -117,6 +117,7 @@ MAP_ARGS = {
     "segment_hillslopes": (("seeds", "mask", "uv", "mapping", "label"), False),
     "subsegment_channel_edges": (("seeds", "mask", "mapping", "link", "label"), False),
     "subsegment_flanks": (("seeds", "mask", "uv", "mapping", "label"), False),
+    "segment_hillslopes_flanks": (("seeds", "mask", "uv", "mapping", "label"), False),
     "fix_right_flanks": (("seeds", "mask", "uv", "mapping", "label"), True),
     "fix_left_flanks": (("seeds", "mask", "uv", "mapping", "label"), True),
 }

@@ -207,9 +207,13 @@ class ChannelMapper:
         if seeds.shape[0] == 0:
             return False
         self._run("subsegment_flanks", seeds, uv=self.uv, mapping=self.mapping, label=self.label)
-        seeds = pick_seeds(mask=self.mask, map=~self.mapping, flag=flank_or_thin, pad=self.pad)
-        if seeds.shape[0] == 0:
-            return False
+        return self._fix_flanks(seeds)
+
+    def _fix_flanks(self, seeds) -> bool:
+        """The last two launches of segment.py:148-243 and the sign fix.
+        seeds: those of subsegment_flanks, which fix_right_flanks picks again
+        from the same mapping (the flank walk writes labels only)."""
+        info = self.info
         self._run("fix_right_flanks", seeds, uv=self.uv, mapping=self.mapping, label=self.label)
         seeds = pick_seeds(mask=self.mask, map=self.mapping, flag=info.is_leftflank, pad=self.pad)
         if seeds.shape[0] == 0:
@@ -220,6 +224,32 @@ class ChannelMapper:
         lab[neg] = -(lab[neg] + (1 << 31))
         self.label = lab.to(self.label.dtype)
         return True
+
+    def segment_hillslopes_flanks(self) -> bool:
+        """segment_hillslopes() then subsegment_flanks(), with the hillslope
+        and flank walks in one launch (slm_segment_hillslopes_flanks): the
+        channel edges run first (neither they nor the hillslope walk read
+        what the other writes), then one walk per unmasked non-thin pixel,
+        then the flank fixes.  The same labels and the same return value as
+        the two steps in order."""
+        info = self.info
+        hs = pick_seeds(mask=self.mask, map=~self.mapping, flag=info.is_thinchannel, pad=self.pad)
+        if hs.shape[0] == 0:
+            return False
+        heads = pick_seeds(mask=self.mask, map=self.mapping, flag=info.is_subsegmenthead, pad=self.pad)
+        if heads.shape[0] == 0:
+            self._run("segment_hillslopes", hs, uv=self.uv, mapping=self.mapping, label=self.label)
+            return False
+        self._run("subsegment_channel_edges", heads, mapping=self.mapping, link=self.link, label=self.label)
+        p = self._params("segment_hillslopes", hs.shape[0])
+        self.workspace = hip.map_call("segment_hillslopes_flanks", p, workspace=self.workspace, stream=self.stream,
+                                      seeds=hs, mask=self.mask_u8, uv=self.uv.reshape(-1),
+                                      mapping=self.mapping.reshape(-1), label=self.label.reshape(-1))
+        # the flank seeds: every unmasked pixel lacks one of the two flags
+        # (left flanks are never thin), the hillslope seeds among them
+        flank_or_thin = int(info.is_leftflank) | int(info.is_thinchannel)
+        seeds = pick_seeds(mask=self.mask, map=~self.mapping, flag=flank_or_thin, pad=self.pad)
+        return self._fix_flanks(seeds)
 
     def map_channels_segments(self, do_map_channels_from_scratch=True, do_map_segments=True) -> bool:
         """mapping.py:627-676."""
@@ -237,4 +267,4 @@ class ChannelMapper:
         if not do_map_segments:
             return True
         return (self.label_confluences() and self.segment_downchannels() and self.link_hillslopes()
-                and self.segment_hillslopes() and self.subsegment_flanks())
+                and self.segment_hillslopes_flanks())

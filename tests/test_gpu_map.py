@@ -129,6 +129,32 @@ def test_chain_stepwise_bit_exact(gpu, kind, contract):
     assert gm.n_segments == solo.n_segments and gm.n_segments > 0
 
 
+@pytest.mark.parametrize("kind", ["v", "y", "y100", "y_adapt", "traced", "pits"])
+def test_chain_fused_walks_bit_exact(gpu, kind):
+    """map_channels_segments as the pipeline runs it (the hillslope and flank
+    walks in one launch, slm_segment_hillslopes_flanks) against the oracle's
+    chain of the reference's separate steps: the same grids and result."""
+    import torch
+    from slm_amd.mapping import ChannelMapper
+    job, m0, slt = _case(kind)
+    print("case", kind, "grid", job.mask.shape, flush=True)
+    gm = ChannelMapper(job.info, torch.from_numpy(job.mask).to(gpu), torch.from_numpy(job.uv).to(gpu),
+                       torch.from_numpy(m0.view(np.int32)).to(gpu), slt=torch.from_numpy(slt).to(gpu))
+    ok = gm.map_channels_segments(do_map_channels_from_scratch=False)
+    torch.cuda.synchronize()
+    solo = OracleMapper(job.info, job.mask, job.uv, m0, slt=slt)
+    ok_o = True
+    for name in STEPS:
+        if not solo.step(name):
+            ok_o = False
+            break
+    assert bool(ok) == ok_o
+    assert np.array_equal(_u32(gm.mapping).reshape(-1), solo.mapping.reshape(-1))
+    assert np.array_equal(_u32(gm.label).reshape(-1), solo.label.reshape(-1))
+    assert np.array_equal(_u32(gm.link).reshape(-1), solo.link.reshape(-1))
+    assert gm.n_segments == solo.n_segments and gm.n_segments > 0
+
+
 @pytest.mark.parametrize("shape,density,grow", [((300, 420), 0.985, 4), ((257, 130), 0.996, 14), ((70, 65), 1.0, 0)])
 def test_thinning_bit_exact(gpu, shape, density, grow):
     """Bit-exact skeleton and iteration count; thick blobs need more than one

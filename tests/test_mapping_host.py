@@ -73,3 +73,35 @@ def test_pick_seeds_matches_reference_order():
                for k, v in kw.items()}
         got = mapping.pick_seeds(pad=2, **tkw).numpy()
         assert np.array_equal(got, want)
+
+
+def test_channel_mapper_fused_walks_match_oracle_mapper(cpu_kernels, monkeypatch):
+    """map_channels_segments runs the channel edges before one hillslope and
+    flank walk (slm_segment_hillslopes_flanks, faked here as the oracle's
+    segment_hillslopes then subsegment_flanks on the same seeds): the labels
+    equal the reference's order of segment_hillslopes, channel edges and
+    flanks."""
+    import torch
+    fake = hip.map_call
+
+    def map_call(name, p, **kw):
+        if name == "segment_hillslopes_flanks":
+            fake("segment_hillslopes", p, **kw)
+            return fake("subsegment_flanks", p, **kw)
+        return fake(name, p, **kw)
+
+    monkeypatch.setattr(hip, "map_call", map_call)
+    job, m0 = workloads.y_valley(n=48, channel_width=3)
+    job.info.set_thresholds(segmentation_threshold=np.uint32(10))
+    slt = np.random.default_rng(2).random(m0.shape).astype(np.float32)
+    gm = mapping.ChannelMapper(job.info, torch.from_numpy(job.mask), torch.from_numpy(job.uv),
+                               torch.from_numpy(m0.view(np.int32).copy()), slt=torch.from_numpy(slt))
+    gm.workspace = torch.empty(16, dtype=torch.uint8)
+    assert gm.map_channels_segments(do_map_channels_from_scratch=False)
+    om = OracleMapper(job.info, job.mask, job.uv, m0, slt=slt)
+    for s in STEPS:
+        assert om.step(s), s
+    assert gm.n_segments == om.n_segments > 0
+    assert np.array_equal(gm.mapping.numpy().view(np.uint32), om.mapping)
+    assert np.array_equal(gm.label.numpy().view(np.uint32), om.label)
+    assert (gm.label.numpy() < 0).any() and (gm.label.numpy() > 0).any()
