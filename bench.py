@@ -299,7 +299,8 @@ def bench_batch(args):
                    "max_n_steps": int(tiles[0].info.max_n_steps), "subpixel_density": 1, "jitter": 1.0,
                    "directions": 2, "host_generation_s": round(gen_s, 1)},
         "batch": {"graph": True, "opts": {"chunk": opts.chunk, "blocks_per_cu": opts.blocks_per_cu,
-                                          "queue_slices": opts.queue_slices, "lds_slots": opts.lds_slots},
+                                          "queue_slices": opts.queue_slices, "lds_slots": opts.lds_slots,
+                                          "refill_batch": opts.refill_batch},
                   "default_opts_ms": round(default_ms, 3), "graph_replay_ms": round(mean_ms, 3),
                   "autotune_table": [list(r) for r in table]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -234,7 +234,9 @@ typedef struct SlmBatchOpts {
     uint32_t queue_slices;   /* work-list slices / queue heads, 1..8 (0: 8, one per XCD) */
     uint32_t lds_slots;      /* per-wave LDS accumulation table: 256, 512 or 1024 slots
                                 (a 16x16, 16x32 or 32x32 pixel window; 0: 512)         */
-    uint32_t reserved[4];
+    uint32_t refill_batch;   /* idle lanes a wave waits for before it refills them, 1..64
+                                (0: 28 when every job has one seed per pixel, else 20) */
+    uint32_t reserved[3];
 } SlmBatchOpts;
 
 size_t slm_trace_fields_batch_workspace_size(const SlmFieldsJob* jobs, uint32_t n_jobs);

@@ -595,10 +595,11 @@ __global__ __launch_bounds__(256) void k_fields(SlmParams P, const float2* __res
     // wave-private chunk of the work queue: one global atomic per kChunk items
     constexpr uint32_t kChunk = 64;
     uint32_t c_next = 0, c_end = 0;
+    const uint32_t batch = fields_refill_batch(d);
     while (true) {
         if (!exhausted) {
             uint64_t need = __ballot(!active);
-            if (__popcll(need) < kRefillMin && __any(active)) need = 0;   // batched refills
+            if (__popcll(need) < batch && __any(active)) need = 0;   // batched refills
             while (need) {
                 if (c_next >= c_end) {
                     // The work order is cut into kQueues slices; a wave starts on
@@ -2086,7 +2087,7 @@ template <bool C, bool PACKED, bool R2, uint32_t SLOTS = kRunSlots>
 __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restrict__ desc, uint32_t n_jobs,
                                                       uint32_t n_work, unsigned long long* __restrict__ steps,
                                                       FieldsCtrl* __restrict__ ctrl, uint32_t force_fallback,
-                                                      uint32_t chunk, uint32_t n_slices,
+                                                      uint32_t chunk, uint32_t n_slices, uint32_t batch,
                                                       unsigned long long* __restrict__ acc_all) {
     if constexpr (!PACKED) {
         if (fields_packed_ok(ctrl, force_fallback)) return;
@@ -2136,7 +2137,7 @@ __global__ __launch_bounds__(256) void k_fields_batch(const BatchDesc* __restric
     while (true) {
         if (!exhausted) {
             uint64_t need = __ballot(!active);
-            if (__popcll(need) < kRefillMin && __any(active)) need = 0;   // batched refills
+            if (__popcll(need) < batch && __any(active)) need = 0;   // batched refills
             while (need) {
                 if (c_next >= c_end) {
                     uint32_t base = 0xffffffffu, lim = 0;
@@ -2412,12 +2413,18 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
             return slm_fail(SLM_E_INVALID_ARG, "job %u: NULL seeds", j);
     }
     uint32_t chunk = 64, blocks_per_cu = 0, slices = kQueues, slots = kRunSlots;
+    // the refill batch of the densest job (fields_refill_batch)
+    uint32_t max_d = 1;
+    for (uint32_t j = 0; j < n_jobs; j++) max_d = std::max(max_d, jobs[j].params.subpixel_seed_point_density);
+    uint32_t batch = fields_refill_batch(max_d);
     if (opts) {
         if (opts->chunk) chunk = opts->chunk;
         if (opts->blocks_per_cu) blocks_per_cu = opts->blocks_per_cu;
         if (opts->queue_slices) slices = opts->queue_slices;
         if (opts->lds_slots) slots = opts->lds_slots;
+        if (opts->refill_batch) batch = opts->refill_batch;
     }
+    if (batch > 64) return slm_fail(SLM_E_INVALID_ARG, "refill_batch %u outside [1, 64]", batch);
     if (chunk < 1 || chunk > 65536) return slm_fail(SLM_E_INVALID_ARG, "chunk %u outside [1, 65536]", chunk);
     if (slices < 1 || slices > kQueues) return slm_fail(SLM_E_INVALID_ARG, "queue_slices %u outside [1, 8]", slices);
     if (slots != 256 && slots != 512 && slots != 1024)
@@ -2520,7 +2527,7 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (n_work) {
 #define SLM_LAUNCH_BATCH(CC, RR, SS)                                                                     \
     hipLaunchKernelGGL((k_fields_batch<CC, true, RR, SS>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc, \
-                       n_jobs, n_work, st, ctrl, force, chunk, slices, acc_all)
+                       n_jobs, n_work, st, ctrl, force, chunk, slices, batch, acc_all)
 #define SLM_LAUNCH_SLOTS(CC, RR)                          \
     do {                                                  \
         if (slots == 256) SLM_LAUNCH_BATCH(CC, RR, 256);   \
@@ -2543,10 +2550,10 @@ extern "C" int slm_trace_fields_batch(const SlmFieldsJob* jobs, uint32_t n_jobs,
     if (n_work) {
         if (C0)
             hipLaunchKernelGGL((k_fields_batch<true, false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, batch, acc_all);
         else
             hipLaunchKernelGGL((k_fields_batch<false, false, false>), dim3(grid), dim3(256), 0, s, (const BatchDesc*)desc,
-                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, acc_all);
+                               n_jobs, n_work, nullptr, ctrl, force, chunk, slices, batch, acc_all);
     }
     return slm_finish(stream, flags);
 }

@@ -498,10 +498,18 @@ struct CycleSkip {
 // (or none is busy).  A refill costs the wave a queue grab, ballots and the
 // new traces' first loads, whatever the number of lanes it serves; refilling
 // every time one lane ends made that cost a large share of the loop (the
-// fields tracers ran 14-18% slower).  Lanes idle meanwhile waste under
-// kRefillMin / 64 of the issue slots.  Which lane runs which trace, and when,
-// changes no result.
-constexpr uint32_t kRefillMin = 8;
+// fields tracers ran 14-18% slower).  Lanes idle meanwhile waste about
+// kRefillMin / 128 of the issue slots.  Which lane runs which trace, and
+// when, changes no result.
+constexpr uint32_t kRefillMin = 20;
+
+// The fields tracers' refill batch for d x d sub-seeds per pixel, measured on
+// a 2048^2 tile (the same traces, 282 steps on average, at d = 1 and d = 2):
+// 28 lanes at d = 1 (8 lanes: 15% slower, 20: 4%), 20 lanes from d = 2 on
+// (8 lanes: 2.5% slower, 24: 5%, 28: 3%).  Neighbouring work items of one
+// pixel trace almost the same path and tend to end together, which moves the
+// best batch down.
+__host__ __device__ constexpr uint32_t fields_refill_batch(uint32_t d) { return d <= 1u ? 28u : 20u; }
 
 template <typename Start, typename Step>
 __device__ __forceinline__ void persist_seeds(uint32_t first, uint32_t n_items, Start start, Step step) {

@@ -337,9 +337,10 @@ def hillslope_lengths(p: SlmParams, seeds, mask, uv, mapping, thin_flag, from_fl
                                        SLM_FLAG_SYNC if sync else 0))
 
 
-def batch_opts(chunk=0, blocks_per_cu=0, queue_slices=0, lds_slots=0) -> SlmBatchOpts:
+def batch_opts(chunk=0, blocks_per_cu=0, queue_slices=0, lds_slots=0, refill_batch=0) -> SlmBatchOpts:
     o = SlmBatchOpts()
     o.chunk, o.blocks_per_cu, o.queue_slices, o.lds_slots = chunk, blocks_per_cu, queue_slices, lds_slots
+    o.refill_batch = refill_batch
     return o
 
 

@@ -110,7 +110,7 @@ assert ctypes.sizeof(SlmFieldsJob) == 192
 class SlmBatchOpts(ctypes.Structure):
     """Mirror of ``struct SlmBatchOpts`` in include/slm_hip.h (32 bytes)."""
     _fields_ = [("chunk", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32), ("queue_slices", ctypes.c_uint32),
-                ("lds_slots", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("lds_slots", ctypes.c_uint32), ("refill_batch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 _libc = ctypes.CDLL(None)

@@ -129,3 +129,25 @@ def test_lds_table_sizes_same_bits(gpu, slots):
             assert int(steps.item()) == total
         for (_, _, _, _, slc, slt), (rc, rt) in zip(jobs, ref):
             assert np.array_equal(hip.as_u32(slc), rc) and np.array_equal(hip.as_u32(slt), rt)
+
+
+@pytest.mark.parametrize("refill,d", [(1, 1), (7, 2), (20, 2), (28, 1), (64, 1)])
+def test_refill_batches_same_bits(gpu, refill, d):
+    """Any refill batch (the idle lanes a wave gathers before it refills them)
+    gives the bits and step count of separate single-tile calls."""
+    import torch
+    from slm_amd import hip
+    jobs, arr = _batch(gpu, n_tiles=4, seed=13, d=d)
+    ref, total = _single(gpu, jobs)
+    steps = torch.zeros(1, dtype=torch.int64, device=gpu)
+    hip.trace_fields_batch(arr, steps=steps, opts=hip.batch_opts(refill_batch=refill), sync=True)
+    assert int(steps.item()) == total
+    for (_, _, _, _, slc, slt), (rc, rt) in zip(jobs, ref):
+        assert np.array_equal(hip.as_u32(slc), rc) and np.array_equal(hip.as_u32(slt), rt)
+
+
+def test_refill_batch_out_of_range_rejected(gpu):
+    from slm_amd import hip
+    jobs, arr = _batch(gpu, n_tiles=2, seed=3)
+    with pytest.raises(hip.SlmError):
+        hip.trace_fields_batch(arr, opts=hip.batch_opts(refill_batch=65), sync=True)
