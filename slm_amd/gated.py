@@ -127,14 +127,18 @@ class DeviceGrid:
                               slc, slt, steps, workspace=workspace)
         return slc, slt, ws
 
-    def region(self, r0, r1, c0, c1):
+    def region(self, r0, r1, c0, c1, n_seeds=None):
         """Job tuple (params, seeds, mask, uv, slc, slt) tracing the grid's
         window [r0, r1) x [c0, c1) as a grid of its own (streamlines stop at
-        its edge); its outputs are fresh int32 buffers."""
+        its edge); its outputs are fresh int32 buffers.  n_seeds: the
+        window's valid pixels, when the caller has counted them (the seed
+        list is then built without waiting for the device)."""
         t = self.torch
         uv = self.uv[r0:r1, c0:c1].contiguous()
         mask = self.mask[r0:r1, c0:c1].contiguous()
-        seeds = (t.nonzero(mask == 0).to(t.float32) - float(self.pad)).contiguous()
+        valid = mask == 0
+        idx = t.nonzero(valid) if n_seeds is None else t.nonzero_static(valid, size=int(n_seeds))
+        seeds = (idx.to(t.float32) - float(self.pad)).contiguous()
         info = copy.copy(self.info)
         info.set_xy(r1 - r0 - 2 * self.pad, c1 - c0 - 2 * self.pad, self.pad)
         p = self.params(seeds.shape[0], info)
@@ -386,20 +390,22 @@ def tile_origins(nx, ny, size, halo):
 
 def predict_tiles(net, grid, origins, size, halo, batch, out):
     """Inference on the tiles at origins (int32 [T, 2]), their cores stitched
-    into out [3, nxy].  Each tile's result depends on that tile alone."""
+    into out [3, nxy], in batches of at most batch tiles (the last one as
+    large as what is left).  Each tile's result depends on that tile alone,
+    whatever batch it is in."""
     import torch
     dev = grid.dev
-    pad_n = (-len(origins)) % batch
-    if pad_n:       # whole batches: extra tiles lie outside the grid (no input, nothing stitched)
-        origins = np.concatenate([origins, np.full((pad_n, 2), -4 * size, np.int32)])
+    if not len(origins):
+        return out
     tiles = torch.from_numpy(np.ascontiguousarray(origins, np.int32)).to(dev)
     net._alloc(batch, size, size)
     mean_ws = torch.empty(batch, dtype=torch.float32, device=dev)
     for b0 in range(0, len(origins), batch):
         tl = tiles[b0:b0 + batch]
-        tile_inputs(grid, tl, batch, size, mean_ws, net.A["x"])
-        head = net.forward(net.A["x"], batch, size, size)
-        tile_stitch(head, tl, batch, size, halo, grid.nx, grid.ny, out)
+        nb = int(tl.shape[0])
+        tile_inputs(grid, tl, nb, size, mean_ws, net.A["x"])
+        head = net.forward(net.A["x"], nb, size, size)
+        tile_stitch(head, tl, nb, size, halo, grid.nx, grid.ny, out)
     return out
 
 
@@ -419,22 +425,27 @@ def plain_map(grid, theta, steps=None, workspace=None):
     return fields_log(slc, slt, grid.mask.reshape(-1), grid.density, theta), ws
 
 
-def _tile_check(pred, truth, valid, max_err, min_f1, min_channel_px):
-    """Per-tile check of predicted against traced fields (views [3, a, b]).
-    The median error and the confusion counts come back in one copy."""
+def _tile_stats(pred, truth, valid):
+    """Device float64 [4] of one tile (views [3, a, b]): the median relative
+    log-density error over the valid pixels, and the channel confusion counts
+    tp, fp, fn."""
     import torch
     v = valid
     t1 = truth[1][v]
     err = ((pred[1][v] - t1).abs() / t1.clamp_min(1e-6))
     pc, tc = (pred[2] > 0) & v, (truth[2] > 0) & v
     med_t = err.median().double() if err.numel() else torch.zeros((), dtype=torch.float64, device=err.device)
-    got = torch.stack([med_t, (pc & tc).sum().double(), (pc & ~tc).sum().double(),
-                       (~pc & tc).sum().double()]).cpu().numpy()
+    return torch.stack([med_t, (pc & tc).sum().double(), (pc & ~tc).sum().double(), (~pc & tc).sum().double()])
+
+
+def _tile_verdict(got, max_err, min_f1, min_channel_px):
+    """(ok, median error, F1, traced channel pixels) from _tile_stats' values."""
     med = float(np.float32(got[0]))
     tp, fp, fn = int(got[1]), int(got[2]), int(got[3])
     f1 = 2 * tp / max(1, 2 * tp + fp + fn) if (tp + fp + fn) else 1.0
     ok = med <= max_err and (f1 >= min_f1 or (tp + fn) < min_channel_px)
     return ok, med, f1, tp + fn
+
 
 
 def _check_windows(grid, origins, chosen, size, halo):
@@ -446,14 +457,17 @@ def _check_windows(grid, origins, chosen, size, halo):
     import torch
     margin = int(math.ceil(grid.max_length_px)) + 2
     c = size - 2 * halo
-    jobs, cores = [], []
+    cores = []
     for t in chosen:
         i0, j0 = int(origins[t][0]) + halo, int(origins[t][1]) + halo
         i1, j1 = min(grid.nx, i0 + c), min(grid.ny, j0 + c)
         r0, r1 = max(0, i0 - margin), min(grid.nx, i1 + margin)
         c0, c1 = max(0, j0 - margin), min(grid.ny, j1 + margin)
-        jobs.append(grid.region(r0, r1, c0, c1))
         cores.append((i0, i1, j0, j1, r0, r1, c0, c1))
+    # the windows' seed counts in one copy, then their seed lists without waiting
+    counts = torch.stack([(grid.mask[r0:r1, c0:c1] == 0).sum() for _, _, _, _, r0, r1, c0, c1 in cores]).cpu()
+    jobs = [grid.region(r0, r1, c0, c1, n_seeds=int(n))
+            for (_, _, _, _, r0, r1, c0, c1), n in zip(cores, counts.tolist())]
     steps = torch.zeros(1, dtype=torch.int64, device=grid.dev)
     hip.trace_fields_batch(hip.fields_jobs(jobs), steps=steps)
     return jobs, cores, steps
@@ -461,15 +475,22 @@ def _check_windows(grid, origins, chosen, size, halo):
 
 def _checked(grid, jobs, cores, theta, fields, max_err, min_f1, min_channel_px):
     """Per checked tile, in order: (check dict, predicted core view of
-    fields, traced core)."""
+    fields, traced core).  The statistics of all the tiles are computed
+    before the first is returned, and come back in one copy."""
+    import torch
     view = fields.view(3, grid.nx, grid.ny)
     gm = grid.mask.view(grid.nx, grid.ny)
+    items, stats = [], []
     for (p, seeds, mask, uv, slc, slt), (i0, i1, j0, j1, r0, r1, c0, c1) in zip(jobs, cores):
         f = fields_log(slc, slt, mask, grid.density, theta).view(3, r1 - r0, c1 - c0)
         tr = f[:, i0 - r0:i1 - r0, j0 - c0:j1 - c0]
         pr = view[:, i0:i1, j0:j1]
-        ok, med, f1, nch = _tile_check(pr, tr, gm[i0:i1, j0:j1] == 0, max_err, min_f1, min_channel_px)
-        yield {"core": [i0, i1, j0, j1], "median_rel_err": med, "f1": f1, "channel_px": nch, "ok": ok}, pr, tr
+        stats.append(_tile_stats(pr, tr, gm[i0:i1, j0:j1] == 0))
+        items.append(([i0, i1, j0, j1], pr, tr))
+    got = torch.stack(stats).cpu().numpy() if stats else []
+    for (core, pr, tr), g in zip(items, got):
+        ok, med, f1, nch = _tile_verdict(g, max_err, min_f1, min_channel_px)
+        yield {"core": core, "median_rel_err": med, "f1": f1, "channel_px": nch, "ok": ok}, pr, tr
 
 
 def _chosen(n_tiles, check_frac, rng_seed):

@@ -204,7 +204,12 @@ class UNet:
         return out
 
     def _alloc(self, N, H, W):
+        """Activation (and, for training, gradient) buffers for N tiles of
+        H x W.  For inference, buffers made for more tiles of the same size
+        serve: a batch is the leading N*H*W rows of each NHWC buffer."""
         if self.shape == (N, H, W):
+            return
+        if self.inference and self.shape is not None and self.shape[1:] == (H, W) and self.shape[0] >= N:
             return
         t, dev, bf = self.torch, self.dev, self.torch.bfloat16
         e = lambda n, h, w, c: t.empty((n * h * w, c), dtype=bf, device=dev)

@@ -182,3 +182,22 @@ def test_training_on_traced_tiles_beats_constant(gpu):
     print("constant %.4f first %.4f last %.4f" % (const, loss[0], loss[-1]), flush=True)
     assert loss[-1] < 0.5 * loss[0]
     assert loss[-1] < 0.7 * const
+
+
+def test_predict_tiles_any_batch_same_bits(gpu):
+    """A tile's prediction depends on that tile alone: the same tiles in one
+    partial batch, in batches of three and one at a time give the same bits
+    (the gate predicts its few checked tiles as one short batch)."""
+    import torch
+    from slm_amd import gated
+    from slm_amd import surrogate as S
+    g = _small_grid(gpu, seed=8)
+    net = S.UNet(S.init_params(seed=4), device=gpu, inference=True)
+    origins = gated.tile_origins(g.nx, g.ny, 64, 8)[:7]
+    outs = []
+    for batch in (16, 3, 1):
+        out = torch.zeros((3, g.nxy), dtype=torch.float32, device=gpu)
+        gated.predict_tiles(net, g, origins, 64, 8, batch, out)
+        outs.append(out)
+    assert outs[0].abs().sum().item() > 0
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2])
