@@ -23,6 +23,29 @@ inline bool slm_res_is_pow2(const SlmParams* p) {
     return (float)u == r && (u & (u - 1u)) == 0u;
 }
 
+// The lanes of a wave that hold one key act through one atomic (neighbouring
+// items mostly share a key, so per-lane atomics queue on the same address):
+// per distinct key other than `none`, the lowest such lane adds the group's
+// size to ctr[key], and the group's lanes take consecutive positions after
+// the old value, in lane order.  Returns the lane's position (0 for `none`).
+// Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t wave_group_add(uint32_t key, uint32_t none, uint32_t* __restrict__ ctr) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t todo = __ballot(key != none);
+    uint32_t at = 0;
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const uint64_t same = __ballot(key == lk) & todo;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if ((same >> lane) & 1ull) at = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return at;
+}
+
 #define HIP_TRY(expr)                                                                  \
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \

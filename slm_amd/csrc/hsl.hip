@@ -354,32 +354,10 @@ __device__ void heap_down(HeapEnt* h, int32_t* pos, int n, int k) {
     pos[e.i] = k;
 }
 
-// The lanes of a wave that hold one key (neighbouring pixels mostly share
-// one) act through one atomic: per distinct non-zero key, the lowest such
-// lane adds the group's size and the group's lanes take consecutive
-// positions after the old value (in lane order).  Returns the lane's
-// position, or 0 for key 0.  Every lane of the wave calls it.
-__device__ __forceinline__ uint32_t fmm_group_add(uint32_t key, uint32_t* __restrict__ ctr) {
-    const uint32_t lane = threadIdx.x & 63u;
-    uint64_t todo = __ballot(key != 0u);
-    uint32_t at = 0;
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const uint32_t lk = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
-        const uint64_t same = __ballot(key == lk) & todo;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&ctr[lk], (uint32_t)__popcll(same));
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if ((same >> lane) & 1ull) at = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        todo &= ~same;
-    }
-    return at;
-}
-
 __global__ void k_fmm_lcount(uint32_t nx, uint32_t ny, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t key = i < nx * ny ? ((const uint32_t*)slot_base(L))[i] : 0u;
-    fmm_group_add(key, (uint32_t*)(slot_base(L) + L.off_lcnt));
+    wave_group_add(key, 0u, (uint32_t*)(slot_base(L) + L.off_lcnt));
 }
 
 // Exclusive scan of the per-label counts [0, n] in place (one 1024-thread
@@ -416,7 +394,7 @@ __global__ __launch_bounds__(1024) void k_fmm_lscan(uint32_t n, FmmLayout L) {
 __global__ void k_fmm_lscatter(uint32_t nx, uint32_t ny, FmmLayout L) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t key = i < nx * ny ? ((const uint32_t*)slot_base(L))[i] : 0u;
-    const uint32_t at = fmm_group_add(key, (uint32_t*)(slot_base(L) + L.off_lcur));
+    const uint32_t at = wave_group_add(key, 0u, (uint32_t*)(slot_base(L) + L.off_lcur));
     if (key) ((uint32_t*)(slot_base(L) + L.off_pts))[at] = i;
 }
 
