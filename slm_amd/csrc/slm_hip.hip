@@ -330,13 +330,11 @@ __device__ __forceinline__ uint32_t seed_tile_key(const SlmParams& P, float2 s) 
     return spread16(px >> 3) | (spread16(py >> 3) << 1);
 }
 
-// The seeds of one Morton tile may come in any order: the tracer's sums
-// commute and each seed record carries its own RNG key.
 __global__ void k_tile_count(SlmParams P, const float2* __restrict__ seeds, uint32_t n,
                              uint32_t* __restrict__ cnt) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t key = k < n ? seed_tile_key(P, seeds[P.seeds_chunk_offset + k]) : 0xFFFFFFFFu;
-    wave_group_add(key, 0xFFFFFFFFu, cnt);
+    if (k >= n) return;
+    atomicAdd(&cnt[seed_tile_key(P, seeds[P.seeds_chunk_offset + k])], 1u);
 }
 
 // Exclusive scan of cnt[0..n) in place by one 1024-thread block; writes cursor copy.
@@ -374,16 +372,15 @@ struct SeedRec {
 __global__ void k_tile_scatter(SlmParams P, const float2* __restrict__ seeds, uint32_t n,
                                uint32_t* __restrict__ cur, SeedRec* __restrict__ order) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const float2 s = k < n ? seeds[P.seeds_chunk_offset + k] : make_float2(0.f, 0.f);
-    const uint32_t key = k < n ? seed_tile_key(P, s) : 0xFFFFFFFFu;
-    const uint32_t at = wave_group_add(key, 0xFFFFFFFFu, cur);
     if (k >= n) return;
+    const float2 s = seeds[P.seeds_chunk_offset + k];
+    const uint32_t key = seed_tile_key(P, s);
     SeedRec r;
     r.x = s.x;
     r.y = s.y;
     r.seed_idx = P.seeds_chunk_offset + k;
     r.pad = 0u;
-    order[at] = r;
+    order[atomicAdd(&cur[key], 1u)] = r;
 }
 
 // ------------------------------------------------------------------ fields tracer
@@ -1993,19 +1990,30 @@ constexpr uint32_t kDescPerLaunch = 8;
         (out) = lo_;                                               \
     } while (0)
 
+// BATCH_FIND for item i of a wave whose items run on from i0 (the wave's
+// first item, the same in every lane): the job of i0 by one search with
+// scalar loads, and a lane's own search only when its item lies at or past
+// the next job's begin (a wave across a job boundary).
+#define BATCH_FIND_WAVE(desc, n_jobs, field, i0, i, out)                                 \
+    do {                                                                                 \
+        const uint32_t w0_ = (uint32_t)__builtin_amdgcn_readfirstlane((int)(i0));        \
+        uint32_t j0_;                                                                    \
+        BATCH_FIND(desc, n_jobs, field, w0_, j0_);                                       \
+        const uint32_t e_ = j0_ + 1u < (n_jobs) ? (desc)[j0_ + 1u].field : 0xFFFFFFFFu;  \
+        (out) = j0_;                                                                     \
+        if ((i) >= e_) BATCH_FIND(desc, n_jobs, field, i, out);                          \
+    } while (0)
+
 // The seed sort and quad build of every job, one launch each for all jobs.
 __global__ void k_batch_count(const BatchDesc* __restrict__ desc, uint32_t n_jobs, uint32_t n_seeds,
                               uint32_t* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = 0xFFFFFFFFu;
-    if (i < n_seeds) {
-        uint32_t j;
-        BATCH_FIND(desc, n_jobs, seed_begin, i, j);
-        const BatchDesc& D = desc[j];
-        const uint32_t k = i - D.seed_begin;
-        key = D.key_begin + seed_tile_key(D.P, D.seeds[D.P.seeds_chunk_offset + k]);
-    }
-    wave_group_add(key, 0xFFFFFFFFu, cnt);
+    if (i >= n_seeds) return;
+    uint32_t j;
+    BATCH_FIND_WAVE(desc, n_jobs, seed_begin, i - (threadIdx.x & 63u), i, j);
+    const BatchDesc& D = desc[j];
+    const uint32_t k = i - D.seed_begin;
+    atomicAdd(&cnt[D.key_begin + seed_tile_key(D.P, D.seeds[D.P.seeds_chunk_offset + k])], 1u);
 }
 
 // Exclusive scan of each job's counters: one 1024-thread block per job.
@@ -2040,30 +2048,25 @@ __global__ __launch_bounds__(1024) void k_batch_scan(const BatchDesc* __restrict
 __global__ void k_batch_scatter(const BatchDesc* __restrict__ desc, uint32_t n_jobs, uint32_t n_seeds,
                                 uint32_t* __restrict__ cur, SeedRec* __restrict__ order_all) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t j = 0, k = 0, key = 0xFFFFFFFFu;
-    float2 sd = make_float2(0.f, 0.f);
-    if (i < n_seeds) {
-        BATCH_FIND(desc, n_jobs, seed_begin, i, j);
-        k = i - desc[j].seed_begin;
-        sd = desc[j].seeds[desc[j].P.seeds_chunk_offset + k];
-        key = desc[j].key_begin + seed_tile_key(desc[j].P, sd);
-    }
-    const uint32_t at = wave_group_add(key, 0xFFFFFFFFu, cur);
     if (i >= n_seeds) return;
+    uint32_t j;
+    BATCH_FIND_WAVE(desc, n_jobs, seed_begin, i - (threadIdx.x & 63u), i, j);
     const BatchDesc& D = desc[j];
+    const uint32_t k = i - D.seed_begin;
+    const float2 sd = D.seeds[D.P.seeds_chunk_offset + k];
     SeedRec r;
     r.x = sd.x;
     r.y = sd.y;
     r.seed_idx = D.P.seeds_chunk_offset + k;
     r.pad = 0u;
-    order_all[D.seed_begin + at] = r;
+    order_all[D.seed_begin + atomicAdd(&cur[D.key_begin + seed_tile_key(D.P, sd)], 1u)] = r;
 }
 
 __global__ void k_batch_quads(const BatchDesc* __restrict__ desc, uint32_t n_jobs, size_t n_pix,
                               const float2* const* __restrict__ uvs, Quad* __restrict__ quad_all) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix; i += (size_t)gridDim.x * blockDim.x) {
         uint32_t j;
-        BATCH_FIND(desc, n_jobs, pix_begin, i, j);
+        BATCH_FIND_WAVE(desc, n_jobs, pix_begin, (uint32_t)i - (threadIdx.x & 63u), (uint32_t)i, j);
         const SlmParams& P = desc[j].P;
         const float2* uv = uvs[j];
         const size_t li = i - desc[j].pix_begin;
@@ -2339,11 +2342,8 @@ __global__ void k_batch_commit(const BatchDesc* __restrict__ desc, uint32_t n_jo
                                const unsigned long long* __restrict__ acc_all) {
     const bool ok = fields_packed_ok(ctrl, force_fallback);
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_pix; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = n_jobs;
-        while (hi - lo > 1u) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (desc[mid].pix_begin <= i) lo = mid; else hi = mid;
-        }
+        uint32_t lo;
+        BATCH_FIND_WAVE(desc, n_jobs, pix_begin, (uint32_t)i - (threadIdx.x & 63u), (uint32_t)i, lo);
         const BatchDesc* D = desc + lo;
         const size_t j = i - D->pix_begin;
         if (ok) {
